@@ -1,0 +1,283 @@
+"""Python host mirror of the residual-and-implicit-update path.
+
+Thin ctypes layer over libflowsolver.so (the C-ABI in include/fs_capi.h).
+It mirrors the reference's driver-level objects: ``Params`` is PARAMS
+(include/params.h:19-76), ``Mesh`` is the preprocessed GRID/SYS_MEM pair
+(mesh_io.C, maps.C, geometry.C, cv_calc.C, init_memory.C), ``Solver`` holds
+the device-resident state and exposes the reference's per-step functions
+(solve.C:44 solve, residual.C:40/359, linear_algebra.C:122, ...).
+
+There is no CPU fallback: constructing a Solver without the built library or
+without a HIP device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libflowsolver.so")
+
+FS_PREC_F64 = 0
+FS_PREC_MIXED = 1
+
+_MESH_ARRAYS = [
+    "x", "tri", "nn_map", "ncsn", "csn", "nnsn", "nsn", "esn", "edges", "c2e", "sub_edge", "sub_elem", "sub_n",
+    "sub_xm", "sub_xg", "el_cent", "bedge", "bedge_n", "bedge_xg", "cv_area", "moments", "edge_weight", "ia", "iau",
+    "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg",
+]
+_FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS"]
+
+
+class FlowSolverError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("fs error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(C.Structure):
+    """PARAMS: the 24 run controls of euler.params (params.C:44-217)."""
+    _fields_ = [
+        ("nsteps", C.c_int), ("iter", C.c_int), ("jufreq", C.c_int), ("CFL", C.c_double), ("ramp", C.c_int),
+        ("gamma", C.c_double), ("alpha", C.c_double), ("mach", C.c_double), ("subits", C.c_int),
+        ("unsteady", C.c_int), ("tacc", C.c_int), ("mintime", C.c_double), ("method", C.c_int), ("order", C.c_int),
+        ("foits", C.c_int), ("soits", C.c_int), ("toits", C.c_int), ("isrestart", C.c_int), ("limit", C.c_int),
+        ("reorder", C.c_int), ("ic", C.c_int), ("p_or_cv", C.c_int), ("start_post_pro", C.c_int),
+        ("post_pro_freq", C.c_int),
+    ]
+
+    @classmethod
+    def make(cls, **kw):
+        p = cls()
+        lib().fs_params_default(C.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise KeyError(k)
+            setattr(p, k, v)
+        return p
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MeshInfo(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nn", "ng", "ne", "nsub", "nbe", "nnz", "ntri", "nquad", "nb", "ncolor",
+                                       "nstencil", "prepared", "order_prepared")]
+
+
+class Timing(C.Structure):
+    _fields_ = [("step_ms", C.c_double), ("edge_ms", C.c_double), ("node_ms", C.c_double), ("gs_ms", C.c_double),
+                ("other_ms", C.c_double), ("flux_evals", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libflowsolver.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FlowSolverError(-7, "libflowsolver.so not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+    sig = {
+        "fs_last_error": (C.c_char_p, []),
+        "fs_device_count": (i, [C.POINTER(i)]),
+        "fs_params_default": (None, [vp]),
+        "fs_params_read": (i, [C.c_char_p, vp]),
+        "fs_params_write": (i, [C.c_char_p, vp]),
+        "fs_mesh_read": (i, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
+        "fs_mesh_write": (i, [vp, C.c_char_p, C.c_char_p]),
+        "fs_mesh_create_bump": (i, [i, i, d, C.c_uint64, C.POINTER(vp)]),
+        "fs_mesh_create_airfoil": (i, [i, i, d, d, d, C.c_uint64, C.POINTER(vp)]),
+        "fs_mesh_prepare": (i, [vp, vp]),
+        "fs_mesh_get_info": (i, [vp, vp]),
+        "fs_mesh_get_array": (i, [vp, i, C.POINTER(vp), C.POINTER(i64), C.POINTER(i)]),
+        "fs_mesh_destroy": (None, [vp]),
+        "fs_solver_create": (i, [vp, vp, i, C.POINTER(vp)]),
+        "fs_solver_destroy": (None, [vp]),
+        "fs_solver_init_freestream": (i, [vp]),
+        "fs_solver_set": (i, [vp, i, vp, i64]),
+        "fs_solver_get": (i, [vp, i, vp, i64]),
+        "fs_build_matrix": (i, [vp, i]),
+        "fs_gradient": (i, [vp]),
+        "fs_derivatives": (i, [vp]),
+        "fs_limiter": (i, [vp, i]),
+        "fs_interior_residual": (i, [vp, i, i]),
+        "fs_boundary_residual": (i, [vp, i]),
+        "fs_linear_solve": (i, [vp, C.POINTER(d)]),
+        "fs_solve": (i, [vp, i, C.POINTER(d), C.POINTER(d)]),
+        "fs_roe_flux_batch": (i, [i, d, vp, vp, vp, vp, vp]),
+        "fs_solve_steady": (i, [vp, i, d, C.POINTER(i), C.POINTER(d)]),
+        "fs_solver_time_steps": (i, [vp, i, i, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise FlowSolverError(rc, lib().fs_last_error().decode())
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().fs_device_count(C.byref(n)))
+    return n.value
+
+
+def read_params(path):
+    p = Params()
+    _check(lib().fs_params_read(path.encode(), C.byref(p)))
+    return p
+
+
+def write_params(path, p):
+    _check(lib().fs_params_write(path.encode(), C.byref(p)))
+
+
+class Mesh:
+    """A grid in the reference format plus its preprocessed maps/geometry."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def bump(cls, nx, ny, jitter=0.2, seed=1):
+        h = C.c_void_p()
+        _check(lib().fs_mesh_create_bump(nx, ny, jitter, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def airfoil(cls, nwall, nrad, h0=1e-3, rmax=30.0, jitter=0.1, seed=3):
+        h = C.c_void_p()
+        _check(lib().fs_mesh_create_airfoil(nwall, nrad, h0, rmax, jitter, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def read(cls, grid_path, bc_path):
+        h = C.c_void_p()
+        _check(lib().fs_mesh_read(grid_path.encode(), bc_path.encode() if bc_path else None, C.byref(h)))
+        return cls(h.value)
+
+    def write(self, grid_path, bc_path):
+        _check(lib().fs_mesh_write(self._h, grid_path.encode(), bc_path.encode() if bc_path else None))
+
+    def prepare(self, params):
+        _check(lib().fs_mesh_prepare(self._h, C.byref(params)))
+        return self
+
+    def info(self):
+        mi = MeshInfo()
+        _check(lib().fs_mesh_get_info(self._h, C.byref(mi)))
+        return {k: getattr(mi, k) for k, _ in mi._fields_}
+
+    def array(self, name):
+        """Copy of a preprocessed array (0-based indices)."""
+        which = _MESH_ARRAYS.index(name)
+        data, cnt, dt = C.c_void_p(), C.c_int64(), C.c_int()
+        _check(lib().fs_mesh_get_array(self._h, which, C.byref(data), C.byref(cnt), C.byref(dt)))
+        n = cnt.value
+        if n == 0:
+            return np.zeros(0, dtype=np.int32 if dt.value == 0 else np.float64)
+        ct = C.c_int32 if dt.value == 0 else C.c_double
+        buf = C.cast(data, C.POINTER(ct * n)).contents
+        return np.frombuffer(buf, dtype=np.int32 if dt.value == 0 else np.float64).copy()
+
+    def close(self):
+        if self._h:
+            lib().fs_mesh_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """Device-resident solver for one mesh on the current HIP device."""
+
+    def __init__(self, mesh, params, precision=FS_PREC_F64):
+        self.mesh = mesh
+        self.params = params
+        self.info = mesh.info()
+        h = C.c_void_p()
+        _check(lib().fs_solver_create(mesh._h, C.byref(params), precision, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().fs_solver_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _count(self, field):
+        i = self.info
+        return {"Q": 4 * (i["nn"] + i["ng"]), "nQ": 4 * (i["nn"] + i["ng"]), "dQ": 4 * i["nn"], "R": 4 * i["nn"],
+                "RHS": 4 * i["nn"], "grad": 8 * i["nn"], "phi": i["nn"], "hess": 20 * i["nn"], "pv": 4 * i["nn"],
+                "ev_con": i["nn"], "dt": i["nn"], "LHS": 16 * i["nnz"]}[field]
+
+    def get(self, field):
+        n = self._count(field)
+        out = np.empty(n, dtype=np.float64)
+        _check(lib().fs_solver_get(self._h, _FIELDS.index(field), out.ctypes.data, n))
+        return out
+
+    def set(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        _check(lib().fs_solver_set(self._h, _FIELDS.index(field), v.ctypes.data, v.size))
+
+    def init_freestream(self):
+        _check(lib().fs_solver_init_freestream(self._h))
+
+    # reference-function mirrors
+    def build_matrix(self, ts):
+        _check(lib().fs_build_matrix(self._h, ts))
+
+    def gradient(self):
+        _check(lib().fs_gradient(self._h))
+
+    def derivatives(self):
+        _check(lib().fs_derivatives(self._h))
+
+    def limiter(self, ts):
+        _check(lib().fs_limiter(self._h, ts))
+
+    def interior_residual(self, jupdate, ts):
+        _check(lib().fs_interior_residual(self._h, jupdate, ts))
+
+    def boundary_residual(self, jupdate):
+        _check(lib().fs_boundary_residual(self._h, jupdate))
+
+    def linear_solve(self):
+        r = C.c_double()
+        _check(lib().fs_linear_solve(self._h, C.byref(r)))
+        return r.value
+
+    def solve(self, ts):
+        dq, r2 = C.c_double(), C.c_double()
+        _check(lib().fs_solve(self._h, ts, C.byref(dq), C.byref(r2)))
+        return dq.value, r2.value
+
+    def solve_steady(self, max_steps, tol):
+        st, dr = C.c_int(), C.c_double()
+        _check(lib().fs_solve_steady(self._h, max_steps, tol, C.byref(st), C.byref(dr)))
+        return st.value, dr.value
+
+    def time_steps(self, ts_first, nsteps):
+        t = Timing()
+        _check(lib().fs_solver_time_steps(self._h, ts_first, nsteps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}

@@ -1,0 +1,175 @@
+/* C-ABI of the MI355X residual-and-implicit-update library (libflowsolver.so).
+ *
+ * Plain pointers and sizes only; every call returns an fs_status (0 = OK).
+ * Device state is resident in HBM between calls; host arrays move only
+ * through the explicit set/get calls.  Nothing here exits the process: the
+ * reference's fatal conditions (printf + exit) become status codes and
+ * fs_last_error() text.
+ *
+ * Each entry point names the reference interface it replaces
+ * (shedsaw/flow-solver, file:line).  The reference passes GRID and SYS_MEM pointers
+ * plus PARAMS by value; here the mesh and solver handles own the same data,
+ * laid out 0-based (reference index minus one).
+ */
+#ifndef FS_CAPI_H
+#define FS_CAPI_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  FS_OK = 0,
+  FS_ERR_ARG = -1,     /* invalid argument or size */
+  FS_ERR_IO = -2,      /* file could not be read/written */
+  FS_ERR_MESH = -3,    /* mesh rejected by preprocessing */
+  FS_ERR_HIP = -4,     /* HIP runtime error or no device */
+  FS_ERR_STATE = -5,   /* non-physical state: the reference would exit() */
+  FS_ERR_NOMEM = -6,   /* device memory exhausted or over the admission budget */
+  FS_ERR_UNSUPPORTED = -7
+} fs_status;
+
+/* Run controls: the 24 fields of the reference PARAMS
+ * (include/params.h:19-76) in the order params.C:44-217 reads them. */
+typedef struct {
+  int nsteps, iter, jufreq;
+  double CFL;
+  int ramp;
+  double gamma, alpha, mach;
+  int subits, unsteady, tacc;
+  double mintime;
+  int method, order, foits, soits, toits;
+  int isrestart, limit, reorder, ic, p_or_cv, start_post_pro, post_pro_freq;
+} fs_params;
+
+typedef struct fs_mesh fs_mesh;
+typedef struct fs_solver fs_solver;
+
+const char* fs_last_error(void);
+int fs_device_count(int* count);
+
+/* ---- parameters (params.C:27-219 read_parameters) ---- */
+void fs_params_default(fs_params* p);
+int fs_params_read(const char* path, fs_params* p);
+int fs_params_write(const char* path, const fs_params* p);
+
+/* ---- meshes: reference grid format (mesh_io.C:31-304 read_mesh,
+ *      mesh_io.C:318-395 write_mesh) and seeded generators ---- */
+int fs_mesh_read(const char* grid_path, const char* bc_path, fs_mesh** out);
+int fs_mesh_write(const fs_mesh* m, const char* grid_path, const char* bc_path);
+int fs_mesh_create_bump(int nx, int ny, double jitter, uint64_t seed, fs_mesh** out);
+int fs_mesh_create_airfoil(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed, fs_mesh** out);
+/* Preprocessing of main.C:149-559: Cuthill_McKee (reorder.C:58), metrics2D
+ * (metrics.C:31), CSN/NSN/ESN/c2e maps (maps.C:35-517), Build_Edge_Data and
+ * Build_Boundary_Edge_Data (geometry.C:119,407), Generate_Gaussian_Data_For_Edge
+ * (geometry.C:993), cv_calc_area_Green / cv_calc_Moments (cv_calc.C:34,981),
+ * Compute_Edge_Weights (gradient.C:923), Build_CRS (init_memory.C:36), the
+ * order-3 stencils (solve.C:57-114 with maps.C:1316 Add_Layer_to_GNSN) and
+ * the k=2 pseudo-inverse (compute_derivatives.C:859-940); plus the
+ * multicolor row coloring used by the device Gauss-Seidel. */
+int fs_mesh_prepare(fs_mesh* m, const fs_params* p);
+
+typedef struct {
+  int nn, ng, ne, nsub, nbe, nnz, ntri, nquad, nb, ncolor, nstencil;
+  int prepared, order_prepared;
+} fs_mesh_info;
+int fs_mesh_get_info(const fs_mesh* m, fs_mesh_info* info);
+
+typedef enum {
+  FS_M_X = 0,          /* f64 2*(nn+ng) node and ghost coordinates */
+  FS_M_TRI,            /* i32 3*ntri */
+  FS_M_NN_MAP,         /* i32 nn file order -> new id (empty without reorder) */
+  FS_M_NCSN, FS_M_CSN, /* i32 */
+  FS_M_NNSN, FS_M_NSN, FS_M_ESN,
+  FS_M_EDGES,          /* i32 2*ne */
+  FS_M_C2E,            /* i32 */
+  FS_M_SUB_EDGE, FS_M_SUB_ELEM,       /* i32 nsub */
+  FS_M_SUB_N,          /* f64 3*nsub */
+  FS_M_SUB_XM,         /* f64 2*nsub */
+  FS_M_SUB_XG,         /* f64 6*nsub */
+  FS_M_EL_CENT,        /* f64 2*nel */
+  FS_M_BEDGE,          /* i32 5*nbe */
+  FS_M_BEDGE_N,        /* f64 3*nbe */
+  FS_M_BEDGE_XG,       /* f64 6*nbe */
+  FS_M_CV_AREA,        /* f64 nn */
+  FS_M_MOMENTS,        /* f64 9*nn */
+  FS_M_EDGE_WEIGHT,    /* f64 4*ne */
+  FS_M_IA, FS_M_IAU, FS_M_JA,         /* i32 */
+  FS_M_COLOR, FS_M_COLOR_OFF, FS_M_COLOR_ROWS, /* i32 */
+  FS_M_ST_OFF, FS_M_ST_NODES,         /* i32 */
+  FS_M_ST_W,           /* f64 5 per stencil entry */
+  FS_M_BBC,            /* i32 nb */
+  FS_M_BSEG_OFF, FS_M_BSEG, /* i32 */
+  FS_M_COUNT
+} fs_mesh_array;
+/* Borrowed host pointer to a preprocessed array; dtype 0 = int32, 1 = f64. */
+int fs_mesh_get_array(const fs_mesh* m, int which, const void** data, int64_t* count, int* dtype);
+void fs_mesh_destroy(fs_mesh* m);
+
+/* ---- solver on one GPU ----
+ * precision FS_PREC_F64: everything in FP64, the reference's algorithms
+ * (finite-difference Jacobian, Gauss-Seidel sweeps) on multicolor order.
+ * precision FS_PREC_MIXED: FP32 flux, reconstruction, limiter, analytic
+ * Jacobian (jacobian.C:301-658) and sweeps inside an FP64 defect-correction
+ * loop (FP64 residual, norms and update). */
+enum { FS_PREC_F64 = 0, FS_PREC_MIXED = 1 };
+int fs_solver_create(const fs_mesh* m, const fs_params* p, int precision, fs_solver** out);
+void fs_solver_destroy(fs_solver* s);
+/* init_pv with ic_0 freestream, ghosts copied (ic.C:36-99, ic.C:808-913) */
+int fs_solver_init_freestream(fs_solver* s);
+typedef enum {
+  FS_F_Q = 0,   /* f64 (nn+ng)*4: state at the start of the step, ghosts included */
+  FS_F_NQ,      /* f64 (nn+ng)*4: Newton iterate */
+  FS_F_DQ,      /* f64 nn*4 */
+  FS_F_R,       /* f64 nn*4 */
+  FS_F_RHS,     /* f64 nn*4 */
+  FS_F_GRAD,    /* f64 nn*8  [n][var][x,y] */
+  FS_F_PHI,     /* f64 nn     limiter scalar per node */
+  FS_F_HESS,    /* f64 nn*20 [n][var][x,y,xx,yy,xy] */
+  FS_F_PV,      /* f64 nn*4 point values */
+  FS_F_EVCON,   /* f64 nn */
+  FS_F_DT,      /* f64 nn */
+  FS_F_LHS,     /* f64 nnz*16 row-major 4x4 blocks */
+  FS_F_COUNT
+} fs_field;
+/* host <-> device copies (synchronous); n is the element count */
+int fs_solver_set(fs_solver* s, int field, const double* host, int64_t n);
+int fs_solver_get(fs_solver* s, int field, double* host, int64_t n);
+
+/* Reference-function mirrors, each one device pass over the resident state.
+ * ts is the pseudo-time step (grid->citer in the reference). */
+int fs_build_matrix(fs_solver* s, int ts);                      /* init_memory.C:159 build_matrix */
+int fs_gradient(fs_solver* s);                                  /* gradient.C:245 Compute_Gradient_LeastSquares */
+int fs_derivatives(fs_solver* s);                               /* compute_derivatives.C:40 Compute_Derivatives(.,3) */
+int fs_limiter(fs_solver* s, int ts);                           /* limiter.C:39 compute_limiter_barth */
+int fs_interior_residual(fs_solver* s, int jupdate, int ts);    /* residual.C:40 build_interior_residuals */
+int fs_boundary_residual(fs_solver* s, int jupdate);            /* residual.C:359 build_boundary_residuals */
+int fs_linear_solve(fs_solver* s, double* rms);                 /* linear_algebra.C:122 linear_solve */
+/* solve.C:44 solve(): one pseudo-time step; returns ||dQ||_2 of the last
+ * Newton iteration and sum(R^2) of the last residual. */
+int fs_solve(fs_solver* s, int ts, double* dq_norm, double* r2);
+/* Batched Roe_flux_centered (flux.C:299): n faces, device pointers;
+ * normals[3*i] = (nx, ny, len), qL/qR/flux 4 per face. */
+int fs_roe_flux_batch(int n, double gamma, const double* d_normals, const double* d_qL, const double* d_qR,
+                      double* d_flux, void* stream);
+
+/* Steady solve to ||R||/||R0|| <= tol (R0 = residual at the first step of the
+ * highest-order phase) or max_steps; returns steps taken and the drop. */
+int fs_solve_steady(fs_solver* s, int max_steps, double tol, int* steps, double* drop);
+
+/* ---- timing helpers for the benchmark: device-side event timing of the
+ *      pseudo-time step and of the dominant kernel ---- */
+typedef struct {
+  double step_ms;        /* mean wall time per pseudo-time step */
+  double edge_ms;        /* mean device time of the edge flux kernel */
+  double node_ms;        /* mean device time of the node assembly kernel */
+  double gs_ms;          /* mean device time of all Gauss-Seidel sweeps */
+  double other_ms;
+  int64_t flux_evals;    /* residual Roe evaluations per step (subedges+bedges) */
+} fs_timing;
+int fs_solver_time_steps(fs_solver* s, int ts_first, int nsteps, fs_timing* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

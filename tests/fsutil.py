@@ -1,0 +1,174 @@
+"""Shared test helpers: load the product mirror, the oracle (the checker) and
+reference-build dumps.  Only tests, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use the oracle."""
+import ctypes as C
+import importlib.util
+import os
+import struct
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_ref", "liboracle.so")
+REF_DRIVER = os.path.join(ORACLE_DIR, "_ref", "ref_driver")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def load_flowsolver():
+    path = os.path.join(ROOT, "flow-solver_amd", "flowsolver.py")
+    spec = importlib.util.spec_from_file_location("flowsolver", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+fsm = load_flowsolver()
+
+
+# ---------------------------------------------------------------- oracle ----
+class FsoMesh(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nn", "ng", "ne", "nsub", "nbe", "nnz", "nb", "nel")] + [
+        ("x", C.c_void_p), ("edges", C.c_void_p), ("sub_edge", C.c_void_p), ("sub_elem", C.c_void_p),
+        ("sub_n", C.c_void_p), ("sub_xm", C.c_void_p), ("sub_xg", C.c_void_p), ("el_cent", C.c_void_p),
+        ("bedge", C.c_void_p), ("bedge_n", C.c_void_p), ("bbc", C.c_void_p), ("cv_area", C.c_void_p),
+        ("moments", C.c_void_p), ("edge_weight", C.c_void_p), ("ia", C.c_void_p), ("iau", C.c_void_p),
+        ("ja", C.c_void_p), ("st_off", C.c_void_p), ("st_nodes", C.c_void_p), ("ncolor", C.c_int),
+        ("color_off", C.c_void_p), ("color_rows", C.c_void_p)]
+
+
+class FsoState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("Q", "nQ", "pQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "W",
+                                          "ev_con", "dt", "LHS")] + [
+        ("citer", C.c_int), ("cfl", C.c_double), ("last_rms", C.c_double), ("last_lin", C.c_double)]
+
+
+_olib = None
+
+
+def oracle_lib():
+    global _olib
+    if _olib is None:
+        if not os.path.exists(ORACLE_LIB):
+            subprocess.run(["make", "-C", ORACLE_DIR, "all"], check=True, capture_output=True)
+        L = C.CDLL(ORACLE_LIB)
+        vp, d, i = C.c_void_p, C.c_double, C.c_int
+        L.fso_roe_flux.argtypes = [d, d, d, d, vp, vp, vp]
+        L.fso_init_freestream.argtypes = [vp, vp, vp]
+        L.fso_init_freestream.restype = None
+        L.fso_build_matrix.argtypes = [vp, vp, vp]
+        L.fso_gradient.argtypes = [vp, vp]
+        L.fso_gradient.restype = None
+        L.fso_build_pinv.argtypes = [vp, vp]
+        L.fso_derivatives.argtypes = [vp, vp]
+        L.fso_derivatives.restype = None
+        L.fso_limiter_barth.argtypes = [vp, vp, vp]
+        L.fso_limiter_barth.restype = None
+        L.fso_interior_residual.argtypes = [vp, vp, vp, i, i]
+        L.fso_interior_residual.restype = None
+        L.fso_boundary_residual.argtypes = [vp, vp, vp, i]
+        L.fso_linear_solve.argtypes = [vp, vp, vp, i]
+        L.fso_linear_solve.restype = d
+        L.fso_solve.argtypes = [vp, vp, vp, i, i]
+        L.fso_solve.restype = d
+        _olib = L
+    return _olib
+
+
+class Oracle:
+    """Holds a mesh (arrays from the product's preprocessing) and an FP64 state
+    for the C restatement."""
+
+    def __init__(self, mesh, params):
+        self.params = params
+        info = mesh.info()
+        self.info = info
+        keep = {}
+
+        def arr(name, dtype):
+            a = np.ascontiguousarray(mesh.array(name), dtype=dtype)
+            keep[name] = a
+            return a.ctypes.data
+
+        m = FsoMesh()
+        m.nn, m.ng, m.ne, m.nsub, m.nbe, m.nnz, m.nb = (info[k] for k in ("nn", "ng", "ne", "nsub", "nbe", "nnz", "nb"))
+        m.nel = info["ntri"] + info["nquad"]
+        for name in ("x", "sub_n", "sub_xm", "sub_xg", "el_cent", "bedge_n", "cv_area", "moments", "edge_weight"):
+            setattr(m, name, arr(name, np.float64))
+        for name in ("edges", "sub_edge", "sub_elem", "bedge", "bbc", "ia", "iau", "ja", "color_off", "color_rows"):
+            setattr(m, name, arr(name, np.int32))
+        m.ncolor = info["ncolor"]
+        if info["nstencil"] > 0:
+            m.st_off = arr("st_off", np.int32)
+            m.st_nodes = arr("st_nodes", np.int32)
+        self._keep = keep
+        self.m = m
+        nn, ng, nnz = info["nn"], info["ng"], info["nnz"]
+        self.a = {
+            "Q": np.zeros((nn + ng) * 4), "nQ": np.zeros((nn + ng) * 4), "pQ": np.zeros((nn + ng) * 4),
+            "dQ": np.zeros(nn * 4), "R": np.zeros(nn * 4), "RHS": np.zeros(nn * 4), "grad": np.zeros(nn * 8),
+            "phi": np.ones(nn * 8), "hess": np.zeros(nn * 20), "pv": np.zeros(nn * 4),
+            "W": np.zeros(max(1, 5 * info["nstencil"])), "ev_con": np.zeros(nn), "dt": np.zeros(nn),
+            "LHS": np.zeros(nnz * 16),
+        }
+        s = FsoState()
+        for k, v in self.a.items():
+            setattr(s, k, v.ctypes.data)
+        s.citer = 0
+        self.s = s
+        self.L = oracle_lib()
+        if info["nstencil"] > 0:
+            rc = self.L.fso_build_pinv(C.byref(self.m), C.byref(self.s))
+            assert rc == 0, rc
+
+    def init_freestream(self):
+        self.L.fso_init_freestream(C.byref(self.m), C.byref(self.params), C.byref(self.s))
+
+    def solve(self, ts, multicolor=0):
+        r = self.L.fso_solve(C.byref(self.m), C.byref(self.params), C.byref(self.s), ts, multicolor)
+        assert r >= 0, "oracle step failed"
+        return r
+
+    def __getitem__(self, k):
+        return self.a[k]
+
+
+def roe_oracle(nx, ny, ln, gamma, ql, qr):
+    out = np.zeros(4)
+    ql = np.ascontiguousarray(ql, dtype=np.float64)
+    qr = np.ascontiguousarray(qr, dtype=np.float64)
+    oracle_lib().fso_roe_flux(nx, ny, ln, gamma, ql.ctypes.data, qr.ctypes.data, out.ctypes.data)
+    return out
+
+
+# ------------------------------------------------------ reference dumps ----
+def read_dump(path):
+    """Arrays written by oracle/ref_driver (name[32], type i32, count i64, data)."""
+    out = {}
+    with open(path, "rb") as f:
+        data = f.read()
+    off = 0
+    while off < len(data):
+        name = data[off:off + 32].split(b"\0", 1)[0].decode()
+        off += 32
+        typ, = struct.unpack_from("<i", data, off)
+        off += 4
+        cnt, = struct.unpack_from("<q", data, off)
+        off += 8
+        dt = np.int32 if typ == 0 else np.float64
+        nbytes = cnt * np.dtype(dt).itemsize
+        out[name] = np.frombuffer(data[off:off + nbytes], dtype=dt).copy()
+        off += nbytes
+    return out
+
+
+def write_case(dirpath, mesh, params):
+    os.makedirs(dirpath, exist_ok=True)
+    mesh.write(os.path.join(dirpath, "grid.mesh"), os.path.join(dirpath, "bc.dat"))
+    fsm.write_params(os.path.join(dirpath, "euler.params"), params)
+
+
+def norm_resid_err(Rgpu, Rref, absflux):
+    """max over nodes/vars of |R - Rref| / (sum of |face flux| at the node)."""
+    return float(np.max(np.abs(Rgpu - Rref) / (absflux + 1e-300)))
