@@ -1,0 +1,121 @@
+"""Benchmark: edge-flux evaluations per second of the residual-and-implicit-
+update path on one MI355X.
+
+Workload (default): bump channel 1025x257 nodes (524,288 triangles, seeded
++-0.2 jitter), M=0.675, k=2 (order 3) with the Barth limiter, FP64, the
+reference's implicit pseudo-time step (finite-difference Jacobian, 20
+symmetric Gauss-Seidel sweeps).  Warmup steps walk through the first-order and
+linear phases into the k=2 phase; the timed steps are k=2-phase steps.
+
+One step = one pseudo-time step (one Newton iteration): residual + Jacobian +
+linear sweeps + update over the whole mesh.  Edge-flux evals per step =
+subedges + boundary edges (the residual's Roe evaluations; the 8 extra
+Jacobian evaluations per subedge are reported separately).
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_edge_kernel(info, recon):
+    """Compulsory HBM bytes of one edge flux+Jacobian launch (FP64)."""
+    ne, nsub, nn = info["ne"], info["nsub"], info["nn"]
+    b = ne * (8 + 8 + 8)            # (L,R), subedge ids, the two block slots
+    b += nsub * 24                  # (nx, ny, len)
+    b += nsub * 32                  # per-subedge flux written
+    b += ne * 2 * 128               # two off-diagonal 4x4 FP64 blocks written
+    b += nn * 32                    # node states read once
+    if recon == 1:
+        b += nsub * 16 + nn * (64 + 8 + 16)   # face point; grad, phi, x per node
+    return b
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--nx", type=int, default=1025)
+    ap.add_argument("--ny", type=int, default=257)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="k=2-phase steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(U, fsm, p, nx, ny, steps):
+    """The oracle (single-threaded C restatement) on the same workload: the
+    untimed steps up to the k=2 phase, then `steps` timed k=2-phase steps."""
+    mesh = fsm.Mesh.bump(nx, ny, 0.2, 2)
+    mesh.prepare(p)
+    info = mesh.info()
+    o = U.Oracle(mesh, p)
+    o.init_freestream()
+    first = p.foits + p.soits + 1
+    for ts in range(1, first):
+        o.solve(ts, multicolor=0)
+    t0 = time.perf_counter()
+    for ts in range(first, first + steps):
+        o.solve(ts, multicolor=0)
+    dt = time.perf_counter() - t0
+    evals = (info["nsub"] + info["nbe"]) * steps
+    return {"value": evals / dt, "unit": "edge-flux evals/s", "cores": 1, "kind": "port",
+            "sample": "%d k=2-phase pseudo-time steps of the same %dx%d bump channel (%.1f s)" % (steps, nx, ny, dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import fsutil as U
+
+    fsm = U.fsm
+    p = fsm.Params.make(nsteps=a.warmup + a.steps, iter=1, jufreq=1, CFL=50.0, ramp=20, gamma=1.4, alpha=0.0,
+                        mach=0.675, subits=20, method=1, order=3, foits=2, soits=2, limit=1, reorder=1)
+    t0 = time.perf_counter()
+    mesh = fsm.Mesh.bump(a.nx, a.ny, 0.2, 2)
+    mesh.prepare(p)
+    t_setup = time.perf_counter() - t0
+    info = mesh.info()
+    solver = fsm.Solver(mesh, p, fsm.FS_PREC_F64)
+    solver.init_freestream()
+    warm = max(a.warmup, p.foits + p.soits)
+    for ts in range(1, warm + 1):
+        solver.solve(ts)
+    tim = solver.time_steps(warm + 1, a.steps)
+    ms = tim["step_ms"]
+    evals = info["nsub"] + info["nbe"]
+    value = evals / (ms * 1e-3)
+    abytes = algorithmic_bytes_edge_kernel(info, recon=0)
+    achieved = abytes / (tim["edge_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "edge-flux evals/s", "value": value, "unit": "edge-flux evals/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": warm, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "bump channel %dx%d, M=0.675, k=2 (order 3), Barth limiter, FP64 implicit step"
+                               % (a.nx, a.ny), "nodes": info["nn"], "subedges": info["nsub"],
+                   "bedges": info["nbe"], "lhs_blocks": info["nnz"], "precision": "fp64", "subits": p.subits},
+        "roofline": {"bound": "hbm", "kernel": "k_edge_flux (Roe flux + FD Jacobian blocks)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": abytes, "avg_ms": tim["edge_ms"]},
+        "kernel_ms": {"edge_flux": tim["edge_ms"], "node_assembly": tim["node_ms"], "gauss_seidel": tim["gs_ms"],
+                      "other": tim["other_ms"]},
+        "jacobian_flux_evals_per_step": 8 * info["nsub"],
+        "setup_s": t_setup,
+    }
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(U, fsm, p, a.nx, a.ny, a.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -172,3 +172,23 @@ def write_case(dirpath, mesh, params):
 def norm_resid_err(Rgpu, Rref, absflux):
     """max over nodes/vars of |R - Rref| / (sum of |face flux| at the node)."""
     return float(np.max(np.abs(Rgpu - Rref) / (absflux + 1e-300)))
+
+
+def oracle_call(orc, name, *args):
+    """Call one oracle function on (mesh, params?, state) with extra ints."""
+    L = orc.L
+    f = getattr(L, name)
+    if name in ("fso_gradient", "fso_derivatives", "fso_build_pinv"):
+        return f(C.byref(orc.m), C.byref(orc.s))
+    return f(C.byref(orc.m), C.byref(orc.params), C.byref(orc.s), *args)
+
+
+def perturbed_state(nn, ng, params, seed, amp=0.05):
+    """Freestream (ic.C:808-840) times (1 + amp*U(-1,1)) per component."""
+    a = params.alpha * np.pi / 180.0
+    u, v = params.mach * np.cos(a), params.mach * np.sin(a)
+    P = 1.0 / params.gamma
+    q = np.array([1.0, u, v, P / (params.gamma - 1.0) + 0.5 * (u * u + v * v)])
+    rng = np.random.default_rng(seed)
+    Q = np.tile(q, nn + ng) * (1.0 + amp * rng.uniform(-1.0, 1.0, 4 * (nn + ng)))
+    return Q

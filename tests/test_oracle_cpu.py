@@ -1,0 +1,211 @@
+"""Pins the oracle (the C restatement of the reference path).
+
+* Against the reference build: step-by-step histories (R, dQ, Q, LHS, phi,
+  grad, ev_con, dt, returned |dQ|) must agree bit for bit, and the converged
+  unlimited k=1 state on the 97x33 configuration to 1e-13.
+* Against mathematics, for what the reference build cannot run here (the k=2
+  factorization needs LAPACK): the pseudo-inverse against numpy's pinv, and the
+  k=2 derivatives exact for control-volume averages of a quadratic.
+* Roe flux: consistency, conservation, supersonic upwinding; LSQ gradient exact
+  for linear fields; Gauss-Seidel converges to the dense solution.
+A planted mistake (sign, index, dropped term) fails at least one of these.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import fsutil as U
+
+fsm = U.fsm
+
+
+def _load(name):
+    return np.load(os.path.join(U.GOLDEN, name + ".npz"))
+
+
+def _setup(g, **over):
+    nx, ny, seed, order, foits, soits, limit, nsteps = (int(v) for v in g["params"])
+    cfl, mach, alpha, ramp, subits = g["params_f"]
+    kw = dict(order=order, foits=foits, soits=soits, limit=limit, mach=mach, alpha=alpha, CFL=cfl, ramp=int(ramp),
+              subits=int(subits), method=1, iter=1, jufreq=1, reorder=1, gamma=1.4, nsteps=nsteps)
+    kw.update(over)
+    p = fsm.Params.make(**kw)
+    m = fsm.Mesh.bump(nx, ny, 0.2, seed)
+    m.prepare(p)
+    o = U.Oracle(m, p)
+    o.init_freestream()
+    return p, m, o, nsteps
+
+
+@pytest.mark.parametrize("case", ["bump17x9_o1", "bump17x9_o3", "cfg1_97x33"])
+def test_step_history_bitexact(case):
+    g = _load(case)
+    p, m, o, nsteps = _setup(g)
+    np.testing.assert_array_equal(o["Q"], g["s_Q0"][4:])
+    for ts in range(1, nsteps + 1):
+        dq = o.solve(ts, multicolor=0)
+        assert dq == g["rms_%d" % ts][0], (ts, dq, g["rms_%d" % ts][0])
+        np.testing.assert_array_equal(o["R"], g["R_%d" % ts][4:])
+        np.testing.assert_array_equal(o["dQ"], g["dQ_%d" % ts][4:])
+        np.testing.assert_array_equal(o["Q"], g["Q_%d" % ts][4:])
+        for f in ("phi", "grad"):
+            if "%s_%d" % (f, ts) in g:
+                np.testing.assert_array_equal(o[f], g["%s_%d" % (f, ts)][8:])
+        for f in ("ev_con", "dt"):
+            if "%s_%d" % (f, ts) in g:
+                np.testing.assert_array_equal(o[f], g["%s_%d" % (f, ts)][1:])
+        if "LHS_%d" % ts in g:
+            np.testing.assert_array_equal(o["LHS"], g["LHS_%d" % ts])
+
+
+def test_converged_state_matches_reference():
+    g = _load("cfg1_97x33_steady")
+    p, m, o, nsteps = _setup(g)
+    for ts in range(1, nsteps + 1):
+        dq = o.solve(ts, multicolor=0)
+        if "rms_%d" % ts in g:
+            assert dq == g["rms_%d" % ts][0]
+    nn = m.info()["nn"]
+    rho, rho_ref = o["Q"][:4 * nn:4], g["Qfinal"][4:4 * nn + 4:4]
+    assert np.linalg.norm(rho - rho_ref) / np.linalg.norm(rho_ref) <= 1e-13
+    # and the state is converged: residual drop below 1e-10 relative to the first linear step
+    assert np.sqrt(o.s.last_rms) < 1e-9
+
+
+def _state(rho, u, v, p, g=1.4):
+    return np.array([rho, rho * u, rho * v, p / (g - 1) + 0.5 * rho * (u * u + v * v)])
+
+
+def _phys_flux(q, nx, ny, ln, g=1.4):
+    rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+    P = (g - 1) * (q[3] - 0.5 * rho * (u * u + v * v))
+    th = u * nx + v * ny
+    return np.array([rho * th, rho * u * th + P * nx, rho * v * th + P * ny, th * (q[3] + P)]) * ln
+
+
+def test_roe_flux_properties():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        a = rng.uniform(0, 2 * np.pi)
+        nx, ny, ln = np.cos(a), np.sin(a), rng.uniform(0.1, 2.0)
+        qL = _state(rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.3, 2))
+        qR = _state(rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.3, 2))
+        f = U.roe_oracle(nx, ny, ln, 1.4, qL, qR)
+        # conservation: f(qL,qR;n) = -f(qR,qL;-n)
+        np.testing.assert_allclose(f, -U.roe_oracle(-nx, -ny, ln, 1.4, qR, qL), rtol=1e-12, atol=1e-13)
+        # consistency: f(q,q) = F(q).n len
+        np.testing.assert_allclose(U.roe_oracle(nx, ny, ln, 1.4, qL, qL), _phys_flux(qL, nx, ny, ln),
+                                   rtol=1e-13, atol=1e-14)
+    # supersonic flow along +n: the upwind (left) flux
+    qL = _state(1.0, 3.0, 0.1, 1 / 1.4)
+    qR = _state(1.1, 3.1, 0.0, 0.8 / 1.4)
+    np.testing.assert_allclose(U.roe_oracle(1.0, 0.0, 1.0, 1.4, qL, qR), _phys_flux(qL, 1, 0, 1), rtol=1e-12)
+
+
+def _prepared(nx, ny, order=3):
+    p = fsm.Params.make(order=order, foits=1, soits=1, limit=1, reorder=1, mach=0.5)
+    m = fsm.Mesh.bump(nx, ny, 0.2, 5)
+    m.prepare(p)
+    return p, m
+
+
+def test_gradient_exact_for_linear_fields():
+    p, m = _prepared(25, 11, order=1)
+    o = U.Oracle(m, p)
+    nn = m.info()["nn"]
+    x = m.array("x")[:2 * nn].reshape(-1, 2)
+    coef = np.array([[1.0, 0.3, -0.2], [0.5, 2.0, 0.7], [-0.1, 0.4, 1.5], [3.0, -1.0, 0.25]])
+    Q = np.zeros((nn + m.info()["ng"], 4))
+    for v in range(4):
+        Q[:nn, v] = coef[v, 0] + coef[v, 1] * x[:, 0] + coef[v, 2] * x[:, 1]
+    o["nQ"][:] = Q.ravel()
+    U.oracle_call(o, "fso_gradient")
+    gr = o["grad"].reshape(nn, 4, 2)
+    for v in range(4):
+        np.testing.assert_allclose(gr[:, v, 0], coef[v, 1], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(gr[:, v, 1], coef[v, 2], rtol=0, atol=1e-10)
+
+
+def _quadratic_cv_averages(m, c):
+    """CV averages of f = c0 + c1 x + c2 y + c3 x^2/2 + c4 y^2/2 + c5 x y from
+    the control-volume moments (exact for quadratics)."""
+    nn = m.info()["nn"]
+    x = m.array("x")[:2 * nn].reshape(-1, 2)
+    M = m.array("moments").reshape(-1, 9)
+    X, Y = x[:, 0], x[:, 1]
+    f = c[0] + c[1] * X + c[2] * Y + 0.5 * c[3] * X * X + 0.5 * c[4] * Y * Y + c[5] * X * Y
+    fx = c[1] + c[3] * X + c[5] * Y
+    fy = c[2] + c[4] * Y + c[5] * X
+    avg = f + fx * M[:, 0] + fy * M[:, 1] + 0.5 * c[3] * M[:, 2] + 0.5 * c[4] * M[:, 3] + c[5] * M[:, 4]
+    return f, fx, fy, avg
+
+
+def test_k2_derivatives_exact_for_quadratics():
+    p, m = _prepared(33, 13)
+    o = U.Oracle(m, p)
+    info = m.info()
+    nn = info["nn"]
+    coefs = [np.array([1.0, 0.2, -0.3, 0.5, -0.25, 0.4]), np.array([0.3, -1.0, 0.5, 2.0, 1.0, -0.5]),
+             np.array([-0.2, 0.1, 0.9, -0.6, 0.3, 0.8]), np.array([2.5, 0.0, 0.0, 1.0, 1.0, 0.0])]
+    Q = np.zeros((nn + info["ng"], 4))
+    ex = []
+    for v, c in enumerate(coefs):
+        f, fx, fy, avg = _quadratic_cv_averages(m, c)
+        Q[:nn, v] = avg
+        ex.append((f, fx, fy, c))
+    o["nQ"][:] = Q.ravel()
+    U.oracle_call(o, "fso_derivatives")
+    h = o["hess"].reshape(nn, 4, 5)
+    pv = o["pv"].reshape(nn, 4)
+    for v, (f, fx, fy, c) in enumerate(ex):
+        np.testing.assert_allclose(h[:, v, 0], fx, atol=1e-8)
+        np.testing.assert_allclose(h[:, v, 1], fy, atol=1e-8)
+        np.testing.assert_allclose(h[:, v, 2], c[3], atol=1e-6)
+        np.testing.assert_allclose(h[:, v, 3], c[4], atol=1e-6)
+        np.testing.assert_allclose(h[:, v, 4], c[5], atol=1e-6)
+        np.testing.assert_allclose(pv[:, v], f, atol=1e-10)
+
+
+def test_pseudo_inverse_vs_numpy_and_product():
+    p, m = _prepared(21, 9)
+    o = U.Oracle(m, p)
+    nn = m.info()["nn"]
+    off, st = m.array("st_off"), m.array("st_nodes")
+    x = m.array("x")[:2 * nn].reshape(-1, 2)
+    M = m.array("moments").reshape(-1, 9)
+    Wp = m.array("st_W")
+    Wo = o["W"]
+    for i in range(0, nn, 7):
+        nb = st[off[i]:off[i + 1]]
+        d = x[nb] - x[i]
+        w = 1.0 / np.hypot(d[:, 0], d[:, 1])
+        A = np.stack([d[:, 0] + M[nb, 0] - M[i, 0], d[:, 1] + M[nb, 1] - M[i, 1],
+                      0.5 * (d[:, 0] ** 2 + 2 * M[nb, 0] * d[:, 0] + M[nb, 2] - M[i, 2]),
+                      0.5 * (d[:, 1] ** 2 + 2 * M[nb, 1] * d[:, 1] + M[nb, 3] - M[i, 3]),
+                      M[nb, 4] + M[nb, 0] * d[:, 1] + M[nb, 1] * d[:, 0] + d[:, 0] * d[:, 1] - M[i, 4]], 1)
+        Wnp = np.linalg.pinv(w[:, None] * A) * w[None, :]
+        mm = len(nb)
+        a, b = 5 * off[i], 5 * off[i] + 5 * mm
+        scale = np.max(np.abs(Wnp))
+        assert np.max(np.abs(Wo[a:b].reshape(5, mm) - Wnp)) <= 1e-9 * scale
+        assert np.max(np.abs(Wp[a:b].reshape(5, mm) - Wnp)) <= 1e-9 * scale
+
+
+def test_gauss_seidel_converges_to_dense_solve():
+    g = _load("bump17x9_o1")
+    p, m, o, _ = _setup(g, subits=400)
+    o.solve(1, multicolor=0)  # assembles LHS, RHS for step 1
+    info = m.info()
+    nn = info["nn"]
+    ia, ja = m.array("ia"), m.array("ja")
+    A = np.zeros((4 * nn, 4 * nn))
+    L = o["LHS"].reshape(-1, 4, 4)
+    for r in range(nn):
+        for k in range(ia[r], ia[r + 1]):
+            A[4 * r:4 * r + 4, 4 * ja[k]:4 * ja[k] + 4] = L[k]
+    x = np.linalg.solve(A, o["RHS"])
+    for mc in (0, 1):
+        o["dQ"][:] = 0
+        U.oracle_call(o, "fso_linear_solve", mc)
+        assert np.max(np.abs(o["dQ"] - x)) <= 1e-10 * np.max(np.abs(x))
