@@ -65,6 +65,7 @@ struct fs_solver {
   fs_params prm{};
   const fs::Mesh* host = nullptr;
   std::vector<int> color_off;
+  std::vector<int> slot_dev;   // reference block slot -> device (color-major) slot
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<DBuf>> bufs;
   fsk::DevMesh dm{};
@@ -342,8 +343,35 @@ int fs_solver_create(const fs_mesh* mh, const fs_params* p, int precision, fs_so
   std::vector<int> edge_sub(2 * (size_t)g.ne, -1), edge_slot(2 * (size_t)g.ne, 0);
   for (int e = 0; e < g.ne; ++e) {
     for (int k = g.esub_off[e], t = 0; k < g.esub_off[e + 1] && t < 2; ++k, ++t) edge_sub[2 * (size_t)e + t] = g.esub_list[k];
-    edge_slot[2 * (size_t)e] = g.edge_slot_lr[e];
-    edge_slot[2 * (size_t)e + 1] = g.edge_slot_rl[e];
+  }
+  // color-major block CSR: rows in color order, each row's blocks in the
+  // reference order (nsn order, diagonal last); slot_dev maps a reference slot
+  // to its device slot
+  s->slot_dev.assign(g.ja.size(), 0);
+  std::vector<int> rstart(g.nn), iau_d(g.nn), ja_d(g.ja.size()), slot_t_d(g.ja.size());
+  {
+    int pos = 0;
+    for (int t = 0; t < g.nn; ++t) {
+      int j = g.color_rows[t];
+      rstart[j] = pos;
+      for (int k = g.ia[j]; k < g.ia[j + 1]; ++k) s->slot_dev[k] = pos++;
+      iau_d[j] = s->slot_dev[g.iau[j]];
+    }
+    for (size_t k = 0; k < g.ja.size(); ++k) {
+      ja_d[s->slot_dev[k]] = g.ja[k];
+      slot_t_d[s->slot_dev[k]] = s->slot_dev[g.slot_t[k]];
+    }
+    for (int e = 0; e < g.ne; ++e) {
+      edge_slot[2 * (size_t)e] = s->slot_dev[g.edge_slot_lr[e]];
+      edge_slot[2 * (size_t)e + 1] = s->slot_dev[g.edge_slot_rl[e]];
+    }
+  }
+  // pseudo-inverse entry-major: Wt[(o+j)*5 + r] = W[5*o + r*m + j]
+  std::vector<double> st_Wt(g.st_W.size());
+  for (int i = 0; i + 1 < (int)g.st_off.size(); ++i) {
+    int o = g.st_off[i], mm = g.st_off[i + 1] - o;
+    for (int j = 0; j < mm; ++j)
+      for (int r = 0; r < 5; ++r) st_Wt[5 * ((size_t)o + j) + r] = g.st_W[5 * (size_t)o + (size_t)r * mm + j];
   }
   std::vector<double> sub_fc(2 * (size_t)g.nsub);
   for (int is = 0; is < g.nsub; ++is) {
@@ -364,10 +392,10 @@ int fs_solver_create(const fs_mesh* mh, const fs_params* p, int precision, fs_so
       (rc = s->upload(&d.nedge_off, g.nedge_off)) || (rc = s->upload(&d.nedge_list, g.nedge_list)) ||
       (rc = s->upload(&d.nbe_off, g.nbe_off)) || (rc = s->upload(&d.nbe_list, g.nbe_list)) ||
       (rc = s->upload(&d.bedge_real, breal)) || (rc = s->upload(&d.bedge_bc, bbc)) ||
-      (rc = s->upload(&d.bedge_n, g.bedge_n)) || (rc = s->upload(&d.ia, g.ia)) || (rc = s->upload(&d.ja, g.ja)) ||
-      (rc = s->upload(&d.iau, g.iau)) || (rc = s->upload(&d.slot_t, g.slot_t)) ||
+      (rc = s->upload(&d.bedge_n, g.bedge_n)) || (rc = s->upload(&d.rstart, rstart)) || (rc = s->upload(&d.ja, ja_d)) ||
+      (rc = s->upload(&d.iau, iau_d)) || (rc = s->upload(&d.slot_t, slot_t_d)) ||
       (rc = s->upload(&d.color_rows, g.color_rows)) || (rc = s->upload(&d.st_off, g.st_off)) ||
-      (rc = s->upload(&d.st_nodes, g.st_nodes)) || (rc = s->upload(&d.st_W, g.st_W)) ||
+      (rc = s->upload(&d.st_nodes, g.st_nodes)) || (rc = s->upload(&d.st_Wt, st_Wt)) ||
       (rc = s->upload(&d.moments, g.moments)))
     return rc;
   fsk::DevState& st = s->ds;
@@ -456,6 +484,14 @@ int fs_solver_set(fs_solver* s, int field, const double* host, int64_t n) {
   if (rc) return rc;
   if (n != cnt) return fail(FS_ERR_ARG, "size mismatch: expected " + std::to_string(cnt));
   HIP_TRY(hipStreamSynchronize(s->stream));
+  if (field == FS_F_LHS) {
+    // reference block order -> device color-major order
+    std::vector<double> tmp((size_t)n);
+    for (size_t k = 0; k < s->slot_dev.size(); ++k)
+      std::memcpy(&tmp[16 * (size_t)s->slot_dev[k]], host + 16 * k, 16 * sizeof(double));
+    HIP_TRY(hipMemcpy(d, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    return FS_OK;
+  }
   HIP_TRY(hipMemcpy(d, host, n * sizeof(double), hipMemcpyHostToDevice));
   return FS_OK;
 }
@@ -467,6 +503,13 @@ int fs_solver_get(fs_solver* s, int field, double* host, int64_t n) {
   if (rc) return rc;
   if (n != cnt) return fail(FS_ERR_ARG, "size mismatch: expected " + std::to_string(cnt));
   HIP_TRY(hipStreamSynchronize(s->stream));
+  if (field == FS_F_LHS) {
+    std::vector<double> tmp((size_t)n);
+    HIP_TRY(hipMemcpy(tmp.data(), d, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < s->slot_dev.size(); ++k)
+      std::memcpy(host + 16 * k, &tmp[16 * (size_t)s->slot_dev[k]], 16 * sizeof(double));
+    return FS_OK;
+  }
   HIP_TRY(hipMemcpy(host, d, n * sizeof(double), hipMemcpyDeviceToHost));
   return FS_OK;
 }
@@ -534,7 +577,8 @@ int fs_linear_solve(fs_solver* s, double* rms) {
     // sqrt(sum |RHS - A dQ|^2 / nn) as linear_algebra.C:348-385 reports
     const fs::Mesh& g = *s->host;
     std::vector<double> L((size_t)s->dm.nnz * 16), b((size_t)g.nn * 4), x((size_t)g.nn * 4);
-    HIP_TRY(hipMemcpy(L.data(), s->ds.LHS, L.size() * sizeof(double), hipMemcpyDeviceToHost));
+    int rc = fs_solver_get(s, FS_F_LHS, L.data(), (int64_t)L.size());
+    if (rc) return rc;
     HIP_TRY(hipMemcpy(b.data(), s->ds.RHS, b.size() * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(x.data(), s->ds.dQ, x.size() * sizeof(double), hipMemcpyDeviceToHost));
     double acc = 0;

@@ -59,42 +59,53 @@ __global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, const double* __
 
 // ---------------------------------------------------------------------------
 // K3: k=2 derivatives, sol = W' (q_j - q_i) over the 2-ring stencil
-// (compute_derivatives.C:1438-1521)
+// (compute_derivatives.C:1438-1521).  16 lanes per node: lane l takes stencil
+// entries l, l+16, ...; W' is stored entry-major ([entry][5]) so the 16 lanes
+// of a node read one contiguous run; the 20 partial sums are combined with a
+// butterfly over the 16 lanes.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_derivatives(DevMesh m, const double* __restrict__ nQ,
                                                         double* __restrict__ hess, double* __restrict__ pv) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = tid >> 4, l = tid & 15;
   if (i >= m.nn) return;
   int o = m.st_off[i], mm = m.st_off[i + 1] - o;
-  const double* W = m.st_W + 5 * (size_t)o;
   double qi[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) qi[v] = nQ[4 * (size_t)i + v];
-  double sol[4][5];
+  double acc[4][5];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
 #pragma unroll
-    for (int r = 0; r < 5; ++r) sol[v][r] = 0.0;
-  for (int j = 0; j < mm; ++j) {
+    for (int r = 0; r < 5; ++r) acc[v][r] = 0.0;
+  for (int j = l; j < mm; j += 16) {
     int node = m.st_nodes[o + j];
+    const double* w = m.st_Wt + 5 * ((size_t)o + j);
     double d[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) d[v] = nQ[4 * (size_t)node + v] - qi[v];
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
-      double w = W[(size_t)r * mm + j];
+      double wr = w[r];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) sol[v][r] += w * d[v];
+      for (int v = 0; v < 4; ++v) acc[v][r] += wr * d[v];
     }
   }
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < 5; ++r) acc[v][r] += __shfl_xor(acc[v][r], off, 16);
   const double* Mi = m.moments + 9 * (size_t)i;
-  double M0 = Mi[0], M1 = Mi[1], M2 = Mi[2], M3 = Mi[3], M4 = Mi[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
+    if (l == v) {
 #pragma unroll
-    for (int r = 0; r < 5; ++r) hess[20 * (size_t)i + 5 * v + r] = sol[v][r];
-    pv[4 * (size_t)i + v] = qi[v] - M0 * sol[v][0] - M1 * sol[v][1] - M2 * sol[v][2] * 0.5 - M3 * sol[v][3] * 0.5 -
-                            M4 * sol[v][4];
+      for (int r = 0; r < 5; ++r) hess[20 * (size_t)i + 5 * v + r] = acc[v][r];
+      pv[4 * (size_t)i + v] = qi[v] - Mi[0] * acc[v][0] - Mi[1] * acc[v][1] - Mi[2] * acc[v][2] * 0.5 -
+                              Mi[3] * acc[v][3] * 0.5 - Mi[4] * acc[v][4];
+    }
   }
 }
 
@@ -104,91 +115,75 @@ __global__ void __launch_bounds__(kBlock) k_derivatives(DevMesh m, const double*
 // incident face point (k=1) or the 3 Gauss points (k=2 phase); one scalar
 // per node (min over the 4 variables).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void barth_candidate(const double* q, const double* qmin, const double* qmax,
-                                                const double* qf, double* ph) {
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    double d = qf[j] - q[j], t;
-    if (d > 0.) {
-      t = dmin(1.0, (qmax[j] - q[j]) / (qf[j] - q[j]));
-      ph[j] = dmin(ph[j], t);
-    } else if (d < 0.) {
-      t = dmin(1.0, (qmin[j] - q[j]) / (qf[j] - q[j]));
-      ph[j] = dmin(ph[j], t);
-    } else {
-      ph[j] = dmin(ph[j], 1.0);
-    }
+__device__ __forceinline__ void barth_candidate1(double q, double qmin, double qmax, double qf, double* ph) {
+  double d = qf - q, t;
+  if (d > 0.) {
+    t = dmin(1.0, (qmax - q) / (qf - q));
+    *ph = dmin(*ph, t);
+  } else if (d < 0.) {
+    t = dmin(1.0, (qmin - q) / (qf - q));
+    *ph = dmin(*ph, t);
+  } else {
+    *ph = dmin(*ph, 1.0);
   }
 }
 
+// 4 lanes per node, lane v limits variable v; the node scalar is the min over
+// the 4 lanes taken in variable order (limiter.C:410-420)
 __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, const double* __restrict__ nQ,
                                                     const double* __restrict__ grad, const double* __restrict__ hess,
                                                     const double* __restrict__ pv, double* __restrict__ phi,
                                                     int recon) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
-  double q[4], qmin[4], qmax[4], ph[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    q[j] = nQ[4 * (size_t)n + j];
-    qmin[j] = q[j];
-    qmax[j] = q[j];
-    ph[j] = 1.0;
-  }
+  double q = nQ[4 * (size_t)n + v];
+  double qmin = q, qmax = q, ph = 1.0;
   for (int k = m.nedge_off[n]; k < m.nedge_off[n + 1]; ++k) {
     int ent = m.nedge_list[k];
     int e = ent & 0x7fffffff;
     int other = ent >= 0 ? m.edges[2 * e + 1] : m.edges[2 * e];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      double o = nQ[4 * (size_t)other + j];
-      qmin[j] = dmin(qmin[j], o);
-      qmax[j] = dmax(qmax[j], o);
-    }
+    double o = nQ[4 * (size_t)other + v];
+    qmin = dmin(qmin, o);
+    qmax = dmax(qmax, o);
   }
   double xn0 = m.x[2 * (size_t)n], xn1 = m.x[2 * (size_t)n + 1];
   if (recon == 3) {
-    double h[20], p0[4];
-#pragma unroll
-    for (int k = 0; k < 20; ++k) h[k] = hess[20 * (size_t)n + k];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) p0[j] = pv[4 * (size_t)n + j];
+    const double* h = hess + 20 * (size_t)n + 5 * v;
+    double h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4];
+    double p0 = pv[4 * (size_t)n + v];
     for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
       int s = m.nsub_list[k] & 0x7fffffff;
       for (int g = 0; g < 3; ++g) {
         double dx0 = m.sub_xg[6 * (size_t)s + 2 * g] - xn0;
         double dx1 = m.sub_xg[6 * (size_t)s + 2 * g + 1] - xn1;
-        double qf[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double* hj = h + 5 * j;
-          qf[j] = p0[j];
-          qf[j] += (hj[0] * dx0 + hj[1] * dx1 + hj[2] * dx0 * dx0 * 0.5 + hj[3] * dx1 * dx1 * 0.5 + hj[4] * dx0 * dx1);
-        }
-        barth_candidate(q, qmin, qmax, qf, ph);
+        double qf = p0;
+        qf += (h0 * dx0 + h1 * dx1 + h2 * dx0 * dx0 * 0.5 + h3 * dx1 * dx1 * 0.5 + h4 * dx0 * dx1);
+        barth_candidate1(q, qmin, qmax, qf, &ph);
       }
     }
   } else {
-    double g[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) g[k] = (recon == 1) ? grad[8 * (size_t)n + k] : 0.0;
+    double g0 = 0.0, g1 = 0.0;
+    if (recon == 1) {
+      g0 = grad[8 * (size_t)n + 2 * v];
+      g1 = grad[8 * (size_t)n + 2 * v + 1];
+    }
     for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
       int s = m.nsub_list[k] & 0x7fffffff;
-      double qf[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) qf[j] = q[j];
+      double qf = q;
       if (recon == 1) {
         double dx0 = m.sub_fc[2 * (size_t)s] - xn0, dx1 = m.sub_fc[2 * (size_t)s + 1] - xn1;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) qf[j] += (g[2 * j] * dx0 + g[2 * j + 1] * dx1);
+        qf += (g0 * dx0 + g1 * dx1);
       }
-      barth_candidate(q, qmin, qmax, qf, ph);
+      barth_candidate1(q, qmin, qmax, qf, &ph);
     }
   }
-  double t = ph[0];
-#pragma unroll
-  for (int j = 1; j < 4; ++j) t = dmin(t, ph[j]);
-  phi[n] = t;
+  double p0 = __shfl(ph, 0, 4), p1 = __shfl(ph, 1, 4), p2 = __shfl(ph, 2, 4), p3 = __shfl(ph, 3, 4);
+  double t = p0;
+  t = dmin(t, p1);
+  t = dmin(t, p2);
+  t = dmin(t, p3);
+  if (v == 0) phi[n] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -306,17 +301,18 @@ __global__ void __launch_bounds__(kBlock) k_bedge(DevMesh m, double* __restrict_
 // the row (each interior face adds dF/dq_L to diag L and subtracts it from
 // block (R,L)) + boundary blocks, RHS = -R.
 // ---------------------------------------------------------------------------
+// 4 lanes per node: lane v owns residual component v and row v of the
+// diagonal block, so the quad reads each 128-byte block contiguously.
 __global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, PhysParams p, int flags) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
   const bool evcon = flags & NODE_EVCON, interior = flags & NODE_INTERIOR, boundary = flags & NODE_BOUNDARY;
   const bool jac = flags & NODE_JAC, rhs = flags & NODE_RHS;
   double q[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) q[j] = s.nQ[4 * (size_t)n + j];
-  double Rn[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) Rn[j] = interior ? 0.0 : s.R[4 * (size_t)n + j];
+  double Rv = interior ? 0.0 : s.R[4 * (size_t)n + v];
   double ev = 0.0;
   if (evcon || interior) {
     for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
@@ -327,14 +323,8 @@ __global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, PhysPara
         ev += (max_ev(q, nx, ny, p.gamma) * len);
       }
       if (interior) {
-        const double* f = s.sub_flux + 4 * (size_t)sb;
-        if (ent >= 0) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) Rn[j] += f[j];
-        } else {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) Rn[j] -= f[j];
-        }
+        double f = s.sub_flux[4 * (size_t)sb + v];
+        if (ent >= 0) Rv += f; else Rv -= f;
       }
     }
   }
@@ -345,54 +335,47 @@ __global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, PhysPara
         double nx = m.bedge_n[3 * (size_t)b], ny = m.bedge_n[3 * (size_t)b + 1], len = m.bedge_n[3 * (size_t)b + 2];
         ev += (max_ev(q, nx, ny, p.gamma) * len);
       }
-      if (boundary) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) Rn[j] += s.bflux[4 * (size_t)b + j];
-      }
+      if (boundary) Rv += s.bflux[4 * (size_t)b + v];
     }
   }
-  double D[16];
-  double* Dg = s.LHS + 16 * (size_t)m.iau[n];
+  double D[4];
+  double* Dg = s.LHS + 16 * (size_t)m.iau[n] + 4 * v;
   if (evcon) {
     double V = m.cv_area[n];
     double dtn = p.cfl * (V / ev);
-    if (!(dtn > 0.0) || !isfinite(dtn) || !isfinite(ev)) atomicOr(s.err, 2);
-    s.ev_con[n] = ev;
-    s.dt[n] = dtn;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) D[k] = 0.0;
+    if (v == 0) {
+      if (!(dtn > 0.0) || !isfinite(dtn) || !isfinite(ev)) atomicOr(s.err, 2);
+      s.ev_con[n] = ev;
+      s.dt[n] = dtn;
+    }
     double vd = V / dtn;
-    D[0] = vd; D[5] = vd; D[10] = vd; D[15] = vd;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) D[c] = (c == v) ? vd : 0.0;
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) D[k] = Dg[k];
+    for (int c = 0; c < 4; ++c) D[c] = Dg[c];
   }
   if (interior && jac) {
-    int d = m.iau[n];
-    for (int k = m.ia[n]; k < m.ia[n + 1]; ++k) {
-      if (k == d) continue;
-      const double* B = s.LHS + 16 * (size_t)m.slot_t[k];
+    for (int k = m.rstart[n]; k < m.iau[n]; ++k) {
+      const double* B = s.LHS + 16 * (size_t)m.slot_t[k] + 4 * v;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) D[c] -= B[c];
+      for (int c = 0; c < 4; ++c) D[c] -= B[c];
     }
   }
   if (boundary && jac) {
     for (int k = m.nbe_off[n]; k < m.nbe_off[n + 1]; ++k) {
-      const double* B = s.bjac + 16 * (size_t)m.nbe_list[k];
+      const double* B = s.bjac + 16 * (size_t)m.nbe_list[k] + 4 * v;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) D[c] += B[c];
+      for (int c = 0; c < 4; ++c) D[c] += B[c];
     }
   }
   if (evcon || jac) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) Dg[k] = D[k];
+    for (int c = 0; c < 4; ++c) Dg[c] = D[c];
   }
   if (interior || boundary) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      s.R[4 * (size_t)n + j] = Rn[j];
-      if (rhs) s.RHS[4 * (size_t)n + j] = -(Rn[j]);
-    }
+    s.R[4 * (size_t)n + v] = Rv;
+    if (rhs) s.RHS[4 * (size_t)n + v] = -(Rv);
   }
 }
 
@@ -411,47 +394,50 @@ __global__ void __launch_bounds__(kBlock) k_zero_offdiag(DevMesh m, double* __re
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const double* __restrict__ LHS,
                                                     double* __restrict__ LU, double* __restrict__ dQ) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= m.nn) return;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.nn) return;
+  int n = m.color_rows[t];
   double A[16];
   const double* D = LHS + 16 * (size_t)m.iau[n];
 #pragma unroll
   for (int k = 0; k < 16; ++k) A[k] = D[k];
   lu4(A);
 #pragma unroll
-  for (int k = 0; k < 16; ++k) LU[16 * (size_t)n + k] = A[k];
+  for (int k = 0; k < 16; ++k) LU[16 * (size_t)t + k] = A[k];
 #pragma unroll
   for (int j = 0; j < 4; ++j) dQ[4 * (size_t)n + j] = 0.0;
 }
 
+// rows [row_begin,row_end) of the color-sorted order; 4 lanes per row, lane v
+// forms component v of RHS - sum_offdiag A dQ (each quad streams whole blocks),
+// then every lane of the quad solves with the factored diagonal.
 __global__ void __launch_bounds__(kBlock) k_gs_color(DevMesh m, const double* __restrict__ LHS,
                                                      const double* __restrict__ LU, const double* __restrict__ RHS,
                                                      double* __restrict__ dQ, int row_begin, int row_end) {
-  int t = row_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = row_begin + (tid >> 2), v = tid & 3;
   if (t >= row_end) return;
   int j = m.color_rows[t];
-  double r[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) r[v] = RHS[4 * (size_t)j + v];
-  int d = m.iau[j];
-  for (int k = m.ia[j]; k < m.ia[j + 1]; ++k) {
-    if (k == d) continue;
-    const double* A = LHS + 16 * (size_t)k;
+  double r = RHS[4 * (size_t)j + v];
+  int kend = m.iau[j];
+  for (int k = m.rstart[j]; k < kend; ++k) {
+    const double* A = LHS + 16 * (size_t)k + 4 * v;
     const double* q = dQ + 4 * (size_t)m.ja[k];
-    double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) r[v] -= (A[v * 4 + 0] * q0 + A[v * 4 + 1] * q1 + A[v * 4 + 2] * q2 + A[v * 4 + 3] * q3);
+    r -= (A[0] * q[0] + A[1] * q[1] + A[2] * q[2] + A[3] * q[3]);
   }
-  double X[4], Ld[16];
+  double rr[4];
+  rr[0] = __shfl(r, 0, 4);
+  rr[1] = __shfl(r, 1, 4);
+  rr[2] = __shfl(r, 2, 4);
+  rr[3] = __shfl(r, 3, 4);
+  double Ld[16], X[4];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) Ld[k] = LU[16 * (size_t)j + k];
-  lu_solve4(Ld, r, X);
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    double old = dQ[4 * (size_t)j + v];
-    double dx = X[v] - old;
-    dQ[4 * (size_t)j + v] = old + dx;
-  }
+  for (int k = 0; k < 16; ++k) Ld[k] = LU[16 * (size_t)t + k];
+  lu_solve4(Ld, rr, X);
+  double xv = v == 0 ? X[0] : v == 1 ? X[1] : v == 2 ? X[2] : X[3];
+  double old = dQ[4 * (size_t)j + v];
+  double dx = xv - old;
+  dQ[4 * (size_t)j + v] = old + dx;
 }
 
 // nQ += dQ over the real nodes (solve.C:901-907)
@@ -532,10 +518,10 @@ void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st) {
   hipLaunchKernelGGL(k_gradient, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.nQ, s.grad);
 }
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_derivatives, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.nQ, s.hess, s.pv);
+  hipLaunchKernelGGL(k_derivatives, dim3(nblocks(16 * m.nn)), dim3(kBlock), 0, st, m, s.nQ, s.hess, s.pv);
 }
 void launch_limiter(const DevMesh& m, const DevState& s, int recon, hipStream_t st) {
-  hipLaunchKernelGGL(k_limiter, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.nQ, s.grad, s.hess, s.pv, s.phi,
+  hipLaunchKernelGGL(k_limiter, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s.nQ, s.grad, s.hess, s.pv, s.phi,
                      recon);
 }
 void launch_edge_flux(const DevMesh& m, const DevState& s, const PhysParams& p, int recon, int jupdate,
@@ -549,7 +535,7 @@ void launch_bedge(const DevMesh& m, const DevState& s, const PhysParams& p, int 
                      s.err);
 }
 void launch_node(const DevMesh& m, const DevState& s, const PhysParams& p, int flags, hipStream_t st) {
-  hipLaunchKernelGGL(k_node, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s, p, flags);
+  hipLaunchKernelGGL(k_node, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s, p, flags);
 }
 void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st) {
   size_t n = (size_t)m.nnz * 16;
@@ -562,7 +548,8 @@ void launch_gs(const DevMesh& m, const DevState& s, const int* color_off, int su
       int c = (it % 2 == 0) ? cc : m.ncolor - 1 - cc;
       int b = color_off[c], e = color_off[c + 1];
       if (e <= b) continue;
-      hipLaunchKernelGGL(k_gs_color, dim3(nblocks(e - b)), dim3(kBlock), 0, st, m, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+      hipLaunchKernelGGL(k_gs_color, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.LHS, s.LU, s.RHS, s.dQ, b,
+                         e);
     }
   }
 }

@@ -35,14 +35,17 @@ struct DevMesh {
   const int* bedge_real;    // nbe
   const int* bedge_bc;      // nbe
   const double* bedge_n;    // 3*nbe
-  const int* ia;            // nn+1
-  const int* ja;            // nnz
+  // Block CSR in color-major device order: the rows of one color are one
+  // contiguous run of blocks.  Row n owns device slots rstart[n]..iau[n]
+  // (off-diagonals in the reference's nsn order, diagonal last).
+  const int* rstart;        // nn
+  const int* ja;            // nnz, device slot order
   const int* iau;           // nn
-  const int* slot_t;        // nnz
+  const int* slot_t;        // nnz, device slot of the transposed block
   const int* color_rows;    // nn (rows grouped by color)
   const int* st_off;        // nn+1 (k=2)
   const int* st_nodes;
-  const double* st_W;       // 5 per stencil entry
+  const double* st_Wt;      // [entry][5] pseudo-inverse columns, weights folded
   const double* moments;    // 9*nn
 };
 
@@ -59,7 +62,7 @@ struct DevState {
   double* ev_con; // nn
   double* dt;     // nn
   double* LHS;    // nnz*16
-  double* LU;     // nn*16
+  double* LU;     // nn*16, indexed by color position
   double* sub_flux;  // nsub*4
   double* bflux;     // nbe*4
   double* bjac;      // nbe*16
