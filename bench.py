@@ -39,6 +39,52 @@ def algorithmic_bytes_edge_kernel(info, recon):
     return b
 
 
+def gs_schedule(ncolor, subits):
+    """Color visiting order of the device sweeps (turnaround repeats dropped)."""
+    out, last = [], -1
+    for it in range(subits):
+        for cc in range(ncolor):
+            c = cc if it % 2 == 0 else ncolor - 1 - cc
+            if c != last:
+                out.append(c)
+                last = c
+    return out
+
+
+def algorithmic_bytes_gauss_seidel(mesh, subits, word):
+    """Compulsory HBM bytes of one linear solve: the diagonal factorisation
+    pass plus every color launch of the sweep schedule.  word = 8 (FP64) or 4.
+    Per launch of color c: its rows' off-diagonal blocks (16 words) and column
+    indices, the rows' factored diagonals (16 words), RHS and dQ (4 words each,
+    dQ read and written), and the neighbours' dQ once (4 words per node of the
+    other colors)."""
+    import numpy as np
+    info = mesh.info()
+    nn = info["nn"]
+    ia = mesh.array("ia")
+    deg = np.diff(ia) - 1
+    co, rows = mesh.array("color_off"), mesh.array("color_rows")
+    per_color = []
+    for c in range(info["ncolor"]):
+        r = rows[co[c]:co[c + 1]]
+        nr, nb = len(r), int(deg[r].sum())
+        per_color.append(nb * (16 * word + 4) + nr * (16 * word + 12 * word + 12) + (nn - nr) * 4 * word)
+    sched = gs_schedule(info["ncolor"], subits)
+    total = sum(per_color[c] for c in sched)
+    return total, len(sched)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (profiles/pmc_traffic_bump1025x257.md gives the commands)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_bump1025x257.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +142,9 @@ def main():
     evals = info["nsub"] + info["nbe"]
     value = evals / (ms * 1e-3)
     abytes = algorithmic_bytes_edge_kernel(info, recon=0)
-    achieved = abytes / (tim["edge_ms"] * 1e-3) / 1e9
+    edge_gbs = abytes / (tim["edge_ms"] * 1e-3) / 1e9
+    gs_bytes, gs_launches = algorithmic_bytes_gauss_seidel(mesh, p.subits, 8)
+    achieved = gs_bytes / (tim["gs_ms"] * 1e-3) / 1e9
     line = {
         "metric": "edge-flux evals/s", "value": value, "unit": "edge-flux evals/s", "n_gpus": 1, "steps": a.steps,
         "warmup": warm, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -104,14 +152,32 @@ def main():
         "config": {"workload": "bump channel %dx%d, M=0.675, k=2 (order 3), Barth limiter, FP64 implicit step"
                                % (a.nx, a.ny), "nodes": info["nn"], "subedges": info["nsub"],
                    "bedges": info["nbe"], "lhs_blocks": info["nnz"], "precision": "fp64", "subits": p.subits},
-        "roofline": {"bound": "hbm", "kernel": "k_edge_flux (Roe flux + FD Jacobian blocks)",
+        "roofline": {"bound": "hbm", "kernel": "k_gs_color (multicolor block Gauss-Seidel sweep, FP64)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": abytes, "avg_ms": tim["edge_ms"]},
+                     "traffic": (None if (a.nx, a.ny) != (1025, 257) else measured_traffic("fsk::k_gs_color")),
+                     "algorithmic_bytes": gs_bytes / gs_launches, "launches_per_step": gs_launches,
+                     "avg_ms": tim["gs_ms"] / gs_launches},
+        "edge_kernel_roofline": {"kernel": "k_edge_flux (Roe flux + FD Jacobian blocks, FP64 VALU)",
+                                 "achieved_GBs": edge_gbs, "algorithmic_bytes": abytes, "avg_ms": tim["edge_ms"]},
         "kernel_ms": {"edge_flux": tim["edge_ms"], "node_assembly": tim["node_ms"], "gauss_seidel": tim["gs_ms"],
-                      "other": tim["other_ms"]},
+                      "diag_lu": tim["lu_ms"], "other": tim["other_ms"]},
         "jacobian_flux_evals_per_step": 8 * info["nsub"],
         "setup_s": t_setup,
     }
+    # the mixed-precision path on the same workload and the same timed steps
+    sm = fsm.Solver(mesh, p, fsm.FS_PREC_MIXED)
+    sm.init_freestream()
+    for ts in range(1, warm + 1):
+        sm.solve(ts)
+    tm = sm.time_steps(warm + 1, a.steps)
+    sm.close()
+    gs32_bytes, _ = algorithmic_bytes_gauss_seidel(mesh, p.subits, 4)
+    line["mixed"] = {"value": evals / (tm["step_ms"] * 1e-3), "ms_per_step": tm["step_ms"],
+                     "gauss_seidel_f32_GBs": gs32_bytes / (tm["gs_ms"] * 1e-3) / 1e9,
+                     "speedup_vs_fp64": ms / tm["step_ms"], "dtype": "f64 residual/norms/update, f32 Jacobian+sweeps",
+                     "kernel_ms": {"edge_flux_f64": tm["edge_ms"], "node_assembly": tm["node_ms"],
+                                   "gauss_seidel_f32": tm["gs_ms"], "diag_lu_f32": tm["lu_ms"],
+                                   "other": tm["other_ms"]}}
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(U, fsm, p, a.nx, a.ny, a.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -77,7 +77,7 @@ struct fs_solver {
   // timing
   bool timing = false;
   hipEvent_t ev[8] = {};
-  double t_edge = 0, t_node = 0, t_gs = 0;
+  double t_edge = 0, t_node = 0, t_gs = 0, t_lu = 0;
 
   template <typename T>
   int alloc(T** out, size_t count) {
@@ -373,11 +373,33 @@ int fs_solver_create(const fs_mesh* mh, const fs_params* p, int precision, fs_so
     for (int j = 0; j < mm; ++j)
       for (int r = 0; r < 5; ++r) st_Wt[5 * ((size_t)o + j) + r] = g.st_W[5 * (size_t)o + (size_t)r * mm + j];
   }
-  std::vector<double> sub_fc(2 * (size_t)g.nsub);
-  for (int is = 0; is < g.nsub; ++is) {
+  // node-incidence arrays: everything a node-centric gather reads, in the
+  // node's ascending incidence order
+  size_t ninc = g.nsub_list.size(), neinc = g.nedge_list.size();
+  std::vector<int> sub_inc(2 * (size_t)g.nsub), inc_other(neinc);
+  std::vector<double> inc_n(4 * ninc), inc_fc(2 * ninc), inc_xg(6 * ninc), inc_gw(2 * neinc);
+  for (size_t k = 0; k < ninc; ++k) {
+    int ent = g.nsub_list[k];
+    int is = ent & 0x7fffffff;
+    sub_inc[2 * (size_t)is + (ent < 0 ? 1 : 0)] = (int)k;
+    inc_n[4 * k] = g.sub_n[3 * (size_t)is];
+    inc_n[4 * k + 1] = g.sub_n[3 * (size_t)is + 1];
+    inc_n[4 * k + 2] = g.sub_n[3 * (size_t)is + 2];
+    inc_n[4 * k + 3] = 0.0;
     int el = g.sub_elem[is];
-    sub_fc[2 * (size_t)is] = (g.el_cent[2 * (size_t)el] + g.sub_xm[2 * (size_t)is]) * 0.5;
-    sub_fc[2 * (size_t)is + 1] = (g.el_cent[2 * (size_t)el + 1] + g.sub_xm[2 * (size_t)is + 1]) * 0.5;
+    // face point of Reconstruct (reconstruct.C:181-186): (centroid + midpoint) * 0.5
+    inc_fc[2 * k] = (g.el_cent[2 * (size_t)el] + g.sub_xm[2 * (size_t)is]) * 0.5;
+    inc_fc[2 * k + 1] = (g.el_cent[2 * (size_t)el + 1] + g.sub_xm[2 * (size_t)is + 1]) * 0.5;
+    for (int c = 0; c < 6; ++c) inc_xg[6 * k + c] = g.sub_xg[6 * (size_t)is + c];
+  }
+  for (size_t k = 0; k < neinc; ++k) {
+    int ent = g.nedge_list[k];
+    int e = ent & 0x7fffffff;
+    bool isR = ent < 0;
+    int other = isR ? g.edges[2 * (size_t)e] : g.edges[2 * (size_t)e + 1];
+    inc_other[k] = other | (isR ? (int)0x80000000u : 0);
+    inc_gw[2 * k] = g.edge_weight[4 * (size_t)e + (isR ? 2 : 0)];
+    inc_gw[2 * k + 1] = g.edge_weight[4 * (size_t)e + (isR ? 3 : 1)];
   }
   std::vector<int> breal(g.nbe), bbc(g.nbe);
   for (int i = 0; i < g.nbe; ++i) {
@@ -387,9 +409,10 @@ int fs_solver_create(const fs_mesh* mh, const fs_params* p, int precision, fs_so
   if ((rc = s->upload(&d.x, xr)) || (rc = s->upload(&d.cv_area, g.cv_area)) || (rc = s->upload(&d.edges, g.edges)) ||
       (rc = s->upload(&d.edge_sub, edge_sub)) || (rc = s->upload(&d.edge_slot, edge_slot)) ||
       (rc = s->upload(&d.edge_w, g.edge_weight)) || (rc = s->upload(&d.sub_n, g.sub_n)) ||
-      (rc = s->upload(&d.sub_fc, sub_fc)) || (rc = s->upload(&d.sub_xg, g.sub_xg)) ||
-      (rc = s->upload(&d.nsub_off, g.nsub_off)) || (rc = s->upload(&d.nsub_list, g.nsub_list)) ||
-      (rc = s->upload(&d.nedge_off, g.nedge_off)) || (rc = s->upload(&d.nedge_list, g.nedge_list)) ||
+      (rc = s->upload(&d.sub_inc, sub_inc)) || (rc = s->upload(&d.inc_n, inc_n)) ||
+      (rc = s->upload(&d.inc_fc, inc_fc)) || (rc = s->upload(&d.inc_xg, inc_xg)) ||
+      (rc = s->upload(&d.inc_other, inc_other)) || (rc = s->upload(&d.inc_gw, inc_gw)) ||
+      (rc = s->upload(&d.nsub_off, g.nsub_off)) || (rc = s->upload(&d.nedge_off, g.nedge_off)) ||
       (rc = s->upload(&d.nbe_off, g.nbe_off)) || (rc = s->upload(&d.nbe_list, g.nbe_list)) ||
       (rc = s->upload(&d.bedge_real, breal)) || (rc = s->upload(&d.bedge_bc, bbc)) ||
       (rc = s->upload(&d.bedge_n, g.bedge_n)) || (rc = s->upload(&d.rstart, rstart)) || (rc = s->upload(&d.ja, ja_d)) ||
@@ -405,10 +428,15 @@ int fs_solver_create(const fs_mesh* mh, const fs_params* p, int precision, fs_so
       (rc = s->alloc(&st.phi, nn)) || (rc = s->alloc(&st.hess, nn * 20)) || (rc = s->alloc(&st.pv, nn * 4)) ||
       (rc = s->alloc(&st.ev_con, nn)) || (rc = s->alloc(&st.dt, nn)) ||
       (rc = s->alloc(&st.LHS, (size_t)d.nnz * 16)) || (rc = s->alloc(&st.LU, nn * 16)) ||
-      (rc = s->alloc(&st.sub_flux, (size_t)g.nsub * 4)) || (rc = s->alloc(&st.bflux, (size_t)g.nbe * 4)) ||
+      (rc = s->alloc(&st.inc_flux, (size_t)g.nsub * 8)) || (rc = s->alloc(&st.bflux, (size_t)g.nbe * 4)) ||
       (rc = s->alloc(&st.bjac, (size_t)g.nbe * 16)) || (rc = s->alloc(&st.partial, 2 * 256)) ||
       (rc = s->alloc(&st.err, 1)) || (rc = s->alloc(&s->d_norms, 4)))
     return rc;
+  if (precision == FS_PREC_MIXED) {
+    if ((rc = s->alloc(&st.LHS32, (size_t)d.nnz * 16)) || (rc = s->alloc(&st.LU32, nn * 16)) ||
+        (rc = s->alloc(&st.RHS32, nn * 4)) || (rc = s->alloc(&st.dQ32, nn * 4)))
+      return rc;
+  }
   HIP_TRY(hipMemset(st.err, 0, sizeof(int)));
   HIP_TRY(hipMemset(st.grad, 0, nn * 8 * sizeof(double)));
   HIP_TRY(hipMemset(st.hess, 0, nn * 20 * sizeof(double)));
@@ -498,6 +526,17 @@ int fs_solver_set(fs_solver* s, int field, const double* host, int64_t n) {
 
 int fs_solver_get(fs_solver* s, int field, double* host, int64_t n) {
   if (!s || !host) return fail(FS_ERR_ARG, "null argument");
+  if (field == FS_F_LHS32) {
+    if (!s->ds.LHS32) return fail(FS_ERR_ARG, "FP32 blocks exist only for a mixed-precision solver");
+    int64_t cnt = (int64_t)s->dm.nnz * 16;
+    if (n != cnt) return fail(FS_ERR_ARG, "size mismatch: expected " + std::to_string(cnt));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<float> tmp((size_t)n);
+    HIP_TRY(hipMemcpy(tmp.data(), s->ds.LHS32, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < s->slot_dev.size(); ++k)
+      for (int c = 0; c < 16; ++c) host[16 * k + c] = (double)tmp[16 * (size_t)s->slot_dev[k] + c];
+    return FS_OK;
+  }
   double* d; int64_t cnt;
   int rc = field_ptr(s, field, &d, &cnt);
   if (rc) return rc;
@@ -596,8 +635,40 @@ int fs_linear_solve(fs_solver* s, double* rms) {
   return FS_OK;
 }
 
+// one Newton iteration of the mixed path: FP64 residual (no Jacobian in the
+// face-flux pass), FP32 analytic Jacobian and sweeps, FP64 update and norms
+static int newton_iteration_mixed(fs_solver* s, int ts, int it, bool record) {
+  const fs_params& p = s->prm;
+  int jac = ((it - 1) % p.jufreq == 0) ? 1 : 0;
+  if (it == 1) jac = 1;
+  Phase ph = phase_for(p, ts);
+  fsk::PhysParams pp = phys(s, ts);
+  hipStream_t st = s->stream;
+  if (ph.gradient) fsk::launch_gradient(s->dm, s->ds, st);
+  if (ph.k2) fsk::launch_derivatives(s->dm, s->ds, st);
+  if (ph.limiter) fsk::launch_limiter(s->dm, s->ds, ph.lim_recon, st);
+  else fill(s->ds.phi, s->dm.nn, 1.0, st);
+  if (record) HIP_TRY(hipEventRecord(s->ev[0], st));
+  fsk::launch_edge_flux(s->dm, s->ds, pp, ph.recon_flux, 0, st);
+  if (record) HIP_TRY(hipEventRecord(s->ev[1], st));
+  fsk::launch_edge_jac32(s->dm, s->ds, (float)p.gamma, ph.recon_flux, jac, st);
+  fsk::launch_bedge(s->dm, s->ds, pp, jac, st);
+  if (record) HIP_TRY(hipEventRecord(s->ev[2], st));
+  fsk::launch_node_mixed(s->dm, s->ds, pp, jac, st);
+  if (record) HIP_TRY(hipEventRecord(s->ev[3], st));
+  fsk::launch_diag_lu32(s->dm, s->ds, st);
+  if (record) HIP_TRY(hipEventRecord(s->ev[5], st));
+  fsk::launch_gs32_sweeps(s->dm, s->ds, s->color_off.data(), p.subits, st);
+  if (record) HIP_TRY(hipEventRecord(s->ev[4], st));
+  fsk::launch_update_mixed(s->dm, s->ds, st);
+  fsk::launch_norms(s->dm, s->ds, s->d_norms, st);
+  HIP_TRY(hipGetLastError());
+  return FS_OK;
+}
+
 // one Newton iteration of solve.C:328-909 on the device
 static int newton_iteration(fs_solver* s, int ts, int it, bool record) {
+  if (s->precision == FS_PREC_MIXED) return newton_iteration_mixed(s, ts, it, record);
   const fs_params& p = s->prm;
   int jac = ((it - 1) % p.jufreq == 0) ? 1 : 0;
   if (it == 1) jac = 1;
@@ -617,7 +688,9 @@ static int newton_iteration(fs_solver* s, int ts, int it, bool record) {
                    fsk::NODE_EVCON | fsk::NODE_INTERIOR | fsk::NODE_BOUNDARY | fsk::NODE_RHS | (jac ? fsk::NODE_JAC : 0),
                    st);
   if (record) HIP_TRY(hipEventRecord(s->ev[3], st));
-  fsk::launch_gs(s->dm, s->ds, s->color_off.data(), p.subits, st);
+  fsk::launch_diag_lu(s->dm, s->ds, st);
+  if (record) HIP_TRY(hipEventRecord(s->ev[5], st));
+  fsk::launch_gs_sweeps(s->dm, s->ds, s->color_off.data(), p.subits, st);
   if (record) HIP_TRY(hipEventRecord(s->ev[4], st));
   fsk::launch_norms(s->dm, s->ds, s->d_norms, st);
   fsk::launch_update(s->dm, s->ds, st);
@@ -638,11 +711,12 @@ int fs_solve(fs_solver* s, int ts, double* dq_norm, double* r2) {
     if (rc) return rc;
     if (rec) {
       HIP_TRY(hipEventSynchronize(s->ev[4]));
-      float a = 0, b = 0, c = 0;
+      float a = 0, b = 0, c = 0, l = 0;
       HIP_TRY(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
       HIP_TRY(hipEventElapsedTime(&b, s->ev[2], s->ev[3]));
-      HIP_TRY(hipEventElapsedTime(&c, s->ev[3], s->ev[4]));
-      s->t_edge += a; s->t_node += b; s->t_gs += c;
+      HIP_TRY(hipEventElapsedTime(&l, s->ev[3], s->ev[5]));
+      HIP_TRY(hipEventElapsedTime(&c, s->ev[5], s->ev[4]));
+      s->t_edge += a; s->t_node += b; s->t_gs += c; s->t_lu += l;
     }
   }
   // Q = nQ for real and ghost nodes (solve.C:913-931)
@@ -688,7 +762,7 @@ int fs_solve_steady(fs_solver* s, int max_steps, double tol, int* steps, double*
 int fs_solver_time_steps(fs_solver* s, int ts_first, int nsteps, fs_timing* t) {
   if (!s || !t || nsteps < 1) return fail(FS_ERR_ARG, "bad arguments");
   s->timing = true;
-  s->t_edge = s->t_node = s->t_gs = 0;
+  s->t_edge = s->t_node = s->t_gs = s->t_lu = 0;
   HIP_TRY(hipStreamSynchronize(s->stream));
   auto t0 = std::chrono::steady_clock::now();
   for (int k = 0; k < nsteps; ++k) {
@@ -699,11 +773,11 @@ int fs_solver_time_steps(fs_solver* s, int ts_first, int nsteps, fs_timing* t) {
   auto t1 = std::chrono::steady_clock::now();
   s->timing = false;
   t->step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / nsteps;
-  t->edge_ms = s->t_edge / (nsteps * (double)s->prm.iter) * s->prm.iter;
   t->node_ms = s->t_node / nsteps;
   t->gs_ms = s->t_gs / nsteps;
+  t->lu_ms = s->t_lu / nsteps;
   t->edge_ms = s->t_edge / nsteps;
-  t->other_ms = t->step_ms - t->edge_ms - t->node_ms - t->gs_ms;
+  t->other_ms = t->step_ms - t->edge_ms - t->node_ms - t->gs_ms - t->lu_ms;
   t->flux_evals = (int64_t)s->dm.nsub + s->dm.nbe;
   return FS_OK;
 }

@@ -10,6 +10,8 @@
 #include "fs_device.hpp"
 #include "fs_kernels.hpp"
 
+#include <vector>
+
 namespace fsk {
 using namespace fsd;
 
@@ -30,26 +32,26 @@ __global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, const double* __
   double g[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) g[k] = 0.0;
+  double qn[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) qn[j] = nQ[4 * (size_t)n + j];
   for (int k = m.nedge_off[n]; k < m.nedge_off[n + 1]; ++k) {
-    int ent = m.nedge_list[k];
-    int e = ent & 0x7fffffff;
-    int L = m.edges[2 * e], R = m.edges[2 * e + 1];
-    const double* w = m.edge_w + 4 * (size_t)e;
-    if (ent >= 0) {
-      double w0 = w[0], w1 = w[1];
+    int ent = m.inc_other[k];
+    int o = ent & 0x7fffffff;
+    double w0 = m.inc_gw[2 * (size_t)k], w1 = m.inc_gw[2 * (size_t)k + 1];
+    if (ent >= 0) {  // this node is L: dQ = Q_R - Q_L, grad_L += w dQ
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        double dq = nQ[4 * (size_t)R + j] - nQ[4 * (size_t)L + j];
+        double dq = nQ[4 * (size_t)o + j] - qn[j];
         g[2 * j] += w0 * dq;
         g[2 * j + 1] += w1 * dq;
       }
-    } else {
-      double w2 = w[2], w3 = w[3];
+    } else {         // this node is R: grad_R -= w dQ
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        double dq = nQ[4 * (size_t)R + j] - nQ[4 * (size_t)L + j];
-        g[2 * j] -= w2 * dq;
-        g[2 * j + 1] -= w3 * dq;
+        double dq = qn[j] - nQ[4 * (size_t)o + j];
+        g[2 * j] -= w0 * dq;
+        g[2 * j + 1] -= w1 * dq;
       }
     }
   }
@@ -140,9 +142,7 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, const double* __r
   double q = nQ[4 * (size_t)n + v];
   double qmin = q, qmax = q, ph = 1.0;
   for (int k = m.nedge_off[n]; k < m.nedge_off[n + 1]; ++k) {
-    int ent = m.nedge_list[k];
-    int e = ent & 0x7fffffff;
-    int other = ent >= 0 ? m.edges[2 * e + 1] : m.edges[2 * e];
+    int other = m.inc_other[k] & 0x7fffffff;
     double o = nQ[4 * (size_t)other + v];
     qmin = dmin(qmin, o);
     qmax = dmax(qmax, o);
@@ -153,10 +153,9 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, const double* __r
     double h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4];
     double p0 = pv[4 * (size_t)n + v];
     for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
-      int s = m.nsub_list[k] & 0x7fffffff;
       for (int g = 0; g < 3; ++g) {
-        double dx0 = m.sub_xg[6 * (size_t)s + 2 * g] - xn0;
-        double dx1 = m.sub_xg[6 * (size_t)s + 2 * g + 1] - xn1;
+        double dx0 = m.inc_xg[6 * (size_t)k + 2 * g] - xn0;
+        double dx1 = m.inc_xg[6 * (size_t)k + 2 * g + 1] - xn1;
         double qf = p0;
         qf += (h0 * dx0 + h1 * dx1 + h2 * dx0 * dx0 * 0.5 + h3 * dx1 * dx1 * 0.5 + h4 * dx0 * dx1);
         barth_candidate1(q, qmin, qmax, qf, &ph);
@@ -169,10 +168,9 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, const double* __r
       g1 = grad[8 * (size_t)n + 2 * v + 1];
     }
     for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
-      int s = m.nsub_list[k] & 0x7fffffff;
       double qf = q;
       if (recon == 1) {
-        double dx0 = m.sub_fc[2 * (size_t)s] - xn0, dx1 = m.sub_fc[2 * (size_t)s + 1] - xn1;
+        double dx0 = m.inc_fc[2 * (size_t)k] - xn0, dx1 = m.inc_fc[2 * (size_t)k + 1] - xn1;
         qf += (g0 * dx0 + g1 * dx1);
       }
       barth_candidate1(q, qmin, qmax, qf, &ph);
@@ -195,7 +193,7 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, const double* __r
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, const double* __restrict__ nQ,
                                                       const double* __restrict__ grad, const double* __restrict__ phi,
-                                                      double* __restrict__ sub_flux, double* __restrict__ LHS,
+                                                      double* __restrict__ inc_flux, double* __restrict__ LHS,
                                                       PhysParams p, int recon, int jupdate) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
@@ -219,7 +217,7 @@ __global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, const double* _
 #pragma unroll
     for (int j = 0; j < 4; ++j) { ql[j] = qL0[j]; qr[j] = qR0[j]; }
     if (recon == 1) {
-      double fc0 = m.sub_fc[2 * (size_t)s], fc1 = m.sub_fc[2 * (size_t)s + 1];
+      double fc0 = m.inc_fc[2 * (size_t)m.sub_inc[2 * (size_t)s]], fc1 = m.inc_fc[2 * (size_t)m.sub_inc[2 * (size_t)s] + 1];
       double dxl0 = fc0 - m.x[2 * (size_t)L], dxl1 = fc1 - m.x[2 * (size_t)L + 1];
       double dxr0 = fc0 - m.x[2 * (size_t)R], dxr1 = fc1 - m.x[2 * (size_t)R + 1];
       double phiL = phi[L], phiR = phi[R];
@@ -232,7 +230,11 @@ __global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, const double* _
     double f[4];
     roe_flux(nx, ny, len, gamma, ql, qr, f);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) sub_flux[4 * (size_t)s + j] = f[j];
+    for (int j = 0; j < 4; ++j) {
+      // R[L] += f and R[R] -= f become additions of +f and -f (bit-identical)
+      inc_flux[4 * (size_t)m.sub_inc[2 * (size_t)s] + j] = f[j];
+      inc_flux[4 * (size_t)m.sub_inc[2 * (size_t)s + 1] + j] = -f[j];
+    }
     if (jupdate) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -315,17 +317,13 @@ __global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, PhysPara
   double Rv = interior ? 0.0 : s.R[4 * (size_t)n + v];
   double ev = 0.0;
   if (evcon || interior) {
+#pragma unroll 4
     for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
-      int ent = m.nsub_list[k];
-      int sb = ent & 0x7fffffff;
       if (evcon) {
-        double nx = m.sub_n[3 * (size_t)sb], ny = m.sub_n[3 * (size_t)sb + 1], len = m.sub_n[3 * (size_t)sb + 2];
-        ev += (max_ev(q, nx, ny, p.gamma) * len);
+        const double* nr = m.inc_n + 4 * (size_t)k;
+        ev += (max_ev(q, nr[0], nr[1], p.gamma) * nr[2]);
       }
-      if (interior) {
-        double f = s.sub_flux[4 * (size_t)sb + v];
-        if (ent >= 0) Rv += f; else Rv -= f;
-      }
+      if (interior) Rv += s.inc_flux[4 * (size_t)k + v];
     }
   }
   if (evcon || boundary) {
@@ -356,6 +354,7 @@ __global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, PhysPara
     for (int c = 0; c < 4; ++c) D[c] = Dg[c];
   }
   if (interior && jac) {
+#pragma unroll 4
     for (int k = m.rstart[n]; k < m.iau[n]; ++k) {
       const double* B = s.LHS + 16 * (size_t)m.slot_t[k] + 4 * v;
 #pragma unroll
@@ -409,8 +408,9 @@ __global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const double* __r
 }
 
 // rows [row_begin,row_end) of the color-sorted order; 4 lanes per row, lane v
-// forms component v of RHS - sum_offdiag A dQ (each quad streams whole blocks),
-// then every lane of the quad solves with the factored diagonal.
+// forms component v of RHS - sum_offdiag A dQ in the row's block order (each
+// quad streams whole 128-byte blocks), then every lane of the quad solves with
+// the factored diagonal.
 __global__ void __launch_bounds__(kBlock) k_gs_color(DevMesh m, const double* __restrict__ LHS,
                                                      const double* __restrict__ LU, const double* __restrict__ RHS,
                                                      double* __restrict__ dQ, int row_begin, int row_end) {
@@ -526,7 +526,7 @@ void launch_limiter(const DevMesh& m, const DevState& s, int recon, hipStream_t 
 }
 void launch_edge_flux(const DevMesh& m, const DevState& s, const PhysParams& p, int recon, int jupdate,
                       hipStream_t st) {
-  hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s.nQ, s.grad, s.phi, s.sub_flux,
+  hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s.nQ, s.grad, s.phi, s.inc_flux,
                      s.LHS, p, recon, jupdate);
 }
 void launch_bedge(const DevMesh& m, const DevState& s, const PhysParams& p, int jupdate, hipStream_t st) {
@@ -541,16 +541,34 @@ void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st) {
   size_t n = (size_t)m.nnz * 16;
   hipLaunchKernelGGL(k_zero_offdiag, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, m, s.LHS);
 }
-void launch_gs(const DevMesh& m, const DevState& s, const int* color_off, int subits, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag_lu, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.LHS, s.LU, s.dQ);
-  for (int it = 0; it < subits; ++it) {
-    for (int cc = 0; cc < m.ncolor; ++cc) {
-      int c = (it % 2 == 0) ? cc : m.ncolor - 1 - cc;
-      int b = color_off[c], e = color_off[c + 1];
-      if (e <= b) continue;
-      hipLaunchKernelGGL(k_gs_color, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.LHS, s.LU, s.RHS, s.dQ, b,
-                         e);
+// Color visiting order of `subits` symmetric sweeps (even sweeps ascend, odd
+// sweeps descend).  A color repeated back to back at a turnaround is dropped:
+// its rows depend only on other colors' dQ, so the repeat would reproduce the
+// same values bit for bit.
+int gs_schedule(int ncolor, int subits, int* out) {
+  int n = 0, last = -1;
+  for (int it = 0; it < subits; ++it)
+    for (int cc = 0; cc < ncolor; ++cc) {
+      int c = (it % 2 == 0) ? cc : ncolor - 1 - cc;
+      if (c == last) continue;
+      out[n++] = c;
+      last = c;
     }
+  return n;
+}
+
+void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_lu, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.LHS, s.LU, s.dQ);
+}
+
+void launch_gs_sweeps(const DevMesh& m, const DevState& s, const int* color_off, int subits, hipStream_t st) {
+  std::vector<int> sched((size_t)subits * m.ncolor + 1);
+  int ns = gs_schedule(m.ncolor, subits, sched.data());
+  for (int i = 0; i < ns; ++i) {
+    int c = sched[i];
+    int b = color_off[c], e = color_off[c + 1];
+    if (e <= b) continue;
+    hipLaunchKernelGGL(k_gs_color, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.LHS, s.LU, s.RHS, s.dQ, b, e);
   }
 }
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st) {

@@ -26,10 +26,17 @@ struct DevMesh {
   const double* sub_n;      // 3*nsub
   const double* sub_fc;     // 2*nsub
   const double* sub_xg;     // 6*nsub
+  // Node-incidence order: node n's incident subedges (ascending subedge
+  // index) occupy slots nsub_off[n]..nsub_off[n+1]; its incident edges
+  // (ascending edge index) occupy nedge_off[n]..nedge_off[n+1].
   const int* nsub_off;      // nn+1
-  const int* nsub_list;     // subedge | side<<31
   const int* nedge_off;     // nn+1
-  const int* nedge_list;    // edge | side<<31
+  const int* sub_inc;       // 2*nsub: incidence slot of each subedge at L and at R
+  const double* inc_n;      // 4 per subedge incidence: (nx, ny, len, 0)
+  const double* inc_fc;     // 2 per subedge incidence: face point (k=1)
+  const double* inc_xg;     // 6 per subedge incidence: Gauss points (k=2)
+  const int* inc_other;     // per edge incidence: other node | (1<<31 if this node is R)
+  const double* inc_gw;     // 2 per edge incidence: LSQ weights of this side
   const int* nbe_off;       // nn+1
   const int* nbe_list;
   const int* bedge_real;    // nbe
@@ -63,11 +70,16 @@ struct DevState {
   double* dt;     // nn
   double* LHS;    // nnz*16
   double* LU;     // nn*16, indexed by color position
-  double* sub_flux;  // nsub*4
+  double* inc_flux;  // 2*nsub*4: subedge flux at each endpoint's incidence slot, signed
   double* bflux;     // nbe*4
   double* bjac;      // nbe*16
   double* partial;   // reduction scratch
   int* err;          // device error flag
+  // mixed precision (FP32 Jacobian blocks and sweeps)
+  float* LHS32;      // nnz*16, color-major slots
+  float* LU32;       // nn*16, color position; reciprocal pivots on the diagonal
+  float* RHS32;      // nn*4
+  float* dQ32;       // nn*4
 };
 
 struct PhysParams {
@@ -85,10 +97,23 @@ void launch_edge_flux(const DevMesh& m, const DevState& s, const PhysParams& p, 
 void launch_bedge(const DevMesh& m, const DevState& s, const PhysParams& p, int jupdate, hipStream_t st);
 void launch_node(const DevMesh& m, const DevState& s, const PhysParams& p, int flags, hipStream_t st);
 void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st);
-void launch_gs(const DevMesh& m, const DevState& s, const int* color_off_host, int subits, hipStream_t st);
+// color visiting order of `subits` symmetric multicolor sweeps; returns its length
+int gs_schedule(int ncolor, int subits, int* out);
+void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st);
+void launch_gs_sweeps(const DevMesh& m, const DevState& s, const int* color_off_host, int subits, hipStream_t st);
+inline void launch_gs(const DevMesh& m, const DevState& s, const int* color_off_host, int subits, hipStream_t st) {
+  launch_diag_lu(m, s, st);
+  launch_gs_sweeps(m, s, color_off_host, subits, st);
+}
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st);
 // deterministic sums over nn*4 values: out[0]=sum R^2, out[1]=sum dQ^2 (device)
 void launch_norms(const DevMesh& m, const DevState& s, double* d_out, hipStream_t st);
+// mixed precision
+void launch_edge_jac32(const DevMesh& m, const DevState& s, float gamma, int recon, int jupdate, hipStream_t st);
+void launch_node_mixed(const DevMesh& m, const DevState& s, const PhysParams& p, int jac, hipStream_t st);
+void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st);
+void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const int* color_off, int subits, hipStream_t st);
+void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_roe_batch(int n, double gamma, const double* normals, const double* qL, const double* qR, double* flux,
                       hipStream_t st);
 

@@ -1,0 +1,320 @@
+// FP32 inner kernels of the mixed-precision path (gfx950, wave64).
+//
+// The outer loop stays FP64: the residual (face fluxes, reconstruction,
+// limiter), its norms, the convergence test and the state update use the FP64
+// kernels.  Inside it the Jacobian blocks and the Gauss-Seidel sweeps run in
+// FP32: the analytic Roe Jacobian of compute_approximate_jacobian_roe
+// (jacobian.C:301-658) at the same reconstructed face states the reference
+// differentiates around, FP32 block storage (64 B per 4x4 block) and FP32
+// sweeps.  The fixed point is the FP64 residual's, so converged states agree
+// with the FP64 path to the FP64 stopping tolerance.
+#include "fs_device.hpp"
+#include "fs_kernels.hpp"
+
+#include <vector>
+
+#pragma clang fp contract(fast)
+
+namespace fsk {
+using namespace fsd;
+
+namespace {
+constexpr int kBlock = 256;
+inline int nblocks(int n, int b = kBlock) { return (n + b - 1) / b; }
+}  // namespace
+
+// Roe matrix T|Lambda|Tinv in FP32 with shared reciprocals
+__device__ __forceinline__ void roe_matrix_f32(float nx, float ny, float g, const float* qL, const float* qR,
+                                               float* Rm) {
+  float irL = 1.0f / qL[0], irR = 1.0f / qR[0];
+  float uL = qL[1] * irL, vL = qL[2] * irL, uR = qR[1] * irR, vR = qR[2] * irR;
+  float gm1 = g - 1.0f;
+  float pL = gm1 * (qL[3] - 0.5f * qL[0] * (uL * uL + vL * vL));
+  float pR = gm1 * (qR[3] - 0.5f * qR[0] * (uR * uR + vR * vR));
+  float hL = (qL[3] + pL) * irL, hR = (qR[3] + pR) * irR;
+  float rb = sqrtf(qL[0] * qR[0]);
+  float bg = rb / (qL[0] + rb);
+  float ub = uL + bg * (uR - uL), vb = vL + bg * (vR - vL), hb = hL + bg * (hR - hL);
+  float th = ub * nx + vb * ny;
+  float V2 = ub * ub + vb * vb;
+  float c = sqrtf(gm1 * (hb - 0.5f * V2));
+  float ic = 1.0f / c, ic2 = ic * ic;
+  float lam[4] = {fabsf(th), fabsf(th), fabsf(th + c), fabsf(th - c)};
+  float T[16], Ti[16];
+  T[0] = 1.0f; T[1] = 0.0f; T[2] = ic2; T[3] = ic2;
+  T[4] = ub; T[5] = -ny; T[6] = ub * ic2 + nx * ic; T[7] = ub * ic2 - nx * ic;
+  T[8] = vb; T[9] = nx; T[10] = vb * ic2 + ny * ic; T[11] = vb * ic2 - ny * ic;
+  T[12] = 0.5f * V2; T[13] = -ub * ny + vb * nx;
+  float igm1 = 1.0f / gm1;
+  T[14] = igm1 + th * ic + 0.5f * V2 * ic2;
+  T[15] = igm1 - th * ic + 0.5f * V2 * ic2;
+  Ti[0] = 1.0f - gm1 * 0.5f * V2 * ic2; Ti[1] = gm1 * ub * ic2; Ti[2] = gm1 * vb * ic2; Ti[3] = -gm1 * ic2;
+  Ti[4] = ub * ny - vb * nx; Ti[5] = -ny; Ti[6] = nx; Ti[7] = 0.0f;
+  Ti[8] = 0.5f * (-c * th + gm1 * 0.5f * V2); Ti[9] = 0.5f * (c * nx - gm1 * ub);
+  Ti[10] = 0.5f * (c * ny - gm1 * vb); Ti[11] = 0.5f * gm1;
+  Ti[12] = 0.5f * (c * th + gm1 * 0.5f * V2); Ti[13] = -0.5f * (c * nx + gm1 * ub);
+  Ti[14] = -0.5f * (c * ny + gm1 * vb); Ti[15] = 0.5f * gm1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float a = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a += T[i * 4 + k] * (lam[k] * Ti[k * 4 + j]);
+      Rm[i * 4 + j] = a;
+    }
+}
+
+// Euler flux Jacobian dF.n/dq (jacobian.C:561-582)
+__device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, const float* q, float* d) {
+  float ir = 1.0f / q[0];
+  float u = q[1] * ir, v = q[2] * ir, E = q[3];
+  float th = u * nx + v * ny;
+  float ph = 0.5f * (g - 1.0f) * (u * u + v * v);
+  float gE = g * E * ir;
+  d[0] = 0.0f; d[1] = nx; d[2] = ny; d[3] = 0.0f;
+  d[4] = ph * nx - u * th; d[5] = (2.0f - g) * u * nx + th; d[6] = u * ny - (g - 1.0f) * v * nx; d[7] = (g - 1.0f) * nx;
+  d[8] = ph * ny - v * th; d[9] = v * nx - (g - 1.0f) * u * ny; d[10] = th + (2.0f - g) * v * ny; d[11] = (g - 1.0f) * ny;
+  d[12] = (2.0f * ph - gE) * th; d[13] = (1.0f - g) * u * th + nx * (gE - ph);
+  d[14] = (1.0f - g) * v * th + ny * (gE - ph); d[15] = g * th;
+}
+
+// Edge-centric FP32 analytic Jacobian: for each of the edge's subedges,
+// dF/dqL = 0.5(A(qL) + |A_roe|) len and dF/dqR = 0.5(A(qR) - |A_roe|) len at
+// the (limited) reconstructed face states; block (L,R) += dF/dqR and block
+// (R,L) -= dF/dqL (jacobian.C:612-658).  Zeros when jupdate is off.
+__global__ void __launch_bounds__(kBlock) k_edge_jac32(DevMesh m, const double* __restrict__ nQ,
+                                                       const double* __restrict__ grad,
+                                                       const double* __restrict__ phi, float* __restrict__ LHS32,
+                                                       float gamma, int recon, int jupdate) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  float offLR[16], offRL[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) { offLR[k] = 0.0f; offRL[k] = 0.0f; }
+  if (jupdate) {
+    int L = m.edges[2 * e], R = m.edges[2 * e + 1];
+    float qL0[4], qR0[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      qL0[j] = (float)nQ[4 * (size_t)L + j];
+      qR0[j] = (float)nQ[4 * (size_t)R + j];
+    }
+    float gL[8], gR[8], phL = 0.f, phR = 0.f, xL0 = 0.f, xL1 = 0.f, xR0 = 0.f, xR1 = 0.f;
+    if (recon == 1) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) { gL[k] = (float)grad[8 * (size_t)L + k]; gR[k] = (float)grad[8 * (size_t)R + k]; }
+      phL = (float)phi[L];
+      phR = (float)phi[R];
+      xL0 = (float)m.x[2 * (size_t)L]; xL1 = (float)m.x[2 * (size_t)L + 1];
+      xR0 = (float)m.x[2 * (size_t)R]; xR1 = (float)m.x[2 * (size_t)R + 1];
+    }
+    for (int t = 0; t < 2; ++t) {
+      int s = m.edge_sub[2 * e + t];
+      if (s < 0) break;
+      float nx = (float)m.sub_n[3 * (size_t)s], ny = (float)m.sub_n[3 * (size_t)s + 1];
+      float len = (float)m.sub_n[3 * (size_t)s + 2];
+      float ql[4], qr[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) { ql[j] = qL0[j]; qr[j] = qR0[j]; }
+      if (recon == 1) {
+        size_t ki = (size_t)m.sub_inc[2 * (size_t)s];
+        float fc0 = (float)m.inc_fc[2 * ki], fc1 = (float)m.inc_fc[2 * ki + 1];
+        float dl0 = fc0 - xL0, dl1 = fc1 - xL1, dr0 = fc0 - xR0, dr1 = fc1 - xR1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          ql[j] += phL * (gL[2 * j] * dl0 + gL[2 * j + 1] * dl1);
+          qr[j] += phR * (gR[2 * j] * dr0 + gR[2 * j + 1] * dr1);
+        }
+      }
+      float Rm[16], A[16];
+      roe_matrix_f32(nx, ny, gamma, ql, qr, Rm);
+      float hl = 0.5f * len;
+      flux_jacobian_f32(nx, ny, gamma, ql, A);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) offRL[k] -= hl * (A[k] + Rm[k]);
+      flux_jacobian_f32(nx, ny, gamma, qr, A);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) offLR[k] += hl * (A[k] - Rm[k]);
+    }
+  }
+  float4* blr = reinterpret_cast<float4*>(LHS32 + 16 * (size_t)m.edge_slot[2 * e]);
+  float4* brl = reinterpret_cast<float4*>(LHS32 + 16 * (size_t)m.edge_slot[2 * e + 1]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    blr[k] = make_float4(offLR[4 * k], offLR[4 * k + 1], offLR[4 * k + 2], offLR[4 * k + 3]);
+    brl[k] = make_float4(offRL[4 * k], offRL[4 * k + 1], offRL[4 * k + 2], offRL[4 * k + 3]);
+  }
+}
+
+// Node assembly of the mixed step: FP64 ev_con/dt and residual gather (same
+// order as the FP64 path), FP32 diagonal block V/dt I - sum of transposed
+// off-diagonals + boundary blocks, FP32 right-hand side -R.
+__global__ void __launch_bounds__(kBlock) k_node_mixed(DevMesh m, DevState s, PhysParams p, int jac) {
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = tid >> 2, v = tid & 3;
+  if (n >= m.nn) return;
+  double q[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) q[j] = s.nQ[4 * (size_t)n + j];
+  double Rv = 0.0, ev = 0.0;
+#pragma unroll 4
+  for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
+    const double* nr = m.inc_n + 4 * (size_t)k;
+    ev += (max_ev(q, nr[0], nr[1], p.gamma) * nr[2]);
+    Rv += s.inc_flux[4 * (size_t)k + v];
+  }
+  for (int k = m.nbe_off[n]; k < m.nbe_off[n + 1]; ++k) {
+    int b = m.nbe_list[k];
+    double nx = m.bedge_n[3 * (size_t)b], ny = m.bedge_n[3 * (size_t)b + 1], len = m.bedge_n[3 * (size_t)b + 2];
+    ev += (max_ev(q, nx, ny, p.gamma) * len);
+    Rv += s.bflux[4 * (size_t)b + v];
+  }
+  double V = m.cv_area[n];
+  double dtn = p.cfl * (V / ev);
+  if (v == 0) {
+    if (!(dtn > 0.0) || !isfinite(dtn) || !isfinite(ev)) atomicOr(s.err, 2);
+    s.ev_con[n] = ev;
+    s.dt[n] = dtn;
+  }
+  double vd = V / dtn;
+  double D[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) D[c] = (c == v) ? vd : 0.0;
+  if (jac) {
+#pragma unroll 4
+    for (int k = m.rstart[n]; k < m.iau[n]; ++k) {
+      float4 B = *reinterpret_cast<const float4*>(s.LHS32 + 16 * (size_t)m.slot_t[k] + 4 * v);
+      D[0] -= B.x; D[1] -= B.y; D[2] -= B.z; D[3] -= B.w;
+    }
+    for (int k = m.nbe_off[n]; k < m.nbe_off[n + 1]; ++k) {
+      const double* B = s.bjac + 16 * (size_t)m.nbe_list[k] + 4 * v;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) D[c] += B[c];
+    }
+  }
+  *reinterpret_cast<float4*>(s.LHS32 + 16 * (size_t)m.iau[n] + 4 * v) =
+      make_float4((float)D[0], (float)D[1], (float)D[2], (float)D[3]);
+  s.R[4 * (size_t)n + v] = Rv;
+  s.RHS[4 * (size_t)n + v] = -Rv;
+  s.RHS32[4 * (size_t)n + v] = (float)(-Rv);
+}
+
+__device__ __forceinline__ void lu4f(float* A) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    float ip = 1.0f / A[k * 4 + k];
+#pragma unroll
+    for (int i = k + 1; i < 4; i++) A[i * 4 + k] = A[i * 4 + k] * ip;
+#pragma unroll
+    for (int j = k + 1; j < 4; j++)
+#pragma unroll
+      for (int i = k + 1; i < 4; i++) A[i * 4 + j] = A[i * 4 + j] - A[i * 4 + k] * A[k * 4 + j];
+  }
+  // store reciprocals of the pivots on the diagonal for the substitution
+#pragma unroll
+  for (int k = 0; k < 4; k++) A[k * 4 + k] = 1.0f / A[k * 4 + k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_diag_lu32(DevMesh m, const float* __restrict__ LHS32,
+                                                      float* __restrict__ LU32, float* __restrict__ dQ32) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.nn) return;
+  int n = m.color_rows[t];
+  float A[16];
+  const float4* D = reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)m.iau[n]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float4 r = D[k];
+    A[4 * k] = r.x; A[4 * k + 1] = r.y; A[4 * k + 2] = r.z; A[4 * k + 3] = r.w;
+  }
+  lu4f(A);
+  float4* o = reinterpret_cast<float4*>(LU32 + 16 * (size_t)t);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o[k] = make_float4(A[4 * k], A[4 * k + 1], A[4 * k + 2], A[4 * k + 3]);
+  *reinterpret_cast<float4*>(dQ32 + 4 * (size_t)n) = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// FP32 multicolor sweep over rows [row_begin,row_end) of the color order;
+// 4 lanes per row (lane v: component v), quad-wide FP32 block rows; the row's
+// index, block and neighbour loads are issued before the arithmetic.
+__global__ void __launch_bounds__(kBlock) k_gs_color32(DevMesh m, const float* __restrict__ LHS32,
+                                                       const float* __restrict__ LU32, const float* __restrict__ RHS32,
+                                                       float* __restrict__ dQ32, int row_begin, int row_end) {
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = row_begin + (tid >> 2), v = tid & 3;
+  if (t >= row_end) return;
+  int j = m.color_rows[t];
+  float r = RHS32[4 * (size_t)j + v];
+  int kb = m.rstart[j], kend = m.iau[j];
+  constexpr int kPre = 8;
+  int nb[kPre];
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) nb[i] = (kb + i < kend) ? m.ja[kb + i] : j;
+  float4 A[kPre], q[kPre];
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) {
+    if (kb + i < kend) {
+      A[i] = *reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)(kb + i) + 4 * v);
+      q[i] = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)nb[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kPre; ++i)
+    if (kb + i < kend) r -= A[i].x * q[i].x + A[i].y * q[i].y + A[i].z * q[i].z + A[i].w * q[i].w;
+  for (int k = kb + kPre; k < kend; ++k) {
+    float4 a = *reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)k + 4 * v);
+    float4 b = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)m.ja[k]);
+    r -= a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+  }
+  float b0 = __shfl(r, 0, 4), b1 = __shfl(r, 1, 4), b2 = __shfl(r, 2, 4), b3 = __shfl(r, 3, 4);
+  const float* U = LU32 + 16 * (size_t)t;
+  float y1 = b1 - U[4] * b0;
+  float y2 = b2 - U[8] * b0 - U[9] * y1;
+  float y3 = b3 - U[12] * b0 - U[13] * y1 - U[14] * y2;
+  float x3 = y3 * U[15];
+  float x2 = (y2 - U[11] * x3) * U[10];
+  float x1 = (y1 - U[6] * x2 - U[7] * x3) * U[5];
+  float x0 = (b0 - U[1] * x1 - U[2] * x2 - U[3] * x3) * U[0];
+  float xv = v == 0 ? x0 : v == 1 ? x1 : v == 2 ? x2 : x3;
+  dQ32[4 * (size_t)j + v] = xv;
+}
+
+// nQ += dQ (FP64 state, FP32 correction) and the FP64 norms' inputs
+__global__ void __launch_bounds__(kBlock) k_update_mixed(int n4, double* __restrict__ nQ,
+                                                         const float* __restrict__ dQ32, double* __restrict__ dQ) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  double d = (double)dQ32[i];
+  dQ[i] = d;
+  nQ[i] += d;
+}
+
+void launch_edge_jac32(const DevMesh& m, const DevState& s, float gamma, int recon, int jupdate, hipStream_t st) {
+  hipLaunchKernelGGL(k_edge_jac32, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s.nQ, s.grad, s.phi, s.LHS32, gamma,
+                     recon, jupdate);
+}
+void launch_node_mixed(const DevMesh& m, const DevState& s, const PhysParams& p, int jac, hipStream_t st) {
+  hipLaunchKernelGGL(k_node_mixed, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s, p, jac);
+}
+void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_lu32, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.LHS32, s.LU32, s.dQ32);
+}
+
+void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const int* color_off, int subits, hipStream_t st) {
+  std::vector<int> sched((size_t)subits * m.ncolor + 1);
+  int ns = gs_schedule(m.ncolor, subits, sched.data());
+  for (int i = 0; i < ns; ++i) {
+    int c = sched[i];
+    int b = color_off[c], e = color_off[c + 1];
+    if (e <= b) continue;
+    hipLaunchKernelGGL(k_gs_color32, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.LHS32, s.LU32, s.RHS32,
+                       s.dQ32, b, e);
+  }
+}
+void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {
+  int n4 = m.nn * 4;
+  hipLaunchKernelGGL(k_update_mixed, dim3(nblocks(n4)), dim3(kBlock), 0, st, n4, s.nQ, s.dQ32, s.dQ);
+}
+
+}  // namespace fsk

@@ -26,7 +26,7 @@ _MESH_ARRAYS = [
     "sub_xm", "sub_xg", "el_cent", "bedge", "bedge_n", "bedge_xg", "cv_area", "moments", "edge_weight", "ia", "iau",
     "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg",
 ]
-_FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS"]
+_FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS", "LHS32"]
 
 
 class FlowSolverError(RuntimeError):
@@ -67,7 +67,7 @@ class MeshInfo(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("step_ms", C.c_double), ("edge_ms", C.c_double), ("node_ms", C.c_double), ("gs_ms", C.c_double),
-                ("other_ms", C.c_double), ("flux_evals", C.c_int64)]
+                ("lu_ms", C.c_double), ("other_ms", C.c_double), ("flux_evals", C.c_int64)]
 
 
 _lib = None
@@ -228,7 +228,7 @@ class Solver:
         i = self.info
         return {"Q": 4 * (i["nn"] + i["ng"]), "nQ": 4 * (i["nn"] + i["ng"]), "dQ": 4 * i["nn"], "R": 4 * i["nn"],
                 "RHS": 4 * i["nn"], "grad": 8 * i["nn"], "phi": i["nn"], "hess": 20 * i["nn"], "pv": 4 * i["nn"],
-                "ev_con": i["nn"], "dt": i["nn"], "LHS": 16 * i["nnz"]}[field]
+                "ev_con": i["nn"], "dt": i["nn"], "LHS": 16 * i["nnz"], "LHS32": 16 * i["nnz"]}[field]
 
     def get(self, field):
         n = self._count(field)

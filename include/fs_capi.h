@@ -130,6 +130,7 @@ typedef enum {
   FS_F_EVCON,   /* f64 nn */
   FS_F_DT,      /* f64 nn */
   FS_F_LHS,     /* f64 nnz*16 row-major 4x4 blocks */
+  FS_F_LHS32,   /* mixed precision only: the FP32 blocks, widened to f64, nnz*16 */
   FS_F_COUNT
 } fs_field;
 /* host <-> device copies (synchronous); n is the element count */
@@ -163,7 +164,8 @@ typedef struct {
   double step_ms;        /* mean wall time per pseudo-time step */
   double edge_ms;        /* mean device time of the edge flux kernel */
   double node_ms;        /* mean device time of the node assembly kernel */
-  double gs_ms;          /* mean device time of all Gauss-Seidel sweeps */
+  double gs_ms;          /* mean device time of the Gauss-Seidel sweep launches */
+  double lu_ms;          /* mean device time of the diagonal-block factorisation */
   double other_ms;
   int64_t flux_evals;    /* residual Roe evaluations per step (subedges+bedges) */
 } fs_timing;

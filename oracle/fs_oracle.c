@@ -442,6 +442,58 @@ static void numerical_jacobian(const fso_mesh* m, const fso_params* p, fso_state
         for (b = 0; b < NV; b++) LHS[(size_t)i * 16 + k * 4 + b] -= dfdqL[k][b];
 }
 
+/* jacobian.C:301-658 compute_approximate_jacobian_roe: dF/dqL = 0.5(A(qL) +
+ * |A_roe|) len, dF/dqR = 0.5(A(qR) - |A_roe|) len, scattered like the
+ * numerical Jacobian */
+static void flux_jac_rows(double g, const double* Q, double nx, double ny, double d[4][4]) {
+  double rho = Q[0], u = Q[1] / Q[0], v = Q[2] / Q[0], E = Q[3];
+  double theta = u * nx + v * ny;
+  double phi = 0.5 * (g - 1.) * (u * u + v * v);
+  d[0][0] = 0.; d[0][1] = nx; d[0][2] = ny; d[0][3] = 0.;
+  d[1][0] = phi * nx - u * theta;
+  d[1][1] = (2. - g) * u * nx + theta;
+  d[1][2] = u * ny - (g - 1.) * v * nx;
+  d[1][3] = (g - 1.) * nx;
+  d[2][0] = phi * ny - v * theta;
+  d[2][1] = v * nx - (g - 1.) * u * ny;
+  d[2][2] = theta + (2. - g) * v * ny;
+  d[2][3] = (g - 1.) * ny;
+  d[3][0] = (2. * phi - g * E / rho) * theta;
+  d[3][1] = (1. - g) * u * theta + nx * (g * E / rho - phi);
+  d[3][2] = (1. - g) * v * theta + ny * (g * E / rho - phi);
+  d[3][3] = g * theta;
+}
+
+static void approximate_jacobian(const fso_mesh* m, const fso_params* p, fso_state* s, int L, int R, const double* QL,
+                                 const double* QR, double nx, double ny, double len) {
+  double Roemat[16], dL[4][4], dR[4][4];
+  int i, k, b;
+  double* LHS = s->LHS;
+  roe_matrix(nx, ny, p->gamma, QL, QR, Roemat);
+  flux_jac_rows(p->gamma, QL, nx, ny, dL);
+  flux_jac_rows(p->gamma, QR, nx, ny, dR);
+  for (i = 0; i < 4; i++)
+    for (k = 0; k < 4; k++) {
+      dL[i][k] = 0.5 * (dL[i][k] + Roemat[i * 4 + k]) * len;
+      dR[i][k] = 0.5 * (dR[i][k] - Roemat[i * 4 + k]) * len;
+    }
+  for (i = 0; i < 4; i++)
+    for (k = 0; k < 4; k++) LHS[(size_t)m->iau[L] * 16 + i * 4 + k] += dL[i][k];
+  for (i = 0; i < 4; i++)
+    for (k = 0; k < 4; k++) LHS[(size_t)m->iau[R] * 16 + i * 4 + k] -= dR[i][k];
+  for (i = m->ia[L]; i < m->ia[L + 1]; i++)
+    if (m->ja[i] == R)
+      for (k = 0; k < 4; k++)
+        for (b = 0; b < 4; b++) LHS[(size_t)i * 16 + k * 4 + b] += dR[k][b];
+  for (i = m->ia[R]; i < m->ia[R + 1]; i++)
+    if (m->ja[i] == L)
+      for (k = 0; k < 4; k++)
+        for (b = 0; b < 4; b++) LHS[(size_t)i * 16 + k * 4 + b] -= dL[k][b];
+}
+
+static int g_analytic_jacobian = 0;
+void fso_set_analytic_jacobian(int on) { g_analytic_jacobian = on; }
+
 /* residual.C:40-342 build_interior_residuals */
 void fso_interior_residual(const fso_mesh* m, const fso_params* p, fso_state* s, int jupdate, int ts) {
   int is, j;
@@ -468,7 +520,8 @@ void fso_interior_residual(const fso_mesh* m, const fso_params* p, fso_state* s,
     /* ro == 3: the Gauss-point flux is overwritten by the nodal flux
      * (residual.C:208-290), so only the nodal flux is formed here */
     fso_roe_flux(nx, ny, len, p->gamma, ql, qr, flux);
-    if (jupdate) numerical_jacobian(m, p, s, L, R, ql, qr, flux, nx, ny, len);
+    if (jupdate && g_analytic_jacobian) approximate_jacobian(m, p, s, L, R, ql, qr, nx, ny, len);
+    else if (jupdate) numerical_jacobian(m, p, s, L, R, ql, qr, flux, nx, ny, len);
     for (j = 0; j < NV; j++) {
       s->R[L * 4 + j] += flux[j];
       s->R[R * 4 + j] -= flux[j];

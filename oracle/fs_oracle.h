@@ -21,6 +21,10 @@
  *   fso_boundary_residual   reference build (R, LHS, ghost states per step)
  *   fso_linear_solve        reference build (dQ per step); dense solve on tiny systems
  *   fso_solve               reference build step histories
+ *   analytic Jacobian       (fso_set_analytic_jacobian) restates jacobian.C:301-658, which the
+ *                           reference never calls: pinned by closed forms (flux Jacobian
+ *                           against finite differences of the physical flux; the Roe matrix
+ *                           against |A| of the averaged state) in the tests
  *   fso_build_pinv          numpy pinv and quadratic exactness (the reference build
  *                           cannot run the k=2 factorization: LAPACK is absent)
  *   fso_derivatives         quadratic exactness of hess/point_value (tests);
@@ -106,6 +110,9 @@ int fso_boundary_residual(const fso_mesh* m, const fso_params* p, fso_state* s, 
 /* multicolor = 0: the reference's sequential symmetric Gauss-Seidel;
  * multicolor = 1: the same sweeps with rows visited color by color. */
 double fso_linear_solve(const fso_mesh* m, const fso_params* p, fso_state* s, int multicolor);
+/* 1: the interior Jacobian is compute_approximate_jacobian_roe (jacobian.C:301-658)
+ * instead of the forward difference; used to check the FP32 analytic blocks */
+void fso_set_analytic_jacobian(int on);
 double fso_solve(const fso_mesh* m, const fso_params* p, fso_state* s, int ts, int multicolor);
 
 #ifdef __cplusplus

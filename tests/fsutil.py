@@ -70,6 +70,8 @@ def oracle_lib():
         L.fso_boundary_residual.argtypes = [vp, vp, vp, i]
         L.fso_linear_solve.argtypes = [vp, vp, vp, i]
         L.fso_linear_solve.restype = d
+        L.fso_set_analytic_jacobian.argtypes = [i]
+        L.fso_set_analytic_jacobian.restype = None
         L.fso_solve.argtypes = [vp, vp, vp, i, i]
         L.fso_solve.restype = d
         _olib = L

@@ -10,8 +10,8 @@ Tolerances (FP64 path):
   * Jacobian blocks: off-diagonal blocks bit-exact (same FD arithmetic);
     diagonal blocks 1e-12 relative (different summation order);
   * one pseudo-time step from an identical state: dQ within 1e-10 relative of
-    the oracle's multicolor sweep (the diagonal's summation order differs at
-    1e-16 and the sweeps amplify it mildly).  Trajectories are not compared
+    the oracle's multicolor sweep (the diagonal and each row product are summed in
+    a different fixed order, 1e-16 apart).  Trajectories are not compared
     step by step: the reference's forward-difference Jacobian (eps=1e-8)
     amplifies any last-bit state difference by 1e8;
   * converged steady state: density L2 relative difference <= 1e-10.
@@ -172,4 +172,58 @@ def test_converged_state_k1_unlimited():
     rel = np.linalg.norm(rho - rho_ref) / np.linalg.norm(rho_ref)
     print("gpu steps %d drop %.2e; oracle steps %d; density L2 rel diff %.3e" % (steps, drop, ts, rel))
     assert drop <= 1e-10
+    assert rel <= 1e-10
+
+
+def test_mixed_fp32_jacobian_blocks():
+    """FP32 analytic blocks of the mixed path against the oracle's FP64
+    restatement of jacobian.C:301-658 at the same (limited, reconstructed) states."""
+    p = _params(order=1, foits=2, subits=0)
+    m = fsm.Mesh.bump(33, 17, 0.2, 11)
+    m.prepare(p)
+    s = fsm.Solver(m, p, fsm.FS_PREC_MIXED)
+    o = U.Oracle(m, p)
+    info = m.info()
+    Q = U.perturbed_state(info["nn"], info["ng"], p, seed=8)
+    _set_state(s, o, Q)
+    ts = 5
+    s.solve(ts)
+    U.oracle_lib().fso_set_analytic_jacobian(1)
+    try:
+        o.solve(ts, multicolor=1)
+    finally:
+        U.oracle_lib().fso_set_analytic_jacobian(0)
+    L32, Lr = s.get("LHS32").reshape(-1, 16), o["LHS"].reshape(-1, 16)
+    err = np.max(np.abs(L32 - Lr)) / np.max(np.abs(Lr))
+    print("FP32 Jacobian blocks vs FP64 analytic: max rel err %.2e" % err)
+    assert err <= 2e-5
+    R, Rr = s.get("R"), o["R"]
+    assert np.max(np.abs(R - Rr)) <= 1e-12 * np.max(np.abs(Rr))
+
+
+@pytest.mark.parametrize("order,limit,soits", [(1, 0, 0), (3, 1, 10)])
+def test_mixed_converged_state_matches_fp64(order, limit, soits):
+    """Mixed precision reaches the same steady state as the FP64 path: the FP64
+    outer residual fixes the converged state; error against the oracle stated."""
+    p, m, s, o = _pair(49, 17, 1, order=order, limit=limit, soits=soits, CFL=100.0, ramp=30, foits=10, subits=20,
+                       alpha=0.0)
+    sm = fsm.Solver(m, p, fsm.FS_PREC_MIXED)
+    sm.init_freestream()
+    steps_m, drop_m = sm.solve_steady(800, 1e-10)
+    o.init_freestream()
+    ts_ref = p.foits + 1 if order == 1 else p.foits + soits + 1
+    r0 = None
+    for ts in range(1, 801):
+        o.solve(ts, multicolor=0)
+        r = np.sqrt(o.s.last_rms)
+        if ts == ts_ref:
+            r0 = r
+        if r0 and r / r0 <= 1e-10:
+            break
+    nn = m.info()["nn"]
+    rho, rho_ref = sm.get("Q")[0:4 * nn:4], o["Q"][0:4 * nn:4]
+    rel = np.linalg.norm(rho - rho_ref) / np.linalg.norm(rho_ref)
+    print("mixed steps %d drop %.2e vs oracle %d steps; density L2 rel err vs FP64 oracle %.3e" %
+          (steps_m, drop_m, ts, rel))
+    assert drop_m <= 1e-10
     assert rel <= 1e-10

@@ -209,3 +209,26 @@ def test_gauss_seidel_converges_to_dense_solve():
         o["dQ"][:] = 0
         U.oracle_call(o, "fso_linear_solve", mc)
         assert np.max(np.abs(o["dQ"] - x)) <= 1e-10 * np.max(np.abs(x))
+
+
+def test_analytic_jacobian_equals_fd_at_equal_states():
+    """At qL = qR the Roe flux's exact derivative is 0.5(A +- |A|) len, which is
+    the analytic Jacobian of jacobian.C:301-658; the forward difference must
+    agree to O(eps) there (uniform freestream: every face has equal states)."""
+    p = fsm.Params.make(order=1, foits=5, limit=1, reorder=1, mach=0.5, alpha=2.0, subits=0, method=1, iter=1,
+                        jufreq=1, CFL=10.0, ramp=5)
+    m = fsm.Mesh.bump(17, 9, 0.2, 7)
+    m.prepare(p)
+    L = U.oracle_lib()
+    out = {}
+    for analytic in (0, 1):
+        o = U.Oracle(m, p)
+        o.init_freestream()
+        L.fso_set_analytic_jacobian(analytic)
+        o.s.citer = 1
+        assert U.oracle_call(o, "fso_build_matrix") == 0
+        U.oracle_call(o, "fso_interior_residual", 1, 1)
+        out[analytic] = o["LHS"].copy()
+    L.fso_set_analytic_jacobian(0)
+    scale = np.max(np.abs(out[1]))
+    assert np.max(np.abs(out[0] - out[1])) <= 1e-6 * scale
