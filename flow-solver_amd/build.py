@@ -28,8 +28,8 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + INC]
 
 HOST_SRCS = ["csrc/host/fs_setup.cpp"]
-HIP_SRCS = ["csrc/kernels/fs_kernels.hip", "csrc/kernels/fs_kernels_f32.hip", "csrc/capi/fs_capi.hip",
-            "csrc/capi/fs_batch.hip"]
+HIP_SRCS = ["csrc/kernels/fs_kernels.hip", "csrc/kernels/fs_kernels_f32.hip", "csrc/capi/fs_engine.hip",
+            "csrc/capi/fs_capi.hip", "csrc/capi/fs_batch.hip"]
 
 
 def hipcc():
@@ -57,8 +57,6 @@ def build(jobs=8, verbose=False):
     hc = hipcc()
     for src in HIP_SRCS:
         path = os.path.join(HERE, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         jobs_list.append([hc] + HIP_FLAGS + ["-c", path, "-o", obj])
         objs.append(obj)

@@ -1,0 +1,14 @@
+// Definitions shared by the C-ABI translation units (not part of the ABI).
+#pragma once
+#include "../host/fs_setup.hpp"
+
+struct fs_mesh {
+  fs::Mesh m;
+  bool prepared = false;
+  int order_prepared = -1;
+};
+
+namespace fse {
+// text of a per-instance device error flag
+const char* device_error_message(int e);
+}  // namespace fse

@@ -1,0 +1,696 @@
+// Packing of prepared meshes into one device image and the pseudo-time step
+// over every active instance (solve.C:44-931), optionally as one hipGraph.
+#include "fs_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace fse {
+
+fs::Params to_params(const fs_params* p) {
+  fs::Params q;
+  q.nsteps = p->nsteps; q.iter = p->iter; q.jufreq = p->jufreq; q.CFL = p->CFL; q.ramp = p->ramp;
+  q.gamma = p->gamma; q.alpha = p->alpha; q.mach = p->mach; q.subits = p->subits; q.unsteady = p->unsteady;
+  q.tacc = p->tacc; q.mintime = p->mintime; q.method = p->method; q.order = p->order; q.foits = p->foits;
+  q.soits = p->soits; q.toits = p->toits; q.isrestart = p->isrestart; q.limit = p->limit; q.reorder = p->reorder;
+  q.ic = p->ic; q.p_or_cv = p->p_or_cv; q.start_post_pro = p->start_post_pro; q.post_pro_freq = p->post_pro_freq;
+  return q;
+}
+void from_params(const fs::Params& q, fs_params* p) {
+  p->nsteps = q.nsteps; p->iter = q.iter; p->jufreq = q.jufreq; p->CFL = q.CFL; p->ramp = q.ramp;
+  p->gamma = q.gamma; p->alpha = q.alpha; p->mach = q.mach; p->subits = q.subits; p->unsteady = q.unsteady;
+  p->tacc = q.tacc; p->mintime = q.mintime; p->method = q.method; p->order = q.order; p->foits = q.foits;
+  p->soits = q.soits; p->toits = q.toits; p->isrestart = q.isrestart; p->limit = q.limit; p->reorder = q.reorder;
+  p->ic = q.ic; p->p_or_cv = q.p_or_cv; p->start_post_pro = q.start_post_pro; p->post_pro_freq = q.post_pro_freq;
+}
+
+double cfl_for(const fs_params& p, int citer) {
+  if (p.method == 1) {
+    if (citer < p.ramp) return 1. + (1. * (citer - 1)) / (1. * p.ramp) * (p.CFL - 1.);
+    return p.CFL;
+  }
+  if (p.method == 0) return 1.0;
+  return p.CFL;
+}
+
+// order 1: first-order fluxes for foits steps, then limited linear
+// reconstruction.  order 3: first order, then soits linear steps, then the
+// k=2 phase whose face states are nodal (the reference's residual at k=2 is
+// first order; the k=2 derivatives feed only the Gauss-point limiter).
+Phase phase_for(const fs_params& p, int ts) {
+  Phase ph{0, false, false, 0};
+  if (p.order == 1) {
+    if (ts > p.foits) {
+      ph.recon_flux = 1;
+      ph.gradient = true;
+      ph.lim = p.limit > 0 ? 1 : 0;
+    }
+  } else if (p.order == 3) {
+    if (ts > p.foits && ts <= p.foits + p.soits) {
+      ph.recon_flux = 1;
+      ph.gradient = true;
+    } else if (ts > p.foits + p.soits) {
+      ph.k2 = true;
+      ph.lim = p.limit > 0 ? 3 : 0;
+    }
+  }
+  return ph;
+}
+
+bool supported(const fs_params& p, std::string* why) {
+  if (p.order != 1 && p.order != 3) { *why = "order must be 1 (k=1) or 3 (k=2)"; return false; }
+  if (p.limit != 0 && p.limit != 1) { *why = "limit must be 0 or 1 (Barth)"; return false; }
+  if (p.method != 1 && p.method != 2) { *why = "method must be implicit (1 ramped CFL or 2 fixed CFL)"; return false; }
+  if (p.unsteady != 0 || p.tacc != 0) { *why = "unsteady time accuracy is not on this path"; return false; }
+  if (p.iter < 1 || p.jufreq < 1 || p.subits < 0) { *why = "iter/jufreq must be >= 1, subits >= 0"; return false; }
+  if (!(p.gamma > 1.0) || !(p.mach >= 0.0)) { *why = "gamma must exceed 1 and mach be non-negative"; return false; }
+  return true;
+}
+
+size_t footprint_bytes(const fs::Mesh& g, int prec) {
+  size_t nn = g.nn, ng = g.ng, ne = g.ne, nsub = g.nsub, nbe = g.nbe, nnz = g.ja.size();
+  size_t ninc = g.nsub_list.size(), neinc = g.nedge_list.size(), nst = g.st_nodes.size();
+  size_t b = 0;
+  // mesh image
+  b += nn * (4 * 4 + 8 * 2 + 8 + 8 * 9 + 4 * 4 + 4 * 3) + ne * 4 * 6 + nsub * (8 * 3 + 4 * 2);
+  b += ninc * (8 * 12) + neinc * (4 + 16) + nbe * (4 * 4 + 8 * 3) + nnz * 8 + (nn + 1) * 4 * 4 + nst * (4 + 40);
+  // state
+  b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + ninc * 4 * 8 + nbe * 8 * 20;
+  if (prec == FS_PREC_MIXED)
+    b += nnz * 16 * 4 + nn * (16 * 4 + 4 * 4 * 2);
+  else
+    b += nnz * 16 * 8 + nn * 16 * 8;
+  b += (nn / fsk::kNormChunkNodes + 2) * (16 + 12);
+  return b;
+}
+
+namespace {
+__global__ void k_fill(double* p, size_t n, double v) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void fill(double* p, size_t n, double v, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+}  // namespace
+
+template <typename T>
+int Engine::alloc(T** out, size_t count) {
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = sizeof(T);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return set_error(FS_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  bufs_.push_back({p});
+  bytes_ += bytes;
+  *out = static_cast<T*>(p);
+  return FS_OK;
+}
+
+template <typename T>
+int Engine::upload(const T** out, const std::vector<T>& v) {
+  T* d = nullptr;
+  int rc = alloc(&d, v.size());
+  if (rc) return rc;
+  if (!v.empty()) FSE_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = d;
+  return FS_OK;
+}
+
+Engine::~Engine() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (h_ip_) (void)hipHostFree(h_ip_);
+  if (h_norms_) (void)hipHostFree(h_norms_);
+  if (h_err_) (void)hipHostFree(h_err_);
+  for (auto& b : bufs_) (void)hipFree(b.p);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector<fs_params>& prm, int prec) {
+  const int B = (int)meshes.size();
+  if (B < 1 || (int)prm.size() != B) return set_error(FS_ERR_ARG, "need one parameter set per mesh");
+  if (prec != FS_PREC_F64 && prec != FS_PREC_MIXED) return set_error(FS_ERR_ARG, "unknown precision");
+  precision = prec;
+  subits = prm[0].subits;
+  iter = prm[0].iter;
+  jufreq = prm[0].jufreq;
+  for (int i = 0; i < B; ++i) {
+    std::string why;
+    if (!supported(prm[i], &why)) return set_error(FS_ERR_UNSUPPORTED, "instance " + std::to_string(i) + ": " + why);
+    if (prm[i].subits != subits || prm[i].iter != iter || prm[i].jufreq != jufreq)
+      return set_error(FS_ERR_ARG, "all instances of a batch share subits, iter and jufreq");
+    const fs::Mesh& g = *meshes[i];
+    if (g.ng != g.nbe) return set_error(FS_ERR_MESH, "ghost count differs from boundary-edge count");
+    if (prm[i].order == 3 && (int)g.st_off.size() != g.nn + 1)
+      return set_error(FS_ERR_ARG, "instance " + std::to_string(i) + ": order 3 needs a mesh prepared with order 3");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(FS_ERR_HIP, "no HIP device available");
+  FSE_HIP(hipGetDevice(&device));
+  FSE_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+  // ---- offsets ----
+  inst.resize(B);
+  long long NN = 0, NE = 0, NS = 0, NB = 0, NNZ = 0, KINC = 0, EINC = 0, BINC = 0, NST = 0;
+  int ncmax = 0;
+  for (int i = 0; i < B; ++i) {
+    const fs::Mesh& g = *meshes[i];
+    Instance& I = inst[i];
+    I.mesh = &g;
+    I.prm = prm[i];
+    double angle = prm[i].alpha * M_PI / 180.;
+    I.uinf = prm[i].mach * cos(angle);
+    I.vinf = prm[i].mach * sin(angle);
+    I.pinf = 1. / prm[i].gamma;
+    I.noff = (int)NN; I.nn = g.nn;
+    I.eoff = (int)NE; I.ne = g.ne;
+    I.soff = (int)NS; I.nsub = g.nsub;
+    I.boff = (int)NB; I.nbe = g.nbe;
+    I.nnz = (int)g.ja.size();
+    I.ncolor = g.ncolor;
+    NN += g.nn; NE += g.ne; NS += g.nsub; NB += g.nbe; NNZ += (long long)g.ja.size();
+    KINC += (long long)g.nsub_list.size(); EINC += (long long)g.nedge_list.size();
+    BINC += (long long)g.nbe_list.size(); NST += (long long)g.st_nodes.size();
+    ncmax = std::max(ncmax, g.ncolor);
+  }
+  if (NN >= (1LL << 30) || NNZ >= (1LL << 31) || KINC >= (1LL << 31) || NST >= (1LL << 31))
+    return set_error(FS_ERR_ARG, "batch too large for 32-bit packed indices");
+  fsk::DevMesh& d = dm;
+  d.nn = (int)NN; d.ng = (int)NB; d.ne = (int)NE; d.nsub = (int)NS; d.nbe = (int)NB; d.nnz = (int)NNZ;
+  d.ncolor = ncmax; d.ninst = B;
+
+  // ---- per-node / per-edge / per-subedge / per-bedge arrays ----
+  std::vector<int> inst_node(NN), inst_edge(NE), inst_bedge(NB);
+  std::vector<double> x(2 * NN), cv(NN), mom(9 * NN);
+  std::vector<int> edges(2 * NE), edge_sub(2 * NE, -1), edge_slot(2 * NE);
+  std::vector<double> sub_n(3 * NS);
+  std::vector<int> nsub_off(NN + 1), nedge_off(NN + 1), nbe_off(NN + 1), st_off(NN + 1);
+  std::vector<int> sub_inc(2 * NS), inc_other(EINC), nbe_list(BINC), st_nodes(NST);
+  std::vector<double> inc_n(4 * KINC), inc_fc(2 * KINC), inc_xg(6 * KINC), inc_gw(2 * EINC), st_Wt(5 * NST);
+  std::vector<int> breal(NB), bbc(NB);
+  std::vector<double> bn(3 * NB);
+  long long kinc = 0, einc = 0, binc = 0, sto = 0;
+  for (int i = 0; i < B; ++i) {
+    const fs::Mesh& g = *meshes[i];
+    const Instance& I = inst[i];
+    const int no = I.noff, eo = I.eoff, so = I.soff, bo = I.boff;
+    for (int n = 0; n < g.nn; ++n) {
+      inst_node[no + n] = i;
+      x[2 * (size_t)(no + n)] = g.x[2 * (size_t)n];
+      x[2 * (size_t)(no + n) + 1] = g.x[2 * (size_t)n + 1];
+      cv[no + n] = g.cv_area[n];
+      for (int c = 0; c < 9; ++c) mom[9 * (size_t)(no + n) + c] = g.moments[9 * (size_t)n + c];
+      nsub_off[no + n] = (int)(kinc + g.nsub_off[n]);
+      nedge_off[no + n] = (int)(einc + g.nedge_off[n]);
+      nbe_off[no + n] = (int)(binc + g.nbe_off[n]);
+      st_off[no + n] = (int)(sto + (g.st_off.empty() ? 0 : g.st_off[n]));
+    }
+    for (int e = 0; e < g.ne; ++e) {
+      inst_edge[eo + e] = i;
+      edges[2 * (size_t)(eo + e)] = g.edges[2 * (size_t)e] + no;
+      edges[2 * (size_t)(eo + e) + 1] = g.edges[2 * (size_t)e + 1] + no;
+      for (int k = g.esub_off[e], t = 0; k < g.esub_off[e + 1] && t < 2; ++k, ++t)
+        edge_sub[2 * (size_t)(eo + e) + t] = g.esub_list[k] + so;
+    }
+    for (int s = 0; s < g.nsub; ++s)
+      for (int c = 0; c < 3; ++c) sub_n[3 * (size_t)(so + s) + c] = g.sub_n[3 * (size_t)s + c];
+    // subedge incidences: normals, face points (reconstruct.C:181-186) and
+    // Gauss points, in each node's ascending subedge order
+    for (size_t k = 0; k < g.nsub_list.size(); ++k) {
+      int ent = g.nsub_list[k];
+      int is = ent & 0x7fffffff;
+      size_t K = (size_t)kinc + k;
+      sub_inc[2 * (size_t)(so + is) + (ent < 0 ? 1 : 0)] = (int)K;
+      for (int c = 0; c < 3; ++c) inc_n[4 * K + c] = g.sub_n[3 * (size_t)is + c];
+      inc_n[4 * K + 3] = 0.0;
+      int el = g.sub_elem[is];
+      inc_fc[2 * K] = (g.el_cent[2 * (size_t)el] + g.sub_xm[2 * (size_t)is]) * 0.5;
+      inc_fc[2 * K + 1] = (g.el_cent[2 * (size_t)el + 1] + g.sub_xm[2 * (size_t)is + 1]) * 0.5;
+      for (int c = 0; c < 6; ++c) inc_xg[6 * K + c] = g.sub_xg[6 * (size_t)is + c];
+    }
+    for (size_t k = 0; k < g.nedge_list.size(); ++k) {
+      int ent = g.nedge_list[k];
+      int e = ent & 0x7fffffff;
+      bool isR = ent < 0;
+      int other = (isR ? g.edges[2 * (size_t)e] : g.edges[2 * (size_t)e + 1]) + no;
+      size_t K = (size_t)einc + k;
+      inc_other[K] = other | (isR ? (int)0x80000000u : 0);
+      inc_gw[2 * K] = g.edge_weight[4 * (size_t)e + (isR ? 2 : 0)];
+      inc_gw[2 * K + 1] = g.edge_weight[4 * (size_t)e + (isR ? 3 : 1)];
+    }
+    for (size_t k = 0; k < g.nbe_list.size(); ++k) nbe_list[(size_t)binc + k] = g.nbe_list[k] + bo;
+    for (int b = 0; b < g.nbe; ++b) {
+      inst_bedge[bo + b] = i;
+      breal[bo + b] = g.bedge[5 * (size_t)b] + no;
+      bbc[bo + b] = g.bbc[g.bedge[5 * (size_t)b + 3]];
+      for (int c = 0; c < 3; ++c) bn[3 * (size_t)(bo + b) + c] = g.bedge_n[3 * (size_t)b + c];
+    }
+    // pseudo-inverse entry-major: Wt[(o+j)*5 + r] = W[5*o + r*m + j]
+    for (int n = 0; n + 1 < (int)g.st_off.size(); ++n) {
+      int o = g.st_off[n], mm = g.st_off[n + 1] - o;
+      for (int j = 0; j < mm; ++j) {
+        st_nodes[(size_t)sto + o + j] = g.st_nodes[o + j] + no;
+        for (int r = 0; r < 5; ++r)
+          st_Wt[5 * ((size_t)sto + o + j) + r] = g.st_W[5 * (size_t)o + (size_t)r * mm + j];
+      }
+    }
+    kinc += (long long)g.nsub_list.size();
+    einc += (long long)g.nedge_list.size();
+    binc += (long long)g.nbe_list.size();
+    sto += (long long)g.st_nodes.size();
+  }
+  nsub_off[NN] = (int)kinc;
+  nedge_off[NN] = (int)einc;
+  nbe_off[NN] = (int)binc;
+  st_off[NN] = (int)sto;
+
+  // ---- color-major block CSR over all instances ----
+  // Global color c holds the color-c rows of every instance, instance by
+  // instance; rows of instances whose last color is c come after the others
+  // (backward launches of c stop before them).  Each row keeps its blocks in
+  // the reference's order (nsn order, diagonal last).
+  std::vector<int> color_rows(NN), pos_inst(NN), rstart(NN), iau_d(NN), ja_d(NNZ), slot_t_d(NNZ);
+  color_off.assign(ncmax + 1, 0);
+  color_nback.assign(ncmax, 0);
+  for (auto& I : inst) I.slot_dev.assign(I.nnz, 0);
+  {
+    long long t = 0, pos = 0;
+    for (int c = 0; c < ncmax; ++c) {
+      color_off[c] = (int)t;
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int i = 0; i < B; ++i) {
+          const fs::Mesh& g = *meshes[i];
+          Instance& I = inst[i];
+          if (c >= g.ncolor || ((g.ncolor - 1 == c) != (pass == 1))) continue;
+          for (int tt = g.color_off[c]; tt < g.color_off[c + 1]; ++tt) {
+            int j = g.color_rows[tt];
+            int J = I.noff + j;
+            color_rows[t] = J;
+            pos_inst[t] = i;
+            ++t;
+            rstart[J] = (int)pos;
+            for (int k = g.ia[j]; k < g.ia[j + 1]; ++k) I.slot_dev[k] = (int)pos++;
+            iau_d[J] = I.slot_dev[g.iau[j]];
+          }
+        }
+        if (pass == 0) color_nback[c] = (int)(t - color_off[c]);
+      }
+    }
+    color_off[ncmax] = (int)t;
+    if (t != NN || pos != NNZ) return set_error(FS_ERR_MESH, "coloring does not cover every row exactly once");
+  }
+  for (int i = 0; i < B; ++i) {
+    const fs::Mesh& g = *meshes[i];
+    const Instance& I = inst[i];
+    for (size_t k = 0; k < g.ja.size(); ++k) {
+      ja_d[I.slot_dev[k]] = g.ja[k] + I.noff;
+      slot_t_d[I.slot_dev[k]] = I.slot_dev[g.slot_t[k]];
+    }
+    for (int e = 0; e < g.ne; ++e) {
+      edge_slot[2 * (size_t)(I.eoff + e)] = I.slot_dev[g.edge_slot_lr[e]];
+      edge_slot[2 * (size_t)(I.eoff + e) + 1] = I.slot_dev[g.edge_slot_rl[e]];
+    }
+  }
+
+  // ---- norm chunks ----
+  std::vector<int> ch_inst, ch_b, ch_e, inst_ch0(B + 1);
+  for (int i = 0; i < B; ++i) {
+    inst_ch0[i] = (int)ch_inst.size();
+    for (int n0 = 0; n0 < inst[i].nn; n0 += fsk::kNormChunkNodes) {
+      ch_inst.push_back(i);
+      ch_b.push_back(inst[i].noff + n0);
+      ch_e.push_back(inst[i].noff + std::min(inst[i].nn, n0 + fsk::kNormChunkNodes));
+    }
+  }
+  inst_ch0[B] = (int)ch_inst.size();
+  nc.nchunk = (int)ch_inst.size();
+
+  int rc = 0;
+  if ((rc = upload(&d.inst_node, inst_node)) || (rc = upload(&d.inst_edge, inst_edge)) ||
+      (rc = upload(&d.inst_bedge, inst_bedge)) || (rc = upload(&d.pos_inst, pos_inst)) || (rc = upload(&d.x, x)) ||
+      (rc = upload(&d.cv_area, cv)) || (rc = upload(&d.edges, edges)) || (rc = upload(&d.edge_sub, edge_sub)) ||
+      (rc = upload(&d.edge_slot, edge_slot)) || (rc = upload(&d.sub_n, sub_n)) ||
+      (rc = upload(&d.nsub_off, nsub_off)) || (rc = upload(&d.nedge_off, nedge_off)) ||
+      (rc = upload(&d.sub_inc, sub_inc)) || (rc = upload(&d.inc_n, inc_n)) || (rc = upload(&d.inc_fc, inc_fc)) ||
+      (rc = upload(&d.inc_xg, inc_xg)) || (rc = upload(&d.inc_other, inc_other)) ||
+      (rc = upload(&d.inc_gw, inc_gw)) || (rc = upload(&d.nbe_off, nbe_off)) ||
+      (rc = upload(&d.nbe_list, nbe_list)) || (rc = upload(&d.bedge_real, breal)) ||
+      (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.rstart, rstart)) ||
+      (rc = upload(&d.ja, ja_d)) || (rc = upload(&d.iau, iau_d)) || (rc = upload(&d.slot_t, slot_t_d)) ||
+      (rc = upload(&d.color_rows, color_rows)) || (rc = upload(&d.st_off, st_off)) ||
+      (rc = upload(&d.st_nodes, st_nodes)) || (rc = upload(&d.st_Wt, st_Wt)) || (rc = upload(&d.moments, mom)) ||
+      (rc = upload(&nc.chunk_inst, ch_inst)) || (rc = upload(&nc.chunk_begin, ch_b)) ||
+      (rc = upload(&nc.chunk_end, ch_e)) || (rc = upload(&nc.inst_chunk0, inst_ch0)))
+    return rc;
+
+  // ---- state ----
+  fsk::DevState& st = ds;
+  size_t nn = NN, ntot = (size_t)NN + NB;
+  if ((rc = alloc(&st.Q, ntot * 4)) || (rc = alloc(&st.nQ, ntot * 4)) || (rc = alloc(&st.dQ, nn * 4)) ||
+      (rc = alloc(&st.R, nn * 4)) || (rc = alloc(&st.RHS, nn * 4)) || (rc = alloc(&st.grad, nn * 8)) ||
+      (rc = alloc(&st.phi, nn)) || (rc = alloc(&st.hess, nn * 20)) || (rc = alloc(&st.pv, nn * 4)) ||
+      (rc = alloc(&st.ev_con, nn)) || (rc = alloc(&st.dt, nn)) || (rc = alloc(&st.inc_flux, (size_t)KINC * 4)) ||
+      (rc = alloc(&st.bflux, (size_t)NB * 4)) || (rc = alloc(&st.bjac, (size_t)NB * 16)) ||
+      (rc = alloc(&st.err, (size_t)B)) || (rc = alloc(&st.inst_norms, 2 * (size_t)B)) ||
+      (rc = alloc(&st.chunk_part, 2 * (size_t)std::max(1, nc.nchunk))) ||
+      (rc = alloc(&d_ip_, (size_t)iter * B)))
+    return rc;
+  if (precision == FS_PREC_MIXED) {
+    if ((rc = alloc(&st.LHS32, (size_t)NNZ * 16)) || (rc = alloc(&st.LU32, nn * 16)) ||
+        (rc = alloc(&st.RHS32, nn * 4)) || (rc = alloc(&st.dQ32, nn * 4)))
+      return rc;
+  } else {
+    if ((rc = alloc(&st.LHS, (size_t)NNZ * 16)) || (rc = alloc(&st.LU, nn * 16))) return rc;
+    FSE_HIP(hipMemset(st.LHS, 0, (size_t)NNZ * 16 * sizeof(double)));
+  }
+  st.ip = d_ip_;
+  FSE_HIP(hipMemset(st.err, 0, (size_t)B * sizeof(int)));
+  FSE_HIP(hipMemset(st.grad, 0, nn * 8 * sizeof(double)));
+  FSE_HIP(hipMemset(st.hess, 0, nn * 20 * sizeof(double)));
+  FSE_HIP(hipMemset(st.pv, 0, nn * 4 * sizeof(double)));
+  FSE_HIP(hipMemset(st.dQ, 0, nn * 4 * sizeof(double)));
+  FSE_HIP(hipMemset(st.R, 0, nn * 4 * sizeof(double)));
+  FSE_HIP(hipMemset(st.RHS, 0, nn * 4 * sizeof(double)));
+  FSE_HIP(hipMemset(st.inst_norms, 0, 2 * (size_t)B * sizeof(double)));
+  FSE_HIP(hipHostMalloc((void**)&h_ip_, (size_t)iter * B * sizeof(fsk::InstParams), hipHostMallocDefault));
+  FSE_HIP(hipHostMalloc((void**)&h_norms_, 2 * (size_t)B * sizeof(double), hipHostMallocDefault));
+  FSE_HIP(hipHostMalloc((void**)&h_err_, (size_t)B * sizeof(int), hipHostMallocDefault));
+  std::memset(h_ip_, 0, (size_t)iter * B * sizeof(fsk::InstParams));
+  std::memset(h_err_, 0, (size_t)B * sizeof(int));
+  fill(st.phi, nn, 1.0, stream);
+  for (auto& e : ev_) FSE_HIP(hipEventCreate(&e));
+  FSE_HIP(hipStreamSynchronize(stream));
+  return FS_OK;
+}
+
+int Engine::init_freestream(int which) {
+  FSE_HIP(hipStreamSynchronize(stream));
+  for (int i = 0; i < ninst(); ++i) {
+    if (which >= 0 && i != which) continue;
+    const Instance& I = inst[i];
+    double angle = I.prm.alpha * M_PI / 180.;
+    double rho = 1.0, c = 1.0;
+    double P = rho * c * c / I.prm.gamma;
+    double u = I.prm.mach * cos(angle), v = I.prm.mach * sin(angle);
+    double et = P / (I.prm.gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+    std::vector<double> Q(4 * ((size_t)I.nn + I.nbe));
+    for (size_t n = 0; n < (size_t)I.nn + I.nbe; ++n) {
+      Q[4 * n] = rho; Q[4 * n + 1] = rho * u; Q[4 * n + 2] = rho * v; Q[4 * n + 3] = et;
+    }
+    int rc = set_field(i, FS_F_Q, Q.data(), (int64_t)Q.size());
+    if (!rc) rc = set_field(i, FS_F_NQ, Q.data(), (int64_t)Q.size());
+    if (rc) return rc;
+  }
+  return FS_OK;
+}
+
+fsk::InstParams Engine::params_for(int i, int it) const {
+  const Instance& I = inst[i];
+  fsk::InstParams p{};
+  p.gamma = I.prm.gamma;
+  p.uinf = I.uinf;
+  p.vinf = I.vinf;
+  p.pinf = I.pinf;
+  p.cfl = cfl_for(I.prm, I.ts);
+  Phase ph = phase_for(I.prm, I.ts);
+  p.recon_flux = ph.recon_flux;
+  p.grad = ph.gradient ? 1 : 0;
+  p.k2 = ph.k2 ? 1 : 0;
+  p.lim = ph.lim;
+  // solve.C:335-338: update on the first iteration and every jufreq-th
+  p.jac = (it == 1 || (it - 1) % jufreq == 0) ? 1 : 0;
+  p.active = I.active ? 1 : 0;
+  return p;
+}
+
+int Engine::upload_params(int it) {
+  (void)it;
+  // the staging buffer may still feed an earlier copy: drain the stream
+  FSE_HIP(hipStreamSynchronize(stream));
+  const int B = ninst();
+  for (int k = 1; k <= iter; ++k)
+    for (int i = 0; i < B; ++i) h_ip_[(size_t)(k - 1) * B + i] = params_for(i, k);
+  FSE_HIP(hipMemcpyAsync(d_ip_, h_ip_, (size_t)iter * B * sizeof(fsk::InstParams), hipMemcpyHostToDevice, stream));
+  return FS_OK;
+}
+
+int Engine::upload_params_custom(const fsk::InstParams& p) {
+  FSE_HIP(hipStreamSynchronize(stream));
+  for (int i = 0; i < ninst(); ++i) h_ip_[i] = p;
+  FSE_HIP(hipMemcpyAsync(d_ip_, h_ip_, (size_t)ninst() * sizeof(fsk::InstParams), hipMemcpyHostToDevice, stream));
+  return FS_OK;
+}
+
+// Bit 0: some active instance needs the LSQ gradient; bit 1: k=2
+// derivatives; -1: nothing is active.
+int Engine::step_key() const {
+  int key = 0;
+  bool any = false;
+  for (const auto& I : inst) {
+    if (!I.active) continue;
+    any = true;
+    Phase ph = phase_for(I.prm, I.ts);
+    if (ph.gradient) key |= 1;
+    if (ph.k2) key |= 2;
+  }
+  return any ? key : -1;
+}
+
+// one Newton iteration of solve.C:328-909 over every active instance
+int Engine::newton_iteration(int it, int key, bool record) {
+  fsk::DevState s = ds;
+  s.ip = d_ip_ + (size_t)(it - 1) * ninst();
+  hipStream_t st = stream;
+  if (key & 1) fsk::launch_gradient(dm, s, st);
+  if (key & 2) fsk::launch_derivatives(dm, s, st);
+  fsk::launch_limiter(dm, s, st);
+  fsk::ColorRanges cr{color_off.data(), color_nback.data()};
+  if (precision == FS_PREC_MIXED) {
+    // FP64 residual without Jacobian, FP32 analytic Jacobian and sweeps
+    if (record) FSE_HIP(hipEventRecord(ev_[0], st));
+    fsk::launch_edge_flux(dm, s, 0, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[1], st));
+    fsk::launch_edge_jac32(dm, s, st);
+    fsk::launch_bedge(dm, s, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[2], st));
+    fsk::launch_node_mixed(dm, s, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[3], st));
+    fsk::launch_diag_lu32(dm, s, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[5], st));
+    fsk::launch_gs32_sweeps(dm, s, cr, subits, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[4], st));
+    fsk::launch_update_mixed(dm, s, st);
+    fsk::launch_norms(dm, s, nc, st);
+  } else {
+    if (record) FSE_HIP(hipEventRecord(ev_[0], st));
+    fsk::launch_edge_flux(dm, s, 1, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[1], st));
+    fsk::launch_bedge(dm, s, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[2], st));
+    fsk::launch_node(dm, s,
+                     fsk::NODE_EVCON | fsk::NODE_INTERIOR | fsk::NODE_BOUNDARY | fsk::NODE_RHS | fsk::NODE_JAC, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[3], st));
+    fsk::launch_diag_lu(dm, s, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[5], st));
+    fsk::launch_gs_sweeps(dm, s, cr, subits, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[4], st));
+    fsk::launch_norms(dm, s, nc, st);
+    fsk::launch_update(dm, s, st);
+  }
+  FSE_HIP(hipGetLastError());
+  return FS_OK;
+}
+
+int Engine::enqueue_step(int key, bool record) {
+  size_t ntot = ((size_t)dm.nn + dm.ng) * 4;
+  // nQ = Q for real and ghost nodes (solve.C:319-325)
+  fsk::launch_copy(ds.nQ, ds.Q, ntot, stream);
+  for (int it = 1; it <= iter; ++it) {
+    int rc = newton_iteration(it, key, record && it == iter);
+    if (rc) return rc;
+  }
+  // Q = nQ for real and ghost nodes (solve.C:913-931)
+  fsk::launch_copy(ds.Q, ds.nQ, ntot, stream);
+  FSE_HIP(hipGetLastError());
+  return FS_OK;
+}
+
+int Engine::step(bool advance, bool graph, bool record) {
+  if (advance)
+    for (auto& I : inst)
+      if (I.active) ++I.ts;
+  int key = step_key();
+  if (key < 0) return FS_OK;
+  int rc = upload_params(1);
+  if (rc) return rc;
+  if (graph && !record) {
+    auto itg = graphs_.find(key);
+    if (itg == graphs_.end()) {
+      hipGraph_t g = nullptr;
+      FSE_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_step(key, false);
+      hipError_t ce = hipStreamEndCapture(stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ce != hipSuccess) return set_error(FS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      hipGraphExec_t ex = nullptr;
+      hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return set_error(FS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      itg = graphs_.emplace(key, ex).first;
+    }
+    FSE_HIP(hipGraphLaunch(itg->second, stream));
+  } else {
+    rc = enqueue_step(key, record);
+    if (rc) return rc;
+  }
+  const int B = ninst();
+  FSE_HIP(hipMemcpyAsync(h_norms_, ds.inst_norms, 2 * (size_t)B * sizeof(double), hipMemcpyDeviceToHost, stream));
+  FSE_HIP(hipMemcpyAsync(h_err_, ds.err, (size_t)B * sizeof(int), hipMemcpyDeviceToHost, stream));
+  FSE_HIP(hipStreamSynchronize(stream));
+  if (record) {
+    float a = 0, b = 0, c = 0, l = 0;
+    FSE_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+    FSE_HIP(hipEventElapsedTime(&b, ev_[2], ev_[3]));
+    FSE_HIP(hipEventElapsedTime(&l, ev_[3], ev_[5]));
+    FSE_HIP(hipEventElapsedTime(&c, ev_[5], ev_[4]));
+    t_edge += a; t_node += b; t_gs += c; t_lu += l;
+  }
+  return check_errors(true);
+}
+
+int Engine::fetch_errors() {
+  FSE_HIP(hipMemcpyAsync(h_err_, ds.err, (size_t)ninst() * sizeof(int), hipMemcpyDeviceToHost, stream));
+  FSE_HIP(hipStreamSynchronize(stream));
+  return check_errors(false);
+}
+
+int Engine::check_errors(bool norms) {
+  bool any = false;
+  for (int i = 0; i < ninst(); ++i) {
+    Instance& I = inst[i];
+    if (!I.active) continue;
+    if (norms) {
+      I.r2 = h_norms_[2 * (size_t)i];
+      I.dq2 = h_norms_[2 * (size_t)i + 1];
+    }
+    I.err = h_err_[i];
+    if (I.err) any = true;
+  }
+  if (any) {
+    std::memset(h_err_, 0, (size_t)ninst() * sizeof(int));
+    FSE_HIP(hipMemsetAsync(ds.err, 0, (size_t)ninst() * sizeof(int), stream));
+    FSE_HIP(hipStreamSynchronize(stream));
+  }
+  return FS_OK;
+}
+
+int Engine::sync() {
+  FSE_HIP(hipStreamSynchronize(stream));
+  return FS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-instance fields
+// ---------------------------------------------------------------------------
+int Engine::field_count(int i, int field, int64_t* n) const {
+  const Instance& I = inst[i];
+  int64_t nn = I.nn, ntot = (int64_t)I.nn + I.nbe;
+  switch (field) {
+    case FS_F_Q: case FS_F_NQ: *n = ntot * 4; return FS_OK;
+    case FS_F_DQ: case FS_F_R: case FS_F_RHS: case FS_F_PV: *n = nn * 4; return FS_OK;
+    case FS_F_GRAD: *n = nn * 8; return FS_OK;
+    case FS_F_PHI: case FS_F_EVCON: case FS_F_DT: *n = nn; return FS_OK;
+    case FS_F_HESS: *n = nn * 20; return FS_OK;
+    case FS_F_LHS: case FS_F_LHS32: *n = (int64_t)I.nnz * 16; return FS_OK;
+    default: return set_error(FS_ERR_ARG, "unknown field id");
+  }
+}
+
+static double* node_field(const fsk::DevState& s, int field, int* per) {
+  switch (field) {
+    case FS_F_Q: *per = 4; return s.Q;
+    case FS_F_NQ: *per = 4; return s.nQ;
+    case FS_F_DQ: *per = 4; return s.dQ;
+    case FS_F_R: *per = 4; return s.R;
+    case FS_F_RHS: *per = 4; return s.RHS;
+    case FS_F_PV: *per = 4; return s.pv;
+    case FS_F_GRAD: *per = 8; return s.grad;
+    case FS_F_PHI: *per = 1; return s.phi;
+    case FS_F_EVCON: *per = 1; return s.ev_con;
+    case FS_F_DT: *per = 1; return s.dt;
+    case FS_F_HESS: *per = 20; return s.hess;
+    default: *per = 0; return nullptr;
+  }
+}
+
+int Engine::get_field(int i, int field, double* host, int64_t n) {
+  if (i < 0 || i >= ninst()) return set_error(FS_ERR_ARG, "instance out of range");
+  int64_t cnt = 0;
+  int rc = field_count(i, field, &cnt);
+  if (rc) return rc;
+  if (n != cnt) return set_error(FS_ERR_ARG, "size mismatch: expected " + std::to_string(cnt));
+  const Instance& I = inst[i];
+  FSE_HIP(hipStreamSynchronize(stream));
+  if (field == FS_F_LHS || field == FS_F_LHS32) {
+    if (field == FS_F_LHS && !ds.LHS) return set_error(FS_ERR_ARG, "FP64 blocks exist only for an FP64 solver");
+    if (field == FS_F_LHS32 && !ds.LHS32) return set_error(FS_ERR_ARG, "FP32 blocks exist only for a mixed-precision solver");
+    size_t all = (size_t)dm.nnz * 16;
+    if (field == FS_F_LHS) {
+      std::vector<double> tmp(all);
+      FSE_HIP(hipMemcpy(tmp.data(), ds.LHS, all * sizeof(double), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < I.slot_dev.size(); ++k)
+        std::memcpy(host + 16 * k, &tmp[16 * (size_t)I.slot_dev[k]], 16 * sizeof(double));
+    } else {
+      std::vector<float> tmp(all);
+      FSE_HIP(hipMemcpy(tmp.data(), ds.LHS32, all * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < I.slot_dev.size(); ++k)
+        for (int c = 0; c < 16; ++c) host[16 * k + c] = (double)tmp[16 * (size_t)I.slot_dev[k] + c];
+    }
+    return FS_OK;
+  }
+  int per = 0;
+  double* d = node_field(ds, field, &per);
+  FSE_HIP(hipMemcpy(host, d + (size_t)per * I.noff, (size_t)per * I.nn * sizeof(double), hipMemcpyDeviceToHost));
+  if (field == FS_F_Q || field == FS_F_NQ)
+    FSE_HIP(hipMemcpy(host + (size_t)per * I.nn, d + (size_t)per * ((size_t)dm.nn + I.boff),
+                      (size_t)per * I.nbe * sizeof(double), hipMemcpyDeviceToHost));
+  return FS_OK;
+}
+
+int Engine::set_field(int i, int field, const double* host, int64_t n) {
+  if (i < 0 || i >= ninst()) return set_error(FS_ERR_ARG, "instance out of range");
+  int64_t cnt = 0;
+  int rc = field_count(i, field, &cnt);
+  if (rc) return rc;
+  if (n != cnt) return set_error(FS_ERR_ARG, "size mismatch: expected " + std::to_string(cnt));
+  const Instance& I = inst[i];
+  FSE_HIP(hipStreamSynchronize(stream));
+  if (field == FS_F_LHS32) return set_error(FS_ERR_ARG, "FP32 blocks are read-only");
+  if (field == FS_F_LHS) {
+    if (!ds.LHS) return set_error(FS_ERR_ARG, "FP64 blocks exist only for an FP64 solver");
+    size_t all = (size_t)dm.nnz * 16;
+    std::vector<double> tmp(all);
+    FSE_HIP(hipMemcpy(tmp.data(), ds.LHS, all * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < I.slot_dev.size(); ++k)
+      std::memcpy(&tmp[16 * (size_t)I.slot_dev[k]], host + 16 * k, 16 * sizeof(double));
+    FSE_HIP(hipMemcpy(ds.LHS, tmp.data(), all * sizeof(double), hipMemcpyHostToDevice));
+    return FS_OK;
+  }
+  int per = 0;
+  double* d = node_field(ds, field, &per);
+  FSE_HIP(hipMemcpy(d + (size_t)per * I.noff, host, (size_t)per * I.nn * sizeof(double), hipMemcpyHostToDevice));
+  if (field == FS_F_Q || field == FS_F_NQ)
+    FSE_HIP(hipMemcpy(d + (size_t)per * ((size_t)dm.nn + I.boff), host + (size_t)per * I.nn,
+                      (size_t)per * I.nbe * sizeof(double), hipMemcpyHostToDevice));
+  return FS_OK;
+}
+
+}  // namespace fse

@@ -1,0 +1,140 @@
+// The device engine behind both the single-mesh solver and the batch: a set
+// of prepared meshes packed into one set of device arrays, their per-instance
+// controls, and the pseudo-time step (solve.C:44-931) over all of them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../host/fs_setup.hpp"
+#include "../kernels/fs_kernels.hpp"
+#include "fs_capi.h"
+
+namespace fse {
+
+// records `msg` as the calling thread's last error and returns `code`
+int set_error(int code, const std::string& msg);
+
+#define FSE_HIP(expr)                                                                                 \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess) return ::fse::set_error(FS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+fs::Params to_params(const fs_params* p);
+void from_params(const fs::Params& q, fs_params* p);
+
+// Which reconstruction a pseudo-time step uses (solve.C:389-473 with the
+// order/foits/soits schedule of solve.C:57-114).
+struct Phase {
+  int recon_flux;  // 0 nodal face states, 1 linear limited
+  bool gradient;
+  bool k2;
+  int lim;         // 0 none (phi = 1), 1 Barth at face points, 3 Barth at Gauss points
+};
+Phase phase_for(const fs_params& p, int ts);
+double cfl_for(const fs_params& p, int citer);   // init_memory.C:200-216
+bool supported(const fs_params& p, std::string* why);
+
+struct Instance {
+  const fs::Mesh* mesh = nullptr;
+  fs_params prm{};
+  double uinf = 0, vinf = 0, pinf = 0;
+  int ts = 0;          // pseudo-time steps taken (grid->citer)
+  bool active = true;
+  int noff = 0, nn = 0, eoff = 0, ne = 0, soff = 0, nsub = 0, boff = 0, nbe = 0;
+  int nnz = 0;
+  int ncolor = 0;
+  std::vector<int> slot_dev;   // reference block slot -> packed device slot
+  int err = 0;                 // device error flags of the last step
+  double r2 = 0, dq2 = 0;      // sum R^2 and sum dQ^2 of the last step
+};
+
+// Device bytes that packing `m` into an engine of precision `prec` costs.
+size_t footprint_bytes(const fs::Mesh& m, int prec);
+
+class Engine {
+ public:
+  Engine() = default;
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+  ~Engine();
+
+  // Packs the meshes (prepared, kept alive by the caller) with one parameter
+  // set each.  All instances share subits, iter and jufreq.
+  int create(const std::vector<const fs::Mesh*>& meshes, const std::vector<fs_params>& prm, int precision);
+
+  int ninst() const { return (int)inst.size(); }
+
+  // Freestream initial state (ic.C:808-840, ghosts ic.C:77-86) for instance i
+  // (or all when i < 0).
+  int init_freestream(int i);
+
+  // Writes the controls of Newton iteration `it` of every instance (from its
+  // step counter and precision) to the device table on the stream.
+  int upload_params(int it);
+  // Controls with a forced limiter/gradient mode for the single-function
+  // mirrors (instance 0 of a one-instance engine).
+  int upload_params_custom(const fsk::InstParams& p);
+  fsk::InstParams params_for(int i, int it) const;
+
+  // One pseudo-time step of every active instance.  With `advance` each
+  // active instance's step counter increments first.  With `graph` the step
+  // runs as one hipGraph launch (captured per phase combination); with
+  // `record` the kernel groups are timed with events (no graph).
+  int step(bool advance, bool graph, bool record);
+
+  // copies the per-instance device error flags into Instance::err (and clears
+  // them on the device)
+  int fetch_errors();
+  int sync();
+
+  // per-instance field access (host <-> device), field ids of fs_capi.h
+  int field_count(int i, int field, int64_t* n) const;
+  int get_field(int i, int field, double* host, int64_t n);
+  int set_field(int i, int field, const double* host, int64_t n);
+
+  size_t device_bytes() const { return bytes_; }
+
+  // state
+  int precision = FS_PREC_F64;
+  int subits = 0, iter = 1, jufreq = 1;
+  std::vector<Instance> inst;
+  fsk::DevMesh dm{};
+  fsk::DevState ds{};
+  fsk::NormChunks nc{};
+  std::vector<int> color_off, color_nback;
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  // timing of the last recorded steps (ms, accumulated)
+  double t_edge = 0, t_node = 0, t_gs = 0, t_lu = 0;
+
+ private:
+  struct Buf {
+    void* p = nullptr;
+  };
+  std::vector<Buf> bufs_;
+  size_t bytes_ = 0;
+  fsk::InstParams* d_ip_ = nullptr;   // iter * ninst
+  fsk::InstParams* h_ip_ = nullptr;   // pinned staging, iter * ninst
+  double* h_norms_ = nullptr;         // pinned, 2 * ninst
+  int* h_err_ = nullptr;              // pinned, ninst
+  hipEvent_t ev_[8] = {};
+  std::map<int, hipGraphExec_t> graphs_;
+
+  template <typename T>
+  int alloc(T** out, size_t count);
+  template <typename T>
+  int upload(const T** out, const std::vector<T>& v);
+  int enqueue_step(int key, bool record);
+  int newton_iteration(int it, int key, bool record);
+  int step_key() const;
+  int check_errors(bool norms);
+};
+
+}  // namespace fse

@@ -7,6 +7,10 @@
 // cv_calc.C:819-953).  Face fluxes are computed once per subedge by an
 // edge-centric kernel and written to a per-subedge buffer; the node kernel
 // reads them back.  No floating-point atomics anywhere.
+//
+// Every element reads its instance's controls (InstParams) and does nothing
+// for an inactive instance, so one launch serves a packed batch of meshes and
+// an instance's arithmetic never depends on the other instances.
 #include "fs_device.hpp"
 #include "fs_kernels.hpp"
 
@@ -25,10 +29,13 @@ __device__ __forceinline__ double dmax(double x, double y) { return x >= y ? x :
 // ---------------------------------------------------------------------------
 // K2: least-squares gradient, node-centric gather (gradient.C:363-403)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, const double* __restrict__ nQ,
-                                                     double* __restrict__ grad) {
+__global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, DevState s) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= m.nn) return;
+  const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
+  if (!ip.active || !ip.grad) return;
+  const double* __restrict__ nQ = s.nQ;
+  double* __restrict__ grad = s.grad;
   double g[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) g[k] = 0.0;
@@ -66,11 +73,15 @@ __global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, const double* __
 // of a node read one contiguous run; the 20 partial sums are combined with a
 // butterfly over the 16 lanes.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_derivatives(DevMesh m, const double* __restrict__ nQ,
-                                                        double* __restrict__ hess, double* __restrict__ pv) {
+__global__ void __launch_bounds__(kBlock) k_derivatives(DevMesh m, DevState s) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int i = tid >> 4, l = tid & 15;
   if (i >= m.nn) return;
+  const InstParams& ip = s.ip[inst_of(m, m.inst_node, i)];
+  if (!ip.active || !ip.k2) return;
+  const double* __restrict__ nQ = s.nQ;
+  double* __restrict__ hess = s.hess;
+  double* __restrict__ pv = s.pv;
   int o = m.st_off[i], mm = m.st_off[i + 1] - o;
   double qi[4];
 #pragma unroll
@@ -131,14 +142,24 @@ __device__ __forceinline__ void barth_candidate1(double q, double qmin, double q
 }
 
 // 4 lanes per node, lane v limits variable v; the node scalar is the min over
-// the 4 lanes taken in variable order (limiter.C:410-420)
-__global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, const double* __restrict__ nQ,
-                                                    const double* __restrict__ grad, const double* __restrict__ hess,
-                                                    const double* __restrict__ pv, double* __restrict__ phi,
-                                                    int recon) {
+// the 4 lanes taken in variable order (limiter.C:410-420).  An instance
+// without a limiter this iteration gets phi = 1 (solve.C:343-347).
+__global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, DevState s) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
+  const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
+  if (!ip.active) return;
+  const int recon = ip.lim;
+  double* __restrict__ phi = s.phi;
+  if (recon == 0) {
+    if (v == 0) phi[n] = 1.0;
+    return;
+  }
+  const double* __restrict__ nQ = s.nQ;
+  const double* __restrict__ grad = s.grad;
+  const double* __restrict__ hess = s.hess;
+  const double* __restrict__ pv = s.pv;
   double q = nQ[4 * (size_t)n + v];
   double qmin = q, qmax = q, ph = 1.0;
   for (int k = m.nedge_off[n]; k < m.nedge_off[n + 1]; ++k) {
@@ -191,13 +212,19 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, const double* __r
 // handles its 1-2 subedges in ascending order, stores each subedge's flux and
 // owns the edge's two off-diagonal blocks outright (no atomics).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, const double* __restrict__ nQ,
-                                                      const double* __restrict__ grad, const double* __restrict__ phi,
-                                                      double* __restrict__ inc_flux, double* __restrict__ LHS,
-                                                      PhysParams p, int recon, int jupdate) {
+__global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, DevState st, int jac_allowed) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
-  const double gamma = p.gamma;
+  const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
+  if (!ip.active) return;
+  const int recon = ip.recon_flux;
+  const int jupdate = jac_allowed && ip.jac;
+  const double gamma = ip.gamma;
+  const double* __restrict__ nQ = st.nQ;
+  const double* __restrict__ grad = st.grad;
+  const double* __restrict__ phi = st.phi;
+  double* __restrict__ inc_flux = st.inc_flux;
+  double* __restrict__ LHS = st.LHS;
   const double eps = 1.0e-08;
   int L = m.edges[2 * e], R = m.edges[2 * e + 1];
   double qL0[4], qR0[4];
@@ -258,10 +285,15 @@ __global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, const double* _
       }
     }
   }
-  double* blr = LHS + 16 * (size_t)m.edge_slot[2 * e];
-  double* brl = LHS + 16 * (size_t)m.edge_slot[2 * e + 1];
+  // the edge owns its two off-diagonal blocks: written every FP64 pass
+  // (zeros when the instance skips the Jacobian, as build_matrix clears
+  // them), never by the mixed path's FP64 residual pass
+  if (jac_allowed) {
+    double* blr = LHS + 16 * (size_t)m.edge_slot[2 * e];
+    double* brl = LHS + 16 * (size_t)m.edge_slot[2 * e + 1];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) { blr[k] = offLR[k]; brl[k] = offRL[k]; }
+    for (int k = 0; k < 16; ++k) { blr[k] = offLR[k]; brl[k] = offRL[k]; }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -270,18 +302,23 @@ __global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, const double* _
 // diagonal approximate Jacobian (jacobian.C:681-962).  Each bedge owns its
 // ghost, so the sequential reference loop is order-independent.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_bedge(DevMesh m, double* __restrict__ nQ, double* __restrict__ bflux,
-                                                  double* __restrict__ bjac, PhysParams p, int jupdate,
-                                                  int* __restrict__ err) {
+__global__ void __launch_bounds__(kBlock) k_bedge(DevMesh m, DevState s) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nbe) return;
+  const int inst = inst_of(m, m.inst_bedge, i);
+  const InstParams& p = s.ip[inst];
+  if (!p.active) return;
+  const int jupdate = p.jac;
+  double* __restrict__ nQ = s.nQ;
+  double* __restrict__ bflux = s.bflux;
+  double* __restrict__ bjac = s.bjac;
   int real = m.bedge_real[i];
   size_t gh = (size_t)m.nn + i;
   double nx = m.bedge_n[3 * (size_t)i], ny = m.bedge_n[3 * (size_t)i + 1], len = m.bedge_n[3 * (size_t)i + 2];
   double qi[4], qg[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) { qi[j] = nQ[4 * (size_t)real + j]; qg[j] = nQ[4 * gh + j]; }
-  if (!ghost_state(m.bedge_bc[i], nx, ny, p.gamma, p.uinf, p.vinf, p.pinf, qi, qg)) atomicOr(err, 1);
+  if (!ghost_state(m.bedge_bc[i], nx, ny, p.gamma, p.uinf, p.vinf, p.pinf, qi, qg)) atomicOr(s.err + inst, 1);
 #pragma unroll
   for (int j = 0; j < 4; ++j) nQ[4 * gh + j] = qg[j];
   double f[4];
@@ -305,12 +342,15 @@ __global__ void __launch_bounds__(kBlock) k_bedge(DevMesh m, double* __restrict_
 // ---------------------------------------------------------------------------
 // 4 lanes per node: lane v owns residual component v and row v of the
 // diagonal block, so the quad reads each 128-byte block contiguously.
-__global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, PhysParams p, int flags) {
+__global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, int flags) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
+  const int inst = inst_of(m, m.inst_node, n);
+  const InstParams& p = s.ip[inst];
+  if (!p.active) return;
   const bool evcon = flags & NODE_EVCON, interior = flags & NODE_INTERIOR, boundary = flags & NODE_BOUNDARY;
-  const bool jac = flags & NODE_JAC, rhs = flags & NODE_RHS;
+  const bool jac = (flags & NODE_JAC) && p.jac, rhs = flags & NODE_RHS;
   double q[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) q[j] = s.nQ[4 * (size_t)n + j];
@@ -342,7 +382,7 @@ __global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, PhysPara
     double V = m.cv_area[n];
     double dtn = p.cfl * (V / ev);
     if (v == 0) {
-      if (!(dtn > 0.0) || !isfinite(dtn) || !isfinite(ev)) atomicOr(s.err, 2);
+      if (!(dtn > 0.0) || !isfinite(dtn) || !isfinite(ev)) atomicOr(s.err + inst, 2);
       s.ev_con[n] = ev;
       s.dt[n] = dtn;
     }
@@ -391,11 +431,13 @@ __global__ void __launch_bounds__(kBlock) k_zero_offdiag(DevMesh m, double* __re
 // identical); each color's rows are independent and run in parallel; sweeps
 // alternate forward and backward over the colors.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const double* __restrict__ LHS,
-                                                    double* __restrict__ LU, double* __restrict__ dQ) {
+__global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const InstParams* __restrict__ ip,
+                                                    const double* __restrict__ LHS, double* __restrict__ LU,
+                                                    double* __restrict__ dQ) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m.nn) return;
   int n = m.color_rows[t];
+  if (!ip[inst_of(m, m.pos_inst, t)].active) return;
   double A[16];
   const double* D = LHS + 16 * (size_t)m.iau[n];
 #pragma unroll
@@ -411,16 +453,19 @@ __global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const double* __r
 // forms component v of RHS - sum_offdiag A dQ in the row's block order (each
 // quad streams whole 128-byte blocks), then every lane of the quad solves with
 // the factored diagonal.
-__global__ void __launch_bounds__(kBlock) k_gs_color(DevMesh m, const double* __restrict__ LHS,
-                                                     const double* __restrict__ LU, const double* __restrict__ RHS,
-                                                     double* __restrict__ dQ, int row_begin, int row_end) {
+__global__ void __launch_bounds__(kBlock) k_gs_color(DevMesh m, const InstParams* __restrict__ ip,
+                                                     const double* __restrict__ LHS, const double* __restrict__ LU,
+                                                     const double* __restrict__ RHS, double* __restrict__ dQ,
+                                                     int row_begin, int row_end) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int t = row_begin + (tid >> 2), v = tid & 3;
   if (t >= row_end) return;
   int j = m.color_rows[t];
+  int inst = inst_of(m, m.pos_inst, t);
   double r = RHS[4 * (size_t)j + v];
-  int kend = m.iau[j];
-  for (int k = m.rstart[j]; k < kend; ++k) {
+  int kb = m.rstart[j], kend = m.iau[j];
+  if (!ip[inst].active) return;
+  for (int k = kb; k < kend; ++k) {
     const double* A = LHS + 16 * (size_t)k + 4 * v;
     const double* q = dQ + 4 * (size_t)m.ja[k];
     r -= (A[0] * q[0] + A[1] * q[1] + A[2] * q[2] + A[3] * q[3]);
@@ -441,23 +486,36 @@ __global__ void __launch_bounds__(kBlock) k_gs_color(DevMesh m, const double* __
 }
 
 // nQ += dQ over the real nodes (solve.C:901-907)
-__global__ void __launch_bounds__(kBlock) k_update(int n4, double* __restrict__ nQ, const double* __restrict__ dQ) {
+__global__ void __launch_bounds__(kBlock) k_update(DevMesh m, const InstParams* __restrict__ ip,
+                                                   double* __restrict__ nQ, const double* __restrict__ dQ) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n4) return;
+  if (i >= 4 * m.nn) return;
+  if (!ip[inst_of(m, m.inst_node, i >> 2)].active) return;
   nQ[i] += dQ[i];
 }
 
-// deterministic two-stage sums of R^2 and dQ^2
-__global__ void __launch_bounds__(kBlock) k_norm_partial(int n4, const double* __restrict__ R,
-                                                         const double* __restrict__ dQ, double* __restrict__ part) {
+__global__ void __launch_bounds__(kBlock) k_copy(size_t n, double* __restrict__ dst, const double* __restrict__ src) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Deterministic per-instance sums of R^2 and dQ^2: one block per chunk of
+// kNormChunkNodes nodes (fixed tree order), then one thread per instance adds
+// its chunks in order.
+__global__ void __launch_bounds__(kBlock) k_norm_chunk(NormChunks c, const InstParams* __restrict__ ip,
+                                                       const double* __restrict__ R, const double* __restrict__ dQ,
+                                                       double* __restrict__ part) {
   __shared__ double sr[kBlock], sd[kBlock];
-  double a = 0.0, b = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
+  int ch = blockIdx.x;
+  if (!ip[c.chunk_inst[ch]].active) return;
+  size_t b = 4 * (size_t)c.chunk_begin[ch], e = 4 * (size_t)c.chunk_end[ch];
+  double a = 0.0, d = 0.0;
+  for (size_t i = b + threadIdx.x; i < e; i += kBlock) {
     a += R[i] * R[i];
-    b += dQ[i] * dQ[i];
+    d += dQ[i] * dQ[i];
   }
   sr[threadIdx.x] = a;
-  sd[threadIdx.x] = b;
+  sd[threadIdx.x] = d;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
@@ -467,33 +525,22 @@ __global__ void __launch_bounds__(kBlock) k_norm_partial(int n4, const double* _
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = sr[0];
-    part[2 * blockIdx.x + 1] = sd[0];
+    part[2 * (size_t)ch] = sr[0];
+    part[2 * (size_t)ch + 1] = sd[0];
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_norm_final(int nparts, const double* __restrict__ part,
-                                                       double* __restrict__ out) {
-  __shared__ double sr[kBlock], sd[kBlock];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-    a += part[2 * i];
-    b += part[2 * i + 1];
+__global__ void __launch_bounds__(kBlock) k_norm_inst(int ninst, NormChunks c, const InstParams* __restrict__ ip,
+                                                      const double* __restrict__ part, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst || !ip[i].active) return;
+  double a = 0.0, d = 0.0;
+  for (int ch = c.inst_chunk0[i]; ch < c.inst_chunk0[i + 1]; ++ch) {
+    a += part[2 * (size_t)ch];
+    d += part[2 * (size_t)ch + 1];
   }
-  sr[threadIdx.x] = a;
-  sd[threadIdx.x] = b;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sr[threadIdx.x] += sr[threadIdx.x + w];
-      sd[threadIdx.x] += sd[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    out[0] = sr[0];
-    out[1] = sd[0];
-  }
+  out[2 * (size_t)i] = a;
+  out[2 * (size_t)i + 1] = d;
 }
 
 __global__ void __launch_bounds__(kBlock) k_roe_batch(int n, double gamma, const double* __restrict__ nrm,
@@ -512,73 +559,82 @@ __global__ void __launch_bounds__(kBlock) k_roe_batch(int n, double gamma, const
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-constexpr int kNormBlocks = 256;
-
+void launch_copy(double* dst, const double* src, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n, dst, src);
+}
 void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_gradient, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.nQ, s.grad);
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_gradient, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s);
 }
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_derivatives, dim3(nblocks(16 * m.nn)), dim3(kBlock), 0, st, m, s.nQ, s.hess, s.pv);
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_derivatives, dim3(nblocks(16 * m.nn)), dim3(kBlock), 0, st, m, s);
 }
-void launch_limiter(const DevMesh& m, const DevState& s, int recon, hipStream_t st) {
-  hipLaunchKernelGGL(k_limiter, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s.nQ, s.grad, s.hess, s.pv, s.phi,
-                     recon);
+void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_limiter, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s);
 }
-void launch_edge_flux(const DevMesh& m, const DevState& s, const PhysParams& p, int recon, int jupdate,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s.nQ, s.grad, s.phi, s.inc_flux,
-                     s.LHS, p, recon, jupdate);
+void launch_edge_flux(const DevMesh& m, const DevState& s, int jac_allowed, hipStream_t st) {
+  if (m.ne == 0) return;
+  hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s, jac_allowed);
 }
-void launch_bedge(const DevMesh& m, const DevState& s, const PhysParams& p, int jupdate, hipStream_t st) {
+void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nbe == 0) return;
-  hipLaunchKernelGGL(k_bedge, dim3(nblocks(m.nbe)), dim3(kBlock), 0, st, m, s.nQ, s.bflux, s.bjac, p, jupdate,
-                     s.err);
+  hipLaunchKernelGGL(k_bedge, dim3(nblocks(m.nbe)), dim3(kBlock), 0, st, m, s);
 }
-void launch_node(const DevMesh& m, const DevState& s, const PhysParams& p, int flags, hipStream_t st) {
-  hipLaunchKernelGGL(k_node, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s, p, flags);
+void launch_node(const DevMesh& m, const DevState& s, int flags, hipStream_t st) {
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_node, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s, flags);
 }
 void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st) {
   size_t n = (size_t)m.nnz * 16;
+  if (n == 0) return;
   hipLaunchKernelGGL(k_zero_offdiag, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, m, s.LHS);
 }
 // Color visiting order of `subits` symmetric sweeps (even sweeps ascend, odd
-// sweeps descend).  A color repeated back to back at a turnaround is dropped:
-// its rows depend only on other colors' dQ, so the repeat would reproduce the
-// same values bit for bit.
-int gs_schedule(int ncolor, int subits, int* out) {
+// sweeps descend).  A color repeated back to back at a turnaround is dropped
+// (the reference's row-sequential sweep has no such repeat); the backward
+// flag lets instances with fewer colors drop their own turnaround repeat.
+int gs_schedule(int ncolor, int subits, GsLaunch* out) {
   int n = 0, last = -1;
   for (int it = 0; it < subits; ++it)
     for (int cc = 0; cc < ncolor; ++cc) {
       int c = (it % 2 == 0) ? cc : ncolor - 1 - cc;
       if (c == last) continue;
-      out[n++] = c;
+      out[n].color = c;
+      out[n].backward = it % 2;
+      ++n;
       last = c;
     }
   return n;
 }
 
 void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag_lu, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.LHS, s.LU, s.dQ);
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_diag_lu, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.ip, s.LHS, s.LU, s.dQ);
 }
 
-void launch_gs_sweeps(const DevMesh& m, const DevState& s, const int* color_off, int subits, hipStream_t st) {
-  std::vector<int> sched((size_t)subits * m.ncolor + 1);
+void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st) {
+  std::vector<GsLaunch> sched((size_t)subits * m.ncolor + 1);
   int ns = gs_schedule(m.ncolor, subits, sched.data());
   for (int i = 0; i < ns; ++i) {
-    int c = sched[i];
-    int b = color_off[c], e = color_off[c + 1];
+    int c = sched[i].color;
+    int b = cr.off[c], e = sched[i].backward ? b + cr.nback[c] : cr.off[c + 1];
     if (e <= b) continue;
-    hipLaunchKernelGGL(k_gs_color, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+    hipLaunchKernelGGL(k_gs_color, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.ip, s.LHS, s.LU, s.RHS,
+                       s.dQ, b, e);
   }
 }
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st) {
-  int n4 = m.nn * 4;
-  hipLaunchKernelGGL(k_update, dim3(nblocks(n4)), dim3(kBlock), 0, st, n4, s.nQ, s.dQ);
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_update, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s.ip, s.nQ, s.dQ);
 }
-void launch_norms(const DevMesh& m, const DevState& s, double* d_out, hipStream_t st) {
-  int n4 = m.nn * 4;
-  hipLaunchKernelGGL(k_norm_partial, dim3(kNormBlocks), dim3(kBlock), 0, st, n4, s.R, s.dQ, s.partial);
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, st, kNormBlocks, s.partial, d_out);
+void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipStream_t st) {
+  if (c.nchunk > 0)
+    hipLaunchKernelGGL(k_norm_chunk, dim3(c.nchunk), dim3(kBlock), 0, st, c, s.ip, s.R, s.dQ, s.chunk_part);
+  hipLaunchKernelGGL(k_norm_inst, dim3(nblocks(m.ninst)), dim3(kBlock), 0, st, m.ninst, c, s.ip, s.chunk_part,
+                     s.inst_norms);
 }
 void launch_roe_batch(int n, double gamma, const double* normals, const double* qL, const double* qR, double* flux,
                       hipStream_t st) {

@@ -1,31 +1,53 @@
-// Device data layout and kernel launchers for one mesh instance.
+// Device data layout and kernel launchers.
+//
+// One launch serves a batch of independent mesh instances packed CSR-style:
+// every per-node / per-edge / per-subedge / per-bedge array is the
+// concatenation of the instances' arrays with their indices shifted by the
+// instance offsets, and every element looks up its instance's controls in a
+// small device table (InstParams).  A single mesh is a batch of one.
 //
 // HBM layout (all arrays 0-based, structure-of-arrays where a kernel streams
 // one field per face):
 //   per node       nQ/Q (4 x f64, interleaved rho,rhou,rhov,E), grad (8),
 //                  phi (1), hess (20), pv (4), R/RHS/dQ (4), cv_area, x (2)
-//   per edge       (L,R), its 1-2 subedges, the two off-diagonal block slots,
-//                  4 LSQ weights
-//   per subedge    (nx,ny,len), face point (2), Gauss points (6), flux (4)
+//   per ghost      nQ/Q after all real nodes: ghost of packed bedge b at nn+b
+//   per edge       (L,R), its 1-2 subedges, the two off-diagonal block slots
+//   per subedge    (nx,ny,len) and, at each endpoint's incidence slot, the
+//                  face point (2), Gauss points (6) and the signed flux (4)
 //   per bedge      real node, bc code, (nx,ny,len), flux (4), block (16)
 //   per node CSR   incident subedges / edges / bedges in ascending order
-//   block CSR      ia/ja/iau + transposed-slot map, LHS 16 x f64 per block
+//   block CSR      color-major: all rows of color c (of every instance) form
+//                  one contiguous run of rows and of blocks
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace fsk {
 
+// Per-instance controls of one Newton iteration, written by the host before
+// the launches and read by every element of the instance.
+struct InstParams {
+  double gamma, uinf, vinf, pinf, cfl;
+  int recon_flux;  // face states: 0 nodal, 1 linear limited
+  int grad;        // LSQ gradient this iteration
+  int k2;          // k=2 derivatives this iteration
+  int lim;         // 0: phi = 1, 1: Barth at face points, 3: Barth at Gauss points
+  int jac;         // Jacobian update this iteration
+  int active;      // instance takes part in this step
+  int pad[2];
+};
+
 struct DevMesh {
-  int nn, ng, ne, nsub, nbe, nnz, ncolor;
+  int nn, ng, ne, nsub, nbe, nnz, ncolor, ninst;
+  const int* inst_node;     // nn
+  const int* inst_edge;     // ne
+  const int* inst_bedge;    // nbe
+  const int* pos_inst;      // nn: instance of each color-order position
   const double* x;          // 2*nn
   const double* cv_area;    // nn
   const int* edges;         // 2*ne
   const int* edge_sub;      // 2*ne (second = -1 for a boundary edge)
   const int* edge_slot;     // 2*ne (slot of block (L,R), slot of block (R,L))
-  const double* edge_w;     // 4*ne
   const double* sub_n;      // 3*nsub
-  const double* sub_fc;     // 2*nsub
-  const double* sub_xg;     // 6*nsub
   // Node-incidence order: node n's incident subedges (ascending subedge
   // index) occupy slots nsub_off[n]..nsub_off[n+1]; its incident edges
   // (ascending edge index) occupy nedge_off[n]..nedge_off[n+1].
@@ -42,15 +64,15 @@ struct DevMesh {
   const int* bedge_real;    // nbe
   const int* bedge_bc;      // nbe
   const double* bedge_n;    // 3*nbe
-  // Block CSR in color-major device order: the rows of one color are one
-  // contiguous run of blocks.  Row n owns device slots rstart[n]..iau[n]
-  // (off-diagonals in the reference's nsn order, diagonal last).
+  // Block CSR in color-major device order.  Row n owns device slots
+  // rstart[n]..iau[n] (off-diagonals in the reference's nsn order, diagonal
+  // last).
   const int* rstart;        // nn
   const int* ja;            // nnz, device slot order
   const int* iau;           // nn
   const int* slot_t;        // nnz, device slot of the transposed block
   const int* color_rows;    // nn (rows grouped by color)
-  const int* st_off;        // nn+1 (k=2)
+  const int* st_off;        // nn+1 (k=2; empty stencils for k=1 instances)
   const int* st_nodes;
   const double* st_Wt;      // [entry][5] pseudo-inverse columns, weights folded
   const double* moments;    // 9*nn
@@ -73,46 +95,70 @@ struct DevState {
   double* inc_flux;  // 2*nsub*4: subedge flux at each endpoint's incidence slot, signed
   double* bflux;     // nbe*4
   double* bjac;      // nbe*16
-  double* partial;   // reduction scratch
-  int* err;          // device error flag
+  int* err;          // per-instance device error flags
   // mixed precision (FP32 Jacobian blocks and sweeps)
   float* LHS32;      // nnz*16, color-major slots
   float* LU32;       // nn*16, color position; reciprocal pivots on the diagonal
   float* RHS32;      // nn*4
   float* dQ32;       // nn*4
+  const InstParams* ip;   // ninst, controls of the current Newton iteration
+  double* inst_norms;     // 2 per instance: sum R^2, sum dQ^2
+  double* chunk_part;     // 2 per reduction chunk
 };
 
-struct PhysParams {
-  double gamma, uinf, vinf, pinf, cfl;
-};
+// Instance of element `idx` (a single-instance launch skips the lookup).
+__device__ __forceinline__ int inst_of(const DevMesh& m, const int* map, int idx) {
+  return m.ninst == 1 ? 0 : map[idx];
+}
 
 enum NodeFlags { NODE_EVCON = 1, NODE_INTERIOR = 2, NODE_BOUNDARY = 4, NODE_JAC = 8, NODE_RHS = 16 };
 
-// recon: 0 nodal, 1 linear limited (k=1), 3 quadratic (k=2 phase: nodal flux)
+// Deterministic per-instance norms: each instance's nodes are cut into
+// fixed-size chunks counted from the instance's first node, so the sums do
+// not depend on what else shares the launch.
+constexpr int kNormChunkNodes = 1024;
+struct NormChunks {
+  int nchunk;
+  const int* chunk_inst;   // nchunk
+  const int* chunk_begin;  // nchunk, packed node index
+  const int* chunk_end;    // nchunk
+  const int* inst_chunk0;  // ninst+1: first chunk of each instance
+};
+
+// One color launch of the symmetric sweeps.  Within a color, the rows of
+// instances whose last color it is are stored last; a backward launch skips
+// them, so every instance sees exactly the schedule of its own color count.
+struct GsLaunch {
+  int color;
+  int backward;
+};
+// color visiting order of `subits` symmetric sweeps over `ncolor` colors;
+// returns its length (out holds at least subits*ncolor entries)
+int gs_schedule(int ncolor, int subits, GsLaunch* out);
+struct ColorRanges {
+  const int* off;     // ncolor+1, host
+  const int* nback;   // ncolor, host: rows of a color visited by backward launches
+};
+
+// Per-instance modes come from DevState::ip.  `jac_allowed` = 0 suppresses the
+// FD Jacobian of the FP64 face pass (the mixed path's FP64 residual).
+void launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
 void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st);
-void launch_limiter(const DevMesh& m, const DevState& s, int recon, hipStream_t st);
-void launch_edge_flux(const DevMesh& m, const DevState& s, const PhysParams& p, int recon, int jupdate,
-                      hipStream_t st);
-void launch_bedge(const DevMesh& m, const DevState& s, const PhysParams& p, int jupdate, hipStream_t st);
-void launch_node(const DevMesh& m, const DevState& s, const PhysParams& p, int flags, hipStream_t st);
+void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st);
+void launch_edge_flux(const DevMesh& m, const DevState& s, int jac_allowed, hipStream_t st);
+void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st);
+void launch_node(const DevMesh& m, const DevState& s, int flags, hipStream_t st);
 void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st);
-// color visiting order of `subits` symmetric multicolor sweeps; returns its length
-int gs_schedule(int ncolor, int subits, int* out);
 void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st);
-void launch_gs_sweeps(const DevMesh& m, const DevState& s, const int* color_off_host, int subits, hipStream_t st);
-inline void launch_gs(const DevMesh& m, const DevState& s, const int* color_off_host, int subits, hipStream_t st) {
-  launch_diag_lu(m, s, st);
-  launch_gs_sweeps(m, s, color_off_host, subits, st);
-}
+void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st);
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st);
-// deterministic sums over nn*4 values: out[0]=sum R^2, out[1]=sum dQ^2 (device)
-void launch_norms(const DevMesh& m, const DevState& s, double* d_out, hipStream_t st);
+void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipStream_t st);
 // mixed precision
-void launch_edge_jac32(const DevMesh& m, const DevState& s, float gamma, int recon, int jupdate, hipStream_t st);
-void launch_node_mixed(const DevMesh& m, const DevState& s, const PhysParams& p, int jac, hipStream_t st);
+void launch_edge_jac32(const DevMesh& m, const DevState& s, hipStream_t st);
+void launch_node_mixed(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st);
-void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const int* color_off, int subits, hipStream_t st);
+void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st);
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_roe_batch(int n, double gamma, const double* normals, const double* qL, const double* qR, double* flux,
                       hipStream_t st);

@@ -83,12 +83,17 @@ __device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, c
 // dF/dqL = 0.5(A(qL) + |A_roe|) len and dF/dqR = 0.5(A(qR) - |A_roe|) len at
 // the (limited) reconstructed face states; block (L,R) += dF/dqR and block
 // (R,L) -= dF/dqL (jacobian.C:612-658).  Zeros when jupdate is off.
-__global__ void __launch_bounds__(kBlock) k_edge_jac32(DevMesh m, const double* __restrict__ nQ,
-                                                       const double* __restrict__ grad,
-                                                       const double* __restrict__ phi, float* __restrict__ LHS32,
-                                                       float gamma, int recon, int jupdate) {
+__global__ void __launch_bounds__(kBlock) k_edge_jac32(DevMesh m, DevState st) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
+  const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
+  if (!ip.active) return;
+  const int recon = ip.recon_flux, jupdate = ip.jac;
+  const float gamma = (float)ip.gamma;
+  const double* __restrict__ nQ = st.nQ;
+  const double* __restrict__ grad = st.grad;
+  const double* __restrict__ phi = st.phi;
+  float* __restrict__ LHS32 = st.LHS32;
   float offLR[16], offRL[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) { offLR[k] = 0.0f; offRL[k] = 0.0f; }
@@ -150,10 +155,14 @@ __global__ void __launch_bounds__(kBlock) k_edge_jac32(DevMesh m, const double* 
 // Node assembly of the mixed step: FP64 ev_con/dt and residual gather (same
 // order as the FP64 path), FP32 diagonal block V/dt I - sum of transposed
 // off-diagonals + boundary blocks, FP32 right-hand side -R.
-__global__ void __launch_bounds__(kBlock) k_node_mixed(DevMesh m, DevState s, PhysParams p, int jac) {
+__global__ void __launch_bounds__(kBlock) k_node_mixed(DevMesh m, DevState s) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
+  const int inst = inst_of(m, m.inst_node, n);
+  const InstParams& p = s.ip[inst];
+  if (!p.active) return;
+  const int jac = p.jac;
   double q[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) q[j] = s.nQ[4 * (size_t)n + j];
@@ -173,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) k_node_mixed(DevMesh m, DevState s, Ph
   double V = m.cv_area[n];
   double dtn = p.cfl * (V / ev);
   if (v == 0) {
-    if (!(dtn > 0.0) || !isfinite(dtn) || !isfinite(ev)) atomicOr(s.err, 2);
+    if (!(dtn > 0.0) || !isfinite(dtn) || !isfinite(ev)) atomicOr(s.err + inst, 2);
     s.ev_con[n] = ev;
     s.dt[n] = dtn;
   }
@@ -216,11 +225,13 @@ __device__ __forceinline__ void lu4f(float* A) {
   for (int k = 0; k < 4; k++) A[k * 4 + k] = 1.0f / A[k * 4 + k];
 }
 
-__global__ void __launch_bounds__(kBlock) k_diag_lu32(DevMesh m, const float* __restrict__ LHS32,
-                                                      float* __restrict__ LU32, float* __restrict__ dQ32) {
+__global__ void __launch_bounds__(kBlock) k_diag_lu32(DevMesh m, const InstParams* __restrict__ ip,
+                                                      const float* __restrict__ LHS32, float* __restrict__ LU32,
+                                                      float* __restrict__ dQ32) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m.nn) return;
   int n = m.color_rows[t];
+  if (!ip[inst_of(m, m.pos_inst, t)].active) return;
   float A[16];
   const float4* D = reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)m.iau[n]);
 #pragma unroll
@@ -238,15 +249,18 @@ __global__ void __launch_bounds__(kBlock) k_diag_lu32(DevMesh m, const float* __
 // FP32 multicolor sweep over rows [row_begin,row_end) of the color order;
 // 4 lanes per row (lane v: component v), quad-wide FP32 block rows; the row's
 // index, block and neighbour loads are issued before the arithmetic.
-__global__ void __launch_bounds__(kBlock) k_gs_color32(DevMesh m, const float* __restrict__ LHS32,
-                                                       const float* __restrict__ LU32, const float* __restrict__ RHS32,
-                                                       float* __restrict__ dQ32, int row_begin, int row_end) {
+__global__ void __launch_bounds__(kBlock) k_gs_color32(DevMesh m, const InstParams* __restrict__ ip,
+                                                       const float* __restrict__ LHS32, const float* __restrict__ LU32,
+                                                       const float* __restrict__ RHS32, float* __restrict__ dQ32,
+                                                       int row_begin, int row_end) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int t = row_begin + (tid >> 2), v = tid & 3;
   if (t >= row_end) return;
   int j = m.color_rows[t];
+  int inst = inst_of(m, m.pos_inst, t);
   float r = RHS32[4 * (size_t)j + v];
   int kb = m.rstart[j], kend = m.iau[j];
+  if (!ip[inst].active) return;
   constexpr int kPre = 8;
   int nb[kPre];
 #pragma unroll
@@ -281,40 +295,44 @@ __global__ void __launch_bounds__(kBlock) k_gs_color32(DevMesh m, const float* _
 }
 
 // nQ += dQ (FP64 state, FP32 correction) and the FP64 norms' inputs
-__global__ void __launch_bounds__(kBlock) k_update_mixed(int n4, double* __restrict__ nQ,
-                                                         const float* __restrict__ dQ32, double* __restrict__ dQ) {
+__global__ void __launch_bounds__(kBlock) k_update_mixed(DevMesh m, const InstParams* __restrict__ ip,
+                                                         double* __restrict__ nQ, const float* __restrict__ dQ32,
+                                                         double* __restrict__ dQ) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n4) return;
+  if (i >= 4 * m.nn) return;
+  if (!ip[inst_of(m, m.inst_node, i >> 2)].active) return;
   double d = (double)dQ32[i];
   dQ[i] = d;
   nQ[i] += d;
 }
 
-void launch_edge_jac32(const DevMesh& m, const DevState& s, float gamma, int recon, int jupdate, hipStream_t st) {
-  hipLaunchKernelGGL(k_edge_jac32, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s.nQ, s.grad, s.phi, s.LHS32, gamma,
-                     recon, jupdate);
+void launch_edge_jac32(const DevMesh& m, const DevState& s, hipStream_t st) {
+  if (m.ne == 0) return;
+  hipLaunchKernelGGL(k_edge_jac32, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s);
 }
-void launch_node_mixed(const DevMesh& m, const DevState& s, const PhysParams& p, int jac, hipStream_t st) {
-  hipLaunchKernelGGL(k_node_mixed, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s, p, jac);
+void launch_node_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_node_mixed, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s);
 }
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag_lu32, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.LHS32, s.LU32, s.dQ32);
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_diag_lu32, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.ip, s.LHS32, s.LU32, s.dQ32);
 }
 
-void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const int* color_off, int subits, hipStream_t st) {
-  std::vector<int> sched((size_t)subits * m.ncolor + 1);
+void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st) {
+  std::vector<GsLaunch> sched((size_t)subits * m.ncolor + 1);
   int ns = gs_schedule(m.ncolor, subits, sched.data());
   for (int i = 0; i < ns; ++i) {
-    int c = sched[i];
-    int b = color_off[c], e = color_off[c + 1];
+    int c = sched[i].color;
+    int b = cr.off[c], e = sched[i].backward ? b + cr.nback[c] : cr.off[c + 1];
     if (e <= b) continue;
-    hipLaunchKernelGGL(k_gs_color32, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.LHS32, s.LU32, s.RHS32,
-                       s.dQ32, b, e);
+    hipLaunchKernelGGL(k_gs_color32, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.ip, s.LHS32, s.LU32,
+                       s.RHS32, s.dQ32, b, e);
   }
 }
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {
-  int n4 = m.nn * 4;
-  hipLaunchKernelGGL(k_update_mixed, dim3(nblocks(n4)), dim3(kBlock), 0, st, n4, s.nQ, s.dQ32, s.dQ);
+  if (m.nn == 0) return;
+  hipLaunchKernelGGL(k_update_mixed, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s.ip, s.nQ, s.dQ32, s.dQ);
 }
 
 }  // namespace fsk

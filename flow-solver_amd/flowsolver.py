@@ -65,6 +65,14 @@ class MeshInfo(C.Structure):
                                        "nstencil", "prepared", "order_prepared")]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ninst", "nn", "ne", "nsub", "nbe", "nnz", "ncolor")] + [
+        ("device_bytes", C.c_int64)]
+
+
+INST_RUNNING, INST_CONVERGED, INST_MAX_STEPS, INST_FAILED, INST_PAUSED = range(5)
+
+
 class Timing(C.Structure):
     _fields_ = [("step_ms", C.c_double), ("edge_ms", C.c_double), ("node_ms", C.c_double), ("gs_ms", C.c_double),
                 ("lu_ms", C.c_double), ("other_ms", C.c_double), ("flux_evals", C.c_int64)]
@@ -112,6 +120,20 @@ def lib():
         "fs_roe_flux_batch": (i, [i, d, vp, vp, vp, vp, vp]),
         "fs_solve_steady": (i, [vp, i, d, C.POINTER(i), C.POINTER(d)]),
         "fs_solver_time_steps": (i, [vp, i, i, vp]),
+        "fs_solver_use_graph": (i, [vp, i]),
+        "fs_batch_create": (i, [i, C.POINTER(vp), vp, i, C.POINTER(vp)]),
+        "fs_batch_destroy": (None, [vp]),
+        "fs_batch_get_info": (i, [vp, vp]),
+        "fs_batch_init_freestream": (i, [vp]),
+        "fs_batch_use_graph": (i, [vp, i]),
+        "fs_batch_set_active": (i, [vp, i, i]),
+        "fs_batch_step": (i, [vp, vp, vp]),
+        "fs_batch_solve_steady": (i, [vp, i, d, vp, vp, vp]),
+        "fs_batch_inst_status": (i, [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(d)]),
+        "fs_batch_get": (i, [vp, i, i, vp, i64]),
+        "fs_batch_set": (i, [vp, i, i, vp, i64]),
+        "fs_batch_time_steps": (i, [vp, i, vp]),
+        "fs_mesh_footprint": (i, [vp, i, C.POINTER(i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -190,6 +212,12 @@ class Mesh:
         buf = C.cast(data, C.POINTER(ct * n)).contents
         return np.frombuffer(buf, dtype=np.int32 if dt.value == 0 else np.float64).copy()
 
+    def footprint(self, precision=FS_PREC_F64):
+        """Device bytes this (prepared) mesh costs inside a solver or batch."""
+        b = C.c_int64()
+        _check(lib().fs_mesh_footprint(self._h, precision, C.byref(b)))
+        return b.value
+
     def close(self):
         if self._h:
             lib().fs_mesh_destroy(self._h)
@@ -225,10 +253,7 @@ class Solver:
             pass
 
     def _count(self, field):
-        i = self.info
-        return {"Q": 4 * (i["nn"] + i["ng"]), "nQ": 4 * (i["nn"] + i["ng"]), "dQ": 4 * i["nn"], "R": 4 * i["nn"],
-                "RHS": 4 * i["nn"], "grad": 8 * i["nn"], "phi": i["nn"], "hess": 20 * i["nn"], "pv": 4 * i["nn"],
-                "ev_con": i["nn"], "dt": i["nn"], "LHS": 16 * i["nnz"], "LHS32": 16 * i["nnz"]}[field]
+        return _field_count(self.info, field)
 
     def get(self, field):
         n = self._count(field)
@@ -280,4 +305,94 @@ class Solver:
     def time_steps(self, ts_first, nsteps):
         t = Timing()
         _check(lib().fs_solver_time_steps(self._h, ts_first, nsteps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def use_graph(self, on=True):
+        """Run each pseudo-time step as one captured hipGraph launch."""
+        _check(lib().fs_solver_use_graph(self._h, 1 if on else 0))
+
+
+def _field_count(info, field):
+    return {"Q": 4 * (info["nn"] + info["ng"]), "nQ": 4 * (info["nn"] + info["ng"]), "dQ": 4 * info["nn"],
+            "R": 4 * info["nn"], "RHS": 4 * info["nn"], "grad": 8 * info["nn"], "phi": info["nn"],
+            "hess": 20 * info["nn"], "pv": 4 * info["nn"], "ev_con": info["nn"], "dt": info["nn"],
+            "LHS": 16 * info["nnz"], "LHS32": 16 * info["nnz"]}[field]
+
+
+class Batch:
+    """Many independent steady solves packed into one device image: every
+    kernel of a pseudo-time step is one launch over all instances."""
+
+    def __init__(self, meshes, params, precision=FS_PREC_F64):
+        if isinstance(params, Params):
+            params = [params] * len(meshes)
+        if len(params) != len(meshes):
+            raise ValueError("one parameter set per mesh")
+        self.meshes = list(meshes)          # kept alive: the batch references them
+        self.params = list(params)
+        self.infos = [m.info() for m in meshes]
+        n = len(meshes)
+        hs = (C.c_void_p * n)(*[m._h.value for m in meshes])
+        ps = (Params * n)(*params)
+        h = C.c_void_p()
+        _check(lib().fs_batch_create(n, C.cast(hs, C.POINTER(C.c_void_p)), ps, precision, C.byref(h)))
+        self._h = h
+        self.n = n
+
+    def close(self):
+        if self._h:
+            lib().fs_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        bi = BatchInfo()
+        _check(lib().fs_batch_get_info(self._h, C.byref(bi)))
+        return {k: getattr(bi, k) for k, _ in bi._fields_}
+
+    def init_freestream(self):
+        _check(lib().fs_batch_init_freestream(self._h))
+
+    def use_graph(self, on=True):
+        _check(lib().fs_batch_use_graph(self._h, 1 if on else 0))
+
+    def set_active(self, i, on):
+        _check(lib().fs_batch_set_active(self._h, i, 1 if on else 0))
+
+    def step(self):
+        r2, dq = np.zeros(self.n), np.zeros(self.n)
+        _check(lib().fs_batch_step(self._h, r2.ctypes.data, dq.ctypes.data))
+        return r2, dq
+
+    def solve_steady(self, max_steps, tol):
+        steps = np.zeros(self.n, dtype=np.int32)
+        drop = np.zeros(self.n)
+        status = np.zeros(self.n, dtype=np.int32)
+        _check(lib().fs_batch_solve_steady(self._h, max_steps, tol, steps.ctypes.data, drop.ctypes.data,
+                                           status.ctypes.data))
+        return steps, drop, status
+
+    def status(self, i):
+        st, steps, drop = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().fs_batch_inst_status(self._h, i, C.byref(st), C.byref(steps), C.byref(drop)))
+        return st.value, steps.value, drop.value
+
+    def get(self, i, field):
+        n = _field_count(self.infos[i], field)
+        out = np.empty(n, dtype=np.float64)
+        _check(lib().fs_batch_get(self._h, i, _FIELDS.index(field), out.ctypes.data, n))
+        return out
+
+    def set(self, i, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        _check(lib().fs_batch_set(self._h, i, _FIELDS.index(field), v.ctypes.data, v.size))
+
+    def time_steps(self, nsteps):
+        t = Timing()
+        _check(lib().fs_batch_time_steps(self._h, nsteps, C.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}

@@ -171,6 +171,53 @@ typedef struct {
 } fs_timing;
 int fs_solver_time_steps(fs_solver* s, int ts_first, int nsteps, fs_timing* t);
 
+/* Run fs_solve's pseudo-time step as one captured hipGraph launch (on != 0)
+ * or as individually enqueued kernels (0, the default).  No counterpart in
+ * the reference (its step is a sequence of CPU loops). */
+int fs_solver_use_graph(fs_solver* s, int on);
+
+/* ---- batches: many independent steady solves (solve.C:44 each) packed
+ *      CSR-style into one device image; every kernel of a pseudo-time step
+ *      is one launch over all instances, optionally one hipGraph launch per
+ *      step.  Instances keep their own mesh, PARAMS and step counter
+ *      (grid->citer); they share subits, iter and jufreq.  An instance's
+ *      results do not depend on the other instances of the batch.  No
+ *      counterpart in the reference, which solves one mesh per process. ---- */
+typedef struct fs_batch fs_batch;
+typedef enum {
+  FS_INST_RUNNING = 0,
+  FS_INST_CONVERGED = 1,   /* residual drop reached the tolerance */
+  FS_INST_MAX_STEPS = 2,   /* step limit reached first */
+  FS_INST_FAILED = 3,      /* non-physical state (fs_batch_inst_error) */
+  FS_INST_PAUSED = 4       /* masked off by fs_batch_set_active */
+} fs_inst_status;
+typedef struct {
+  int ninst, nn, ne, nsub, nbe, nnz, ncolor;
+  int64_t device_bytes;
+} fs_batch_info;
+/* meshes prepared (and kept alive by the caller until fs_batch_destroy) */
+int fs_batch_create(int n, const fs_mesh* const* meshes, const fs_params* params, int precision, fs_batch** out);
+void fs_batch_destroy(fs_batch* b);
+int fs_batch_get_info(const fs_batch* b, fs_batch_info* info);
+int fs_batch_init_freestream(fs_batch* b);
+int fs_batch_use_graph(fs_batch* b, int on);
+/* convergence mask: a masked-off instance is skipped by every kernel */
+int fs_batch_set_active(fs_batch* b, int i, int active);
+/* one pseudo-time step of every active instance (each step counter advances);
+ * r2[i] = sum R^2, dq[i] = ||dQ||_2 of instance i (either may be NULL) */
+int fs_batch_step(fs_batch* b, double* r2, double* dq);
+/* steady solves: each instance stops on its own when ||R||/||R0|| <= tol (R0
+ * as fs_solve_steady) and is masked off; runs until every instance stopped
+ * or max_steps steps.  steps/drop/status: one entry per instance (NULL ok). */
+int fs_batch_solve_steady(fs_batch* b, int max_steps, double tol, int* steps, double* drop, int* status);
+int fs_batch_inst_status(const fs_batch* b, int i, int* status, int* steps, double* drop);
+int fs_batch_get(fs_batch* b, int i, int field, double* host, int64_t n);
+int fs_batch_set(fs_batch* b, int i, int field, const double* host, int64_t n);
+/* nsteps timed pseudo-time steps of the active instances (step counters advance) */
+int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t);
+/* device bytes a mesh costs inside a batch/solver of the given precision */
+int fs_mesh_footprint(const fs_mesh* m, int precision, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,96 @@
+"""C-ABI checks that need no GPU: every entry point declared in
+include/fs_capi.h is exported by the built library, the device entry points
+fail with a status (not a crash, not a CPU fallback) when no HIP device is
+present, and the batch's color-launch schedule gives every instance exactly
+the sweep order it has when solved alone."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import fsutil as U
+
+fsm = U.fsm
+HEADER = os.path.join(U.ROOT, "include", "fs_capi.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(fs_[a-z0-9_]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_symbols_exported():
+    names = declared_functions()
+    assert len(names) >= 40, names
+    lib = C.CDLL(fsm.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def _no_device():
+    return fsm.device_count() == 0
+
+
+@pytest.mark.skipif(not _no_device(), reason="a HIP device is present")
+def test_device_entry_points_fail_without_device():
+    p = fsm.Params.make(order=1, subits=4)
+    m = fsm.Mesh.bump(9, 5, 0.2, 1).prepare(p)
+    with pytest.raises(fsm.FlowSolverError) as e:
+        fsm.Solver(m, p)
+    assert e.value.code == -4
+    with pytest.raises(fsm.FlowSolverError) as e:
+        fsm.Batch([m, m], p)
+    assert e.value.code == -4
+
+
+def test_footprint_grows_with_mesh():
+    p = fsm.Params.make(order=3, subits=4)
+    small = fsm.Mesh.bump(17, 9, 0.2, 1).prepare(p)
+    big = fsm.Mesh.bump(33, 17, 0.2, 1).prepare(p)
+    fs, fb = small.footprint(fsm.FS_PREC_F64), big.footprint(fsm.FS_PREC_F64)
+    assert 0 < fs < fb
+    # FP32 blocks halve the largest array
+    assert big.footprint(fsm.FS_PREC_MIXED) < fb
+    nn, nnz = big.info()["nn"], big.info()["nnz"]
+    assert fb >= nnz * 128 + nn * 128
+
+
+def _schedule(ncolor, subits):
+    """(color, backward) launches of the device sweep order."""
+    out, last = [], -1
+    for it in range(subits):
+        for cc in range(ncolor):
+            c = cc if it % 2 == 0 else ncolor - 1 - cc
+            if c != last:
+                out.append((c, it % 2))
+                last = c
+    return out
+
+
+def _visits(ncolor_inst, ncolor_max, subits):
+    """Colors instance sees inside a batch: empty colors are no-ops and a
+    backward launch of the instance's last color skips it."""
+    seen = []
+    for c, bwd in _schedule(ncolor_max, subits):
+        if c >= ncolor_inst or (bwd and c == ncolor_inst - 1):
+            continue
+        seen.append(c)
+    return seen
+
+
+@pytest.mark.parametrize("ncmax", [1, 2, 3, 4, 5, 7])
+def test_batch_schedule_equals_alone(ncmax):
+    for nci in range(1, ncmax + 1):
+        for subits in range(0, 13):
+            alone = [c for c, _ in _schedule(nci, subits)]
+            assert _visits(nci, ncmax, subits) == alone, (nci, ncmax, subits)
+
+
+def test_schedule_drops_turnaround_repeat():
+    # every sweep after the first starts at the color the previous one ended
+    # on; that launch is dropped
+    s = _schedule(4, 3)
+    assert [c for c, _ in s] == [0, 1, 2, 3, 2, 1, 0, 1, 2, 3]

@@ -1,0 +1,121 @@
+"""Batched solves on the GPU: instances of different meshes, Mach numbers,
+angles, reconstruction orders and limiter settings packed into one launch per
+kernel.  An instance's arithmetic does not depend on its neighbours, so every
+result must equal the same solve run alone bit for bit (a stronger bar than
+the 1e-10 tolerance the serving design asks for).  Also: graph replay equals
+eager launches, and a masked-off instance is left untouched."""
+import numpy as np
+import pytest
+
+import fsutil as U
+
+fsm = U.fsm
+pytestmark = pytest.mark.gpu
+
+
+def _params(**kw):
+    base = dict(nsteps=0, iter=1, jufreq=1, CFL=100.0, ramp=30, gamma=1.4, alpha=0.0, mach=0.5, subits=12,
+                method=1, order=1, foits=5, soits=0, limit=0, reorder=1)
+    base.update(kw)
+    return fsm.Params.make(**base)
+
+
+def _cases():
+    """(mesh factory, params): uneven sizes, both orders, both geometries."""
+    return [
+        (lambda: fsm.Mesh.bump(33, 17, 0.2, 1), _params(mach=0.5, order=1)),
+        (lambda: fsm.Mesh.bump(41, 13, 0.2, 2), _params(mach=0.6, alpha=1.0, order=3, soits=4, limit=1)),
+        (lambda: fsm.Mesh.airfoil(48, 10, 2e-2, 8.0, 0.05, 3), _params(mach=0.4, alpha=2.0, order=1)),
+        (lambda: fsm.Mesh.bump(17, 9, 0.1, 4), _params(mach=0.3, alpha=-1.0, order=3, soits=2, limit=0)),
+        (lambda: fsm.Mesh.bump(25, 11, 0.2, 5), _params(mach=0.55, order=1, limit=1, foits=3)),
+        # 3 colors where the others have 4: its backward launches of color 2
+        # must be skipped for it to see its own sweep order
+        (lambda: fsm.Mesh.bump(3, 3, 0.0, 1), _params(mach=0.4, order=1)),
+    ]
+
+
+def _build(precision):
+    cases = _cases()
+    meshes = []
+    for make, p in cases:
+        m = make()
+        m.prepare(p)
+        meshes.append(m)
+    return meshes, [p for _, p in cases]
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED])
+def test_batch_equals_alone(precision):
+    meshes, params = _build(precision)
+    print("case shapes", [(m.info()["nn"], m.info()["ncolor"]) for m in meshes], flush=True)
+    assert len({m.info()["ncolor"] for m in meshes}) > 1
+    b = fsm.Batch(meshes, params, precision)
+    b.init_freestream()
+    steps, drop, status = b.solve_steady(120, 1e-6)
+    print("batch steps", steps, "drop", drop, "status", status, flush=True)
+    for i, (m, p) in enumerate(zip(meshes, params)):
+        s = fsm.Solver(m, p, precision)
+        s.init_freestream()
+        st, dr = s.solve_steady(120, 1e-6)
+        same = np.array_equal(s.get("Q"), b.get(i, "Q"))
+        print("instance %d alone steps %d drop %.3e  batch steps %d drop %.3e  bit-identical %s" %
+              (i, st, dr, steps[i], drop[i], same), flush=True)
+        assert st == steps[i]
+        assert dr == drop[i]
+        assert same
+        assert np.array_equal(s.get("R"), b.get(i, "R"))
+        s.close()
+    # the convergence mask removed instances at different steps
+    assert status.tolist().count(fsm.INST_CONVERGED) >= 1
+    assert len(set(steps.tolist())) > 1
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED])
+def test_graph_replay_equals_eager(precision):
+    p = _params(order=3, soits=2, limit=1, foits=2)
+    m = fsm.Mesh.bump(33, 17, 0.2, 7)
+    m.prepare(p)
+    eager = fsm.Solver(m, p, precision)
+    graph = fsm.Solver(m, p, precision)
+    graph.use_graph(True)
+    eager.init_freestream()
+    graph.init_freestream()
+    # phases change at ts 3 and 5: three graphs are captured and each is
+    # replayed several times, with eager work and synchronisation in between
+    for ts in range(1, 12):
+        de, re = eager.solve(ts)
+        dg, rg = graph.solve(ts)
+        qe, qg = eager.get("Q"), graph.get("Q")
+        print("ts %d eager |dQ| %.15e graph %.15e identical %s" % (ts, de, dg, np.array_equal(qe, qg)), flush=True)
+        assert np.array_equal(qe, qg)
+        assert de == dg and re == rg
+
+
+def test_mask_freezes_instance():
+    meshes, params = _build(fsm.FS_PREC_F64)
+    meshes, params = meshes[:2], params[:2]
+    b = fsm.Batch(meshes, params, fsm.FS_PREC_F64)
+    b.use_graph(True)
+    b.init_freestream()
+    for _ in range(3):
+        b.step()
+    q1 = b.get(1, "Q")
+    b.set_active(1, False)
+    for _ in range(3):
+        b.step()
+    assert np.array_equal(b.get(1, "Q"), q1)
+    assert b.status(1)[0] == fsm.INST_PAUSED
+    # instance 0 kept stepping: equal to 6 steps alone
+    s = fsm.Solver(meshes[0], params[0], fsm.FS_PREC_F64)
+    s.init_freestream()
+    for ts in range(1, 7):
+        s.solve(ts)
+    assert np.array_equal(s.get("Q"), b.get(0, "Q"))
+    # resuming continues from the frozen state and step counter
+    b.set_active(1, True)
+    b.step()
+    s1 = fsm.Solver(meshes[1], params[1], fsm.FS_PREC_F64)
+    s1.init_freestream()
+    for ts in range(1, 5):
+        s1.solve(ts)
+    assert np.array_equal(s1.get("Q"), b.get(1, "Q"))
