@@ -276,7 +276,10 @@ int fs_derivatives(fs_solver* s) {
 int fs_limiter(fs_solver* s, int ts) {
   if (!s) return fail(FS_ERR_ARG, "null solver");
   fsk::InstParams p = mirror_params(s, ts);
-  p.lim = fse::phase_for(s->eng.inst[0].prm, ts).k2 ? 3 : 1;
+  // the mirror of compute_limiter_barth / compute_limiter_venkatakrishnan
+  // (limit 2) at the points of the current phase
+  p.lim = fsk::LIM_ON | (fse::phase_for(s->eng.inst[0].prm, ts).k2 ? fsk::LIM_GAUSS : 0) |
+          (s->eng.inst[0].prm.limit == 2 ? fsk::LIM_VENKAT : 0);
   if (int rc = s->eng.upload_params_custom(p)) return rc;
   fsk::launch_limiter(s->eng.dm, s->eng.ds, s->eng.stream);
   return finish(s, false);

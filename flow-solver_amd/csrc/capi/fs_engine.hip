@@ -40,11 +40,14 @@ double cfl_for(const fs_params& p, int citer) {
 // first order; the k=2 derivatives feed only the Gauss-point limiter).
 Phase phase_for(const fs_params& p, int ts) {
   Phase ph{0, false, false, 0};
+  // solve.C:440-471: the limiter runs in the phase of the final order only;
+  // limit 1 is Barth, 2 Venkatakrishnan
+  const int kind = p.limit == 2 ? fsk::LIM_VENKAT : 0;
   if (p.order == 1) {
     if (ts > p.foits) {
       ph.recon_flux = 1;
       ph.gradient = true;
-      ph.lim = p.limit > 0 ? 1 : 0;
+      ph.lim = p.limit > 0 ? (fsk::LIM_ON | kind) : 0;
     }
   } else if (p.order == 3) {
     if (ts > p.foits && ts <= p.foits + p.soits) {
@@ -52,7 +55,7 @@ Phase phase_for(const fs_params& p, int ts) {
       ph.gradient = true;
     } else if (ts > p.foits + p.soits) {
       ph.k2 = true;
-      ph.lim = p.limit > 0 ? 3 : 0;
+      ph.lim = p.limit > 0 ? (fsk::LIM_ON | fsk::LIM_GAUSS | kind) : 0;
     }
   }
   return ph;
@@ -60,7 +63,7 @@ Phase phase_for(const fs_params& p, int ts) {
 
 bool supported(const fs_params& p, std::string* why) {
   if (p.order != 1 && p.order != 3) { *why = "order must be 1 (k=1) or 3 (k=2)"; return false; }
-  if (p.limit != 0 && p.limit != 1) { *why = "limit must be 0 or 1 (Barth)"; return false; }
+  if (p.limit < 0 || p.limit > 2) { *why = "limit must be 0, 1 (Barth) or 2 (Venkatakrishnan)"; return false; }
   if (p.method != 1 && p.method != 2) { *why = "method must be implicit (1 ramped CFL or 2 fixed CFL)"; return false; }
   if (p.unsteady != 0 || p.tacc != 0) { *why = "unsteady time accuracy is not on this path"; return false; }
   if (p.iter < 1 || p.jufreq < 1 || p.subits < 0) { *why = "iter/jufreq must be >= 1, subits >= 0"; return false; }

@@ -34,7 +34,7 @@ struct Phase {
   int recon_flux;  // 0 nodal face states, 1 linear limited
   bool gradient;
   bool k2;
-  int lim;         // 0 none (phi = 1), 1 Barth at face points, 3 Barth at Gauss points
+  int lim;         // fsk::LimMode bits, 0 none (phi = 1)
 };
 Phase phase_for(const fs_params& p, int ts);
 double cfl_for(const fs_params& p, int citer);   // init_memory.C:200-216

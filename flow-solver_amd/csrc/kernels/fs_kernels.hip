@@ -128,31 +128,46 @@ __global__ void __launch_bounds__(kBlock) k_derivatives(DevMesh m, DevState s) {
 // incident face point (k=1) or the 3 Gauss points (k=2 phase); one scalar
 // per node (min over the 4 variables).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void barth_candidate1(double q, double qmin, double qmax, double qf, double* ph) {
+// One face-point candidate: r = (Qmax - q)/(qf - q) above, (Qmin - q)/(qf - q)
+// below; Barth takes min(1, r) (limiter.C:154-246), Venkatakrishnan
+// (r^2 + 2r)/(r^2 + r + 2) (limiter.C:696-730).
+template <bool kVenkat>
+__device__ __forceinline__ void limiter_candidate(double q, double qmin, double qmax, double qf, double* ph) {
   double d = qf - q, t;
   if (d > 0.) {
-    t = dmin(1.0, (qmax - q) / (qf - q));
+    t = (qmax - q) / (qf - q);
+    t = kVenkat ? (t * t + 2. * t) / (t * t + t + 2.) : dmin(1.0, t);
     *ph = dmin(*ph, t);
   } else if (d < 0.) {
-    t = dmin(1.0, (qmin - q) / (qf - q));
+    t = (qmin - q) / (qf - q);
+    t = kVenkat ? (t * t + 2. * t) / (t * t + t + 2.) : dmin(1.0, t);
     *ph = dmin(*ph, t);
   } else {
     *ph = dmin(*ph, 1.0);
   }
 }
 
+__device__ __forceinline__ void candidate(bool venkat, double q, double qmin, double qmax, double qf, double* ph) {
+  if (venkat)
+    limiter_candidate<true>(q, qmin, qmax, qf, ph);
+  else
+    limiter_candidate<false>(q, qmin, qmax, qf, ph);
+}
+
 // 4 lanes per node, lane v limits variable v; the node scalar is the min over
 // the 4 lanes taken in variable order (limiter.C:410-420).  An instance
-// without a limiter this iteration gets phi = 1 (solve.C:343-347).
+// without a limiter this iteration gets phi = 1 (solve.C:343-347).  Mode bits
+// (InstParams::lim): 1 limiter on, 2 Gauss points of the k=2 phase,
+// 4 Venkatakrishnan's function instead of Barth's.
 __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, DevState s) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
   if (!ip.active) return;
-  const int recon = ip.lim;
+  const int lim = ip.lim;
   double* __restrict__ phi = s.phi;
-  if (recon == 0) {
+  if (!(lim & LIM_ON)) {
     if (v == 0) phi[n] = 1.0;
     return;
   }
@@ -169,7 +184,8 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, DevState s) {
     qmax = dmax(qmax, o);
   }
   double xn0 = m.x[2 * (size_t)n], xn1 = m.x[2 * (size_t)n + 1];
-  if (recon == 3) {
+  const bool venkat = lim & LIM_VENKAT;
+  if (lim & LIM_GAUSS) {
     const double* h = hess + 20 * (size_t)n + 5 * v;
     double h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4];
     double p0 = pv[4 * (size_t)n + v];
@@ -179,22 +195,19 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, DevState s) {
         double dx1 = m.inc_xg[6 * (size_t)k + 2 * g + 1] - xn1;
         double qf = p0;
         qf += (h0 * dx0 + h1 * dx1 + h2 * dx0 * dx0 * 0.5 + h3 * dx1 * dx1 * 0.5 + h4 * dx0 * dx1);
-        barth_candidate1(q, qmin, qmax, qf, &ph);
+        candidate(venkat, q, qmin, qmax, qf, &ph);
       }
     }
   } else {
-    double g0 = 0.0, g1 = 0.0;
-    if (recon == 1) {
-      g0 = grad[8 * (size_t)n + 2 * v];
-      g1 = grad[8 * (size_t)n + 2 * v + 1];
-    }
+    // face points of the linear reconstruction (the limiter runs only in
+    // the gradient phase of k=1)
+    double g0 = grad[8 * (size_t)n + 2 * v];
+    double g1 = grad[8 * (size_t)n + 2 * v + 1];
     for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
       double qf = q;
-      if (recon == 1) {
-        double dx0 = m.inc_fc[2 * (size_t)k] - xn0, dx1 = m.inc_fc[2 * (size_t)k + 1] - xn1;
-        qf += (g0 * dx0 + g1 * dx1);
-      }
-      barth_candidate1(q, qmin, qmax, qf, &ph);
+      double dx0 = m.inc_fc[2 * (size_t)k] - xn0, dx1 = m.inc_fc[2 * (size_t)k + 1] - xn1;
+      qf += (g0 * dx0 + g1 * dx1);
+      candidate(venkat, q, qmin, qmax, qf, &ph);
     }
   }
   double p0 = __shfl(ph, 0, 4), p1 = __shfl(ph, 1, 4), p2 = __shfl(ph, 2, 4), p3 = __shfl(ph, 3, 4);

@@ -30,11 +30,13 @@ struct InstParams {
   int recon_flux;  // face states: 0 nodal, 1 linear limited
   int grad;        // LSQ gradient this iteration
   int k2;          // k=2 derivatives this iteration
-  int lim;         // 0: phi = 1, 1: Barth at face points, 3: Barth at Gauss points
+  int lim;         // LimMode bits; 0: phi = 1
   int jac;         // Jacobian update this iteration
   int active;      // instance takes part in this step
   int pad[2];
 };
+
+enum LimMode { LIM_ON = 1, LIM_GAUSS = 2, LIM_VENKAT = 4 };
 
 struct DevMesh {
   int nn, ng, ne, nsub, nbe, nnz, ncolor, ninst;

@@ -142,7 +142,9 @@ int fs_solver_get(fs_solver* s, int field, double* host, int64_t n);
 int fs_build_matrix(fs_solver* s, int ts);                      /* init_memory.C:159 build_matrix */
 int fs_gradient(fs_solver* s);                                  /* gradient.C:245 Compute_Gradient_LeastSquares */
 int fs_derivatives(fs_solver* s);                               /* compute_derivatives.C:40 Compute_Derivatives(.,3) */
-int fs_limiter(fs_solver* s, int ts);                           /* limiter.C:39 compute_limiter_barth */
+/* limiter.C:39 compute_limiter_barth, or limiter.C:511 compute_limiter_venkatakrishnan
+ * when the solver's PARAMS has limit = 2 */
+int fs_limiter(fs_solver* s, int ts);
 int fs_interior_residual(fs_solver* s, int jupdate, int ts);    /* residual.C:40 build_interior_residuals */
 int fs_boundary_residual(fs_solver* s, int jupdate);            /* residual.C:359 build_boundary_residuals */
 int fs_linear_solve(fs_solver* s, double* rms);                 /* linear_algebra.C:122 linear_solve */

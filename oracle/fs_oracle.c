@@ -324,17 +324,21 @@ static void recon_quad_ul(const fso_state* s, const double* x, int node, const d
   }
 }
 
-/* one Barth candidate update for one side (limiter.C:154-246) */
-static void barth_side(const fso_state* s, const double* qmin, const double* qmax, int node, const double* qf,
-                       double* phi) {
+/* one candidate update for one side: Barth min(1, r) (limiter.C:154-246) or
+ * Venkatakrishnan's (r^2 + 2r) / (r^2 + r + 2) (limiter.C:696-730), with
+ * r = (Qmax - q) / (qf - q) above and (Qmin - q) / (qf - q) below */
+static void limiter_side(const fso_state* s, const double* qmin, const double* qmax, int node, const double* qf,
+                         double* phi, int venkat) {
   int j;
   for (j = 0; j < NV; j++) {
     double q = s->nQ[node * 4 + j], d = qf[j] - q, temp;
     if (d > 0.) {
-      temp = OMIN(1.0, (qmax[node * 4 + j] - q) / (qf[j] - q));
+      temp = (qmax[node * 4 + j] - q) / (qf[j] - q);
+      temp = venkat ? (temp * temp + 2. * temp) / (temp * temp + temp + 2.) : OMIN(1.0, temp);
       phi[node * 8 + j * 2] = OMIN(phi[node * 8 + j * 2], temp);
     } else if (d < 0.) {
-      temp = OMIN(1.0, (qmin[node * 4 + j] - q) / (qf[j] - q));
+      temp = (qmin[node * 4 + j] - q) / (qf[j] - q);
+      temp = venkat ? (temp * temp + 2. * temp) / (temp * temp + temp + 2.) : OMIN(1.0, temp);
       phi[node * 8 + j * 2] = OMIN(phi[node * 8 + j * 2], temp);
     } else {
       phi[node * 8 + j * 2] = OMIN(phi[node * 8 + j * 2], 1.0);
@@ -342,8 +346,9 @@ static void barth_side(const fso_state* s, const double* qmin, const double* qma
   }
 }
 
-/* limiter.C:39-497 compute_limiter_barth */
-void fso_limiter_barth(const fso_mesh* m, const fso_params* p, fso_state* s) {
+/* limiter.C:39-497 compute_limiter_barth (venkat = 0) and limiter.C:511-983
+ * compute_limiter_venkatakrishnan (venkat = 1): same stencil, same points */
+static void limiter(const fso_mesh* m, const fso_params* p, fso_state* s, int venkat) {
   int i, j, is;
   double* qmin = (double*)malloc((size_t)m->nn * 4 * sizeof(double));
   double* qmax = (double*)malloc((size_t)m->nn * 4 * sizeof(double));
@@ -376,8 +381,8 @@ void fso_limiter_barth(const fso_mesh* m, const fso_params* p, fso_state* s) {
         const double* gp = &m->sub_xg[6 * is + 2 * k];
         recon_quad_ul(s, m->x, L, gp, ql);
         recon_quad_ul(s, m->x, R, gp, qr);
-        barth_side(s, qmin, qmax, L, ql, phi);
-        barth_side(s, qmin, qmax, R, qr, phi);
+        limiter_side(s, qmin, qmax, L, ql, phi, venkat);
+        limiter_side(s, qmin, qmax, R, qr, phi, venkat);
       }
     } else {
       /* Reconstruct_UL, linear (reconstruct.C:600-616) */
@@ -393,8 +398,8 @@ void fso_limiter_barth(const fso_mesh* m, const fso_params* p, fso_state* s) {
           qr[j] += (s->grad[R * 8 + 2 * j] * dxr0 + s->grad[R * 8 + 2 * j + 1] * dxr1);
         }
       }
-      barth_side(s, qmin, qmax, L, ql, phi);
-      barth_side(s, qmin, qmax, R, qr, phi);
+      limiter_side(s, qmin, qmax, L, ql, phi, venkat);
+      limiter_side(s, qmin, qmax, R, qr, phi, venkat);
     }
   }
   for (i = 0; i < m->nn; i++)
@@ -408,6 +413,9 @@ void fso_limiter_barth(const fso_mesh* m, const fso_params* p, fso_state* s) {
   free(qmin);
   free(qmax);
 }
+
+void fso_limiter_barth(const fso_mesh* m, const fso_params* p, fso_state* s) { limiter(m, p, s, 0); }
+void fso_limiter_venkat(const fso_mesh* m, const fso_params* p, fso_state* s) { limiter(m, p, s, 1); }
 
 /* jacobian.C:45-185 forward-difference Jacobian around the face states */
 static void numerical_jacobian(const fso_mesh* m, const fso_params* p, fso_state* s, int L, int R, const double* QL,
@@ -754,6 +762,7 @@ double fso_solve(const fso_mesh* m, const fso_params* p, fso_state* s, int ts, i
     if (p->order == 1 && p->limit > 0 && ts > p->foits) do_lim = 1;
     if (p->order == 3 && p->limit > 0 && ts > p->foits + p->soits) do_lim = 1;
     if (do_lim && p->limit == 1) fso_limiter_barth(m, p, s);
+    if (do_lim && p->limit == 2) fso_limiter_venkat(m, p, s);
     fso_interior_residual(m, p, s, jac, ts);
     if (fso_boundary_residual(m, p, s, jac) != 0) return -1.0;
     for (i = 0; i < m->nn * 4; i++) rms += s->R[i] * s->R[i];

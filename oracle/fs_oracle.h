@@ -17,6 +17,7 @@
  *   fso_build_matrix        reference build step histories (ev_con, dt, LHS diag)
  *   fso_gradient            reference build; exact for linear fields
  *   fso_limiter_barth       reference build (phi per step)
+ *   fso_limiter_venkat      reference build (phi per step, k=1 face points)
  *   fso_interior_residual   reference build (R, LHS per step)
  *   fso_boundary_residual   reference build (R, LHS, ghost states per step)
  *   fso_linear_solve        reference build (dQ per step); dense solve on tiny systems
@@ -29,7 +30,7 @@
  *                           cannot run the k=2 factorization: LAPACK is absent)
  *   fso_derivatives         quadratic exactness of hess/point_value (tests);
  *                           no reference-build vector exists for the k=2 phase
- *   fso_limiter_barth (k=2 Gauss points), k=2 phase of fso_solve:
+ *   fso_limiter_barth/_venkat (k=2 Gauss points), k=2 phase of fso_solve:
  *                           parity unpinned against the reference build (it cannot
  *                           enter the k=2 phase here); pinned by the k=2 invariants
  */
@@ -105,6 +106,7 @@ void fso_gradient(const fso_mesh* m, fso_state* s);
 int fso_build_pinv(const fso_mesh* m, fso_state* s);
 void fso_derivatives(const fso_mesh* m, fso_state* s);
 void fso_limiter_barth(const fso_mesh* m, const fso_params* p, fso_state* s);
+void fso_limiter_venkat(const fso_mesh* m, const fso_params* p, fso_state* s);
 void fso_interior_residual(const fso_mesh* m, const fso_params* p, fso_state* s, int jupdate, int ts);
 int fso_boundary_residual(const fso_mesh* m, const fso_params* p, fso_state* s, int jupdate);
 /* multicolor = 0: the reference's sequential symmetric Gauss-Seidel;

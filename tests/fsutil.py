@@ -65,6 +65,8 @@ def oracle_lib():
         L.fso_derivatives.restype = None
         L.fso_limiter_barth.argtypes = [vp, vp, vp]
         L.fso_limiter_barth.restype = None
+        L.fso_limiter_venkat.argtypes = [vp, vp, vp]
+        L.fso_limiter_venkat.restype = None
         L.fso_interior_residual.argtypes = [vp, vp, vp, i, i]
         L.fso_interior_residual.restype = None
         L.fso_boundary_residual.argtypes = [vp, vp, vp, i]

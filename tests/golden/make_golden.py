@@ -72,6 +72,9 @@ def main():
     # tiny meshes: complete step histories through the first-order and linear phases
     run_case("bump17x9_o1", 17, 9, 7, params(order=1, foits=2), 6, False, full, lhs_steps=(1, 3, 6))
     run_case("bump17x9_o3", 17, 9, 7, params(order=3, foits=2, soits=3), 5, True, full, lhs_steps=(3, 5))
+    # Venkatakrishnan limiter (limit=2) at the k=1 face points
+    run_case("bump17x9_o1_venkat", 17, 9, 7, params(order=1, foits=2, limit=2, mach=0.675), 6, False, full,
+             lhs_steps=(6,))
     # the CPU-runnable configuration: 97x33 bump channel, M=0.5, k=1, Barth limiter
     run_case("cfg1_97x33", 97, 33, 1, params(order=3, foits=20, soits=20, mach=0.5, alpha=0.0, CFL=100.0, ramp=30,
                                               subits=20), 2, True, ["rms", "R", "dQ", "Q"])

@@ -48,9 +48,9 @@ def _set_state(s, o, Q):
     o["nQ"][:] = Q
 
 
-@pytest.mark.parametrize("ts,order,soits", [(1, 1, 0), (5, 1, 0), (4, 3, 4)])
-def test_residual_and_jacobian_mirrors(ts, order, soits):
-    p, m, s, o = _pair(33, 17, 11, order=order, soits=soits)
+@pytest.mark.parametrize("ts,order,soits,limit", [(1, 1, 0, 1), (5, 1, 0, 1), (4, 3, 4, 1), (5, 1, 0, 2)])
+def test_residual_and_jacobian_mirrors(ts, order, soits, limit):
+    p, m, s, o = _pair(33, 17, 11, order=order, soits=soits, limit=limit)
     info = m.info()
     Q = U.perturbed_state(info["nn"], info["ng"], p, seed=5)
     _set_state(s, o, Q)
@@ -65,7 +65,7 @@ def test_residual_and_jacobian_mirrors(ts, order, soits):
         np.testing.assert_array_equal(s.get("grad"), o["grad"])
     if order == 1 and ts > p.foits:
         s.limiter(ts)
-        U.oracle_call(o, "fso_limiter_barth")
+        U.oracle_call(o, "fso_limiter_venkat" if limit == 2 else "fso_limiter_barth")
         np.testing.assert_array_equal(s.get("phi"), o["phi"][0::8])
     s.interior_residual(1, ts)
     U.oracle_call(o, "fso_interior_residual", 1, ts)
@@ -122,7 +122,7 @@ def test_roe_batch_bitexact():
     assert np.array_equal(out, ref), np.max(np.abs(out - ref))
 
 
-@pytest.mark.parametrize("order,soits,limit", [(1, 0, 1), (3, 2, 1)])
+@pytest.mark.parametrize("order,soits,limit", [(1, 0, 1), (3, 2, 1), (1, 0, 2), (3, 2, 2)])
 def test_step_history(order, soits, limit):
     """fs_solve against the oracle's solve with the same multicolor sweep order."""
     p, m, s, o = _pair(33, 17, 2, order=order, soits=soits, limit=limit, subits=10)

@@ -38,7 +38,7 @@ def _setup(g, **over):
     return p, m, o, nsteps
 
 
-@pytest.mark.parametrize("case", ["bump17x9_o1", "bump17x9_o3", "cfg1_97x33"])
+@pytest.mark.parametrize("case", ["bump17x9_o1", "bump17x9_o3", "bump17x9_o1_venkat", "cfg1_97x33"])
 def test_step_history_bitexact(case):
     g = _load(case)
     p, m, o, nsteps = _setup(g)
