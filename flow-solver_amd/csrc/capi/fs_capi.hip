@@ -26,6 +26,8 @@ int set_error(int code, const std::string& msg) {
   return code;
 }
 
+const char* last_error_text() { return g_err.c_str(); }
+
 const char* device_error_message(int e) {
   if (e & 1) return "boundary state outside the characteristic cases (cvbc.C:330-382 exit)";
   return "non-positive or non-finite time step (init_memory.C:257-268 exit)";

@@ -11,4 +11,6 @@ struct fs_mesh {
 namespace fse {
 // text of a per-instance device error flag
 const char* device_error_message(int e);
+// the calling thread's last error text
+const char* last_error_text();
 }  // namespace fse

@@ -434,6 +434,7 @@ fsk::InstParams Engine::params_for(int i, int it) const {
 int Engine::upload_params(int it) {
   (void)it;
   // the staging buffer may still feed an earlier copy: drain the stream
+  // (a no-op at a step boundary, where step_end has synchronised)
   FSE_HIP(hipStreamSynchronize(stream));
   const int B = ninst();
   for (int k = 1; k <= iter; ++k)
@@ -524,6 +525,42 @@ int Engine::enqueue_step(int key, bool record) {
 }
 
 int Engine::step(bool advance, bool graph, bool record) {
+  if (record) {
+    if (advance)
+      for (auto& I : inst)
+        if (I.active) ++I.ts;
+    int key = step_key();
+    if (key < 0) return FS_OK;
+    int rc = upload_params(1);
+    if (!rc) rc = enqueue_step(key, true);
+    if (!rc) rc = enqueue_readback();
+    if (rc) return rc;
+    FSE_HIP(hipStreamSynchronize(stream));
+    float a = 0, b = 0, c = 0, l = 0;
+    FSE_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+    FSE_HIP(hipEventElapsedTime(&b, ev_[2], ev_[3]));
+    FSE_HIP(hipEventElapsedTime(&l, ev_[3], ev_[5]));
+    FSE_HIP(hipEventElapsedTime(&c, ev_[5], ev_[4]));
+    t_edge += a; t_node += b; t_gs += c; t_lu += l;
+    return check_errors(true);
+  }
+  int rc = step_begin(advance, graph);
+  if (rc) return rc;
+  return step_end();
+}
+
+int Engine::enqueue_readback() {
+  const int B = ninst();
+  FSE_HIP(hipMemcpyAsync(h_norms_, ds.inst_norms, 2 * (size_t)B * sizeof(double), hipMemcpyDeviceToHost, stream));
+  FSE_HIP(hipMemcpyAsync(h_err_, ds.err, (size_t)B * sizeof(int), hipMemcpyDeviceToHost, stream));
+  return FS_OK;
+}
+
+int Engine::step_begin(bool advance, bool graph) {
+  if (pending_) {
+    int rc = step_end();
+    if (rc) return rc;
+  }
   if (advance)
     for (auto& I : inst)
       if (I.active) ++I.ts;
@@ -531,7 +568,7 @@ int Engine::step(bool advance, bool graph, bool record) {
   if (key < 0) return FS_OK;
   int rc = upload_params(1);
   if (rc) return rc;
-  if (graph && !record) {
+  if (graph) {
     auto itg = graphs_.find(key);
     if (itg == graphs_.end()) {
       hipGraph_t g = nullptr;
@@ -551,21 +588,19 @@ int Engine::step(bool advance, bool graph, bool record) {
     }
     FSE_HIP(hipGraphLaunch(itg->second, stream));
   } else {
-    rc = enqueue_step(key, record);
+    rc = enqueue_step(key, false);
     if (rc) return rc;
   }
-  const int B = ninst();
-  FSE_HIP(hipMemcpyAsync(h_norms_, ds.inst_norms, 2 * (size_t)B * sizeof(double), hipMemcpyDeviceToHost, stream));
-  FSE_HIP(hipMemcpyAsync(h_err_, ds.err, (size_t)B * sizeof(int), hipMemcpyDeviceToHost, stream));
+  rc = enqueue_readback();
+  if (rc) return rc;
+  pending_ = true;
+  return FS_OK;
+}
+
+int Engine::step_end() {
+  if (!pending_) return FS_OK;
+  pending_ = false;
   FSE_HIP(hipStreamSynchronize(stream));
-  if (record) {
-    float a = 0, b = 0, c = 0, l = 0;
-    FSE_HIP(hipEventElapsedTime(&a, ev_[0], ev_[1]));
-    FSE_HIP(hipEventElapsedTime(&b, ev_[2], ev_[3]));
-    FSE_HIP(hipEventElapsedTime(&l, ev_[3], ev_[5]));
-    FSE_HIP(hipEventElapsedTime(&c, ev_[5], ev_[4]));
-    t_edge += a; t_node += b; t_gs += c; t_lu += l;
-  }
   return check_errors(true);
 }
 

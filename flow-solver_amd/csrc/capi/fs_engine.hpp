@@ -87,6 +87,11 @@ class Engine {
   // runs as one hipGraph launch (captured per phase combination); with
   // `record` the kernel groups are timed with events (no graph).
   int step(bool advance, bool graph, bool record);
+  // The same step split in two so several engines can overlap on the device:
+  // step_begin enqueues the step and the norm/error readback and returns;
+  // step_end waits for them and updates Instance::r2/dq2/err.
+  int step_begin(bool advance, bool graph);
+  int step_end();
 
   // copies the per-instance device error flags into Instance::err (and clears
   // them on the device)
@@ -135,6 +140,8 @@ class Engine {
   int newton_iteration(int it, int key, bool record);
   int step_key() const;
   int check_errors(bool norms);
+  int enqueue_readback();
+  bool pending_ = false;
 };
 
 }  // namespace fse

@@ -70,7 +70,18 @@ class BatchInfo(C.Structure):
         ("device_bytes", C.c_int64)]
 
 
-INST_RUNNING, INST_CONVERGED, INST_MAX_STEPS, INST_FAILED, INST_PAUSED = range(5)
+INST_RUNNING, INST_CONVERGED, INST_MAX_STEPS, INST_FAILED, INST_PAUSED, INST_CANCELLED = range(6)
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device_budget", C.c_int64), ("precision", C.c_int), ("max_steps", C.c_int), ("tol", C.c_double),
+                ("prep_threads", C.c_int), ("use_graph", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("steps", C.c_int), ("drop", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("device_bytes", C.c_int64), ("nn", C.c_int), ("ng", C.c_int)]
 
 
 class Timing(C.Structure):
@@ -134,6 +145,16 @@ def lib():
         "fs_batch_set": (i, [vp, i, i, vp, i64]),
         "fs_batch_time_steps": (i, [vp, i, vp]),
         "fs_mesh_footprint": (i, [vp, i, C.POINTER(i64)]),
+        "fs_server_config_default": (None, [vp]),
+        "fs_server_create": (i, [vp, C.POINTER(vp)]),
+        "fs_server_destroy": (None, [vp]),
+        "fs_server_submit": (i, [vp, vp, vp, C.POINTER(i64)]),
+        "fs_server_wait": (i, [vp, d, vp, C.POINTER(i)]),
+        "fs_server_get_state": (i, [vp, i64, vp, i64]),
+        "fs_server_release": (i, [vp, i64]),
+        "fs_server_cancel": (i, [vp, i64]),
+        "fs_server_counts": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "fs_server_clock": (d, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -396,3 +417,68 @@ class Batch:
         t = Timing()
         _check(lib().fs_batch_time_steps(self._h, nsteps, C.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
+
+
+class Server:
+    """The serving request loop (C++ worker and device threads inside the
+    library): submit steady-solve requests, wait for finished ones."""
+
+    def __init__(self, precision=FS_PREC_MIXED, max_steps=2000, tol=1e-10, device_budget=0, prep_threads=8,
+                 use_graph=True):
+        cfg = ServerConfig()
+        lib().fs_server_config_default(C.byref(cfg))
+        cfg.precision, cfg.max_steps, cfg.tol = precision, max_steps, tol
+        cfg.device_budget, cfg.prep_threads, cfg.use_graph = device_budget, prep_threads, 1 if use_graph else 0
+        h = C.c_void_p()
+        _check(lib().fs_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().fs_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, mesh, params):
+        """Queue a solve; the server takes the mesh over (the Mesh object is
+        emptied)."""
+        rid = C.c_int64()
+        _check(lib().fs_server_submit(self._h, mesh._h, C.byref(params), C.byref(rid)))
+        mesh._h = C.c_void_p()
+        return rid.value
+
+    def wait(self, timeout=-1.0):
+        """Next finished request as a dict, or None after `timeout` seconds."""
+        info, got = RequestInfo(), C.c_int()
+        _check(lib().fs_server_wait(self._h, timeout, C.byref(info), C.byref(got)))
+        if not got.value:
+            return None
+        out = {k: getattr(info, k) for k, _ in info._fields_}
+        if info.status == INST_FAILED:
+            out["error"] = lib().fs_last_error().decode()
+        return out
+
+    def state(self, info):
+        n = 4 * (info["nn"] + info["ng"])
+        out = np.empty(n)
+        _check(lib().fs_server_get_state(self._h, info["id"], out.ctypes.data, n))
+        return out
+
+    def release(self, rid):
+        _check(lib().fs_server_release(self._h, rid))
+
+    def cancel(self, rid):
+        _check(lib().fs_server_cancel(self._h, rid))
+
+    def counts(self):
+        q, r, f = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().fs_server_counts(self._h, C.byref(q), C.byref(r), C.byref(f)))
+        return q.value, r.value, f.value
+
+    def clock(self):
+        return lib().fs_server_clock(self._h)

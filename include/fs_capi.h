@@ -191,7 +191,8 @@ typedef enum {
   FS_INST_CONVERGED = 1,   /* residual drop reached the tolerance */
   FS_INST_MAX_STEPS = 2,   /* step limit reached first */
   FS_INST_FAILED = 3,      /* non-physical state (fs_batch_inst_error) */
-  FS_INST_PAUSED = 4       /* masked off by fs_batch_set_active */
+  FS_INST_PAUSED = 4,      /* masked off by fs_batch_set_active */
+  FS_INST_CANCELLED = 5    /* serving: cancelled before it finished */
 } fs_inst_status;
 typedef struct {
   int ninst, nn, ne, nsub, nbe, nnz, ncolor;
@@ -219,6 +220,49 @@ int fs_batch_set(fs_batch* b, int i, int field, const double* host, int64_t n);
 int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t);
 /* device bytes a mesh costs inside a batch/solver of the given precision */
 int fs_mesh_footprint(const fs_mesh* m, int precision, int64_t* bytes);
+
+/* ---- serving: a request loop that accepts steady-solve requests (a mesh
+ *      and its PARAMS: Mach, angle of attack, order ...), prepares meshes on
+ *      host worker threads, admits prepared requests at pseudo-time-step
+ *      boundaries while their device footprint fits the budget, keeps every
+ *      admitted solve resident in HBM, and overlaps the steps of all resident
+ *      solves on the device (requests admitted together share one batched
+ *      engine; each engine runs on its own stream).  A solve ends when
+ *      ||R||/||R0|| <= tol (as fs_solve_steady), at max_steps, or on a
+ *      non-physical state.  No counterpart in the reference. ---- */
+typedef struct fs_server fs_server;
+typedef struct {
+  int64_t device_budget;   /* bytes of device memory for resident solves (0: 80% of free) */
+  int precision;           /* FS_PREC_F64 or FS_PREC_MIXED */
+  int max_steps;
+  double tol;
+  int prep_threads;        /* host preprocessing workers */
+  int use_graph;           /* one hipGraph launch per engine step */
+} fs_server_config;
+typedef struct {
+  int64_t id;
+  int status;              /* fs_inst_status; FS_INST_FAILED also for requests rejected at admission */
+  int steps;
+  double drop;
+  double t_submit, t_admit, t_done;   /* seconds on the server clock (fs_server_clock) */
+  int64_t device_bytes;
+  int nn, ng;
+} fs_request_info;
+void fs_server_config_default(fs_server_config* c);
+int fs_server_create(const fs_server_config* c, fs_server** out);
+/* stops the loop; outstanding requests are dropped */
+void fs_server_destroy(fs_server* s);
+/* takes ownership of `mesh` (prepared or not; it is prepared with `p` on a worker) */
+int fs_server_submit(fs_server* s, fs_mesh* mesh, const fs_params* p, int64_t* id);
+/* waits up to timeout_s for the next finished request; *got = 0 on timeout */
+int fs_server_wait(fs_server* s, double timeout_s, fs_request_info* out, int* got);
+/* final state of a finished request: (nn+ng)*4 values, ghosts after the real nodes */
+int fs_server_get_state(fs_server* s, int64_t id, double* Q, int64_t n);
+/* forgets a finished request (its state is freed) */
+int fs_server_release(fs_server* s, int64_t id);
+int fs_server_cancel(fs_server* s, int64_t id);
+int fs_server_counts(fs_server* s, int* queued, int* running, int* finished);
+double fs_server_clock(fs_server* s);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,99 @@
+"""Serving on the GPU: requests submitted to the request loop while others
+are on the device are admitted as memory frees up, their steps overlap, and
+every result equals the same solve run alone (bit for bit; the stated
+tolerance of the serving design is 1e-10 in the density L2 norm)."""
+import numpy as np
+import pytest
+
+import fsutil as U
+
+fsm = U.fsm
+pytestmark = pytest.mark.gpu
+
+
+def _params(**kw):
+    base = dict(nsteps=0, iter=1, jufreq=1, CFL=100.0, ramp=20, gamma=1.4, alpha=0.0, mach=0.5, subits=16,
+                method=1, order=1, foits=5, soits=0, limit=0, reorder=1)
+    base.update(kw)
+    return fsm.Params.make(**base)
+
+
+CASES = [
+    (lambda: fsm.Mesh.bump(41, 17, 0.2, 11), dict(mach=0.5, alpha=0.5)),
+    (lambda: fsm.Mesh.bump(25, 9, 0.2, 12), dict(mach=0.4, order=3, soits=3, limit=1)),
+    (lambda: fsm.Mesh.airfoil(40, 9, 2e-2, 8.0, 0.05, 13), dict(mach=0.45, alpha=2.0)),
+    (lambda: fsm.Mesh.bump(33, 13, 0.1, 14), dict(mach=0.35, alpha=-1.0, order=3, soits=2)),
+    (lambda: fsm.Mesh.bump(17, 9, 0.2, 15), dict(mach=0.55)),
+    (lambda: fsm.Mesh.airfoil(32, 8, 2e-2, 8.0, 0.05, 16), dict(mach=0.3, order=3, soits=2, limit=2)),
+]
+MAX_STEPS, TOL = 200, 1e-8
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED, fsm.FS_PREC_F64])
+def test_serving_results_equal_alone(precision):
+    params = [_params(**kw) for _, kw in CASES]
+    meshes = []
+    for (make, _), p in zip(CASES, params):
+        m = make()
+        m.prepare(p)
+        meshes.append(m)
+    foot = [m.footprint(precision) for m in meshes]
+    # a budget that holds about half the requests: the rest wait for memory
+    budget = int(sum(sorted(foot)[-3:]) * 1.05)
+    srv = fsm.Server(precision=precision, max_steps=MAX_STEPS, tol=TOL, device_budget=budget, prep_threads=4)
+    ids = {}
+    for i, (m, p) in enumerate(zip(meshes, params)):
+        ids[srv.submit(m, p)] = i
+    results = {}
+    while len(results) < len(ids):
+        info = srv.wait(120.0)
+        assert info is not None, "server timed out"
+        results[ids[info["id"]]] = (info, srv.state(info))
+        srv.release(info["id"])
+    srv.close()
+    admits = sorted(r[0]["t_admit"] for r in results.values())
+    print("admission times", ["%.3f" % t for t in admits], flush=True)
+    assert admits[-1] > admits[0], "no request waited for memory"
+    spans = [(r[0]["t_admit"], r[0]["t_done"]) for r in results.values()]
+    overlap = any(a0 < b1 and b0 < a1 for i, (a0, a1) in enumerate(spans) for (b0, b1) in spans[i + 1:])
+    assert overlap, "no two solves were resident together"
+    for i, ((make, _), p) in enumerate(zip(CASES, params)):
+        info, Q = results[i]
+        m = make()
+        m.prepare(p)
+        s = fsm.Solver(m, p, precision)
+        s.init_freestream()
+        steps, drop = s.solve_steady(MAX_STEPS, TOL)
+        Qa = s.get("Q")
+        nn = m.info()["nn"]
+        rel = np.linalg.norm(Q[0:4 * nn:4] - Qa[0:4 * nn:4]) / np.linalg.norm(Qa[0:4 * nn:4])
+        print("request %d status %d steps %d drop %.2e | alone steps %d drop %.2e | density L2 diff %.1e identical %s"
+              % (i, info["status"], info["steps"], info["drop"], steps, drop, rel, np.array_equal(Q, Qa)), flush=True)
+        assert info["status"] in (fsm.INST_CONVERGED, fsm.INST_MAX_STEPS)
+        assert info["steps"] == steps
+        assert rel <= 1e-10
+        assert np.array_equal(Q, Qa)
+        s.close()
+
+
+def test_serving_rejects_oversized_and_cancels():
+    p = _params()
+    m = fsm.Mesh.bump(33, 13, 0.2, 3)
+    m.prepare(p)
+    srv = fsm.Server(precision=fsm.FS_PREC_MIXED, max_steps=50, tol=1e-12, device_budget=1024, prep_threads=1)
+    rid = srv.submit(m, p)
+    info = srv.wait(60.0)
+    assert info is not None and info["id"] == rid
+    assert info["status"] == fsm.INST_FAILED and "budget" in info["error"]
+    srv.close()
+    srv = fsm.Server(precision=fsm.FS_PREC_MIXED, max_steps=100000, tol=1e-30, prep_threads=1)
+    m = fsm.Mesh.bump(33, 13, 0.2, 3)
+    rid = srv.submit(m, p)
+    import time
+    t0 = time.time()
+    while srv.counts()[1] == 0 and time.time() - t0 < 60:
+        time.sleep(0.01)
+    srv.cancel(rid)
+    info = srv.wait(60.0)
+    assert info is not None and info["status"] == fsm.INST_CANCELLED
+    srv.close()
