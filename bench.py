@@ -12,6 +12,17 @@ linear sweeps + update over the whole mesh.  Edge-flux evals per step =
 subedges + boundary edges (the residual's Roe evaluations; the 8 extra
 Jacobian evaluations per subedge are reported separately).
 
+Also measured (extra objects of the same line):
+  * "mixed": the same workload and steps on the mixed-precision path;
+  * "batch": 256 independent solves packed into one launch per kernel
+    (bump channels and airfoil O-meshes, 5k-200k triangles log-uniform,
+    M ~ U[0.3,0.8], alpha ~ U[-2,4] deg, k in {1,2}; seeded), FP64 and mixed:
+    throughput over full steps, then every solve run to a 1e-10 drop under
+    per-instance convergence masks and the converged states compared;
+  * "serving": a seeded Poisson stream of steady-solve requests (10k-2M
+    triangles) through the C++ request loop, p50/p99 time-to-converged and
+    solves/s, mixed and FP64 on the same stream.
+
 Prints one JSON line.
 """
 import argparse
@@ -22,6 +33,7 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "flow-solver_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
@@ -94,7 +106,132 @@ def parse():
     ap.add_argument("--ny", type=int, default=257)
     ap.add_argument("--cpu-steps", type=int, default=2, help="k=2-phase steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=256, help="instances of the batch section (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-max-steps", type=int, default=1500, help="steady batch solve limit (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=30.0, help="length of the request stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=0.5, help="offered requests per second")
+    ap.add_argument("--serve-max-steps", type=int, default=2000)
     return ap.parse_args()
+
+
+# solve controls of the batch and serving workloads: implicit ramped CFL, 40
+# sweeps, first-order start-up; k=1 unlimited (the Barth-limited k=1 iteration
+# stalls short of a 1e-10 drop), k=2 with the Barth limiter (its k=2 phase
+# residual is first order, so the limiter does not move the solution)
+WORKLOAD_CONTROLS = dict(CFL=200.0, ramp=30, subits=40)
+
+
+def workload_params(W, cases):
+    return [W.case_params(c, limit=(0 if c["k"] == 1 else 1), **WORKLOAD_CONTROLS) for c in cases]
+
+
+def bench_batch(a):
+    import numpy as np
+    import workloads as W
+    fsm = W.fsm
+    cases = W.batch_cases(a.batch)
+    params = workload_params(W, cases)
+    t0 = time.perf_counter()
+    meshes = W.build_cases(cases, params, workers=16)
+    t_setup = time.perf_counter() - t0
+    infos = [m.info() for m in meshes]
+    out = {"instances": a.batch, "triangles": sum(i["ntri"] for i in infos), "nodes": sum(i["nn"] for i in infos),
+           "subedges": sum(i["nsub"] for i in infos), "bedges": sum(i["nbe"] for i in infos),
+           "k2_instances": sum(1 for c in cases if c["k"] == 2), "setup_s": t_setup,
+           "controls": dict(WORKLOAD_CONTROLS, tol=1e-10), "graph": True}
+    warm = max(p.foits + p.soits for p in params)
+    rho = {}
+    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+        b = fsm.Batch(meshes, params, prec)
+        b.use_graph(True)
+        b.init_freestream()
+        for _ in range(warm):
+            b.step()
+        t = b.time_steps(a.batch_steps)
+        res = {"value": t["flux_evals"] / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s",
+               "ms_per_step": t["step_ms"], "device_bytes": b.info()["device_bytes"]}
+        b.close()
+        if a.batch_max_steps > 0:
+            b = fsm.Batch(meshes, params, prec)
+            b.use_graph(True)
+            b.init_freestream()
+            t0 = time.perf_counter()
+            steps, drop, status = b.solve_steady(a.batch_max_steps, 1e-10)
+            wall = time.perf_counter() - t0
+            conv = status == fsm.INST_CONVERGED
+            rho[name] = {i: b.get(i, "Q")[0:4 * infos[i]["nn"]:4] for i in range(a.batch) if conv[i]}
+            res["steady"] = {"wall_s": wall, "converged": int(conv.sum()),
+                             "failed": int((status == fsm.INST_FAILED).sum()),
+                             "max_steps": int((status == fsm.INST_MAX_STEPS).sum()),
+                             "steps_median": float(np.median(steps[conv])) if conv.any() else None,
+                             "instance_steps_total": int(steps.sum()),
+                             "solves_per_s": float(conv.sum()) / wall}
+            b.close()
+        out[name] = res
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if rho:
+        both = sorted(set(rho["fp64"]) & set(rho["mixed"]))
+        diffs = [float(np.linalg.norm(rho["mixed"][i] - rho["fp64"][i]) / np.linalg.norm(rho["fp64"][i]))
+                 for i in both]
+        out["converged_both"] = len(both)
+        out["max_density_l2_diff_mixed_vs_fp64"] = max(diffs) if diffs else None
+        out["time_to_solution_speedup"] = out["fp64"]["steady"]["wall_s"] / out["mixed"]["steady"]["wall_s"]
+    return out
+
+
+def bench_serving(a, precision):
+    """Poisson arrivals at a fixed offered rate; each request's mesh is built
+    before the clock starts and handed to the server at its arrival time (the
+    server prepares it on its workers).  Latency = done - submit."""
+    import threading
+    import numpy as np
+    import workloads as W
+    fsm = W.fsm
+    rng = np.random.Generator(np.random.PCG64(77))
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / a.serve_rate)
+        if t > a.serve_seconds:
+            break
+        arrivals.append(t)
+    cases = W.serving_cases(len(arrivals))
+    params = workload_params(W, cases)
+    meshes = [W.make_mesh(c) for c in cases]
+    srv = fsm.Server(precision=precision, max_steps=a.serve_max_steps, tol=1e-10, prep_threads=16)
+    res = []
+
+    def collect():
+        while len(res) < len(cases):
+            info = srv.wait(600.0)
+            if info is None:
+                break
+            res.append(info)
+            srv.release(info["id"])
+
+    th = threading.Thread(target=collect)
+    th.start()
+    w0 = time.perf_counter()
+    for arr, m, p in zip(arrivals, meshes, params):
+        d = arr - (time.perf_counter() - w0)
+        if d > 0:
+            time.sleep(d)
+        srv.submit(m, p)
+    th.join()
+    wall = time.perf_counter() - w0
+    srv.close()
+    st = np.array([r["status"] for r in res])
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+    dev = np.array([r["t_done"] - r["t_admit"] for r in res])
+    conv = st == fsm.INST_CONVERGED
+    q = lambda v, p: float(np.percentile(v, p)) if len(v) else None
+    return {"requests": len(res), "offered_rate_per_s": a.serve_rate, "stream_s": a.serve_seconds, "wall_s": wall,
+            "converged": int(conv.sum()), "max_steps": int((st == fsm.INST_MAX_STEPS).sum()),
+            "failed": int((st == fsm.INST_FAILED).sum()),
+            "p50_time_to_converged_s": q(lat[conv], 50), "p99_time_to_converged_s": q(lat[conv], 99),
+            "p50_admit_to_converged_s": q(dev[conv], 50), "p99_admit_to_converged_s": q(dev[conv], 99),
+            "solves_per_s": float(conv.sum()) / wall,
+            "nodes_median": float(np.median([r["nn"] for r in res])) if res else None}
 
 
 def cpu_baseline(U, fsm, p, nx, ny, steps):
@@ -178,6 +315,12 @@ def main():
                      "kernel_ms": {"edge_flux_f64": tm["edge_ms"], "node_assembly": tm["node_ms"],
                                    "gauss_seidel_f32": tm["gs_ms"], "diag_lu_f32": tm["lu_ms"],
                                    "other": tm["other_ms"]}}
+    if a.batch > 0:
+        line["batch"] = bench_batch(a)
+    if a.serve_seconds > 0:
+        sv = {"mixed": bench_serving(a, fsm.FS_PREC_MIXED), "fp64": bench_serving(a, fsm.FS_PREC_F64)}
+        sv["controls"] = dict(WORKLOAD_CONTROLS, tol=1e-10, max_steps=a.serve_max_steps)
+        line["serving"] = sv
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(U, fsm, p, a.nx, a.ny, a.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -92,6 +92,10 @@ class Engine {
   // step_end waits for them and updates Instance::r2/dq2/err.
   int step_begin(bool advance, bool graph);
   int step_end();
+  bool step_pending() const { return pending_; }
+  // the enqueued step has completed or failed (step_end will not block and
+  // reports a failure)
+  bool step_ready() const { return !pending_ || hipStreamQuery(stream) != hipErrorNotReady; }
 
   // copies the per-instance device error flags into Instance::err (and clears
   // them on the device)

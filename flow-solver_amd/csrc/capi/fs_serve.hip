@@ -3,11 +3,13 @@
 // Threads: `prep_threads` host workers run the mesh preprocessing of queued
 // requests; one device thread owns every engine.  At each pseudo-time-step
 // boundary the device thread admits prepared requests (first come, first
-// served) while their footprint fits the device budget, packs the admitted
-// ones into one new batched engine, enqueues one step of every live engine
-// (each on its own stream, so the steps overlap on the device), then waits
-// for all of them and retires solves that met the stopping test.  An engine
-// whose solves have all finished is destroyed and its memory returns to the
+// served) while their footprint fits the device budget and packs the admitted
+// ones into one new batched engine.  Every engine runs on its own stream and
+// advances on its own: whenever its previous step has finished, the device
+// thread retires the solves that met the stopping test and enqueues the next
+// step at once, so the steps of all resident solves overlap on the device and
+// a small solve is not held to the pace of a large one.  An engine whose
+// solves have all finished is destroyed and its memory returns to the
 // budget.  A solve's arithmetic depends only on its own mesh and controls,
 // so its result does not depend on what it shares the device with.
 #include <hip/hip_runtime.h>
@@ -194,81 +196,91 @@ struct fs_server {
     groups.push_back(std::move(g));
   }
 
-  void step_all() {
-    for (auto& g : groups) {
-      if (g->live == 0) continue;
-      {
-        // cancellations take effect at the step boundary
-        std::lock_guard<std::mutex> lk(mu);
-        for (size_t i = 0; i < g->reqs.size(); ++i) {
-          Request* r = g->reqs[i];
-          if (r && r->cancel) {
-            g->eng->inst[i].active = false;
-            g->reqs[i] = nullptr;
-            --g->live;
-            --running;
-            finish_locked(r, FS_INST_CANCELLED, "cancelled");
-          }
+  // Bookkeeping after a finished step of engine g: stopping test per solve
+  // (as fs_solve_steady), final state copied out, finished solves masked off.
+  void retire_finished(Group* g) {
+    fse::Engine& E = *g->eng;
+    for (int i = 0; i < E.ninst(); ++i) {
+      fse::Instance& I = E.inst[i];
+      Request* r = g->reqs[i];
+      if (!I.active || !r) continue;
+      r->steps = I.ts;
+      int status = FS_INST_RUNNING;
+      std::string err;
+      if (I.err) {
+        status = FS_INST_FAILED;
+        err = fse::device_error_message(I.err);
+      } else {
+        double rn = std::sqrt(I.r2);
+        int tr = ts_ref(I.prm);
+        if (I.ts == tr || (I.ts > tr && r->r0 < 0)) r->r0 = rn;
+        if (r->r0 > 0) r->drop = rn / r->r0;
+        if (!(rn == rn)) {
+          status = FS_INST_FAILED;
+          err = "non-finite residual";
+        } else if (r->r0 > 0 && r->drop <= cfg.tol) {
+          status = FS_INST_CONVERGED;
+        } else if (I.ts >= cfg.max_steps) {
+          status = FS_INST_MAX_STEPS;
         }
       }
-      if (g->live == 0) continue;
-      if (g->eng->step_begin(true, cfg.use_graph != 0)) fail_group(g.get(), fse::last_error_text());
-    }
-    for (auto& g : groups) {
-      if (g->live == 0) continue;
-      if (g->eng->step_end()) {
-        fail_group(g.get(), fse::last_error_text());
-        continue;
+      if (status == FS_INST_RUNNING) continue;
+      int64_t n = 4 * ((int64_t)I.nn + I.nbe);
+      r->Q.assign((size_t)n, 0.0);
+      if (E.get_field(i, FS_F_Q, r->Q.data(), n)) {
+        status = FS_INST_FAILED;
+        err = fse::last_error_text();
       }
-      fse::Engine& E = *g->eng;
-      for (int i = 0; i < E.ninst(); ++i) {
-        fse::Instance& I = E.inst[i];
+      I.active = false;
+      --g->live;
+      g->reqs[i] = nullptr;
+      std::lock_guard<std::mutex> lk(mu);
+      --running;
+      finish_locked(r, status, err);
+    }
+  }
+
+  // Advances engine g if its previous step has finished: bookkeeping, then
+  // the next step is enqueued at once.  Engines progress independently, so a
+  // small solve is not held to the step rate of a large one.  Returns whether
+  // anything happened.
+  bool service(Group* g) {
+    if (g->live == 0) return false;
+    fse::Engine& E = *g->eng;
+    if (E.step_pending()) {
+      if (!E.step_ready()) return false;
+      if (E.step_end()) {
+        fail_group(g, fse::last_error_text());
+        return true;
+      }
+      retire_finished(g);
+    }
+    {
+      // cancellations take effect at the step boundary
+      std::lock_guard<std::mutex> lk(mu);
+      for (size_t i = 0; i < g->reqs.size(); ++i) {
         Request* r = g->reqs[i];
-        if (!I.active || !r) continue;
-        r->steps = I.ts;
-        int status = FS_INST_RUNNING;
-        std::string err;
-        if (I.err) {
-          status = FS_INST_FAILED;
-          err = fse::device_error_message(I.err);
-        } else {
-          double rn = std::sqrt(I.r2);
-          int tr = ts_ref(I.prm);
-          if (I.ts == tr || (I.ts > tr && r->r0 < 0)) r->r0 = rn;
-          if (r->r0 > 0) r->drop = rn / r->r0;
-          if (!(rn == rn)) {
-            status = FS_INST_FAILED;
-            err = "non-finite residual";
-          } else if (r->r0 > 0 && r->drop <= cfg.tol) {
-            status = FS_INST_CONVERGED;
-          } else if (I.ts >= cfg.max_steps) {
-            status = FS_INST_MAX_STEPS;
-          }
+        if (r && r->cancel) {
+          E.inst[i].active = false;
+          g->reqs[i] = nullptr;
+          --g->live;
+          --running;
+          finish_locked(r, FS_INST_CANCELLED, "cancelled");
         }
-        if (status == FS_INST_RUNNING) continue;
-        // keep the final state, then drop the instance from the engine
-        int64_t n = 4 * ((int64_t)I.nn + I.nbe);
-        r->Q.assign((size_t)n, 0.0);
-        if (E.get_field(i, FS_F_Q, r->Q.data(), n)) {
-          status = FS_INST_FAILED;
-          err = fse::last_error_text();
-        }
-        I.active = false;
-        --g->live;
-        g->reqs[i] = nullptr;
-        std::lock_guard<std::mutex> lk(mu);
-        --running;
-        finish_locked(r, status, err);
       }
     }
-    // retire engines without live solves; their meshes are no longer needed
+    if (g->live > 0 && E.step_begin(true, cfg.use_graph != 0)) fail_group(g, fse::last_error_text());
+    return true;
+  }
+
+  void retire_groups() {
     for (auto it = groups.begin(); it != groups.end();) {
       if ((*it)->live > 0) {
         ++it;
         continue;
       }
       int64_t b = (*it)->bytes;
-      it = groups.erase(it);
+      it = groups.erase(it);  // the engine's destructor waits for its stream
       std::lock_guard<std::mutex> lk(mu);
       bytes_in_use -= b;
     }
@@ -283,8 +295,10 @@ struct fs_server {
         if (stop) break;
       }
       admit();
-      if (groups.empty()) continue;
-      step_all();
+      bool any = false;
+      for (auto& g : groups) any = service(g.get()) || any;
+      retire_groups();
+      if (!any) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
     groups.clear();
   }

@@ -19,18 +19,12 @@ Triangle counts: a bump channel of nx*ny nodes has 2(nx-1)(ny-1) triangles
 import concurrent.futures as cf
 import math
 import os
+import sys
 
 import numpy as np
 
-try:
-    from . import flowsolver as fsm  # package import
-except ImportError:  # loaded by path
-    import importlib.util as _ilu
-
-    _spec = _ilu.spec_from_file_location("flowsolver", os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                                                                    "flowsolver.py"))
-    fsm = _ilu.module_from_spec(_spec)
-    _spec.loader.exec_module(fsm)
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import flowsolver as fsm  # noqa: E402
 
 
 def bump_dims(tri):

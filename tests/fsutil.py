@@ -2,7 +2,6 @@
 reference-build dumps.  Only tests, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg use the oracle."""
 import ctypes as C
-import importlib.util
 import os
 import struct
 import subprocess
@@ -17,11 +16,14 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 def load_flowsolver():
-    path = os.path.join(ROOT, "flow-solver_amd", "flowsolver.py")
-    spec = importlib.util.spec_from_file_location("flowsolver", path)
-    mod = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(mod)
-    return mod
+    """The product's host mirror, imported once under its own name (so the
+    workload module and the tests share one module object)."""
+    import sys
+    pkg = os.path.join(ROOT, "flow-solver_amd")
+    if pkg not in sys.path:
+        sys.path.insert(0, pkg)
+    import flowsolver
+    return flowsolver
 
 
 fsm = load_flowsolver()
