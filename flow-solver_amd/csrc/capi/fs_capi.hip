@@ -134,6 +134,34 @@ int fs_mesh_create_bump(int nx, int ny, double jitter, uint64_t seed, fs_mesh** 
   return FS_OK;
 }
 
+int fs_mesh_create(int nn, const double* x, int ntri, const int* tri, int nquad, const int* quad, int nb,
+                   const int* bseg_off, const int* bseg, const int* bbc, fs_mesh** out) {
+  if (!out || nn < 3 || !x || ntri < 0 || nquad < 0 || ntri + nquad < 1 || nb < 1 || !bseg_off || !bbc ||
+      (ntri > 0 && !tri) || (nquad > 0 && !quad))
+    return fail(FS_ERR_ARG, "bad mesh arrays");
+  auto h = std::make_unique<fs_mesh>();
+  fs::Mesh& m = h->m;
+  m.nn = nn;
+  m.x.assign(x, x + 2 * (size_t)nn);
+  m.ntri = ntri;
+  m.tri.assign(tri, tri + 3 * (size_t)ntri);
+  m.nquad = nquad;
+  if (nquad > 0) m.quad.assign(quad, quad + 4 * (size_t)nquad);
+  m.nb = nb;
+  m.bseg_off.assign(bseg_off, bseg_off + nb + 1);
+  if (m.bseg_off[0] != 0) return fail(FS_ERR_ARG, "bseg_off[0] must be 0");
+  for (int b = 0; b < nb; ++b)
+    if (m.bseg_off[b + 1] < m.bseg_off[b]) return fail(FS_ERR_ARG, "bseg_off not ascending");
+  if (m.bseg_off[nb] > 0 && !bseg) return fail(FS_ERR_ARG, "null bseg");
+  m.bseg.assign(bseg, bseg + 2 * (size_t)m.bseg_off[nb]);
+  m.bbc.assign(bbc, bbc + nb);
+  for (int v : m.tri) if (v < 0 || v >= nn) return fail(FS_ERR_ARG, "triangle node out of range");
+  for (int v : m.quad) if (v < 0 || v >= nn) return fail(FS_ERR_ARG, "quad node out of range");
+  for (int v : m.bseg) if (v < 0 || v >= nn) return fail(FS_ERR_ARG, "segment node out of range");
+  *out = h.release();
+  return FS_OK;
+}
+
 int fs_mesh_create_airfoil(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed, fs_mesh** out) {
   if (!out || nwall < 8 || nrad < 3 || !(h0 > 0) || !(rmax > 1) || jitter < 0 || jitter >= 0.5)
     return fail(FS_ERR_ARG, "bad airfoil O-mesh arguments");

@@ -58,6 +58,10 @@ int fs_params_write(const char* path, const fs_params* p);
 int fs_mesh_read(const char* grid_path, const char* bc_path, fs_mesh** out);
 int fs_mesh_write(const fs_mesh* m, const char* grid_path, const char* bc_path);
 int fs_mesh_create_bump(int nx, int ny, double jitter, uint64_t seed, fs_mesh** out);
+/* A mesh from arrays (0-based): the contents of the grid file that
+ * mesh_io.C:31-214 reads into GRID (x, c2n[Tri], c2n[Quad], bs, bbc). */
+int fs_mesh_create(int nn, const double* x, int ntri, const int* tri, int nquad, const int* quad, int nb,
+                   const int* bseg_off, const int* bseg, const int* bbc, fs_mesh** out);
 int fs_mesh_create_airfoil(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed, fs_mesh** out);
 /* Preprocessing of main.C:149-559: Cuthill_McKee (reorder.C:58), metrics2D
  * (metrics.C:31), CSN/NSN/ESN/c2e maps (maps.C:35-517), Build_Edge_Data and
