@@ -384,7 +384,16 @@ int fs_solve(fs_solver* s, int ts, double* dq_norm, double* r2) {
 int fs_roe_flux_batch(int n, double gamma, const double* d_normals, const double* d_qL, const double* d_qR,
                       double* d_flux, void* stream) {
   if (n < 0 || (n > 0 && (!d_normals || !d_qL || !d_qR || !d_flux))) return fail(FS_ERR_ARG, "bad arguments");
-  fsk::launch_roe_batch(n, gamma, d_normals, d_qL, d_qR, d_flux, (hipStream_t)stream);
+  fsk::launch_roe_batch(n, 0, gamma, d_normals, d_qL, d_qR, d_flux, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return FS_OK;
+}
+
+int fs_roe_flux_batch_form(int n, int form, double gamma, const double* d_normals, const double* d_qL,
+                           const double* d_qR, double* d_flux, void* stream) {
+  if (n < 0 || (form != 0 && form != 1) || (n > 0 && (!d_normals || !d_qL || !d_qR || !d_flux)))
+    return fail(FS_ERR_ARG, "bad arguments");
+  fsk::launch_roe_batch(n, form, gamma, d_normals, d_qL, d_qR, d_flux, (hipStream_t)stream);
   HIP_TRY(hipGetLastError());
   return FS_OK;
 }

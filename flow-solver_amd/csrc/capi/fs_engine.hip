@@ -72,7 +72,8 @@ bool supported(const fs_params& p, std::string* why) {
 }
 
 size_t footprint_bytes(const fs::Mesh& g, int prec) {
-  size_t nn = g.nn, ng = g.ng, ne = g.ne, nsub = g.nsub, nbe = g.nbe, nnz = g.ja.size();
+  // block slots: the sliced layout pads each 16-row slice to its largest row
+  size_t nn = g.nn, ng = g.ng, ne = g.ne, nsub = g.nsub, nbe = g.nbe, nnz = g.ja.size() + g.nn / 2;
   size_t ninc = g.nsub_list.size(), neinc = g.nedge_list.size(), nst = g.st_nodes.size();
   size_t b = 0;
   // mesh image
@@ -192,6 +193,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   std::vector<double> x(2 * NN), cv(NN), mom(9 * NN);
   std::vector<int> edges(2 * NE), edge_sub(2 * NE, -1), edge_slot(2 * NE);
   std::vector<double> sub_n(3 * NS);
+  std::vector<int> sub_edge_d(NS);
   std::vector<int> nsub_off(NN + 1), nedge_off(NN + 1), nbe_off(NN + 1), st_off(NN + 1);
   std::vector<int> sub_inc(2 * NS), inc_other(EINC), nbe_list(BINC), st_nodes(NST);
   std::vector<double> inc_n(4 * KINC), inc_fc(2 * KINC), inc_xg(6 * KINC), inc_gw(2 * EINC), st_Wt(5 * NST);
@@ -220,8 +222,10 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       for (int k = g.esub_off[e], t = 0; k < g.esub_off[e + 1] && t < 2; ++k, ++t)
         edge_sub[2 * (size_t)(eo + e) + t] = g.esub_list[k] + so;
     }
-    for (int s = 0; s < g.nsub; ++s)
+    for (int s = 0; s < g.nsub; ++s) {
       for (int c = 0; c < 3; ++c) sub_n[3 * (size_t)(so + s) + c] = g.sub_n[3 * (size_t)s + c];
+      sub_edge_d[so + s] = g.sub_edge[s] + eo;
+    }
     // subedge incidences: normals, face points (reconstruct.C:181-186) and
     // Gauss points, in each node's ascending subedge order
     for (size_t k = 0; k < g.nsub_list.size(); ++k) {
@@ -272,17 +276,15 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   nbe_off[NN] = (int)binc;
   st_off[NN] = (int)sto;
 
-  // ---- color-major block CSR over all instances ----
+  // ---- rows in color order over all instances ----
   // Global color c holds the color-c rows of every instance, instance by
   // instance; rows of instances whose last color is c come after the others
-  // (backward launches of c stop before them).  Each row keeps its blocks in
-  // the reference's order (nsn order, diagonal last).
-  std::vector<int> color_rows(NN), pos_inst(NN), rstart(NN), iau_d(NN), ja_d(NNZ), slot_t_d(NNZ);
+  // (backward launches of c stop before them).
+  std::vector<int> color_rows(NN), pos_inst(NN), pos_of(NN), pdeg(NN);
   color_off.assign(ncmax + 1, 0);
   color_nback.assign(ncmax, 0);
-  for (auto& I : inst) I.slot_dev.assign(I.nnz, 0);
   {
-    long long t = 0, pos = 0;
+    long long t = 0;
     for (int c = 0; c < ncmax; ++c) {
       color_off[c] = (int)t;
       for (int pass = 0; pass < 2; ++pass) {
@@ -292,33 +294,71 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
           if (c >= g.ncolor || ((g.ncolor - 1 == c) != (pass == 1))) continue;
           for (int tt = g.color_off[c]; tt < g.color_off[c + 1]; ++tt) {
             int j = g.color_rows[tt];
+            if (g.iau[j] != g.ia[j + 1] - 1) return set_error(FS_ERR_MESH, "diagonal block is not last in its row");
             int J = I.noff + j;
             color_rows[t] = J;
             pos_inst[t] = i;
+            pos_of[J] = (int)t;
+            pdeg[t] = g.ia[j + 1] - g.ia[j] - 1;
             ++t;
-            rstart[J] = (int)pos;
-            for (int k = g.ia[j]; k < g.ia[j + 1]; ++k) I.slot_dev[k] = (int)pos++;
-            iau_d[J] = I.slot_dev[g.iau[j]];
           }
         }
         if (pass == 0) color_nback[c] = (int)(t - color_off[c]);
       }
     }
     color_off[ncmax] = (int)t;
-    if (t != NN || pos != NNZ) return set_error(FS_ERR_MESH, "coloring does not cover every row exactly once");
+    if (t != NN) return set_error(FS_ERR_MESH, "coloring does not cover every row exactly once");
+  }
+  // ---- sliced block slots ----
+  // slice s = positions 16s..16s+15; its off-diagonal blocks occupy
+  // 16 * (largest row degree of the slice) slots starting at sbase[s]
+  const long long nslice = (NN + 15) / 16;
+  std::vector<int> sbase(nslice + 1, 0);
+  {
+    long long acc = 0;
+    for (long long sl = 0; sl < nslice; ++sl) {
+      int mx = 0;
+      for (long long t = 16 * sl; t < std::min(NN, 16 * sl + 16); ++t) mx = std::max(mx, pdeg[t]);
+      sbase[sl] = (int)acc;
+      acc += 16LL * mx;
+      if (acc + NN >= (1LL << 31)) return set_error(FS_ERR_ARG, "batch too large for 32-bit block slots");
+    }
+    sbase[nslice] = (int)acc;
+  }
+  const long long NOFF = sbase[nslice], NSLOT = NOFF + NN;
+  std::vector<int> ja_d(NOFF, 0), iau_d(NN), roff(NN + 1, 0), tslot;
+  tslot.reserve(NNZ);
+  for (auto& I : inst) I.slot_dev.assign(I.nnz, 0);
+  for (int i = 0; i < B; ++i) {
+    const fs::Mesh& g = *meshes[i];
+    Instance& I = inst[i];
+    for (int j = 0; j < g.nn; ++j) {
+      int J = I.noff + j;
+      int t = pos_of[J];
+      long long base = (long long)sbase[t >> 4] + (t & 15);
+      for (int k = g.ia[j]; k < g.ia[j + 1] - 1; ++k) {
+        long long slot = base + 16LL * (k - g.ia[j]);
+        I.slot_dev[k] = (int)slot;
+        ja_d[slot] = g.ja[k] + I.noff;
+      }
+      I.slot_dev[g.iau[j]] = (int)(NOFF + t);
+      iau_d[J] = (int)(NOFF + t);
+    }
   }
   for (int i = 0; i < B; ++i) {
     const fs::Mesh& g = *meshes[i];
     const Instance& I = inst[i];
-    for (size_t k = 0; k < g.ja.size(); ++k) {
-      ja_d[I.slot_dev[k]] = g.ja[k] + I.noff;
-      slot_t_d[I.slot_dev[k]] = I.slot_dev[g.slot_t[k]];
+    for (int j = 0; j < g.nn; ++j) {
+      roff[I.noff + j] = (int)tslot.size();
+      for (int k = g.ia[j]; k < g.ia[j + 1] - 1; ++k) tslot.push_back(I.slot_dev[g.slot_t[k]]);
     }
     for (int e = 0; e < g.ne; ++e) {
       edge_slot[2 * (size_t)(I.eoff + e)] = I.slot_dev[g.edge_slot_lr[e]];
       edge_slot[2 * (size_t)(I.eoff + e) + 1] = I.slot_dev[g.edge_slot_rl[e]];
     }
   }
+  roff[NN] = (int)tslot.size();
+  d.nnz = (int)NSLOT;
 
   // ---- norm chunks ----
   std::vector<int> ch_inst, ch_b, ch_e, inst_ch0(B + 1);
@@ -337,14 +377,15 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   if ((rc = upload(&d.inst_node, inst_node)) || (rc = upload(&d.inst_edge, inst_edge)) ||
       (rc = upload(&d.inst_bedge, inst_bedge)) || (rc = upload(&d.pos_inst, pos_inst)) || (rc = upload(&d.x, x)) ||
       (rc = upload(&d.cv_area, cv)) || (rc = upload(&d.edges, edges)) || (rc = upload(&d.edge_sub, edge_sub)) ||
-      (rc = upload(&d.edge_slot, edge_slot)) || (rc = upload(&d.sub_n, sub_n)) ||
+      (rc = upload(&d.edge_slot, edge_slot)) || (rc = upload(&d.sub_n, sub_n)) || (rc = upload(&d.sub_edge, sub_edge_d)) ||
       (rc = upload(&d.nsub_off, nsub_off)) || (rc = upload(&d.nedge_off, nedge_off)) ||
       (rc = upload(&d.sub_inc, sub_inc)) || (rc = upload(&d.inc_n, inc_n)) || (rc = upload(&d.inc_fc, inc_fc)) ||
       (rc = upload(&d.inc_xg, inc_xg)) || (rc = upload(&d.inc_other, inc_other)) ||
       (rc = upload(&d.inc_gw, inc_gw)) || (rc = upload(&d.nbe_off, nbe_off)) ||
       (rc = upload(&d.nbe_list, nbe_list)) || (rc = upload(&d.bedge_real, breal)) ||
-      (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.rstart, rstart)) ||
-      (rc = upload(&d.ja, ja_d)) || (rc = upload(&d.iau, iau_d)) || (rc = upload(&d.slot_t, slot_t_d)) ||
+      (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.roff, roff)) ||
+      (rc = upload(&d.tslot, tslot)) || (rc = upload(&d.ja, ja_d)) || (rc = upload(&d.iau, iau_d)) ||
+      (rc = upload(&d.pdeg, pdeg)) || (rc = upload(&d.sbase, sbase)) ||
       (rc = upload(&d.color_rows, color_rows)) || (rc = upload(&d.st_off, st_off)) ||
       (rc = upload(&d.st_nodes, st_nodes)) || (rc = upload(&d.st_Wt, st_Wt)) || (rc = upload(&d.moments, mom)) ||
       (rc = upload(&nc.chunk_inst, ch_inst)) || (rc = upload(&nc.chunk_begin, ch_b)) ||
@@ -364,12 +405,12 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = alloc(&d_ip_, (size_t)iter * B)))
     return rc;
   if (precision == FS_PREC_MIXED) {
-    if ((rc = alloc(&st.LHS32, (size_t)NNZ * 16)) || (rc = alloc(&st.LU32, nn * 16)) ||
+    if ((rc = alloc(&st.LHS32, (size_t)NSLOT * 16)) || (rc = alloc(&st.LU32, nn * 16)) ||
         (rc = alloc(&st.RHS32, nn * 4)) || (rc = alloc(&st.dQ32, nn * 4)))
       return rc;
   } else {
-    if ((rc = alloc(&st.LHS, (size_t)NNZ * 16)) || (rc = alloc(&st.LU, nn * 16))) return rc;
-    FSE_HIP(hipMemset(st.LHS, 0, (size_t)NNZ * 16 * sizeof(double)));
+    if ((rc = alloc(&st.LHS, (size_t)NSLOT * 16)) || (rc = alloc(&st.LU, nn * 16))) return rc;
+    FSE_HIP(hipMemset(st.LHS, 0, (size_t)NSLOT * 16 * sizeof(double)));
   }
   st.ip = d_ip_;
   FSE_HIP(hipMemset(st.err, 0, (size_t)B * sizeof(int)));

@@ -30,7 +30,7 @@ __device__ __forceinline__ double dmax(double x, double y) { return x >= y ? x :
 // K2: least-squares gradient, node-centric gather (gradient.C:363-403)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, DevState s) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = xcd_block() * blockDim.x + threadIdx.x;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
   if (!ip.active || !ip.grad) return;
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, DevState s) {
 // butterfly over the 16 lanes.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_derivatives(DevMesh m, DevState s) {
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = xcd_block() * blockDim.x + threadIdx.x;
   int i = tid >> 4, l = tid & 15;
   if (i >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, i)];
@@ -160,7 +160,7 @@ __device__ __forceinline__ void candidate(bool venkat, double q, double qmin, do
 // (InstParams::lim): 1 limiter on, 2 Gauss points of the k=2 phase,
 // 4 Venkatakrishnan's function instead of Barth's.
 __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, DevState s) {
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = xcd_block() * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
@@ -225,13 +225,16 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, DevState s) {
 // handles its 1-2 subedges in ascending order, stores each subedge's flux and
 // owns the edge's two off-diagonal blocks outright (no atomics).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, DevState st, int jac_allowed) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// The FP64 parity pass: reference Roe operation order and the FD Jacobian
+// blocks (kJac = true is the only instantiation launched).
+template <bool kJac>
+__global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, DevState st) {
+  int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
   if (!ip.active) return;
   const int recon = ip.recon_flux;
-  const int jupdate = jac_allowed && ip.jac;
+  const bool jupdate = kJac && ip.jac;
   const double gamma = ip.gamma;
   const double* __restrict__ nQ = st.nQ;
   const double* __restrict__ grad = st.grad;
@@ -301,12 +304,51 @@ __global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, DevState st, in
   // the edge owns its two off-diagonal blocks: written every FP64 pass
   // (zeros when the instance skips the Jacobian, as build_matrix clears
   // them), never by the mixed path's FP64 residual pass
-  if (jac_allowed) {
+  if (kJac) {
     double* blr = LHS + 16 * (size_t)m.edge_slot[2 * e];
     double* brl = LHS + 16 * (size_t)m.edge_slot[2 * e + 1];
 #pragma unroll
     for (int k = 0; k < 16; ++k) { blr[k] = offLR[k]; brl[k] = offRL[k]; }
   }
+}
+
+// Residual-only face fluxes of the mixed path's FP64 outer residual, one
+// thread per subedge (characteristic-wave Roe flux, same reconstruction):
+// every face is independent, so no thread loops over an edge's subedges.
+__global__ void __launch_bounds__(kBlock) k_sub_flux(DevMesh m, DevState st) {
+  int s = xcd_block() * blockDim.x + threadIdx.x;
+  if (s >= m.nsub) return;
+  int e = m.sub_edge[s];
+  const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
+  if (!ip.active) return;
+  const double* __restrict__ nQ = st.nQ;
+  int L = m.edges[2 * e], R = m.edges[2 * e + 1];
+  int kL = m.sub_inc[2 * (size_t)s], kR = m.sub_inc[2 * (size_t)s + 1];
+  double nx = m.sub_n[3 * (size_t)s], ny = m.sub_n[3 * (size_t)s + 1], len = m.sub_n[3 * (size_t)s + 2];
+  double ql[4], qr[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    ql[j] = nQ[4 * (size_t)L + j];
+    qr[j] = nQ[4 * (size_t)R + j];
+  }
+  if (ip.recon_flux == 1) {
+    const double* __restrict__ grad = st.grad;
+    double fc0 = m.inc_fc[2 * (size_t)kL], fc1 = m.inc_fc[2 * (size_t)kL + 1];
+    double dxl0 = fc0 - m.x[2 * (size_t)L], dxl1 = fc1 - m.x[2 * (size_t)L + 1];
+    double dxr0 = fc0 - m.x[2 * (size_t)R], dxr1 = fc1 - m.x[2 * (size_t)R + 1];
+    double phiL = st.phi[L], phiR = st.phi[R];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      ql[j] += (phiL * grad[8 * (size_t)L + 2 * j] * dxl0 + phiL * grad[8 * (size_t)L + 2 * j + 1] * dxl1);
+      qr[j] += (phiR * grad[8 * (size_t)R + 2 * j] * dxr0 + phiR * grad[8 * (size_t)R + 2 * j + 1] * dxr1);
+    }
+  }
+  double f[4];
+  roe_flux_wave(nx, ny, len, ip.gamma, ql, qr, f);
+  double4* fl = reinterpret_cast<double4*>(st.inc_flux + 4 * (size_t)kL);
+  double4* fr = reinterpret_cast<double4*>(st.inc_flux + 4 * (size_t)kR);
+  *fl = make_double4(f[0], f[1], f[2], f[3]);
+  *fr = make_double4(-f[0], -f[1], -f[2], -f[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -356,7 +398,7 @@ __global__ void __launch_bounds__(kBlock) k_bedge(DevMesh m, DevState s) {
 // 4 lanes per node: lane v owns residual component v and row v of the
 // diagonal block, so the quad reads each 128-byte block contiguously.
 __global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, int flags) {
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = xcd_block() * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
   const int inst = inst_of(m, m.inst_node, n);
@@ -408,8 +450,8 @@ __global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, int flag
   }
   if (interior && jac) {
 #pragma unroll 4
-    for (int k = m.rstart[n]; k < m.iau[n]; ++k) {
-      const double* B = s.LHS + 16 * (size_t)m.slot_t[k] + 4 * v;
+    for (int q = m.roff[n]; q < m.roff[n + 1]; ++q) {
+      const double* B = s.LHS + 16 * (size_t)m.tslot[q] + 4 * v;
 #pragma unroll
       for (int c = 0; c < 4; ++c) D[c] -= B[c];
     }
@@ -447,7 +489,7 @@ __global__ void __launch_bounds__(kBlock) k_zero_offdiag(DevMesh m, double* __re
 __global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const InstParams* __restrict__ ip,
                                                     const double* __restrict__ LHS, double* __restrict__ LU,
                                                     double* __restrict__ dQ) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
@@ -463,23 +505,26 @@ __global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const InstParams*
 }
 
 // rows [row_begin,row_end) of the color-sorted order; 4 lanes per row, lane v
-// forms component v of RHS - sum_offdiag A dQ in the row's block order (each
-// quad streams whole 128-byte blocks), then every lane of the quad solves with
-// the factored diagonal.
+// forms component v of RHS - sum_offdiag A dQ in the row's block order (a
+// wave's 16 rows read block e of their rows as one contiguous 2 KB run of the
+// sliced layout), then every lane of the quad solves with the factored
+// diagonal.
 __global__ void __launch_bounds__(kBlock) k_gs_color(DevMesh m, const InstParams* __restrict__ ip,
                                                      const double* __restrict__ LHS, const double* __restrict__ LU,
                                                      const double* __restrict__ RHS, double* __restrict__ dQ,
                                                      int row_begin, int row_end) {
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = xcd_block() * blockDim.x + threadIdx.x;
   int t = row_begin + (tid >> 2), v = tid & 3;
   if (t >= row_end) return;
   int j = m.color_rows[t];
   int inst = inst_of(m, m.pos_inst, t);
+  int deg = m.pdeg[t];
+  size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
   double r = RHS[4 * (size_t)j + v];
-  int kb = m.rstart[j], kend = m.iau[j];
   if (!ip[inst].active) return;
-  for (int k = kb; k < kend; ++k) {
-    const double* A = LHS + 16 * (size_t)k + 4 * v;
+  for (int e = 0; e < deg; ++e) {
+    size_t k = base + 16 * (size_t)e;
+    const double* A = LHS + 16 * k + 4 * v;
     const double* q = dQ + 4 * (size_t)m.ja[k];
     r -= (A[0] * q[0] + A[1] * q[1] + A[2] * q[2] + A[3] * q[3]);
   }
@@ -556,6 +601,7 @@ __global__ void __launch_bounds__(kBlock) k_norm_inst(int ninst, NormChunks c, c
   out[2 * (size_t)i + 1] = d;
 }
 
+template <int kForm>
 __global__ void __launch_bounds__(kBlock) k_roe_batch(int n, double gamma, const double* __restrict__ nrm,
                                                       const double* __restrict__ qL, const double* __restrict__ qR,
                                                       double* __restrict__ flux) {
@@ -564,7 +610,10 @@ __global__ void __launch_bounds__(kBlock) k_roe_batch(int n, double gamma, const
   double a[4], b[4], f[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) { a[j] = qL[4 * (size_t)i + j]; b[j] = qR[4 * (size_t)i + j]; }
-  roe_flux(nrm[3 * (size_t)i], nrm[3 * (size_t)i + 1], nrm[3 * (size_t)i + 2], gamma, a, b, f);
+  if (kForm == 0)
+    roe_flux(nrm[3 * (size_t)i], nrm[3 * (size_t)i + 1], nrm[3 * (size_t)i + 2], gamma, a, b, f);
+  else
+    roe_flux_wave(nrm[3 * (size_t)i], nrm[3 * (size_t)i + 1], nrm[3 * (size_t)i + 2], gamma, a, b, f);
 #pragma unroll
   for (int j = 0; j < 4; ++j) flux[4 * (size_t)i + j] = f[j];
 }
@@ -590,7 +639,10 @@ void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 void launch_edge_flux(const DevMesh& m, const DevState& s, int jac_allowed, hipStream_t st) {
   if (m.ne == 0) return;
-  hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s, jac_allowed);
+  if (jac_allowed)
+    hipLaunchKernelGGL(k_edge_flux<true>, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s);
+  else if (m.nsub > 0)
+    hipLaunchKernelGGL(k_sub_flux, dim3(nblocks(m.nsub)), dim3(kBlock), 0, st, m, s);
 }
 void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nbe == 0) return;
@@ -649,10 +701,13 @@ void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipS
   hipLaunchKernelGGL(k_norm_inst, dim3(nblocks(m.ninst)), dim3(kBlock), 0, st, m.ninst, c, s.ip, s.chunk_part,
                      s.inst_norms);
 }
-void launch_roe_batch(int n, double gamma, const double* normals, const double* qL, const double* qR, double* flux,
-                      hipStream_t st) {
+void launch_roe_batch(int n, int form, double gamma, const double* normals, const double* qL, const double* qR,
+                      double* flux, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_roe_batch, dim3(nblocks(n)), dim3(kBlock), 0, st, n, gamma, normals, qL, qR, flux);
+  if (form == 0)
+    hipLaunchKernelGGL(k_roe_batch<0>, dim3(nblocks(n)), dim3(kBlock), 0, st, n, gamma, normals, qL, qR, flux);
+  else
+    hipLaunchKernelGGL(k_roe_batch<1>, dim3(nblocks(n)), dim3(kBlock), 0, st, n, gamma, normals, qL, qR, flux);
 }
 
 }  // namespace fsk

@@ -50,6 +50,7 @@ struct DevMesh {
   const int* edge_sub;      // 2*ne (second = -1 for a boundary edge)
   const int* edge_slot;     // 2*ne (slot of block (L,R), slot of block (R,L))
   const double* sub_n;      // 3*nsub
+  const int* sub_edge;      // nsub: the subedge's edge
   // Node-incidence order: node n's incident subedges (ascending subedge
   // index) occupy slots nsub_off[n]..nsub_off[n+1]; its incident edges
   // (ascending edge index) occupy nedge_off[n]..nedge_off[n+1].
@@ -66,13 +67,19 @@ struct DevMesh {
   const int* bedge_real;    // nbe
   const int* bedge_bc;      // nbe
   const double* bedge_n;    // 3*nbe
-  // Block CSR in color-major device order.  Row n owns device slots
-  // rstart[n]..iau[n] (off-diagonals in the reference's nsn order, diagonal
-  // last).
-  const int* rstart;        // nn
-  const int* ja;            // nnz, device slot order
-  const int* iau;           // nn
-  const int* slot_t;        // nnz, device slot of the transposed block
+  // Block storage.  Rows are visited in color order (position t = index in
+  // color_rows).  Off-diagonal blocks are sliced: positions are cut into
+  // slices of 16, and off-diagonal block e (the reference's nsn order) of the
+  // row at position t lives in slot sbase[t/16] + 16 e + t%16, so a sweep
+  // wave reading block e of 16 consecutive rows reads one contiguous run.
+  // The diagonal block of node n is slot iau[n] (all diagonals after the
+  // off-diagonal slots, in position order).
+  const int* roff;          // nn+1: node n's off-diagonal entries roff[n]..roff[n+1]
+  const int* tslot;         // per entry: slot of the transposed block (m,n) of entry (n,m)
+  const int* iau;           // nn: slot of the diagonal block
+  const int* ja;            // per off-diagonal slot: column node (padding slots: 0)
+  const int* pdeg;          // nn, by position: off-diagonal blocks of the row
+  const int* sbase;         // nn/16+1: first slot of each slice
   const int* color_rows;    // nn (rows grouped by color)
   const int* st_off;        // nn+1 (k=2; empty stencils for k=1 instances)
   const int* st_nodes;
@@ -92,7 +99,7 @@ struct DevState {
   double* pv;     // nn*4
   double* ev_con; // nn
   double* dt;     // nn
-  double* LHS;    // nnz*16
+  double* LHS;    // nnz*16, nnz = block slots (padding included)
   double* LU;     // nn*16, indexed by color position
   double* inc_flux;  // 2*nsub*4: subedge flux at each endpoint's incidence slot, signed
   double* bflux;     // nbe*4
@@ -107,6 +114,16 @@ struct DevState {
   double* inst_norms;     // 2 per instance: sum R^2, sum dQ^2
   double* chunk_part;     // 2 per reduction chunk
 };
+
+// Workgroups are dispatched to the 8 XCDs round-robin.  Renumbered this way,
+// XCD x runs one contiguous range of the grid, so the neighbour gathers of
+// adjacent rows and faces are served by the same L2 instead of being fetched
+// into eight.  A bijection of [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int q = G >> 3, r = G & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
 
 // Instance of element `idx` (a single-instance launch skips the lookup).
 __device__ __forceinline__ int inst_of(const DevMesh& m, const int* map, int idx) {
@@ -162,7 +179,8 @@ void launch_node_mixed(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st);
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st);
-void launch_roe_batch(int n, double gamma, const double* normals, const double* qL, const double* qR, double* flux,
-                      hipStream_t st);
+// form 0: reference operation order (bit-exact); 1: characteristic-wave form
+void launch_roe_batch(int n, int form, double gamma, const double* normals, const double* qL, const double* qR,
+                      double* flux, hipStream_t st);
 
 }  // namespace fsk

@@ -84,7 +84,7 @@ __device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, c
 // the (limited) reconstructed face states; block (L,R) += dF/dqR and block
 // (R,L) -= dF/dqL (jacobian.C:612-658).  Zeros when jupdate is off.
 __global__ void __launch_bounds__(kBlock) k_edge_jac32(DevMesh m, DevState st) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
   if (!ip.active) return;
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) k_edge_jac32(DevMesh m, DevState st) {
 // order as the FP64 path), FP32 diagonal block V/dt I - sum of transposed
 // off-diagonals + boundary blocks, FP32 right-hand side -R.
 __global__ void __launch_bounds__(kBlock) k_node_mixed(DevMesh m, DevState s) {
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = xcd_block() * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
   const int inst = inst_of(m, m.inst_node, n);
@@ -192,8 +192,8 @@ __global__ void __launch_bounds__(kBlock) k_node_mixed(DevMesh m, DevState s) {
   for (int c = 0; c < 4; ++c) D[c] = (c == v) ? vd : 0.0;
   if (jac) {
 #pragma unroll 4
-    for (int k = m.rstart[n]; k < m.iau[n]; ++k) {
-      float4 B = *reinterpret_cast<const float4*>(s.LHS32 + 16 * (size_t)m.slot_t[k] + 4 * v);
+    for (int q = m.roff[n]; q < m.roff[n + 1]; ++q) {
+      float4 B = *reinterpret_cast<const float4*>(s.LHS32 + 16 * (size_t)m.tslot[q] + 4 * v);
       D[0] -= B.x; D[1] -= B.y; D[2] -= B.z; D[3] -= B.w;
     }
     for (int k = m.nbe_off[n]; k < m.nbe_off[n + 1]; ++k) {
@@ -228,7 +228,7 @@ __device__ __forceinline__ void lu4f(float* A) {
 __global__ void __launch_bounds__(kBlock) k_diag_lu32(DevMesh m, const InstParams* __restrict__ ip,
                                                       const float* __restrict__ LHS32, float* __restrict__ LU32,
                                                       float* __restrict__ dQ32) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
@@ -247,37 +247,40 @@ __global__ void __launch_bounds__(kBlock) k_diag_lu32(DevMesh m, const InstParam
 }
 
 // FP32 multicolor sweep over rows [row_begin,row_end) of the color order;
-// 4 lanes per row (lane v: component v), quad-wide FP32 block rows; the row's
-// index, block and neighbour loads are issued before the arithmetic.
+// 4 lanes per row (lane v: component v), quad-wide FP32 block rows of the
+// sliced layout; the row's index, block and neighbour loads are issued
+// before the arithmetic.
 __global__ void __launch_bounds__(kBlock) k_gs_color32(DevMesh m, const InstParams* __restrict__ ip,
                                                        const float* __restrict__ LHS32, const float* __restrict__ LU32,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
                                                        int row_begin, int row_end) {
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int tid = xcd_block() * blockDim.x + threadIdx.x;
   int t = row_begin + (tid >> 2), v = tid & 3;
   if (t >= row_end) return;
   int j = m.color_rows[t];
   int inst = inst_of(m, m.pos_inst, t);
+  int deg = m.pdeg[t];
+  size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
   float r = RHS32[4 * (size_t)j + v];
-  int kb = m.rstart[j], kend = m.iau[j];
   if (!ip[inst].active) return;
-  constexpr int kPre = 8;
+  constexpr int kPre = 6;  // 60 VGPRs: full occupancy
   int nb[kPre];
 #pragma unroll
-  for (int i = 0; i < kPre; ++i) nb[i] = (kb + i < kend) ? m.ja[kb + i] : j;
+  for (int i = 0; i < kPre; ++i) nb[i] = (i < deg) ? m.ja[base + 16 * i] : j;
   float4 A[kPre], q[kPre];
 #pragma unroll
   for (int i = 0; i < kPre; ++i) {
-    if (kb + i < kend) {
-      A[i] = *reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)(kb + i) + 4 * v);
+    if (i < deg) {
+      A[i] = *reinterpret_cast<const float4*>(LHS32 + 16 * (base + 16 * i) + 4 * v);
       q[i] = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)nb[i]);
     }
   }
 #pragma unroll
   for (int i = 0; i < kPre; ++i)
-    if (kb + i < kend) r -= A[i].x * q[i].x + A[i].y * q[i].y + A[i].z * q[i].z + A[i].w * q[i].w;
-  for (int k = kb + kPre; k < kend; ++k) {
-    float4 a = *reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)k + 4 * v);
+    if (i < deg) r -= A[i].x * q[i].x + A[i].y * q[i].y + A[i].z * q[i].z + A[i].w * q[i].w;
+  for (int e = kPre; e < deg; ++e) {
+    size_t k = base + 16 * (size_t)e;
+    float4 a = *reinterpret_cast<const float4*>(LHS32 + 16 * k + 4 * v);
     float4 b = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)m.ja[k]);
     r -= a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
   }

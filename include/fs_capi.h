@@ -159,6 +159,11 @@ int fs_solve(fs_solver* s, int ts, double* dq_norm, double* r2);
  * normals[3*i] = (nx, ny, len), qL/qR/flux 4 per face. */
 int fs_roe_flux_batch(int n, double gamma, const double* d_normals, const double* d_qL, const double* d_qR,
                       double* d_flux, void* stream);
+/* The same with a choice of arithmetic: form 0 is the reference's operation
+ * order (bit-exact, as fs_roe_flux_batch); form 1 the characteristic-wave form
+ * the mixed path's FP64 residual uses (same flux to a few ulps of |F|). */
+int fs_roe_flux_batch_form(int n, int form, double gamma, const double* d_normals, const double* d_qL,
+                           const double* d_qR, double* d_flux, void* stream);
 
 /* Steady solve to ||R||/||R0|| <= tol (R0 = residual at the first step of the
  * highest-order phase) or max_steps; returns steps taken and the drop. */

@@ -122,6 +122,52 @@ def test_roe_batch_bitexact():
     assert np.array_equal(out, ref), np.max(np.abs(out - ref))
 
 
+def test_roe_wave_form_matches_reference_form():
+    """The characteristic-wave Roe flux of the mixed path's FP64 residual
+    against the oracle's restatement of flux.C:299-541: per face, within 1e-13
+    of the sum of the magnitudes of the two physical fluxes (the terms the
+    flux is formed from)."""
+    import ctypes as C
+    rng = np.random.default_rng(4)
+    n = 8192
+    gamma = 1.4
+    th = rng.uniform(0, 2 * np.pi, n)
+    nrm = np.stack([np.cos(th), np.sin(th), rng.uniform(0.01, 1.0, n)], 1).ravel()
+
+    def states():
+        rho = rng.uniform(0.3, 3.0, n)
+        u = rng.uniform(-2.0, 2.0, n)
+        v = rng.uniform(-2.0, 2.0, n)
+        pr = rng.uniform(0.2, 3.0, n)
+        return np.stack([rho, rho * u, rho * v, pr / (gamma - 1) + 0.5 * rho * (u * u + v * v)], 1).ravel()
+
+    qL, qR = states(), states()
+    hip = C.CDLL("libamdhip64.so")
+    ptrs = []
+    for arr in (nrm, qL, qR, np.zeros(4 * n)):
+        d = C.c_void_p()
+        assert hip.hipMalloc(C.byref(d), C.c_size_t(arr.nbytes)) == 0
+        assert hip.hipMemcpy(d, arr.ctypes.data_as(C.c_void_p), C.c_size_t(arr.nbytes), 1) == 0
+        ptrs.append(d)
+    lib = fsm.lib()
+    lib.fs_roe_flux_batch_form.argtypes = [C.c_int, C.c_int, C.c_double] + [C.c_void_p] * 5
+    assert lib.fs_roe_flux_batch_form(n, 1, gamma, ptrs[0], ptrs[1], ptrs[2], ptrs[3], None) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    out = np.zeros(4 * n)
+    assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), ptrs[3], C.c_size_t(out.nbytes), 2) == 0
+    for d in ptrs:
+        hip.hipFree(d)
+    worst = 0.0
+    for i in range(n):
+        nx, ny, ln = nrm[3 * i:3 * i + 3]
+        ref = U.roe_oracle(nx, ny, ln, gamma, qL[4 * i:4 * i + 4], qR[4 * i:4 * i + 4])
+        scale = U.roe_oracle(nx, ny, ln, gamma, qL[4 * i:4 * i + 4], qL[4 * i:4 * i + 4])
+        scale = np.abs(scale) + np.abs(U.roe_oracle(nx, ny, ln, gamma, qR[4 * i:4 * i + 4], qR[4 * i:4 * i + 4]))
+        worst = max(worst, float(np.max(np.abs(out[4 * i:4 * i + 4] - ref) / (scale + 1e-300))))
+    print("wave-form Roe flux: max error / (|F(qL)| + |F(qR)|) = %.2e" % worst)
+    assert worst <= 1e-13
+
+
 @pytest.mark.parametrize("order,soits,limit", [(1, 0, 1), (3, 2, 1), (1, 0, 2), (3, 2, 2)])
 def test_step_history(order, soits, limit):
     """fs_solve against the oracle's solve with the same multicolor sweep order."""
