@@ -2,6 +2,7 @@
 // (a one-instance device engine with its own HIP stream).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -181,6 +182,27 @@ int fs_mesh_prepare(fs_mesh* m, const fs_params* p) {
   return FS_OK;
 }
 
+int fs_restart_write(const fs_mesh* m, const char* grid_path, const double* Q, int64_t n, int citer, double cfl,
+                     int with_ghosts) {
+  if (!m || !grid_path || !Q) return fail(FS_ERR_ARG, "null argument");
+  if (!m->prepared) return fail(FS_ERR_ARG, "restart files are written from a prepared mesh (solver numbering)");
+  const fs::Mesh& g = m->m;
+  if (n != 4 * ((int64_t)g.nn + g.ng)) return fail(FS_ERR_ARG, "size mismatch: expected 4*(nn+ng)");
+  if (!fs::write_restart(grid_path, g, Q, g.ng, citer, cfl, with_ghosts != 0))
+    return fail(FS_ERR_IO, std::string("cannot write ") + grid_path);
+  return FS_OK;
+}
+
+int fs_mesh_restart_info(const fs_mesh* m, int* has_restart, int* citer, double* cfl, int* has_ghosts) {
+  if (!m) return fail(FS_ERR_ARG, "null mesh");
+  const fs::Mesh& g = m->m;
+  if (has_restart) *has_restart = g.has_restart ? 1 : 0;
+  if (citer) *citer = g.restart_citer;
+  if (cfl) *cfl = g.restart_cfl;
+  if (has_ghosts) *has_ghosts = g.restart_ghost_Q.empty() ? 0 : 1;
+  return FS_OK;
+}
+
 int fs_mesh_get_info(const fs_mesh* m, fs_mesh_info* info) {
   if (!m || !info) return fail(FS_ERR_ARG, "null argument");
   const fs::Mesh& g = m->m;
@@ -257,6 +279,25 @@ void fs_solver_destroy(fs_solver* s) { delete s; }
 int fs_solver_init_freestream(fs_solver* s) {
   if (!s) return fail(FS_ERR_ARG, "null solver");
   return s->eng.init_freestream(0);
+}
+
+int fs_solver_init_restart(fs_solver* s) {
+  if (!s) return fail(FS_ERR_ARG, "null solver");
+  const fs::Mesh& g = *s->eng.inst[0].mesh;
+  if (!g.has_restart) return fail(FS_ERR_ARG, "the solver's mesh carries no restart payload");
+  std::vector<double> Q(4 * ((size_t)g.nn + g.ng));
+  std::copy(g.restart_Q.begin(), g.restart_Q.end(), Q.begin());
+  if (g.restart_ghost_Q.size() == 4 * (size_t)g.ng) {
+    std::copy(g.restart_ghost_Q.begin(), g.restart_ghost_Q.end(), Q.begin() + 4 * (size_t)g.nn);
+  } else {
+    for (int b = 0; b < g.nbe; ++b) {
+      int real = g.bedge[5 * (size_t)b], gh = g.nn + g.bedge[5 * (size_t)b + 1];
+      for (int v = 0; v < 4; ++v) Q[4 * (size_t)gh + v] = Q[4 * (size_t)real + v];
+    }
+  }
+  int rc = s->eng.set_field(0, FS_F_Q, Q.data(), (int64_t)Q.size());
+  if (!rc) rc = s->eng.set_field(0, FS_F_NQ, Q.data(), (int64_t)Q.size());
+  return rc;
 }
 
 int fs_solver_set(fs_solver* s, int field, const double* host, int64_t n) {

@@ -138,11 +138,61 @@ bool read_grid(const std::string& path, Mesh* m, std::string* err) {
     }
     m->bseg_off[b + 1] = m->bseg_off[b] + ns;
   }
+  // optional restart payload (mesh_io.C:217-263): constants (last step, last
+  // CFL) and the state of every real node; then, as an extension the
+  // reference's reader skips, the ghost states
+  m->has_restart = false;
+  m->restart_Q.clear();
+  m->restart_ghost_Q.clear();
+  bool more = line();
+  while (more && buf[0] != '#' && std::strspn(buf, " \t\r\n") == std::strlen(buf)) more = line();  // blank lines
+  if (more && buf[0] == '#' && std::strstr(buf, "Constants")) {
+    int nconst = 0, nvar = 0;
+    if (!line() || std::sscanf(buf, "%d", &nconst) != 1 || nconst != 2) return fail("restart constants");
+    if (!line() || std::sscanf(buf, "%d", &m->restart_citer) != 1) return fail("restart step");
+    if (!line() || std::sscanf(buf, "%lf", &m->restart_cfl) != 1) return fail("restart CFL");
+    if (!line() || !line() || std::sscanf(buf, "%d", &nvar) != 1 || nvar != 4) return fail("restart variable count");
+    m->restart_Q.assign(4 * (size_t)m->nn, 0.0);
+    for (int n = 0; n < m->nn; ++n) {
+      double* q = &m->restart_Q[4 * (size_t)n];
+      if (!line() || std::sscanf(buf, "%lf %lf %lf %lf", q, q + 1, q + 2, q + 3) != 4) return fail("restart state");
+    }
+    m->has_restart = true;
+    int ng = 0;
+    if (line() && line() && std::sscanf(buf, "%d", &ng) == 1 && ng > 0) {
+      m->restart_ghost_Q.assign(4 * (size_t)ng, 0.0);
+      for (int g = 0; g < ng; ++g) {
+        double* q = &m->restart_ghost_Q[4 * (size_t)g];
+        if (!line() || std::sscanf(buf, "%lf %lf %lf %lf", q, q + 1, q + 2, q + 3) != 4) return fail("restart ghost state");
+      }
+    }
+  }
   std::fclose(fp);
   for (int v : m->tri) if (v < 0 || v >= m->nn) { if (err) *err = "triangle node out of range"; return false; }
   for (int v : m->quad) if (v < 0 || v >= m->nn) { if (err) *err = "quad node out of range"; return false; }
   for (int v : m->bseg) if (v < 0 || v >= m->nn) { if (err) *err = "segment node out of range"; return false; }
   if ((int)m->bbc.size() != m->nb) m->bbc.assign(m->nb, 0);
+  return true;
+}
+
+bool write_restart(const std::string& path, const Mesh& m, const double* Q, int ng, int citer, double cfl,
+                   bool ghosts) {
+  if (!write_grid(path, m)) return false;
+  FILE* fp = std::fopen(path.c_str(), "a");
+  if (!fp) return false;
+  // mesh_io.C:380-389 layout of the payload
+  std::fprintf(fp, "#Number of Constants\n%d\n%d\n%.17E\n", 2, citer, cfl);
+  std::fprintf(fp, "#Number of Variables\n%d\n", 4);
+  for (int n = 0; n < m.nn; ++n)
+    std::fprintf(fp, "%.17e %.17e %.17e %.17e\n", Q[4 * n], Q[4 * n + 1], Q[4 * n + 2], Q[4 * n + 3]);
+  if (ghosts) {
+    std::fprintf(fp, "#Number of ghost states (bedge order)\n%d\n", ng);
+    for (int g = 0; g < ng; ++g) {
+      const double* q = Q + 4 * ((size_t)m.nn + g);
+      std::fprintf(fp, "%.17e %.17e %.17e %.17e\n", q[0], q[1], q[2], q[3]);
+    }
+  }
+  std::fclose(fp);
   return true;
 }
 

@@ -48,6 +48,12 @@ struct Mesh {
   std::vector<int> bseg;      // 2*nseg: (n0,n1) boundary-major
   std::vector<int> bbc;       // nb boundary condition codes
   std::vector<int> nn_map;    // file order -> new id (empty if no reorder)
+  // restart payload of the grid file (mesh_io.C:217-263), if it has one
+  bool has_restart = false;
+  int restart_citer = 0;
+  double restart_cfl = 0.0;
+  std::vector<double> restart_Q;        // 4*nn
+  std::vector<double> restart_ghost_Q;  // 4*ng when the file carries the ghost extension
 
   // ---- integer maps (maps.C) ----
   std::vector<int> ncsn, csn;   // cells surrounding node (global elem ids)
@@ -113,6 +119,12 @@ void make_airfoil_omesh(int nwall, int nrad, double h0, double rmax,
 bool write_grid(const std::string& path, const Mesh& m);
 bool read_grid(const std::string& path, Mesh* m, std::string* err);
 bool write_bc(const std::string& path, const Mesh& m);
+// The grid in its current numbering plus the restart payload of
+// write_mesh (mesh_io.C:318-395): last step, last CFL and Q of the nn real
+// nodes; with `ghosts`, the ng ghost states after it (an extension the
+// reference's reader does not read, so its files stay readable there).
+bool write_restart(const std::string& path, const Mesh& m, const double* Q, int ng, int citer, double cfl,
+                   bool ghosts);
 bool read_bc(const std::string& path, Mesh* m, std::string* err);
 
 // ---------------- preprocessing ----------------

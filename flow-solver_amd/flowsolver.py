@@ -155,6 +155,11 @@ def lib():
         "fs_server_cancel": (i, [vp, i64]),
         "fs_server_counts": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
         "fs_server_clock": (d, [vp]),
+        "fs_restart_write": (i, [vp, C.c_char_p, vp, i64, i, d, i]),
+        "fs_mesh_restart_info": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(d), C.POINTER(i)]),
+        "fs_solver_init_restart": (i, [vp]),
+        "fs_mesh_create": (i, [i, vp, i, vp, i, vp, i, vp, vp, vp, C.POINTER(vp)]),
+        "fs_roe_flux_batch_form": (i, [i, i, d, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -233,6 +238,18 @@ class Mesh:
         buf = C.cast(data, C.POINTER(ct * n)).contents
         return np.frombuffer(buf, dtype=np.int32 if dt.value == 0 else np.float64).copy()
 
+    def restart_info(self):
+        """(has_restart, last step, last CFL, has ghost states) of the grid file read."""
+        h, it, cfl, g = C.c_int(), C.c_int(), C.c_double(), C.c_int()
+        _check(lib().fs_mesh_restart_info(self._h, C.byref(h), C.byref(it), C.byref(cfl), C.byref(g)))
+        return bool(h.value), it.value, cfl.value, bool(g.value)
+
+    def write_restart(self, grid_path, Q, citer, cfl, ghosts=True):
+        """Restart file of this prepared mesh (solver numbering) with state Q."""
+        q = np.ascontiguousarray(Q, dtype=np.float64).ravel()
+        _check(lib().fs_restart_write(self._h, grid_path.encode(), q.ctypes.data, q.size, citer, cfl,
+                                      1 if ghosts else 0))
+
     def footprint(self, precision=FS_PREC_F64):
         """Device bytes this (prepared) mesh costs inside a solver or batch."""
         b = C.c_int64()
@@ -288,6 +305,10 @@ class Solver:
 
     def init_freestream(self):
         _check(lib().fs_solver_init_freestream(self._h))
+
+    def init_restart(self):
+        """State from the mesh's restart payload (ghost states when saved)."""
+        _check(lib().fs_solver_init_restart(self._h))
 
     # reference-function mirrors
     def build_matrix(self, ts):

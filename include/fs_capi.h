@@ -72,6 +72,16 @@ int fs_mesh_create_airfoil(int nwall, int nrad, double h0, double rmax, double j
  * the k=2 pseudo-inverse (compute_derivatives.C:859-940); plus the
  * multicolor row coloring used by the device Gauss-Seidel. */
 int fs_mesh_prepare(fs_mesh* m, const fs_params* p);
+/* Restart files (mesh_io.C:318-395 write_mesh, mesh_io.C:217-263 read_mesh
+ * with isrestart = 1): the prepared mesh in its solver numbering plus the
+ * last step, last CFL and the state of the nn real nodes; with_ghosts appends
+ * the ng ghost states (an extension the reference's reader skips), which
+ * makes a continuation bit-identical to an uninterrupted run.  Q holds
+ * (nn+ng)*4 values as fs_solver_get(FS_F_Q).  Read a restart file with
+ * fs_mesh_read and prepare it with isrestart = 1 (no renumbering). */
+int fs_restart_write(const fs_mesh* m, const char* grid_path, const double* Q, int64_t n, int citer, double cfl,
+                     int with_ghosts);
+int fs_mesh_restart_info(const fs_mesh* m, int* has_restart, int* citer, double* cfl, int* has_ghosts);
 
 typedef struct {
   int nn, ng, ne, nsub, nbe, nnz, ntri, nquad, nb, ncolor, nstencil;
@@ -121,6 +131,9 @@ int fs_solver_create(const fs_mesh* m, const fs_params* p, int precision, fs_sol
 void fs_solver_destroy(fs_solver* s);
 /* init_pv with ic_0 freestream, ghosts copied (ic.C:36-99, ic.C:808-913) */
 int fs_solver_init_freestream(fs_solver* s);
+/* Q and nQ from the solver mesh's restart payload (main.C:604-620: ghosts
+ * copy their real node unless the file carries the ghost states). */
+int fs_solver_init_restart(fs_solver* s);
 typedef enum {
   FS_F_Q = 0,   /* f64 (nn+ng)*4: state at the start of the step, ghosts included */
   FS_F_NQ,      /* f64 (nn+ng)*4: Newton iterate */

@@ -119,3 +119,40 @@ def test_mask_freezes_instance():
     for ts in range(1, 5):
         s1.solve(ts)
     assert np.array_equal(s1.get("Q"), b.get(1, "Q"))
+
+
+def test_restart_continuation_bitexact(tmp_path):
+    """Steps 1..5, restart file with the ghost states, then steps 6..10 from
+    the file on a fresh mesh and solver: identical to ten uninterrupted steps.
+    Without the ghost extension (the reference's own restart, which copies
+    each real node into its ghosts, main.C:604-620) the run continues but the
+    ghost memory of the characteristic boundary update is lost."""
+    p = _params(order=1, limit=1, foits=2)
+    m = fsm.Mesh.bump(33, 13, 0.2, 9)
+    m.prepare(p)
+    ref = fsm.Solver(m, p)
+    ref.init_freestream()
+    for ts in range(1, 6):
+        ref.solve(ts)
+    bc = str(tmp_path / "bc.dat")
+    with open(bc, "w") as f:
+        f.write("".join("%d\n" % v for v in m.array("bbc")))
+    out = {}
+    for ghosts in (True, False):
+        path = str(tmp_path / ("restart_%d.mesh" % ghosts))
+        m.write_restart(path, ref.get("Q"), 5, p.CFL, ghosts=ghosts)
+        pr = fsm.Params.make(**dict(p.as_dict(), isrestart=1))
+        m2 = fsm.Mesh.read(path, bc)
+        m2.prepare(pr)
+        s2 = fsm.Solver(m2, pr)
+        s2.init_restart()
+        for ts in range(6, 11):
+            s2.solve(ts)
+        out[ghosts] = s2.get("Q")
+    for ts in range(6, 11):
+        ref.solve(ts)
+    q = ref.get("Q")
+    print("continuation with ghost states identical: %s; without: max |dQ| %.2e" %
+          (np.array_equal(out[True], q), np.max(np.abs(out[False] - q))))
+    assert np.array_equal(out[True], q)
+    assert np.all(np.isfinite(out[False]))

@@ -150,3 +150,36 @@ def test_airfoil_omesh():
     assert np.all(m.array("cv_area") > 0)
     x = m.array("x")[:2 * info["nn"]].reshape(-1, 2)
     assert np.max(np.hypot(x[:, 0] - 0.5, x[:, 1])) <= 30.0 + 1e-9
+
+
+def test_restart_file_roundtrip(tmp_path):
+    """A restart file (write_mesh's layout, mesh_io.C:318-395, plus the ghost
+    extension) read back with isrestart = 1 gives the same preprocessed mesh
+    and the same state, bit for bit (%.17e keeps every double)."""
+    p = fsm.Params.make(order=3, foits=2, soits=2, reorder=1)
+    m = fsm.Mesh.bump(21, 9, 0.2, 4).prepare(p)
+    info = m.info()
+    rng = np.random.default_rng(1)
+    Q = rng.uniform(0.5, 2.0, 4 * (info["nn"] + info["ng"]))
+    path = str(tmp_path / "restart.mesh")
+    bc = str(tmp_path / "bc.dat")
+    with open(bc, "w") as f:
+        f.write("".join("%d\n" % v for v in m.array("bbc")))
+    m.write_restart(path, Q, 37, 12.5, ghosts=True)
+    r = fsm.Mesh.read(path, bc)
+    assert r.restart_info() == (True, 37, 12.5, True)
+    pr = fsm.Params.make(order=3, foits=2, soits=2, reorder=1, isrestart=1)
+    r.prepare(pr)
+    for name in ("x", "edges", "sub_n", "bedge", "ia", "ja", "color", "color_rows", "st_nodes", "st_W", "cv_area"):
+        np.testing.assert_array_equal(r.array(name), m.array(name), err_msg=name)
+    # the reference's layout: constants, then one line of 4 values per real node
+    lines = open(path).read().splitlines()
+    k = lines.index("#Number of Constants")
+    assert lines[k + 1:k + 3] == ["2", "37"] and float(lines[k + 3]) == 12.5
+    assert lines[k + 4] == "#Number of Variables" and lines[k + 5] == "4"
+    vals = np.array([float(v) for v in " ".join(lines[k + 6:k + 6 + info["nn"]]).split()])
+    np.testing.assert_array_equal(vals, Q[:4 * info["nn"]])
+    # without the extension the file ends after the real-node states
+    m.write_restart(path, Q, 37, 12.5, ghosts=False)
+    assert len(open(path).read().splitlines()) == k + 6 + info["nn"]
+    assert fsm.Mesh.read(path, bc).restart_info() == (True, 37, 12.5, False)
