@@ -166,10 +166,16 @@ def bench_batch(a):
                              "max_steps": int((status == fsm.INST_MAX_STEPS).sum()),
                              "steps_median": float(np.median(steps[conv])) if conv.any() else None,
                              "instance_steps_total": int(steps.sum()),
+                             "steps_per_instance": [int(v) for v in steps],
                              "solves_per_s": float(conv.sum()) / wall}
             b.close()
         out[name] = res
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    # the plain FP64 GPU path: the same solves one at a time (no batching),
+    # each timed alone over steps of its final phase
+    plain = plain_sequential(fsm, meshes, params, fsm.FS_PREC_F64, warm, 3)
+    out["plain_fp64_sequential"] = plain
+    out["speedup_mixed_batched_vs_plain_fp64"] = out["mixed"]["value"] / plain["value"]
     if rho:
         both = sorted(set(rho["fp64"]) & set(rho["mixed"]))
         diffs = [float(np.linalg.norm(rho["mixed"][i] - rho["fp64"][i]) / np.linalg.norm(rho["fp64"][i]))
@@ -177,7 +183,36 @@ def bench_batch(a):
         out["converged_both"] = len(both)
         out["max_density_l2_diff_mixed_vs_fp64"] = max(diffs) if diffs else None
         out["time_to_solution_speedup"] = out["fp64"]["steady"]["wall_s"] / out["mixed"]["steady"]["wall_s"]
+        # an instance's FP64 trajectory is the same alone and batched, so the
+        # plain path needs the same steps: its time to solution is the sum of
+        # steps x alone step time
+        st = out["fp64"]["steady"]["steps_per_instance"]
+        t_plain = sum(n * ms for n, ms in zip(st, plain["ms_per_step_per_instance"])) * 1e-3
+        out["plain_fp64_time_to_solution_s_estimated"] = t_plain
+        out["time_to_solution_speedup_vs_plain_fp64"] = t_plain / out["mixed"]["steady"]["wall_s"]
+    for k in ("fp64", "mixed"):
+        if "steady" in out[k]:
+            out[k]["steady"].pop("steps_per_instance", None)
+    out["plain_fp64_sequential"].pop("ms_per_step_per_instance", None)
     return out
+
+
+def plain_sequential(fsm, meshes, params, prec, warm, steps):
+    """Each solve alone on the single-mesh solver (hipGraph steps), timed
+    over `steps` steps after `warm` untimed ones."""
+    ms, evals = [], 0
+    for m, p in zip(meshes, params):
+        s = fsm.Solver(m, p, prec)
+        s.use_graph(True)
+        s.init_freestream()
+        for ts in range(1, warm + 1):
+            s.solve(ts)
+        t = s.time_steps(warm + 1, steps)
+        ms.append(t["step_ms"])
+        evals += t["flux_evals"]
+        s.close()
+    return {"value": evals / (sum(ms) * 1e-3), "unit": "edge-flux evals/s", "ms_per_step_sum": sum(ms),
+            "instances": len(ms), "ms_per_step_per_instance": ms}
 
 
 def bench_serving(a, precision):
