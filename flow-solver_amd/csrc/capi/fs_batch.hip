@@ -104,6 +104,18 @@ int fs_batch_use_graph(fs_batch* b, int on) {
   return FS_OK;
 }
 
+int fs_batch_autotune(fs_batch* b, double* seconds) {
+  if (!b) return fail(FS_ERR_ARG, "null batch");
+  return b->eng.autotune(seconds);
+}
+
+int fs_batch_get_tuning(const fs_batch* b, int* values, int* from_cache) {
+  if (!b || !values) return fail(FS_ERR_ARG, "null argument");
+  b->eng.tuning_values(values);
+  if (from_cache) *from_cache = b->eng.tuned_from_cache() ? 1 : 0;
+  return FS_OK;
+}
+
 int fs_batch_set_active(fs_batch* b, int i, int active) {
   if (!b || i < 0 || i >= b->eng.ninst()) return fail(FS_ERR_ARG, "instance out of range");
   fs_batch::Track& T = b->track[i];

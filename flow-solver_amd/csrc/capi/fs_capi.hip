@@ -316,6 +316,18 @@ int fs_solver_use_graph(fs_solver* s, int on) {
   return FS_OK;
 }
 
+int fs_solver_autotune(fs_solver* s, double* seconds) {
+  if (!s) return fail(FS_ERR_ARG, "null solver");
+  return s->eng.autotune(seconds);
+}
+
+int fs_solver_get_tuning(const fs_solver* s, int* values, int* from_cache) {
+  if (!s || !values) return fail(FS_ERR_ARG, "null argument");
+  s->eng.tuning_values(values);
+  if (from_cache) *from_cache = s->eng.tuned_from_cache() ? 1 : 0;
+  return FS_OK;
+}
+
 int fs_build_matrix(fs_solver* s, int ts) {
   if (!s) return fail(FS_ERR_ARG, "null solver");
   if (int rc = need_f64(s)) return rc;

@@ -3,8 +3,14 @@
 #include "fs_engine.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
+#include <sys/stat.h>
 
 namespace fse {
 
@@ -90,6 +96,20 @@ size_t footprint_bytes(const fs::Mesh& g, int prec) {
 }
 
 namespace {
+bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
+bool valid_tuning(const fsk::Tuning& t) {
+  return valid_block(t.gs_block) && (t.gs_lanes == 1 || t.gs_lanes == 4) && valid_block(t.gs32_block) &&
+         (t.gs32_pre == 4 || t.gs32_pre == 6 || t.gs32_pre == 8) && valid_block(t.node_block) &&
+         valid_block(t.face_block) && valid_block(t.cell_block);
+}
+// FS_TUNE_CACHE, else $HOME/.cache/flowsolver/tune_cache.txt
+std::string tune_cache_path() {
+  if (const char* e = std::getenv("FS_TUNE_CACHE")) return e;
+  const char* home = std::getenv("HOME");
+  if (!home) return "";
+  return std::string(home) + "/.cache/flowsolver/tune_cache.txt";
+}
+
 __global__ void k_fill(double* p, size_t n, double v) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -429,6 +449,21 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   fill(st.phi, nn, 1.0, stream);
   for (auto& e : ev_) FSE_HIP(hipEventCreate(&e));
   FSE_HIP(hipStreamSynchronize(stream));
+  // launch shapes of an earlier tuning run with the same key
+  {
+    std::string path = tune_cache_path(), key = tune_key(), line;
+    std::ifstream in(path);
+    while (in && std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string k;
+      fsk::Tuning t;
+      if (ls >> k >> t.gs_block >> t.gs_lanes >> t.gs32_block >> t.gs32_pre >> t.node_block >> t.face_block >>
+          t.cell_block && k == key && valid_tuning(t)) {
+        dm.tune = t;
+        tuned_from_cache_ = true;
+      }
+    }
+  }
   return FS_OK;
 }
 
@@ -583,6 +618,7 @@ int Engine::step(bool advance, bool graph, bool record) {
     FSE_HIP(hipEventElapsedTime(&l, ev_[3], ev_[5]));
     FSE_HIP(hipEventElapsedTime(&c, ev_[5], ev_[4]));
     t_edge += a; t_node += b; t_gs += c; t_lu += l;
+    ++steps_run_;
     return check_errors(true);
   }
   int rc = step_begin(advance, graph);
@@ -641,6 +677,7 @@ int Engine::step_begin(bool advance, bool graph) {
 int Engine::step_end() {
   if (!pending_) return FS_OK;
   pending_ = false;
+  ++steps_run_;
   FSE_HIP(hipStreamSynchronize(stream));
   return check_errors(true);
 }
@@ -673,6 +710,138 @@ int Engine::check_errors(bool norms) {
 
 int Engine::sync() {
   FSE_HIP(hipStreamSynchronize(stream));
+  return FS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuning of launch shapes
+// ---------------------------------------------------------------------------
+std::string Engine::tune_key() const {
+  hipDeviceProp_t prop;
+  std::string arch = "unknown";
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    arch = prop.gcnArchName;
+    size_t c = arch.find(':');
+    if (c != std::string::npos) arch = arch.substr(0, c);
+  }
+  bool k2 = false;
+  for (const auto& I : inst) k2 = k2 || I.prm.order == 3;
+  int lg = 0;
+  while ((1LL << (lg + 1)) <= (long long)std::max(1, dm.nn)) ++lg;
+  std::ostringstream k;
+  k << arch << ":" << (precision == FS_PREC_MIXED ? "mixed" : "f64") << ":" << (k2 ? "k2" : "k1") << ":n" << lg;
+  return k.str();
+}
+
+int Engine::autotune(double* seconds) {
+  if (steps_run_ < 1) return set_error(FS_ERR_STATE, "autotune needs one completed step (run a step first)");
+  auto t_start = std::chrono::steady_clock::now();
+  int rc = sync();
+  if (rc) return rc;
+  fsk::DevState s = ds;  // controls of the last step's first Newton iteration
+  const int key = step_key() < 0 ? 0 : step_key();
+  fsk::ColorRanges cr{color_off.data(), color_nback.data()};
+  const int blocks[] = {64, 128, 256, 512};
+  hipStream_t st = stream;
+  // mean device time of `reps` launches of `f` after one untimed launch
+  auto time_it = [&](auto&& f, int reps, float* ms) -> int {
+    f();
+    FSE_HIP(hipEventRecord(ev_[6], st));
+    for (int r = 0; r < reps; ++r) f();
+    FSE_HIP(hipEventRecord(ev_[7], st));
+    FSE_HIP(hipEventSynchronize(ev_[7]));
+    FSE_HIP(hipGetLastError());
+    float t = 0;
+    FSE_HIP(hipEventElapsedTime(&t, ev_[6], ev_[7]));
+    *ms = t / reps;
+    return FS_OK;
+  };
+  fsk::Tuning& tu = dm.tune;
+  float best, t;
+  if (precision == FS_PREC_MIXED) {
+    best = 1e30f;
+    fsk::Tuning keep = tu;
+    for (int pre : {4, 6, 8})
+      for (int b : blocks) {
+        tu.gs32_pre = pre;
+        tu.gs32_block = b;
+        if ((rc = time_it([&] { fsk::launch_gs32_sweeps(dm, s, cr, 2, st); }, 2, &t))) return rc;
+        if (t < best) { best = t; keep.gs32_pre = pre; keep.gs32_block = b; }
+      }
+    tu.gs32_pre = keep.gs32_pre;
+    tu.gs32_block = keep.gs32_block;
+  } else {
+    best = 1e30f;
+    fsk::Tuning keep = tu;
+    for (int lanes : {4, 1})
+      for (int b : blocks) {
+        tu.gs_lanes = lanes;
+        tu.gs_block = b;
+        if ((rc = time_it([&] { fsk::launch_gs_sweeps(dm, s, cr, 2, st); }, 2, &t))) return rc;
+        if (t < best) { best = t; keep.gs_lanes = lanes; keep.gs_block = b; }
+      }
+    tu.gs_lanes = keep.gs_lanes;
+    tu.gs_block = keep.gs_block;
+  }
+  const int nflags = fsk::NODE_EVCON | fsk::NODE_INTERIOR | fsk::NODE_BOUNDARY | fsk::NODE_RHS | fsk::NODE_JAC;
+  int keepb = tu.node_block;
+  best = 1e30f;
+  for (int b : blocks) {
+    tu.node_block = b;
+    if (precision == FS_PREC_MIXED)
+      rc = time_it([&] { fsk::launch_node_mixed(dm, s, st); }, 2, &t);
+    else
+      rc = time_it([&] { fsk::launch_node(dm, s, nflags, st); }, 2, &t);
+    if (rc) return rc;
+    if (t < best) { best = t; keepb = b; }
+  }
+  tu.node_block = keepb;
+  keepb = tu.face_block;
+  best = 1e30f;
+  for (int b : blocks) {
+    tu.face_block = b;
+    if (precision == FS_PREC_MIXED)
+      rc = time_it([&] { fsk::launch_edge_flux(dm, s, 0, st); fsk::launch_edge_jac32(dm, s, st); }, 2, &t);
+    else
+      rc = time_it([&] { fsk::launch_edge_flux(dm, s, 1, st); }, 2, &t);
+    if (rc) return rc;
+    if (t < best) { best = t; keepb = b; }
+  }
+  tu.face_block = keepb;
+  keepb = tu.cell_block;
+  best = 1e30f;
+  for (int b : blocks) {
+    tu.cell_block = b;
+    rc = time_it([&] {
+      if (key & 1) fsk::launch_gradient(dm, s, st);
+      if (key & 2) fsk::launch_derivatives(dm, s, st);
+      fsk::launch_limiter(dm, s, st);
+      if (precision == FS_PREC_MIXED) fsk::launch_diag_lu32(dm, s, st);
+      else fsk::launch_diag_lu(dm, s, st);
+    }, 2, &t);
+    if (rc) return rc;
+    if (t < best) { best = t; keepb = b; }
+  }
+  tu.cell_block = keepb;
+  // captured graphs hold the old launch shapes
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+  // record the choice
+  std::string path = tune_cache_path();
+  if (!path.empty()) {
+    size_t slash = path.rfind('/');
+    if (slash != std::string::npos) {
+      std::string dir = path.substr(0, slash);
+      for (size_t i = 1; i <= dir.size(); ++i)
+        if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+    }
+    std::ofstream out(path, std::ios::app);
+    if (out)
+      out << tune_key() << " " << tu.gs_block << " " << tu.gs_lanes << " " << tu.gs32_block << " " << tu.gs32_pre << " "
+          << tu.node_block << " " << tu.face_block << " " << tu.cell_block << "\n";
+  }
+  if (seconds)
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   return FS_OK;
 }
 

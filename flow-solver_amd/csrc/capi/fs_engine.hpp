@@ -109,6 +109,21 @@ class Engine {
 
   size_t device_bytes() const { return bytes_; }
 
+  // Times every candidate launch shape of the tunable kernels on this
+  // engine's own data and keeps the fastest (coordinate search; the engine
+  // must have run a step, and the next step recomputes everything the trial
+  // launches overwrite).  The choice is stored in a cache file keyed by
+  // (arch, precision, k=2 present, log2 of the node count) and applied by
+  // later engines with the same key at creation.
+  int autotune(double* seconds);
+  bool tuned_from_cache() const { return tuned_from_cache_; }
+  int steps_run() const { return steps_run_; }
+  void tuning_values(int* v) const {
+    const fsk::Tuning& t = dm.tune;
+    v[0] = t.gs_block; v[1] = t.gs_lanes; v[2] = t.gs32_block; v[3] = t.gs32_pre;
+    v[4] = t.node_block; v[5] = t.face_block; v[6] = t.cell_block;
+  }
+
   // state
   int precision = FS_PREC_F64;
   int subits = 0, iter = 1, jufreq = 1;
@@ -146,6 +161,9 @@ class Engine {
   int check_errors(bool norms);
   int enqueue_readback();
   bool pending_ = false;
+  bool tuned_from_cache_ = false;
+  int steps_run_ = 0;
+  std::string tune_key() const;
 };
 
 }  // namespace fse

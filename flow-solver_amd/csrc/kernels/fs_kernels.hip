@@ -29,7 +29,7 @@ __device__ __forceinline__ double dmax(double x, double y) { return x >= y ? x :
 // ---------------------------------------------------------------------------
 // K2: least-squares gradient, node-centric gather (gradient.C:363-403)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, DevState s) {
+__global__ void __launch_bounds__(kMaxBlock) k_gradient(DevMesh m, DevState s) {
   int n = xcd_block() * blockDim.x + threadIdx.x;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient(DevMesh m, DevState s) {
 // of a node read one contiguous run; the 20 partial sums are combined with a
 // butterfly over the 16 lanes.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_derivatives(DevMesh m, DevState s) {
+__global__ void __launch_bounds__(kMaxBlock) k_derivatives(DevMesh m, DevState s) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
   int i = tid >> 4, l = tid & 15;
   if (i >= m.nn) return;
@@ -159,7 +159,7 @@ __device__ __forceinline__ void candidate(bool venkat, double q, double qmin, do
 // without a limiter this iteration gets phi = 1 (solve.C:343-347).  Mode bits
 // (InstParams::lim): 1 limiter on, 2 Gauss points of the k=2 phase,
 // 4 Venkatakrishnan's function instead of Barth's.
-__global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, DevState s) {
+__global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_limiter(DevMesh m, DevState s) {
 // The FP64 parity pass: reference Roe operation order and the FD Jacobian
 // blocks (kJac = true is the only instantiation launched).
 template <bool kJac>
-__global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, DevState st) {
+__global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st) {
   int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) k_edge_flux(DevMesh m, DevState st) {
 // Residual-only face fluxes of the mixed path's FP64 outer residual, one
 // thread per subedge (characteristic-wave Roe flux, same reconstruction):
 // every face is independent, so no thread loops over an edge's subedges.
-__global__ void __launch_bounds__(kBlock) k_sub_flux(DevMesh m, DevState st) {
+__global__ void __launch_bounds__(kMaxBlock) k_sub_flux(DevMesh m, DevState st) {
   int s = xcd_block() * blockDim.x + threadIdx.x;
   if (s >= m.nsub) return;
   int e = m.sub_edge[s];
@@ -397,7 +397,7 @@ __global__ void __launch_bounds__(kBlock) k_bedge(DevMesh m, DevState s) {
 // ---------------------------------------------------------------------------
 // 4 lanes per node: lane v owns residual component v and row v of the
 // diagonal block, so the quad reads each 128-byte block contiguously.
-__global__ void __launch_bounds__(kBlock) k_node(DevMesh m, DevState s, int flags) {
+__global__ void __launch_bounds__(kMaxBlock) k_node(DevMesh m, DevState s, int flags) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(kBlock) k_zero_offdiag(DevMesh m, double* __re
 // identical); each color's rows are independent and run in parallel; sweeps
 // alternate forward and backward over the colors.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const InstParams* __restrict__ ip,
+__global__ void __launch_bounds__(kMaxBlock) k_diag_lu(DevMesh m, const InstParams* __restrict__ ip,
                                                     const double* __restrict__ LHS, double* __restrict__ LU,
                                                     double* __restrict__ dQ) {
   int t = xcd_block() * blockDim.x + threadIdx.x;
@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(kBlock) k_diag_lu(DevMesh m, const InstParams*
 // wave's 16 rows read block e of their rows as one contiguous 2 KB run of the
 // sliced layout), then every lane of the quad solves with the factored
 // diagonal.
-__global__ void __launch_bounds__(kBlock) k_gs_color(DevMesh m, const InstParams* __restrict__ ip,
+__global__ void __launch_bounds__(kMaxBlock) k_gs_color(DevMesh m, const InstParams* __restrict__ ip,
                                                      const double* __restrict__ LHS, const double* __restrict__ LU,
                                                      const double* __restrict__ RHS, double* __restrict__ dQ,
                                                      int row_begin, int row_end) {
@@ -541,6 +541,42 @@ __global__ void __launch_bounds__(kBlock) k_gs_color(DevMesh m, const InstParams
   double old = dQ[4 * (size_t)j + v];
   double dx = xv - old;
   dQ[4 * (size_t)j + v] = old + dx;
+}
+
+// The same sweep with one thread per row (all four components): the same
+// products summed in the same order as the 4-lane form.
+__global__ void __launch_bounds__(kMaxBlock) k_gs_row(DevMesh m, const InstParams* __restrict__ ip,
+                                                      const double* __restrict__ LHS, const double* __restrict__ LU,
+                                                      const double* __restrict__ RHS, double* __restrict__ dQ,
+                                                      int row_begin, int row_end) {
+  int t = row_begin + xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= row_end) return;
+  int j = m.color_rows[t];
+  int inst = inst_of(m, m.pos_inst, t);
+  int deg = m.pdeg[t];
+  size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
+  double r[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) r[v] = RHS[4 * (size_t)j + v];
+  if (!ip[inst].active) return;
+  for (int e = 0; e < deg; ++e) {
+    size_t k = base + 16 * (size_t)e;
+    const double* A = LHS + 16 * k;
+    const double* q = dQ + 4 * (size_t)m.ja[k];
+    double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) r[v] -= (A[4 * v] * q0 + A[4 * v + 1] * q1 + A[4 * v + 2] * q2 + A[4 * v + 3] * q3);
+  }
+  double Ld[16], X[4];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) Ld[k] = LU[16 * (size_t)t + k];
+  lu_solve4(Ld, r, X);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    double old = dQ[4 * (size_t)j + v];
+    double dx = X[v] - old;
+    dQ[4 * (size_t)j + v] = old + dx;
+  }
 }
 
 // nQ += dQ over the real nodes (solve.C:901-907)
@@ -627,22 +663,22 @@ void launch_copy(double* dst, const double* src, size_t n, hipStream_t st) {
 }
 void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_gradient, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s);
+  hipLaunchKernelGGL(k_gradient, dim3(nblocks(m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
 }
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_derivatives, dim3(nblocks(16 * m.nn)), dim3(kBlock), 0, st, m, s);
+  hipLaunchKernelGGL(k_derivatives, dim3(nblocks(16 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
 }
 void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_limiter, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s);
+  hipLaunchKernelGGL(k_limiter, dim3(nblocks(4 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
 }
 void launch_edge_flux(const DevMesh& m, const DevState& s, int jac_allowed, hipStream_t st) {
   if (m.ne == 0) return;
   if (jac_allowed)
-    hipLaunchKernelGGL(k_edge_flux<true>, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s);
+    hipLaunchKernelGGL(k_edge_flux<true>, dim3(nblocks(m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
   else if (m.nsub > 0)
-    hipLaunchKernelGGL(k_sub_flux, dim3(nblocks(m.nsub)), dim3(kBlock), 0, st, m, s);
+    hipLaunchKernelGGL(k_sub_flux, dim3(nblocks(m.nsub, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
 }
 void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nbe == 0) return;
@@ -650,7 +686,7 @@ void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 void launch_node(const DevMesh& m, const DevState& s, int flags, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_node, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s, flags);
+  hipLaunchKernelGGL(k_node, dim3(nblocks(4 * m.nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st, m, s, flags);
 }
 void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st) {
   size_t n = (size_t)m.nnz * 16;
@@ -677,7 +713,8 @@ int gs_schedule(int ncolor, int subits, GsLaunch* out) {
 
 void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_diag_lu, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.ip, s.LHS, s.LU, s.dQ);
+  hipLaunchKernelGGL(k_diag_lu, dim3(nblocks(m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s.ip, s.LHS,
+                     s.LU, s.dQ);
 }
 
 void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st) {
@@ -687,8 +724,12 @@ void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr
     int c = sched[i].color;
     int b = cr.off[c], e = sched[i].backward ? b + cr.nback[c] : cr.off[c + 1];
     if (e <= b) continue;
-    hipLaunchKernelGGL(k_gs_color, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.ip, s.LHS, s.LU, s.RHS,
-                       s.dQ, b, e);
+    const int bs = m.tune.gs_block;
+    if (m.tune.gs_lanes == 1)
+      hipLaunchKernelGGL(k_gs_row, dim3(nblocks(e - b, bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+    else
+      hipLaunchKernelGGL(k_gs_color, dim3(nblocks(4 * (e - b), bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS,
+                         s.dQ, b, e);
   }
 }
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st) {

@@ -38,8 +38,23 @@ struct InstParams {
 
 enum LimMode { LIM_ON = 1, LIM_GAUSS = 2, LIM_VENKAT = 4 };
 
+// Launch shapes chosen by the autotuner.  None of them changes what a
+// thread computes or the order it sums in, so every choice gives the same
+// results bit for bit.
+constexpr int kMaxBlock = 512;
+struct Tuning {
+  int gs_block = 256;     // FP64 sweep workgroup size
+  int gs_lanes = 4;       // FP64 sweep: 4 lanes per row (one per component) or 1 thread per row
+  int gs32_block = 256;   // FP32 sweep workgroup size
+  int gs32_pre = 6;       // FP32 sweep: row blocks whose loads are issued before the arithmetic (4, 6, 8)
+  int node_block = 256;   // node assembly
+  int face_block = 256;   // subedge / edge kernels
+  int cell_block = 256;   // gradient, limiter, derivatives, diagonal factorisation
+};
+
 struct DevMesh {
   int nn, ng, ne, nsub, nbe, nnz, ncolor, ninst;
+  Tuning tune;
   const int* inst_node;     // nn
   const int* inst_edge;     // ne
   const int* inst_bedge;    // nbe

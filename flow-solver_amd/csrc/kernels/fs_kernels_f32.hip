@@ -83,7 +83,7 @@ __device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, c
 // dF/dqL = 0.5(A(qL) + |A_roe|) len and dF/dqR = 0.5(A(qR) - |A_roe|) len at
 // the (limited) reconstructed face states; block (L,R) += dF/dqR and block
 // (R,L) -= dF/dqL (jacobian.C:612-658).  Zeros when jupdate is off.
-__global__ void __launch_bounds__(kBlock) k_edge_jac32(DevMesh m, DevState st) {
+__global__ void __launch_bounds__(kMaxBlock) k_edge_jac32(DevMesh m, DevState st) {
   int e = xcd_block() * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
   const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kBlock) k_edge_jac32(DevMesh m, DevState st) {
 // Node assembly of the mixed step: FP64 ev_con/dt and residual gather (same
 // order as the FP64 path), FP32 diagonal block V/dt I - sum of transposed
 // off-diagonals + boundary blocks, FP32 right-hand side -R.
-__global__ void __launch_bounds__(kBlock) k_node_mixed(DevMesh m, DevState s) {
+__global__ void __launch_bounds__(kMaxBlock) k_node_mixed(DevMesh m, DevState s) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
@@ -225,7 +225,7 @@ __device__ __forceinline__ void lu4f(float* A) {
   for (int k = 0; k < 4; k++) A[k * 4 + k] = 1.0f / A[k * 4 + k];
 }
 
-__global__ void __launch_bounds__(kBlock) k_diag_lu32(DevMesh m, const InstParams* __restrict__ ip,
+__global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstParams* __restrict__ ip,
                                                       const float* __restrict__ LHS32, float* __restrict__ LU32,
                                                       float* __restrict__ dQ32) {
   int t = xcd_block() * blockDim.x + threadIdx.x;
@@ -250,7 +250,8 @@ __global__ void __launch_bounds__(kBlock) k_diag_lu32(DevMesh m, const InstParam
 // 4 lanes per row (lane v: component v), quad-wide FP32 block rows of the
 // sliced layout; the row's index, block and neighbour loads are issued
 // before the arithmetic.
-__global__ void __launch_bounds__(kBlock) k_gs_color32(DevMesh m, const InstParams* __restrict__ ip,
+template <int kPre>
+__global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const InstParams* __restrict__ ip,
                                                        const float* __restrict__ LHS32, const float* __restrict__ LU32,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
                                                        int row_begin, int row_end) {
@@ -263,7 +264,6 @@ __global__ void __launch_bounds__(kBlock) k_gs_color32(DevMesh m, const InstPara
   size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
   float r = RHS32[4 * (size_t)j + v];
   if (!ip[inst].active) return;
-  constexpr int kPre = 6;  // 60 VGPRs: full occupancy
   int nb[kPre];
 #pragma unroll
   for (int i = 0; i < kPre; ++i) nb[i] = (i < deg) ? m.ja[base + 16 * i] : j;
@@ -311,15 +311,16 @@ __global__ void __launch_bounds__(kBlock) k_update_mixed(DevMesh m, const InstPa
 
 void launch_edge_jac32(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.ne == 0) return;
-  hipLaunchKernelGGL(k_edge_jac32, dim3(nblocks(m.ne)), dim3(kBlock), 0, st, m, s);
+  hipLaunchKernelGGL(k_edge_jac32, dim3(nblocks(m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
 }
 void launch_node_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_node_mixed, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s);
+  hipLaunchKernelGGL(k_node_mixed, dim3(nblocks(4 * m.nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st, m, s);
 }
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_diag_lu32, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.ip, s.LHS32, s.LU32, s.dQ32);
+  hipLaunchKernelGGL(k_diag_lu32, dim3(nblocks(m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s.ip,
+                     s.LHS32, s.LU32, s.dQ32);
 }
 
 void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st) {
@@ -329,8 +330,14 @@ void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& 
     int c = sched[i].color;
     int b = cr.off[c], e = sched[i].backward ? b + cr.nback[c] : cr.off[c + 1];
     if (e <= b) continue;
-    hipLaunchKernelGGL(k_gs_color32, dim3(nblocks(4 * (e - b))), dim3(kBlock), 0, st, m, s.ip, s.LHS32, s.LU32,
-                       s.RHS32, s.dQ32, b, e);
+    const int bs = m.tune.gs32_block;
+    const dim3 grid(nblocks(4 * (e - b), bs));
+    if (m.tune.gs32_pre <= 4)
+      hipLaunchKernelGGL(k_gs_color32<4>, grid, dim3(bs), 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b, e);
+    else if (m.tune.gs32_pre <= 6)
+      hipLaunchKernelGGL(k_gs_color32<6>, grid, dim3(bs), 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b, e);
+    else
+      hipLaunchKernelGGL(k_gs_color32<8>, grid, dim3(bs), 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b, e);
   }
 }
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {

@@ -27,6 +27,16 @@ _MESH_ARRAYS = [
     "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg",
 ]
 _FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS", "LHS32"]
+_TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block"]
+
+
+def _tuning(fn, h):
+    v = np.zeros(len(_TUNING), dtype=np.int32)
+    fc = C.c_int()
+    _check(fn(h, v.ctypes.data, C.byref(fc)))
+    out = dict(zip(_TUNING, (int(x) for x in v)))
+    out["from_cache"] = bool(fc.value)
+    return out
 
 
 class FlowSolverError(RuntimeError):
@@ -158,6 +168,10 @@ def lib():
         "fs_restart_write": (i, [vp, C.c_char_p, vp, i64, i, d, i]),
         "fs_mesh_restart_info": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(d), C.POINTER(i)]),
         "fs_solver_init_restart": (i, [vp]),
+        "fs_solver_autotune": (i, [vp, C.POINTER(d)]),
+        "fs_solver_get_tuning": (i, [vp, vp, C.POINTER(i)]),
+        "fs_batch_autotune": (i, [vp, C.POINTER(d)]),
+        "fs_batch_get_tuning": (i, [vp, vp, C.POINTER(i)]),
         "fs_mesh_create": (i, [i, vp, i, vp, i, vp, i, vp, vp, vp, C.POINTER(vp)]),
         "fs_roe_flux_batch_form": (i, [i, i, d, vp, vp, vp, vp, vp]),
     }
@@ -353,6 +367,16 @@ class Solver:
         """Run each pseudo-time step as one captured hipGraph launch."""
         _check(lib().fs_solver_use_graph(self._h, 1 if on else 0))
 
+    def autotune(self):
+        """Tune launch shapes on this solver's data (after a step); returns
+        the chosen shapes and the seconds spent."""
+        t = C.c_double()
+        _check(lib().fs_solver_autotune(self._h, C.byref(t)))
+        return dict(self.tuning(), seconds=t.value)
+
+    def tuning(self):
+        return _tuning(lib().fs_solver_get_tuning, self._h)
+
 
 def _field_count(info, field):
     return {"Q": 4 * (info["nn"] + info["ng"]), "nQ": 4 * (info["nn"] + info["ng"]), "dQ": 4 * info["nn"],
@@ -402,6 +426,14 @@ class Batch:
 
     def use_graph(self, on=True):
         _check(lib().fs_batch_use_graph(self._h, 1 if on else 0))
+
+    def autotune(self):
+        t = C.c_double()
+        _check(lib().fs_batch_autotune(self._h, C.byref(t)))
+        return dict(self.tuning(), seconds=t.value)
+
+    def tuning(self):
+        return _tuning(lib().fs_batch_get_tuning, self._h)
 
     def set_active(self, i, on):
         _check(lib().fs_batch_set_active(self._h, i, 1 if on else 0))

@@ -200,6 +200,19 @@ int fs_solver_time_steps(fs_solver* s, int ts_first, int nsteps, fs_timing* t);
  * the reference (its step is a sequence of CPU loops). */
 int fs_solver_use_graph(fs_solver* s, int on);
 
+/* Launch-shape autotuning (no counterpart in the reference).  Times every
+ * candidate workgroup size / lanes-per-row / sweep prefetch depth of the
+ * tunable kernels on the handle's own data (after at least one step) and
+ * keeps the fastest; none of the choices changes any result.  The choice is
+ * appended to the cache file (FS_TUNE_CACHE, default
+ * $HOME/.cache/flowsolver/tune_cache.txt) under (arch, precision, k=2
+ * present, log2 node count), and handles created later with the same key
+ * start from it.  values: gs_block, gs_lanes, gs32_block, gs32_pre,
+ * node_block, face_block, cell_block. */
+#define FS_TUNING_VALUES 7
+int fs_solver_autotune(fs_solver* s, double* seconds);
+int fs_solver_get_tuning(const fs_solver* s, int* values, int* from_cache);
+
 /* ---- batches: many independent steady solves (solve.C:44 each) packed
  *      CSR-style into one device image; every kernel of a pseudo-time step
  *      is one launch over all instances, optionally one hipGraph launch per
@@ -226,6 +239,8 @@ void fs_batch_destroy(fs_batch* b);
 int fs_batch_get_info(const fs_batch* b, fs_batch_info* info);
 int fs_batch_init_freestream(fs_batch* b);
 int fs_batch_use_graph(fs_batch* b, int on);
+int fs_batch_autotune(fs_batch* b, double* seconds);
+int fs_batch_get_tuning(const fs_batch* b, int* values, int* from_cache);
 /* convergence mask: a masked-off instance is skipped by every kernel */
 int fs_batch_set_active(fs_batch* b, int i, int active);
 /* one pseudo-time step of every active instance (each step counter advances);

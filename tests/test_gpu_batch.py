@@ -156,3 +156,37 @@ def test_restart_continuation_bitexact(tmp_path):
           (np.array_equal(out[True], q), np.max(np.abs(out[False] - q))))
     assert np.array_equal(out[True], q)
     assert np.all(np.isfinite(out[False]))
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED])
+def test_autotune_keeps_results_and_caches(precision, tmp_path, monkeypatch):
+    """Tuned launch shapes give the same states bit for bit; the choice is
+    written to the cache and picked up by the next engine with the same key."""
+    cache = tmp_path / "tune_cache.txt"
+    monkeypatch.setenv("FS_TUNE_CACHE", str(cache))
+    meshes, params = _build(precision)
+
+    def run(tune):
+        b = fsm.Batch(meshes, params, precision)
+        b.use_graph(True)
+        b.init_freestream()
+        b.step()
+        info = b.autotune() if tune else b.tuning()
+        for _ in range(8):
+            b.step()
+        q = [b.get(i, "Q") for i in range(len(meshes))]
+        b.close()
+        return q, info
+
+    q0, t0 = run(False)
+    assert not cache.exists() or cache.read_text() == ""
+    q1, t1 = run(True)
+    print("tuned shapes", t1, flush=True)
+    assert cache.exists() and len(cache.read_text().splitlines()) == 1
+    for a, b in zip(q0, q1):
+        assert np.array_equal(a, b)
+    b = fsm.Batch(meshes, params, precision)
+    t2 = b.tuning()
+    b.close()
+    assert t2["from_cache"]
+    assert {k: t2[k] for k in t2 if k != "from_cache"} == {k: t1[k] for k in t2 if k != "from_cache"}
