@@ -112,6 +112,10 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="length of the request stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.5, help="offered requests per second")
     ap.add_argument("--serve-max-steps", type=int, default=2000)
+    ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-solve-at-a-time FP64 baseline")
+    ap.add_argument("--batch-precisions", default="fp64,mixed")
+    ap.add_argument("--sections", default="config2,batch,serving,cpu",
+                    help="parts to run (a profiled run picks one; the headline needs config2)")
     return ap.parse_args()
 
 
@@ -142,7 +146,10 @@ def bench_batch(a):
            "controls": dict(WORKLOAD_CONTROLS, tol=1e-10), "graph": True}
     warm = max(p.foits + p.soits for p in params)
     rho = {}
+    wanted = a.batch_precisions.split(",")
     for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+        if name not in wanted:
+            continue
         b = fsm.Batch(meshes, params, prec)
         b.use_graph(True)
         b.init_freestream()
@@ -172,7 +179,11 @@ def bench_batch(a):
                              "solves_per_s": float(conv.sum()) / wall}
             b.close()
         out[name] = res
+    if "fp64" not in out or "mixed" not in out:
+        return out
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if a.no_batch_plain:
+        return out
     # the plain FP64 GPU path: the same solves one at a time (no batching),
     # each timed alone over steps of its final phase
     plain = plain_sequential(fsm, meshes, params, fsm.FS_PREC_F64, warm, 4)
@@ -193,7 +204,7 @@ def bench_batch(a):
         out["plain_fp64_time_to_solution_s_estimated"] = t_plain
         out["time_to_solution_speedup_vs_plain_fp64"] = t_plain / out["mixed"]["steady"]["wall_s"]
     for k in ("fp64", "mixed"):
-        if "steady" in out[k]:
+        if k in out and "steady" in out[k]:
             out[k]["steady"].pop("steps_per_instance", None)
     out["plain_fp64_sequential"].pop("ms_per_step_per_instance", None)
     return out
@@ -310,6 +321,14 @@ def main():
     import fsutil as U
 
     fsm = U.fsm
+    sections = set(a.sections.split(","))
+    if "config2" not in sections:
+        line = {"metric": "edge-flux evals/s", "value": None, "unit": "edge-flux evals/s", "n_gpus": 1,
+                "note": "headline section not run (--sections %s)" % a.sections}
+        if "batch" in sections and a.batch > 0:
+            line["batch"] = bench_batch(a)
+        print(json.dumps(line), flush=True)
+        return
     p = fsm.Params.make(nsteps=a.warmup + a.steps, iter=1, jufreq=1, CFL=50.0, ramp=20, gamma=1.4, alpha=0.0,
                         mach=0.675, subits=20, method=1, order=3, foits=2, soits=2, limit=1, reorder=1)
     t0 = time.perf_counter()
@@ -363,13 +382,13 @@ def main():
                      "kernel_ms": {"edge_flux_f64": tm["edge_ms"], "node_assembly": tm["node_ms"],
                                    "gauss_seidel_f32": tm["gs_ms"], "diag_lu_f32": tm["lu_ms"],
                                    "other": tm["other_ms"]}}
-    if a.batch > 0:
+    if "batch" in sections and a.batch > 0:
         line["batch"] = bench_batch(a)
-    if a.serve_seconds > 0:
+    if "serving" in sections and a.serve_seconds > 0:
         sv = {"mixed": bench_serving(a, fsm.FS_PREC_MIXED), "fp64": bench_serving(a, fsm.FS_PREC_F64)}
         sv["controls"] = dict(WORKLOAD_CONTROLS, tol=1e-10, max_steps=a.serve_max_steps)
         line["serving"] = sv
-    if not a.no_cpu_baseline:
+    if "cpu" in sections and not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(U, fsm, p, a.nx, a.ny, a.cpu_steps)
     print(json.dumps(line), flush=True)
 
