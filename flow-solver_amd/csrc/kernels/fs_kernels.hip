@@ -68,58 +68,72 @@ __global__ void __launch_bounds__(kMaxBlock) k_gradient(DevMesh m, DevState s) {
 
 // ---------------------------------------------------------------------------
 // K3: k=2 derivatives, sol = W' (q_j - q_i) over the 2-ring stencil
-// (compute_derivatives.C:1438-1521).  16 lanes per node: lane l takes stencil
-// entries l, l+16, ...; W' is stored entry-major ([entry][5]) so the 16 lanes
-// of a node read one contiguous run; the 20 partial sums are combined with a
-// butterfly over the 16 lanes.
+// (compute_derivatives.C:1438-1521).  16 lanes per node: lane v + 4p takes
+// variable v over the stencil entries p, p+4, ...; the 4 partial sums of
+// each moment are combined with a fixed butterfly (xor 4, then xor 8).  W' is
+// stored entry-major ([entry][5]), so the lanes of a node read 4 consecutive
+// entries per step and the 4 variable lanes of an entry share its cache line.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMaxBlock) k_derivatives(DevMesh m, DevState s) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
-  int i = tid >> 4, l = tid & 15;
+  int i = tid >> 4, v = tid & 3, p = (tid >> 2) & 3;
   if (i >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, i)];
   if (!ip.active || !ip.k2) return;
   const double* __restrict__ nQ = s.nQ;
-  double* __restrict__ hess = s.hess;
-  double* __restrict__ pv = s.pv;
-  int o = m.st_off[i], mm = m.st_off[i + 1] - o;
-  double qi[4];
+  const int* __restrict__ st_nodes = m.st_nodes;
+  const double* __restrict__ Wt = m.st_Wt;
+  const int o = m.st_off[i], e = m.st_off[i + 1];
+  const double qi = nQ[4 * (size_t)i + v];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+  // the first kHead entries of this lane: all index and state loads are
+  // issued before the first product, then the rest of a long stencil
+  constexpr int kHead = 5;
+  int nd[kHead];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) qi[v] = nQ[4 * (size_t)i + v];
-  double acc[4][5];
+  for (int k = 0; k < kHead; ++k) {
+    int j = o + p + 4 * k;
+    nd[k] = (j < e) ? st_nodes[j] : -1;
+  }
+  double d[kHead];
 #pragma unroll
-  for (int v = 0; v < 4; ++v)
+  for (int k = 0; k < kHead; ++k) d[k] = (nd[k] >= 0) ? nQ[4 * (size_t)nd[k] + v] - qi : 0.0;
 #pragma unroll
-    for (int r = 0; r < 5; ++r) acc[v][r] = 0.0;
-  for (int j = l; j < mm; j += 16) {
-    int node = m.st_nodes[o + j];
-    const double* w = m.st_Wt + 5 * ((size_t)o + j);
-    double d[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) d[v] = nQ[4 * (size_t)node + v] - qi[v];
-#pragma unroll
-    for (int r = 0; r < 5; ++r) {
-      double wr = w[r];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v][r] += wr * d[v];
-    }
+  for (int k = 0; k < kHead; ++k) {
+    if (nd[k] < 0) break;
+    const double* w = Wt + 5 * (size_t)(o + p + 4 * k);
+    a0 += w[0] * d[k];
+    a1 += w[1] * d[k];
+    a2 += w[2] * d[k];
+    a3 += w[3] * d[k];
+    a4 += w[4] * d[k];
+  }
+  for (int j = o + p + 4 * kHead; j < e; j += 4) {
+    const double dj = nQ[4 * (size_t)st_nodes[j] + v] - qi;
+    const double* w = Wt + 5 * (size_t)j;
+    a0 += w[0] * dj;
+    a1 += w[1] * dj;
+    a2 += w[2] * dj;
+    a3 += w[3] * dj;
+    a4 += w[4] * dj;
   }
 #pragma unroll
-  for (int off = 8; off > 0; off >>= 1)
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-      for (int r = 0; r < 5; ++r) acc[v][r] += __shfl_xor(acc[v][r], off, 16);
+  for (int off = 4; off <= 8; off <<= 1) {
+    a0 += __shfl_xor(a0, off, 16);
+    a1 += __shfl_xor(a1, off, 16);
+    a2 += __shfl_xor(a2, off, 16);
+    a3 += __shfl_xor(a3, off, 16);
+    a4 += __shfl_xor(a4, off, 16);
+  }
+  if (p != 0) return;
+  double* __restrict__ h = s.hess + 20 * (size_t)i + 5 * v;
+  h[0] = a0;
+  h[1] = a1;
+  h[2] = a2;
+  h[3] = a3;
+  h[4] = a4;
   const double* Mi = m.moments + 9 * (size_t)i;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    if (l == v) {
-#pragma unroll
-      for (int r = 0; r < 5; ++r) hess[20 * (size_t)i + 5 * v + r] = acc[v][r];
-      pv[4 * (size_t)i + v] = qi[v] - Mi[0] * acc[v][0] - Mi[1] * acc[v][1] - Mi[2] * acc[v][2] * 0.5 -
-                              Mi[3] * acc[v][3] * 0.5 - Mi[4] * acc[v][4];
-    }
-  }
+  s.pv[4 * (size_t)i + v] = qi - Mi[0] * a0 - Mi[1] * a1 - Mi[2] * a2 * 0.5 - Mi[3] * a3 * 0.5 - Mi[4] * a4;
 }
 
 // ---------------------------------------------------------------------------
@@ -130,21 +144,17 @@ __global__ void __launch_bounds__(kMaxBlock) k_derivatives(DevMesh m, DevState s
 // ---------------------------------------------------------------------------
 // One face-point candidate: r = (Qmax - q)/(qf - q) above, (Qmin - q)/(qf - q)
 // below; Barth takes min(1, r) (limiter.C:154-246), Venkatakrishnan
-// (r^2 + 2r)/(r^2 + r + 2) (limiter.C:696-730).
+// (r^2 + 2r)/(r^2 + r + 2) (limiter.C:696-730).  The numerator is selected
+// first so a wave divides once per candidate whichever side its lanes take;
+// the quotient is the same operation on the same operands as in the
+// two-branch form.
 template <bool kVenkat>
 __device__ __forceinline__ void limiter_candidate(double q, double qmin, double qmax, double qf, double* ph) {
-  double d = qf - q, t;
-  if (d > 0.) {
-    t = (qmax - q) / (qf - q);
-    t = kVenkat ? (t * t + 2. * t) / (t * t + t + 2.) : dmin(1.0, t);
-    *ph = dmin(*ph, t);
-  } else if (d < 0.) {
-    t = (qmin - q) / (qf - q);
-    t = kVenkat ? (t * t + 2. * t) / (t * t + t + 2.) : dmin(1.0, t);
-    *ph = dmin(*ph, t);
-  } else {
-    *ph = dmin(*ph, 1.0);
-  }
+  const double d = qf - q;
+  const double num = (d > 0.) ? (qmax - q) : (qmin - q);
+  double t = num / d;
+  t = kVenkat ? (t * t + 2. * t) / (t * t + t + 2.) : dmin(1.0, t);
+  *ph = dmin(*ph, (d > 0. || d < 0.) ? t : 1.0);
 }
 
 __device__ __forceinline__ void candidate(bool venkat, double q, double qmin, double qmax, double qf, double* ph) {
@@ -154,93 +164,114 @@ __device__ __forceinline__ void candidate(bool venkat, double q, double qmin, do
     limiter_candidate<false>(q, qmin, qmax, qf, ph);
 }
 
-// 4 lanes per node, lane v limits variable v; the node scalar is the min over
-// the 4 lanes taken in variable order (limiter.C:410-420).  An instance
-// without a limiter this iteration gets phi = 1 (solve.C:343-347).  Mode bits
-// (InstParams::lim): 1 limiter on, 2 Gauss points of the k=2 phase,
-// 4 Venkatakrishnan's function instead of Barth's.
+// 16 lanes per node: lane v + 4p limits variable v over the node's edges and
+// face points p, p+4, ...; min and max are exact, so combining the 4 parts
+// and then the 4 variables (limiter.C:410-420) with shuffles gives the
+// sequential loops' values.  An instance without a limiter this iteration
+// gets phi = 1 (solve.C:343-347).  Mode bits (InstParams::lim): 1 limiter
+// on, 2 Gauss points of the k=2 phase, 4 Venkatakrishnan's function instead
+// of Barth's.
 __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
-  int n = tid >> 2, v = tid & 3;
+  int n = tid >> 4, v = tid & 3, p = (tid >> 2) & 3;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
   if (!ip.active) return;
   const int lim = ip.lim;
   double* __restrict__ phi = s.phi;
   if (!(lim & LIM_ON)) {
-    if (v == 0) phi[n] = 1.0;
+    if ((tid & 15) == 0) phi[n] = 1.0;
     return;
   }
   const double* __restrict__ nQ = s.nQ;
-  const double* __restrict__ grad = s.grad;
-  const double* __restrict__ hess = s.hess;
-  const double* __restrict__ pv = s.pv;
-  double q = nQ[4 * (size_t)n + v];
+  const double q = nQ[4 * (size_t)n + v];
   double qmin = q, qmax = q, ph = 1.0;
-  for (int k = m.nedge_off[n]; k < m.nedge_off[n + 1]; ++k) {
-    int other = m.inc_other[k] & 0x7fffffff;
-    double o = nQ[4 * (size_t)other + v];
-    qmin = dmin(qmin, o);
-    qmax = dmax(qmax, o);
+  const int eb = m.nedge_off[n], ee = m.nedge_off[n + 1];
+  // two edges per lane and pass, both loads in flight together
+  for (int k = eb + p; k < ee; k += 8) {
+    const bool two = k + 4 < ee;
+    int o0 = m.inc_other[k] & 0x7fffffff;
+    int o1 = two ? (m.inc_other[k + 4] & 0x7fffffff) : o0;
+    double q0 = nQ[4 * (size_t)o0 + v], q1 = nQ[4 * (size_t)o1 + v];
+    qmin = dmin(dmin(qmin, q0), q1);
+    qmax = dmax(dmax(qmax, q0), q1);
   }
-  double xn0 = m.x[2 * (size_t)n], xn1 = m.x[2 * (size_t)n + 1];
+#pragma unroll
+  for (int off = 4; off <= 8; off <<= 1) {
+    qmin = dmin(qmin, __shfl_xor(qmin, off, 16));
+    qmax = dmax(qmax, __shfl_xor(qmax, off, 16));
+  }
+  const double xn0 = m.x[2 * (size_t)n], xn1 = m.x[2 * (size_t)n + 1];
   const bool venkat = lim & LIM_VENKAT;
+  const int sb = m.nsub_off[n], se = m.nsub_off[n + 1];
   if (lim & LIM_GAUSS) {
-    const double* h = hess + 20 * (size_t)n + 5 * v;
-    double h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4];
-    double p0 = pv[4 * (size_t)n + v];
-    for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
-      for (int g = 0; g < 3; ++g) {
-        double dx0 = m.inc_xg[6 * (size_t)k + 2 * g] - xn0;
-        double dx1 = m.inc_xg[6 * (size_t)k + 2 * g + 1] - xn1;
-        double qf = p0;
-        qf += (h0 * dx0 + h1 * dx1 + h2 * dx0 * dx0 * 0.5 + h3 * dx1 * dx1 * 0.5 + h4 * dx0 * dx1);
-        candidate(venkat, q, qmin, qmax, qf, &ph);
+    const double* h = s.hess + 20 * (size_t)n + 5 * v;
+    const double h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4];
+    const double p0 = s.pv[4 * (size_t)n + v];
+    // up to three face incidences per lane and pass, their Gauss points
+    // loaded before the first candidate
+    constexpr int kU = 3;
+    for (int k0 = sb + p; k0 < se; k0 += 4 * kU) {
+      double xg[kU][6];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int k = k0 + 4 * u;
+        const double* src = m.inc_xg + 6 * (size_t)(k < se ? k : k0);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) xg[u][c] = src[c];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (k0 + 4 * u >= se) break;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          double dx0 = xg[u][2 * g] - xn0;
+          double dx1 = xg[u][2 * g + 1] - xn1;
+          double qf = p0;
+          qf += (h0 * dx0 + h1 * dx1 + h2 * dx0 * dx0 * 0.5 + h3 * dx1 * dx1 * 0.5 + h4 * dx0 * dx1);
+          candidate(venkat, q, qmin, qmax, qf, &ph);
+        }
       }
     }
   } else {
     // face points of the linear reconstruction (the limiter runs only in
     // the gradient phase of k=1)
-    double g0 = grad[8 * (size_t)n + 2 * v];
-    double g1 = grad[8 * (size_t)n + 2 * v + 1];
-    for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
+    const double g0 = s.grad[8 * (size_t)n + 2 * v];
+    const double g1 = s.grad[8 * (size_t)n + 2 * v + 1];
+    for (int k = sb + p; k < se; k += 4) {
       double qf = q;
       double dx0 = m.inc_fc[2 * (size_t)k] - xn0, dx1 = m.inc_fc[2 * (size_t)k + 1] - xn1;
       qf += (g0 * dx0 + g1 * dx1);
       candidate(venkat, q, qmin, qmax, qf, &ph);
     }
   }
-  double p0 = __shfl(ph, 0, 4), p1 = __shfl(ph, 1, 4), p2 = __shfl(ph, 2, 4), p3 = __shfl(ph, 3, 4);
-  double t = p0;
-  t = dmin(t, p1);
-  t = dmin(t, p2);
-  t = dmin(t, p3);
-  if (v == 0) phi[n] = t;
+#pragma unroll
+  for (int off = 1; off <= 8; off <<= 1) ph = dmin(ph, __shfl_xor(ph, off, 16));
+  if ((tid & 15) == 0) phi[n] = ph;
 }
 
 // ---------------------------------------------------------------------------
 // K5+K7: edge-centric face fluxes and forward-difference Jacobian blocks
 // (residual.C:112-333 with Reconstruct, reconstruct.C:181-205, and
-// compute_numerical_jacobian_roe, jacobian.C:45-185).  One thread per edge
-// handles its 1-2 subedges in ascending order, stores each subedge's flux and
-// owns the edge's two off-diagonal blocks outright (no atomics).
+// compute_numerical_jacobian_roe, jacobian.C:45-185).  4 lanes per edge:
+// every lane evaluates the face flux of the edge's 1-2 subedges in ascending
+// order, lane c stores flux component c and perturbs state component c, so
+// it owns column c of the edge's two off-diagonal blocks (no atomics).  Each
+// block entry is accumulated over the subedges in the reference's order.
 // ---------------------------------------------------------------------------
-// The FP64 parity pass: reference Roe operation order and the FD Jacobian
-// blocks (kJac = true is the only instantiation launched).
-template <bool kJac>
 __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st) {
-  int e = xcd_block() * blockDim.x + threadIdx.x;
+  int tid = xcd_block() * blockDim.x + threadIdx.x;
+  int e = tid >> 2, c = tid & 3;
   if (e >= m.ne) return;
   const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
   if (!ip.active) return;
   const int recon = ip.recon_flux;
-  const bool jupdate = kJac && ip.jac;
+  const bool jupdate = ip.jac;
   const double gamma = ip.gamma;
   const double* __restrict__ nQ = st.nQ;
   const double* __restrict__ grad = st.grad;
   const double* __restrict__ phi = st.phi;
   double* __restrict__ inc_flux = st.inc_flux;
-  double* __restrict__ LHS = st.LHS;
   const double eps = 1.0e-08;
   int L = m.edges[2 * e], R = m.edges[2 * e + 1];
   double qL0[4], qR0[4];
@@ -249,18 +280,19 @@ __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st)
     qL0[j] = nQ[4 * (size_t)L + j];
     qR0[j] = nQ[4 * (size_t)R + j];
   }
-  double offLR[16], offRL[16];
+  double colLR[4], colRL[4];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) { offLR[k] = 0.0; offRL[k] = 0.0; }
+  for (int j = 0; j < 4; ++j) { colLR[j] = 0.0; colRL[j] = 0.0; }
   for (int t = 0; t < 2; ++t) {
     int s = m.edge_sub[2 * e + t];
     if (s < 0) break;
     double nx = m.sub_n[3 * (size_t)s], ny = m.sub_n[3 * (size_t)s + 1], len = m.sub_n[3 * (size_t)s + 2];
+    const size_t kL = (size_t)m.sub_inc[2 * (size_t)s], kR = (size_t)m.sub_inc[2 * (size_t)s + 1];
     double ql[4], qr[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) { ql[j] = qL0[j]; qr[j] = qR0[j]; }
     if (recon == 1) {
-      double fc0 = m.inc_fc[2 * (size_t)m.sub_inc[2 * (size_t)s]], fc1 = m.inc_fc[2 * (size_t)m.sub_inc[2 * (size_t)s] + 1];
+      double fc0 = m.inc_fc[2 * kL], fc1 = m.inc_fc[2 * kL + 1];
       double dxl0 = fc0 - m.x[2 * (size_t)L], dxl1 = fc1 - m.x[2 * (size_t)L + 1];
       double dxr0 = fc0 - m.x[2 * (size_t)R], dxr1 = fc1 - m.x[2 * (size_t)R + 1];
       double phiL = phi[L], phiR = phi[R];
@@ -272,43 +304,33 @@ __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st)
     }
     double f[4];
     roe_flux(nx, ny, len, gamma, ql, qr, f);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      // R[L] += f and R[R] -= f become additions of +f and -f (bit-identical)
-      inc_flux[4 * (size_t)m.sub_inc[2 * (size_t)s] + j] = f[j];
-      inc_flux[4 * (size_t)m.sub_inc[2 * (size_t)s + 1] + j] = -f[j];
-    }
+    // R[L] += f and R[R] -= f become additions of +f and -f (bit-identical)
+    const double fcomp = (c == 0) ? f[0] : (c == 1) ? f[1] : (c == 2) ? f[2] : f[3];
+    inc_flux[4 * kL + c] = fcomp;
+    inc_flux[4 * kR + c] = -fcomp;
     if (jupdate) {
+      double qp[4], fp[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        double qp[4], fp[4];
+      for (int j = 0; j < 4; ++j) qp[j] = (j == c) ? ql[j] + eps : ql[j];
+      roe_flux(nx, ny, len, gamma, qp, qr, fp);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) qp[j] = ql[j];
-        qp[i] += eps;
-        roe_flux(nx, ny, len, gamma, qp, qr, fp);
+      for (int j = 0; j < 4; ++j) colRL[j] -= (fp[j] - f[j]) / eps;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) offRL[j * 4 + i] -= (fp[j] - f[j]) / eps;
-      }
+      for (int j = 0; j < 4; ++j) qp[j] = (j == c) ? qr[j] + eps : qr[j];
+      roe_flux(nx, ny, len, gamma, ql, qp, fp);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        double qp[4], fp[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) qp[j] = qr[j];
-        qp[i] += eps;
-        roe_flux(nx, ny, len, gamma, ql, qp, fp);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) offLR[j * 4 + i] += (fp[j] - f[j]) / eps;
-      }
+      for (int j = 0; j < 4; ++j) colLR[j] += (fp[j] - f[j]) / eps;
     }
   }
   // the edge owns its two off-diagonal blocks: written every FP64 pass
   // (zeros when the instance skips the Jacobian, as build_matrix clears
   // them), never by the mixed path's FP64 residual pass
-  if (kJac) {
-    double* blr = LHS + 16 * (size_t)m.edge_slot[2 * e];
-    double* brl = LHS + 16 * (size_t)m.edge_slot[2 * e + 1];
+  double* blr = st.LHS + 16 * (size_t)m.edge_slot[2 * e];
+  double* brl = st.LHS + 16 * (size_t)m.edge_slot[2 * e + 1];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) { blr[k] = offLR[k]; brl[k] = offRL[k]; }
+  for (int j = 0; j < 4; ++j) {
+    blr[4 * j + c] = colLR[j];
+    brl[4 * j + c] = colRL[j];
   }
 }
 
@@ -594,8 +616,11 @@ __global__ void __launch_bounds__(kBlock) k_copy(size_t n, double* __restrict__ 
 }
 
 // Deterministic per-instance sums of R^2 and dQ^2: one block per chunk of
-// kNormChunkNodes nodes (fixed tree order), then one thread per instance adds
-// its chunks in order.
+// kNormChunkNodes nodes (fixed tree order), then one wave per instance adds
+// its chunks, lane l taking chunks l, l+64, ... counted from the instance's
+// first chunk, followed by a fixed butterfly.  Both orders depend only on the
+// instance's own node count, so an instance's norms are the same alone and
+// in a batch.
 __global__ void __launch_bounds__(kBlock) k_norm_chunk(NormChunks c, const InstParams* __restrict__ ip,
                                                        const double* __restrict__ R, const double* __restrict__ dQ,
                                                        double* __restrict__ part) {
@@ -626,15 +651,22 @@ __global__ void __launch_bounds__(kBlock) k_norm_chunk(NormChunks c, const InstP
 
 __global__ void __launch_bounds__(kBlock) k_norm_inst(int ninst, NormChunks c, const InstParams* __restrict__ ip,
                                                       const double* __restrict__ part, double* __restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, l = threadIdx.x & 63;
   if (i >= ninst || !ip[i].active) return;
   double a = 0.0, d = 0.0;
-  for (int ch = c.inst_chunk0[i]; ch < c.inst_chunk0[i + 1]; ++ch) {
+  for (int ch = c.inst_chunk0[i] + l; ch < c.inst_chunk0[i + 1]; ch += 64) {
     a += part[2 * (size_t)ch];
     d += part[2 * (size_t)ch + 1];
   }
-  out[2 * (size_t)i] = a;
-  out[2 * (size_t)i + 1] = d;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    d += __shfl_xor(d, off, 64);
+  }
+  if (l == 0) {
+    out[2 * (size_t)i] = a;
+    out[2 * (size_t)i + 1] = d;
+  }
 }
 
 template <int kForm>
@@ -671,12 +703,12 @@ void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_limiter, dim3(nblocks(4 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
+  hipLaunchKernelGGL(k_limiter, dim3(nblocks(16 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
 }
 void launch_edge_flux(const DevMesh& m, const DevState& s, int jac_allowed, hipStream_t st) {
   if (m.ne == 0) return;
   if (jac_allowed)
-    hipLaunchKernelGGL(k_edge_flux<true>, dim3(nblocks(m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
+    hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(4 * m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
   else if (m.nsub > 0)
     hipLaunchKernelGGL(k_sub_flux, dim3(nblocks(m.nsub, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
 }
@@ -739,7 +771,7 @@ void launch_update(const DevMesh& m, const DevState& s, hipStream_t st) {
 void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipStream_t st) {
   if (c.nchunk > 0)
     hipLaunchKernelGGL(k_norm_chunk, dim3(c.nchunk), dim3(kBlock), 0, st, c, s.ip, s.R, s.dQ, s.chunk_part);
-  hipLaunchKernelGGL(k_norm_inst, dim3(nblocks(m.ninst)), dim3(kBlock), 0, st, m.ninst, c, s.ip, s.chunk_part,
+  hipLaunchKernelGGL(k_norm_inst, dim3(nblocks(64 * m.ninst)), dim3(kBlock), 0, st, m.ninst, c, s.ip, s.chunk_part,
                      s.inst_norms);
 }
 void launch_roe_batch(int n, int form, double gamma, const double* normals, const double* qL, const double* qR,
