@@ -197,6 +197,14 @@ int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t) {
   return FS_OK;
 }
 
+int fs_batch_body_forces(fs_batch* b, double* forces) {
+  if (!b || !forces) return fail(FS_ERR_ARG, "null argument");
+  fse::Engine& E = b->eng;
+  std::vector<int> citer(E.ninst());
+  for (int i = 0; i < E.ninst(); ++i) citer[i] = E.inst[i].ts;
+  return E.body_forces(citer.data(), forces);
+}
+
 int fs_mesh_footprint(const fs_mesh* m, int precision, int64_t* bytes) {
   if (!m || !bytes) return fail(FS_ERR_ARG, "null argument");
   if (!m->prepared) return fail(FS_ERR_ARG, "mesh not prepared");

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -448,6 +449,87 @@ int fs_roe_flux_batch_form(int n, int form, double gamma, const double* d_normal
     return fail(FS_ERR_ARG, "bad arguments");
   fsk::launch_roe_batch(n, form, gamma, d_normals, d_qL, d_qR, d_flux, (hipStream_t)stream);
   HIP_TRY(hipGetLastError());
+  return FS_OK;
+}
+
+int fs_body_forces(fs_solver* s, int citer, double* forces) {
+  if (!s || !forces) return fail(FS_ERR_ARG, "null argument");
+  return s->eng.body_forces(&citer, forces);
+}
+
+namespace {
+// write_surface_cp's nodal branch (grid_io.C:1270-1330): the unique nodes of
+// boundary b's segments in order of first appearance, Cp from Q.
+int surface_cp_values(fs_solver* s, int b, std::vector<double>* xs, std::vector<double>* cps) {
+  const fse::Instance& I = s->eng.inst[0];
+  const fs::Mesh& g = *I.mesh;
+  if (b < 0 || b >= g.nb) return fail(FS_ERR_ARG, "boundary out of range");
+  xs->clear();
+  cps->clear();
+  const int bc = g.bbc[b];
+  if (bc == 0 || (bc % 2 == 0)) return FS_OK;
+  std::vector<double> Q(4 * (size_t)(g.nn + g.ng));
+  int rc = s->eng.get_field(0, FS_F_Q, Q.data(), (int64_t)Q.size());
+  if (rc) return rc;
+  const fs_params& p = I.prm;
+  const double pinf = 1. / p.gamma, minf = p.mach;
+  std::vector<int> seen;
+  for (int sg = g.bseg_off[b]; sg < g.bseg_off[b + 1]; ++sg)
+    for (int e = 0; e < 2; ++e) {
+      const int n = g.bseg[2 * (size_t)sg + e];
+      if (std::find(seen.begin(), seen.end(), n) != seen.end()) continue;
+      seen.push_back(n);
+      const double* q = &Q[4 * (size_t)n];
+      const double r = q[0], u = q[1] / r, v = q[2] / r, en = q[3];
+      const double P = (p.gamma - 1.) * (en - 0.5 * r * (u * u + v * v));
+      xs->push_back(g.x[2 * (size_t)n]);
+      cps->push_back((P / pinf - 1.) * (2. / (p.gamma * minf * minf)));
+    }
+  return FS_OK;
+}
+}  // namespace
+
+int fs_surface_cp(fs_solver* s, int b, double* x, double* cp, int64_t cap, int64_t* count) {
+  if (!s || !count) return fail(FS_ERR_ARG, "null argument");
+  std::vector<double> xs, cps;
+  int rc = surface_cp_values(s, b, &xs, &cps);
+  if (rc) return rc;
+  *count = (int64_t)xs.size();
+  if (!x || !cp) return FS_OK;
+  if (cap < (int64_t)xs.size()) return fail(FS_ERR_ARG, "output arrays too small");
+  std::copy(xs.begin(), xs.end(), x);
+  std::copy(cps.begin(), cps.end(), cp);
+  return FS_OK;
+}
+
+int fs_write_surface_cp(fs_solver* s, const char* dir) {
+  if (!s || !dir) return fail(FS_ERR_ARG, "null argument");
+  const fs::Mesh& g = *s->eng.inst[0].mesh;
+  for (int b = 0; b < g.nb; ++b) {
+    const int bc = g.bbc[b];
+    if (bc == 0 || (bc % 2 == 0)) continue;
+    std::vector<double> xs, cps;
+    int rc = surface_cp_values(s, b, &xs, &cps);
+    if (rc) return rc;
+    // the reference prints "%f  %f" of the floats and sorts the file with
+    // `sort -g`: numeric order of the first field, then the whole line
+    std::vector<std::string> lines;
+    char buf[128];
+    for (size_t k = 0; k < xs.size(); ++k) {
+      std::snprintf(buf, sizeof buf, "%f  %f\n", (double)(float)xs[k], (double)(float)cps[k]);
+      lines.push_back(buf);
+    }
+    std::stable_sort(lines.begin(), lines.end(), [](const std::string& a, const std::string& c) {
+      double va = std::strtod(a.c_str(), nullptr), vc = std::strtod(c.c_str(), nullptr);
+      if (va != vc) return va < vc;
+      return a < c;
+    });
+    std::string path = std::string(dir) + "/surface_" + std::to_string(b + 1) + "_pressure.dat";
+    FILE* fp = std::fopen(path.c_str(), "w");
+    if (!fp) return fail(FS_ERR_IO, "cannot open " + path);
+    for (const auto& l : lines) std::fputs(l.c_str(), fp);
+    std::fclose(fp);
+  }
   return FS_OK;
 }
 

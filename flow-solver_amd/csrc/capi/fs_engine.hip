@@ -84,9 +84,9 @@ size_t footprint_bytes(const fs::Mesh& g, int prec) {
   size_t b = 0;
   // mesh image
   b += nn * (4 * 4 + 8 * 2 + 8 + 8 * 9 + 4 * 4 + 4 * 3) + ne * 4 * 6 + nsub * (8 * 3 + 4 * 2);
-  b += ninc * (8 * 12) + neinc * (4 + 16) + nbe * (4 * 4 + 8 * 3) + nnz * 8 + (nn + 1) * 4 * 4 + nst * (4 + 40);
+  b += ninc * (8 * 12) + neinc * (4 + 16) + nbe * (4 * 4 + 8 * 9) + nnz * 8 + (nn + 1) * 4 * 4 + nst * (4 + 40);
   // state
-  b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + ninc * 4 * 8 + nbe * 8 * 20;
+  b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + ninc * 4 * 8 + nbe * 8 * 22;
   if (prec == FS_PREC_MIXED)
     b += nnz * 16 * 4 + nn * (16 * 4 + 4 * 4 * 2);
   else
@@ -151,6 +151,8 @@ Engine::~Engine() {
   if (h_ip_) (void)hipHostFree(h_ip_);
   if (h_norms_) (void)hipHostFree(h_norms_);
   if (h_err_) (void)hipHostFree(h_err_);
+  if (h_fp_) (void)hipHostFree(h_fp_);
+  if (h_bforce_) (void)hipHostFree(h_bforce_);
   for (auto& b : bufs_) (void)hipFree(b.p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -218,7 +220,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   std::vector<int> sub_inc(2 * NS), inc_other(EINC), nbe_list(BINC), st_nodes(NST);
   std::vector<double> inc_n(4 * KINC), inc_fc(2 * KINC), inc_xg(6 * KINC), inc_gw(2 * EINC), st_Wt(5 * NST);
   std::vector<int> breal(NB), bbc(NB);
-  std::vector<double> bn(3 * NB);
+  std::vector<double> bn(3 * NB), bxg(6 * NB);
   long long kinc = 0, einc = 0, binc = 0, sto = 0;
   for (int i = 0; i < B; ++i) {
     const fs::Mesh& g = *meshes[i];
@@ -276,6 +278,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       breal[bo + b] = g.bedge[5 * (size_t)b] + no;
       bbc[bo + b] = g.bbc[g.bedge[5 * (size_t)b + 3]];
       for (int c = 0; c < 3; ++c) bn[3 * (size_t)(bo + b) + c] = g.bedge_n[3 * (size_t)b + c];
+      for (int c = 0; c < 6; ++c) bxg[6 * (size_t)(bo + b) + c] = g.bedge_xg[6 * (size_t)b + c];
     }
     // pseudo-inverse entry-major: Wt[(o+j)*5 + r] = W[5*o + r*m + j]
     for (int n = 0; n + 1 < (int)g.st_off.size(); ++n) {
@@ -403,7 +406,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = upload(&d.inc_xg, inc_xg)) || (rc = upload(&d.inc_other, inc_other)) ||
       (rc = upload(&d.inc_gw, inc_gw)) || (rc = upload(&d.nbe_off, nbe_off)) ||
       (rc = upload(&d.nbe_list, nbe_list)) || (rc = upload(&d.bedge_real, breal)) ||
-      (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.roff, roff)) ||
+      (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.bedge_xg, bxg)) || (rc = upload(&d.roff, roff)) ||
       (rc = upload(&d.tslot, tslot)) || (rc = upload(&d.ja, ja_d)) || (rc = upload(&d.iau, iau_d)) ||
       (rc = upload(&d.pdeg, pdeg)) || (rc = upload(&d.sbase, sbase)) ||
       (rc = upload(&d.color_rows, color_rows)) || (rc = upload(&d.st_off, st_off)) ||
@@ -420,6 +423,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = alloc(&st.phi, nn)) || (rc = alloc(&st.hess, nn * 20)) || (rc = alloc(&st.pv, nn * 4)) ||
       (rc = alloc(&st.ev_con, nn)) || (rc = alloc(&st.dt, nn)) || (rc = alloc(&st.inc_flux, (size_t)KINC * 4)) ||
       (rc = alloc(&st.bflux, (size_t)NB * 4)) || (rc = alloc(&st.bjac, (size_t)NB * 16)) ||
+      (rc = alloc(&st.bforce, (size_t)NB * 2)) || (rc = alloc(&d_fp_, (size_t)B)) ||
       (rc = alloc(&st.err, (size_t)B)) || (rc = alloc(&st.inst_norms, 2 * (size_t)B)) ||
       (rc = alloc(&st.chunk_part, 2 * (size_t)std::max(1, nc.nchunk))) ||
       (rc = alloc(&d_ip_, (size_t)iter * B)))
@@ -444,6 +448,9 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   FSE_HIP(hipHostMalloc((void**)&h_ip_, (size_t)iter * B * sizeof(fsk::InstParams), hipHostMallocDefault));
   FSE_HIP(hipHostMalloc((void**)&h_norms_, 2 * (size_t)B * sizeof(double), hipHostMallocDefault));
   FSE_HIP(hipHostMalloc((void**)&h_err_, (size_t)B * sizeof(int), hipHostMallocDefault));
+  FSE_HIP(hipHostMalloc((void**)&h_fp_, (size_t)B * sizeof(fsk::ForceParams), hipHostMallocDefault));
+  FSE_HIP(hipHostMalloc((void**)&h_bforce_, 2 * std::max<size_t>(1, (size_t)NB) * sizeof(double),
+                        hipHostMallocDefault));
   std::memset(h_ip_, 0, (size_t)iter * B * sizeof(fsk::InstParams));
   std::memset(h_err_, 0, (size_t)B * sizeof(int));
   fill(st.phi, nn, 1.0, stream);
@@ -680,6 +687,45 @@ int Engine::step_end() {
   ++steps_run_;
   FSE_HIP(hipStreamSynchronize(stream));
   return check_errors(true);
+}
+
+// Compute_Body_Forces (grid_io.C:1390-1697) of every instance from its
+// resident state; citer[i] (grid->citer) selects the boundary reconstruction
+// as Reconstruct_Gauss_Boundary does (reconstruct.C:1733-1770).  The device
+// evaluates the per-bedge terms; their sum runs here in the reference's
+// bedge order over the instance's walls (odd bc), so the result is the
+// reference's sequential sum.
+int Engine::body_forces(const int* citer, double* forces) {
+  const int B = ninst();
+  for (int i = 0; i < B; ++i) {
+    const fs_params& p = inst[i].prm;
+    const Phase ph = phase_for(p, citer[i]);
+    const double angle = p.alpha * M_PI / 180;
+    h_fp_[i].rorder = ph.k2 ? 3 : ph.recon_flux;
+    h_fp_[i].gamma = p.gamma;
+    h_fp_[i].ca = std::cos(angle);
+    h_fp_[i].sa = std::sin(angle);
+  }
+  FSE_HIP(hipMemcpyAsync(d_fp_, h_fp_, (size_t)B * sizeof(fsk::ForceParams), hipMemcpyHostToDevice, stream));
+  fsk::launch_body_terms(dm, ds, d_fp_, stream);
+  FSE_HIP(hipGetLastError());
+  if (dm.nbe > 0)
+    FSE_HIP(hipMemcpyAsync(h_bforce_, ds.bforce, 2 * (size_t)dm.nbe * sizeof(double), hipMemcpyDeviceToHost, stream));
+  FSE_HIP(hipStreamSynchronize(stream));
+  for (int i = 0; i < B; ++i) {
+    const Instance& I = inst[i];
+    const fs::Mesh& g = *I.mesh;
+    double f0 = 0., f1 = 0.;
+    for (int b = 0; b < I.nbe; ++b) {
+      const int bc = g.bbc[g.bedge[5 * (size_t)b + 3]];
+      if (bc == 0 || (bc % 2 == 0)) continue;
+      f0 += h_bforce_[2 * (size_t)(I.boff + b)];
+      f1 += h_bforce_[2 * (size_t)(I.boff + b) + 1];
+    }
+    forces[2 * (size_t)i] = f0;
+    forces[2 * (size_t)i + 1] = f1;
+  }
+  return FS_OK;
 }
 
 int Engine::fetch_errors() {

@@ -97,6 +97,10 @@ class Engine {
   // reports a failure)
   bool step_ready() const { return !pending_ || hipStreamQuery(stream) != hipErrorNotReady; }
 
+  // Compute_Body_Forces of every instance from its resident state (lift,
+  // drag: 2 per instance); citer[i] is the step the state comes from.
+  int body_forces(const int* citer, double* forces);
+
   // copies the per-instance device error flags into Instance::err (and clears
   // them on the device)
   int fetch_errors();
@@ -148,6 +152,9 @@ class Engine {
   fsk::InstParams* h_ip_ = nullptr;   // pinned staging, iter * ninst
   double* h_norms_ = nullptr;         // pinned, 2 * ninst
   int* h_err_ = nullptr;              // pinned, ninst
+  fsk::ForceParams* d_fp_ = nullptr;  // ninst
+  fsk::ForceParams* h_fp_ = nullptr;  // pinned staging, ninst
+  double* h_bforce_ = nullptr;        // pinned, 2 * nbe
   hipEvent_t ev_[8] = {};
   std::map<int, hipGraphExec_t> graphs_;
 

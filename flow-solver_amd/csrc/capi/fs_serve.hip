@@ -46,6 +46,7 @@ struct Request {
   bool finished = false;
   std::string error;
   std::vector<double> Q;
+  double forces[2] = {0., 0.};
 };
 
 struct Group {
@@ -231,6 +232,16 @@ struct fs_server {
         status = FS_INST_FAILED;
         err = fse::last_error_text();
       }
+      if (status == FS_INST_CONVERGED || status == FS_INST_MAX_STEPS) {
+        // main.C:874-876: the body forces of the final state
+        std::vector<int> citer(E.ninst());
+        std::vector<double> f(2 * (size_t)E.ninst());
+        for (int k = 0; k < E.ninst(); ++k) citer[k] = E.inst[k].ts;
+        if (E.body_forces(citer.data(), f.data()) == FS_OK) {
+          r->forces[0] = f[2 * (size_t)i];
+          r->forces[1] = f[2 * (size_t)i + 1];
+        }
+      }
       I.active = false;
       --g->live;
       g->reqs[i] = nullptr;
@@ -396,6 +407,8 @@ int fs_server_wait(fs_server* s, double timeout_s, fs_request_info* out, int* go
   out->device_bytes = r->bytes;
   out->nn = r->nn;
   out->ng = r->ng;
+  out->forces[0] = r->forces[0];
+  out->forces[1] = r->forces[1];
   *got = 1;
   if (!r->error.empty()) fse::set_error(FS_ERR_STATE, r->error);
   return FS_OK;

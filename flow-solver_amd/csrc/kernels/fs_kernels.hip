@@ -669,6 +669,71 @@ __global__ void __launch_bounds__(kBlock) k_norm_inst(int ninst, NormChunks c, c
   }
 }
 
+// ---------------------------------------------------------------------------
+// Body forces (grid_io.C:1390-1697, straight boundaries): one thread per
+// boundary edge; the three Gauss points of the bedge's half segment, the
+// boundary reconstruction of Reconstruct_Gauss_Boundary (reconstruct.C:
+// 1708-1890, PV_EXT) and the pressure, weighted 5/9, 8/9, 5/9 and scaled by
+// len/2.  The per-bedge terms are summed in bedge order on the host, which
+// keeps the reference's sequential sum.  Bedges that are not walls are
+// skipped there.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_body_terms(DevMesh m, DevState s, const ForceParams* __restrict__ fp) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= m.nbe) return;
+  const int bc = m.bedge_bc[b];
+  if (bc == 0 || (bc % 2 == 0)) {
+    s.bforce[2 * (size_t)b] = 0.0;
+    s.bforce[2 * (size_t)b + 1] = 0.0;
+    return;
+  }
+  const ForceParams f = fp[inst_of(m, m.inst_bedge, b)];
+  const int real = m.bedge_real[b];
+  const double nx = -m.bedge_n[3 * (size_t)b], ny = -m.bedge_n[3 * (size_t)b + 1];
+  const double len = m.bedge_n[3 * (size_t)b + 2];
+  const double x0 = m.x[2 * (size_t)real], x1 = m.x[2 * (size_t)real + 1];
+  const double phi = s.phi[real], sigma = phi;
+  double base[4], c1[4][2], c2[4][3];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    base[j] = (f.rorder == 3) ? s.pv[4 * (size_t)real + j] : s.nQ[4 * (size_t)real + j];
+    if (f.rorder == 1) {
+      c1[j][0] = s.grad[8 * (size_t)real + 2 * j];
+      c1[j][1] = s.grad[8 * (size_t)real + 2 * j + 1];
+    } else if (f.rorder == 3) {
+      const double* h = s.hess + 20 * (size_t)real + 5 * j;
+      c1[j][0] = h[0];
+      c1[j][1] = h[1];
+      c2[j][0] = h[2];
+      c2[j][1] = h[3];
+      c2[j][2] = h[4];
+    }
+  }
+  const double w[3] = {5. / 9., 8. / 9., 5. / 9.};
+  double temp1 = 0., temp2 = 0.;
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    const double dx0 = m.bedge_xg[6 * (size_t)b + 2 * g] - x0;
+    const double dx1 = m.bedge_xg[6 * (size_t)b + 2 * g + 1] - x1;
+    double q[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      q[j] = base[j];
+      if (f.rorder == 1)
+        q[j] += (phi * c1[j][0] * dx0 + phi * c1[j][1] * dx1);
+      else if (f.rorder == 3)
+        q[j] += (phi * c1[j][0] * dx0 + phi * c1[j][1] * dx1 + sigma * c2[j][0] * dx0 * dx0 * 0.5 +
+                 sigma * c2[j][1] * dx1 * dx1 * 0.5 + sigma * c2[j][2] * dx0 * dx1);
+    }
+    const double r = q[0], u = q[1] / r, v = q[2] / r, e = q[3];
+    const double P = (f.gamma - 1.) * (e - 0.5 * r * (u * u + v * v));
+    temp1 += (w[g] * (P * ny * f.ca - P * nx * f.sa));
+    temp2 += (w[g] * (P * nx * f.ca + P * ny * f.sa));
+  }
+  s.bforce[2 * (size_t)b] = temp1 * len * 0.5;
+  s.bforce[2 * (size_t)b + 1] = temp2 * len * 0.5;
+}
+
 template <int kForm>
 __global__ void __launch_bounds__(kBlock) k_roe_batch(int n, double gamma, const double* __restrict__ nrm,
                                                       const double* __restrict__ qL, const double* __restrict__ qR,
@@ -773,6 +838,10 @@ void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipS
     hipLaunchKernelGGL(k_norm_chunk, dim3(c.nchunk), dim3(kBlock), 0, st, c, s.ip, s.R, s.dQ, s.chunk_part);
   hipLaunchKernelGGL(k_norm_inst, dim3(nblocks(64 * m.ninst)), dim3(kBlock), 0, st, m.ninst, c, s.ip, s.chunk_part,
                      s.inst_norms);
+}
+void launch_body_terms(const DevMesh& m, const DevState& s, const ForceParams* fp, hipStream_t st) {
+  if (m.nbe == 0) return;
+  hipLaunchKernelGGL(k_body_terms, dim3(nblocks(m.nbe)), dim3(kBlock), 0, st, m, s, fp);
 }
 void launch_roe_batch(int n, int form, double gamma, const double* normals, const double* qL, const double* qR,
                       double* flux, hipStream_t st) {

@@ -82,6 +82,7 @@ struct DevMesh {
   const int* bedge_real;    // nbe
   const int* bedge_bc;      // nbe
   const double* bedge_n;    // 3*nbe
+  const double* bedge_xg;   // 6*nbe: Gauss points of the bedge's half segment
   // Block storage.  Rows are visited in color order (position t = index in
   // color_rows).  Off-diagonal blocks are sliced: positions are cut into
   // slices of 16, and off-diagonal block e (the reference's nsn order) of the
@@ -119,6 +120,7 @@ struct DevState {
   double* inc_flux;  // 2*nsub*4: subedge flux at each endpoint's incidence slot, signed
   double* bflux;     // nbe*4
   double* bjac;      // nbe*16
+  double* bforce;    // nbe*2: per-bedge body-force terms
   int* err;          // per-instance device error flags
   // mixed precision (FP32 Jacobian blocks and sweeps)
   float* LHS32;      // nnz*16, color-major slots
@@ -195,6 +197,14 @@ void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st);
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st);
 // form 0: reference operation order (bit-exact); 1: characteristic-wave form
+// Per-instance controls of the body-force pass: the reconstruction of the
+// phase of the step the state comes from (0 nodal, 1 limited linear, 3 k=2
+// point value + moments), gamma, and cos/sin of the angle of attack.
+struct ForceParams {
+  int rorder;
+  double gamma, ca, sa;
+};
+void launch_body_terms(const DevMesh& m, const DevState& s, const ForceParams* fp, hipStream_t st);
 void launch_roe_batch(int n, int form, double gamma, const double* normals, const double* qL, const double* qR,
                       double* flux, hipStream_t st);
 

@@ -91,7 +91,7 @@ class ServerConfig(C.Structure):
 class RequestInfo(C.Structure):
     _fields_ = [("id", C.c_int64), ("status", C.c_int), ("steps", C.c_int), ("drop", C.c_double),
                 ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
-                ("device_bytes", C.c_int64), ("nn", C.c_int), ("ng", C.c_int)]
+                ("device_bytes", C.c_int64), ("nn", C.c_int), ("ng", C.c_int), ("forces", C.c_double * 2)]
 
 
 class Timing(C.Structure):
@@ -138,6 +138,10 @@ def lib():
         "fs_boundary_residual": (i, [vp, i]),
         "fs_linear_solve": (i, [vp, C.POINTER(d)]),
         "fs_solve": (i, [vp, i, C.POINTER(d), C.POINTER(d)]),
+        "fs_body_forces": (i, [vp, i, vp]),
+        "fs_surface_cp": (i, [vp, i, vp, vp, i64, C.POINTER(i64)]),
+        "fs_write_surface_cp": (i, [vp, C.c_char_p]),
+        "fs_batch_body_forces": (i, [vp, vp]),
         "fs_roe_flux_batch": (i, [i, d, vp, vp, vp, vp, vp]),
         "fs_solve_steady": (i, [vp, i, d, C.POINTER(i), C.POINTER(d)]),
         "fs_solver_time_steps": (i, [vp, i, i, vp]),
@@ -353,6 +357,23 @@ class Solver:
         _check(lib().fs_solve(self._h, ts, C.byref(dq), C.byref(r2)))
         return dq.value, r2.value
 
+    def body_forces(self, citer):
+        """Compute_Body_Forces of the resident state after step `citer`: (lift, drag)."""
+        f = np.zeros(2)
+        _check(lib().fs_body_forces(self._h, citer, f.ctypes.data))
+        return f
+
+    def surface_cp(self, b):
+        """write_surface_cp's nodal values for boundary b: (x, cp) of its unique nodes."""
+        n = C.c_int64()
+        _check(lib().fs_surface_cp(self._h, b, None, None, 0, C.byref(n)))
+        x, cp = np.zeros(n.value), np.zeros(n.value)
+        _check(lib().fs_surface_cp(self._h, b, x.ctypes.data, cp.ctypes.data, n.value, C.byref(n)))
+        return x, cp
+
+    def write_surface_cp(self, directory):
+        _check(lib().fs_write_surface_cp(self._h, os.fsencode(directory)))
+
     def solve_steady(self, max_steps, tol):
         st, dr = C.c_int(), C.c_double()
         _check(lib().fs_solve_steady(self._h, max_steps, tol, C.byref(st), C.byref(dr)))
@@ -466,6 +487,12 @@ class Batch:
         v = np.ascontiguousarray(values, dtype=np.float64).ravel()
         _check(lib().fs_batch_set(self._h, i, _FIELDS.index(field), v.ctypes.data, v.size))
 
+    def body_forces(self):
+        """Compute_Body_Forces of every instance after its last step: (n, 2) lift, drag."""
+        f = np.zeros(2 * self.n)
+        _check(lib().fs_batch_body_forces(self._h, f.ctypes.data))
+        return f.reshape(-1, 2)
+
     def time_steps(self, nsteps):
         t = Timing()
         _check(lib().fs_batch_time_steps(self._h, nsteps, C.byref(t)))
@@ -512,6 +539,7 @@ class Server:
         if not got.value:
             return None
         out = {k: getattr(info, k) for k, _ in info._fields_}
+        out["forces"] = (info.forces[0], info.forces[1])
         if info.status == INST_FAILED:
             out["error"] = lib().fs_last_error().decode()
         return out

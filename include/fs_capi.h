@@ -178,6 +178,20 @@ int fs_roe_flux_batch(int n, double gamma, const double* d_normals, const double
 int fs_roe_flux_batch_form(int n, int form, double gamma, const double* d_normals, const double* d_qL,
                            const double* d_qR, double* d_flux, void* stream);
 
+/* grid_io.C:1390 Compute_Body_Forces: forces[0] (lift) and forces[1] (drag)
+ * integrated over the walls (odd bc <= 10) from the resident state after
+ * step `citer` (grid->citer selects the boundary reconstruction, as
+ * reconstruct.C:1708 Reconstruct_Gauss_Boundary does). */
+int fs_body_forces(fs_solver* s, int citer, double forces[2]);
+/* grid_io.C:841 write_surface_cp, nodal values: for boundary b (0-based) the
+ * x coordinate and Cp of its unique segment nodes in order of first
+ * appearance (count 0 for a boundary that is not a wall).  With x or cp NULL
+ * only *count is set. */
+int fs_surface_cp(fs_solver* s, int b, double* x, double* cp, int64_t cap, int64_t* count);
+/* the same written as the reference writes it: dir/surface_<b+1>_pressure.dat
+ * for every wall, "%f  %f" lines in `sort -g` order */
+int fs_write_surface_cp(fs_solver* s, const char* dir);
+
 /* Steady solve to ||R||/||R0|| <= tol (R0 = residual at the first step of the
  * highest-order phase) or max_steps; returns steps taken and the drop. */
 int fs_solve_steady(fs_solver* s, int max_steps, double tol, int* steps, double* drop);
@@ -253,6 +267,9 @@ int fs_batch_solve_steady(fs_batch* b, int max_steps, double tol, int* steps, do
 int fs_batch_inst_status(const fs_batch* b, int i, int* status, int* steps, double* drop);
 int fs_batch_get(fs_batch* b, int i, int field, double* host, int64_t n);
 int fs_batch_set(fs_batch* b, int i, int field, const double* host, int64_t n);
+/* grid_io.C:1390 Compute_Body_Forces of every instance from its state after
+ * its last step: forces[2*i] lift, forces[2*i+1] drag */
+int fs_batch_body_forces(fs_batch* b, double* forces);
 /* nsteps timed pseudo-time steps of the active instances (step counters advance) */
 int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t);
 /* device bytes a mesh costs inside a batch/solver of the given precision */
@@ -284,6 +301,7 @@ typedef struct {
   double t_submit, t_admit, t_done;   /* seconds on the server clock (fs_server_clock) */
   int64_t device_bytes;
   int nn, ng;
+  double forces[2];        /* grid_io.C:1390 Compute_Body_Forces of the final state (lift, drag) */
 } fs_request_info;
 void fs_server_config_default(fs_server_config* c);
 int fs_server_create(const fs_server_config* c, fs_server** out);

@@ -416,6 +416,29 @@ double linear_solve(G* grid, S* smem, P p) {
   return rms;
 }
 
+// grid_io.C:1390-1697: lift (Forces[0]) and drag (Forces[1]) over the walls
+// (odd bc, straight boundaries) from the GRID's state after step grid->citer
+template <class G, class P>
+void Compute_Body_Forces(G* grid, P p, double* Forces) {
+  Twin& t = detail::twin(grid, detail::to_fs(p));
+  detail::lazy_all(grid);
+  detail::put(t, FS_F_NQ, grid->nQ, 4, true);
+  detail::put(t, FS_F_GRAD, grid->grad, 8, false);
+  detail::put_phi(t, grid->phi);
+  detail::put_hess(t, grid->hess);
+  detail::put(t, FS_F_PV, grid->point_value, 4, false);
+  detail::check(fs_body_forces(t.solver, grid->citer, Forces), "Compute_Body_Forces");
+}
+
+// grid_io.C:841-1375: surface_<b>_pressure.dat for every wall in the working
+// directory, Cp of the GRID's Q at the unique segment nodes
+template <class G, class P>
+void write_surface_cp(G* grid, P p) {
+  Twin& t = detail::twin(grid, detail::to_fs(p));
+  detail::put(t, FS_F_Q, grid->Q, 4, true);
+  detail::check(fs_write_surface_cp(t.solver, "."), "write_surface_cp");
+}
+
 // releases the device twin of a GRID
 template <class G>
 void detach(G* grid) {

@@ -780,3 +780,105 @@ double fso_solve(const fso_mesh* m, const fso_params* p, fso_state* s, int ts, i
   memcpy(s->Q, s->nQ, (size_t)ntot * sizeof(double));
   return sqrt(dql2);
 }
+
+/* Reconstruct_Gauss_Boundary (reconstruct.C:1708-1890) with PV_EXT = 1,
+ * RECON_PRIM = 0 and node_order = p->order: the phase of step `citer` picks
+ * nodal values (0), the limited linear reconstruction (1) or the k=2
+ * point-value quadratic (3). */
+static void recon_gauss_boundary(const fso_mesh* m, const fso_params* p, const fso_state* s, int citer, int node,
+                                 const double* gp, double* q) {
+  int j, rorder = 0;
+  double phi = s->phi[node * 8 + 0], sigma = s->phi[node * 8 + 1];
+  double dx0, dx1;
+  for (j = 0; j < NV; j++) q[j] = s->nQ[node * 4 + j];
+  if (p->order == 1) rorder = citer <= p->foits ? 0 : 1;
+  else if (p->order == 3) {
+    if (citer <= p->foits) rorder = 0;
+    else if (citer <= p->foits + p->soits) rorder = 1;
+    else {
+      rorder = 3;
+      for (j = 0; j < NV; j++) q[j] = s->pv[node * 4 + j];
+    }
+  }
+  dx0 = gp[0] - m->x[2 * node];
+  dx1 = gp[1] - m->x[2 * node + 1];
+  if (rorder == 1) {
+    for (j = 0; j < NV; j++)
+      q[j] += (phi * s->grad[node * 8 + j * 2 + 0] * dx0 + phi * s->grad[node * 8 + j * 2 + 1] * dx1);
+  } else if (rorder == 3) {
+    const double* h = &s->hess[node * 20];
+    for (j = 0; j < NV; j++)
+      q[j] += (phi * h[j * 5 + 0] * dx0 + phi * h[j * 5 + 1] * dx1 + sigma * h[j * 5 + 2] * dx0 * dx0 * 0.5 +
+               sigma * h[j * 5 + 3] * dx1 * dx1 * 0.5 + sigma * h[j * 5 + 4] * dx0 * dx1);
+  }
+}
+
+static double pressure_of(double gamma, const double* q) {
+  double r = q[0], u = q[1] / r, v = q[2] / r, e = q[3];
+  return (gamma - 1.) * (e - 0.5 * r * (u * u + v * v));
+}
+
+void fso_body_forces(const fso_mesh* m, const fso_params* p, const fso_state* s, int citer, double forces[2]) {
+  int i, g;
+  const double angle = p->alpha * M_PI / 180;
+  const double t[3] = {sqrt(15.0) / 5.0, 0., sqrt(15.0) / (-5.0)};
+  const double w[3] = {5. / 9., 8. / 9., 5. / 9.};
+  forces[0] = 0.;
+  forces[1] = 0.;
+  for (i = 0; i < m->nbe; i++) {
+    const int* be = &m->bedge[5 * i];
+    int real = be[0], b = be[3], seg = m->bseg_off[b] + be[4];
+    int bc = m->bbc[b];
+    int nL, nR;
+    double nx, ny, len, xi, yi, xmid, ymid, xL, xR, yL, yR, temp1 = 0., temp2 = 0.;
+    if (bc == 0 || (bc % 2 == 0)) continue;
+    nx = -m->bedge_n[3 * i];
+    ny = -m->bedge_n[3 * i + 1];
+    len = m->bedge_n[3 * i + 2];
+    nL = m->bseg[2 * seg];
+    nR = m->bseg[2 * seg + 1];
+    xi = m->x[2 * real];
+    yi = m->x[2 * real + 1];
+    xmid = 0.5 * (m->x[2 * nL] + m->x[2 * nR]);
+    ymid = 0.5 * (m->x[2 * nL + 1] + m->x[2 * nR + 1]);
+    if (real == nL) { xL = xi; xR = xmid; yL = yi; yR = ymid; }
+    else { xL = xmid; xR = xi; yL = ymid; yR = yi; }
+    for (g = 0; g < 3; g++) {
+      double gp[2], q[4], P;
+      gp[0] = (xL + xR) * 0.5 + (xR - xL) * 0.5 * t[g];
+      gp[1] = (yL + yR) * 0.5 + (yR - yL) * 0.5 * t[g];
+      recon_gauss_boundary(m, p, s, citer, real, gp, q);
+      P = pressure_of(p->gamma, q);
+      temp1 += (w[g] * (P * ny * cos(angle) - P * nx * sin(angle)));
+      temp2 += (w[g] * (P * nx * cos(angle) + P * ny * sin(angle)));
+    }
+    temp1 = temp1 * len * 0.5;
+    temp2 = temp2 * len * 0.5;
+    forces[0] += temp1;
+    forces[1] += temp2;
+  }
+}
+
+int fso_surface_cp(const fso_mesh* m, const fso_params* p, const fso_state* s, int b, double* x, double* cp) {
+  int sgi, k, e, count = 0;
+  const double pinf = 1. / p->gamma, minf = p->mach;
+  int bc = m->bbc[b];
+  int* seen;
+  if (bc == 0 || (bc % 2 == 0)) return 0;
+  seen = (int*)malloc(sizeof(int) * (size_t)(2 * (m->bseg_off[b + 1] - m->bseg_off[b]) + 1));
+  for (sgi = m->bseg_off[b]; sgi < m->bseg_off[b + 1]; sgi++)
+    for (e = 0; e < 2; e++) {
+      int n = m->bseg[2 * sgi + e], unique = 1;
+      for (k = 0; k < count; k++)
+        if (seen[k] == n) unique = 0;
+      if (unique) {
+        double P = pressure_of(p->gamma, &s->Q[n * 4]);
+        seen[count] = n;
+        x[count] = m->x[2 * n];
+        cp[count] = (P / pinf - 1.) * (2. / (p->gamma * minf * minf));
+        count++;
+      }
+    }
+  free(seen);
+  return count;
+}

@@ -30,6 +30,10 @@
  *                           cannot run the k=2 factorization: LAPACK is absent)
  *   fso_derivatives         quadratic exactness of hess/point_value (tests);
  *                           no reference-build vector exists for the k=2 phase
+ *   fso_body_forces         reference build (forces per step, first-order and linear
+ *                           phases); k=2 phase: quadratic exactness of the Gauss-point
+ *                           reconstruction and its reduction to the linear phase (tests)
+ *   fso_surface_cp          closed form (uniform state gives Cp of the freestream pressure)
  *   fso_limiter_barth/_venkat (k=2 Gauss points), k=2 phase of fso_solve:
  *                           parity unpinned against the reference build (it cannot
  *                           enter the k=2 phase here); pinned by the k=2 invariants
@@ -64,6 +68,8 @@ typedef struct {
   int ncolor;
   const int* color_off;     /* ncolor+1 */
   const int* color_rows;    /* nn */
+  const int* bseg_off;      /* nb+1: segments of boundary b */
+  const int* bseg;          /* 2 per segment (n0, n1) */
 } fso_mesh;
 
 typedef struct {
@@ -116,6 +122,13 @@ double fso_linear_solve(const fso_mesh* m, const fso_params* p, fso_state* s, in
  * instead of the forward difference; used to check the FP32 analytic blocks */
 void fso_set_analytic_jacobian(int on);
 double fso_solve(const fso_mesh* m, const fso_params* p, fso_state* s, int ts, int multicolor);
+/* Compute_Body_Forces (grid_io.C:1390-1697), straight boundaries (bc <= 10):
+ * forces[0] (lift) and forces[1] (drag) from the state after step `citer`. */
+void fso_body_forces(const fso_mesh* m, const fso_params* p, const fso_state* s, int citer, double forces[2]);
+/* write_surface_cp (grid_io.C:841-1375, the nodal branch): for boundary b
+ * (0-based) the unique segment nodes in first-appearance order, their x and
+ * Cp from Q.  Returns the count (0 for a boundary that is not a wall). */
+int fso_surface_cp(const fso_mesh* m, const fso_params* p, const fso_state* s, int b, double* x, double* cp);
 
 #ifdef __cplusplus
 }

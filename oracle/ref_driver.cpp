@@ -3,7 +3,7 @@
 // This program is compiled against the reference sources in place (see
 // oracle/Makefile) and is used only to write golden vectors and as an
 // optional CPU baseline.  It replays the reference driver's setup sequence
-// (main.C:105-648) and pseudo-time loop (main.C:789-815) and dumps every
+// (main.C:105-648) and pseudo-time loop (main.C:789-876) and dumps every
 // array the parity tests compare.  It never enters the product.
 //
 // Usage: ref_driver <case_dir> <out.bin> <nsteps> [stencil] [dump_every]
@@ -25,6 +25,7 @@
 #include "cv_calc.h"
 #include "geometry.h"
 #include "gradient.h"
+#include "grid_io.h"
 #include "grid.h"
 #include "ic.h"
 #include "init_memory.h"
@@ -177,8 +178,12 @@ int main(int argc, char** argv) {
     double ts0 = now_s();
     double rms = solve(&grid, &smem, p, i);
     t_solve += now_s() - ts0;
+    // main.C:874-876: body forces from the state the step left
+    double forces[2] = {0., 0.};
+    Compute_Body_Forces(&grid, p, forces);
     if (dump_every > 0 && (i % dump_every == 0 || i == nsteps)) {
       snprintf(nm, sizeof nm, "rms_%d", i);      dumpd(nm, &rms, 1);
+      snprintf(nm, sizeof nm, "forces_%d", i);   dumpd(nm, forces, 2);
       snprintf(nm, sizeof nm, "R_%d", i);        dumpd(nm, grid.R, (long)(nn + 1) * 4);
       snprintf(nm, sizeof nm, "dQ_%d", i);       dumpd(nm, grid.dQ, (long)(nn + 1) * 4);
       snprintf(nm, sizeof nm, "Q_%d", i);        dumpd(nm, grid.Q, (long)(nn + ng + 1) * 4);

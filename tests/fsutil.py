@@ -37,7 +37,7 @@ class FsoMesh(C.Structure):
         ("bedge", C.c_void_p), ("bedge_n", C.c_void_p), ("bbc", C.c_void_p), ("cv_area", C.c_void_p),
         ("moments", C.c_void_p), ("edge_weight", C.c_void_p), ("ia", C.c_void_p), ("iau", C.c_void_p),
         ("ja", C.c_void_p), ("st_off", C.c_void_p), ("st_nodes", C.c_void_p), ("ncolor", C.c_int),
-        ("color_off", C.c_void_p), ("color_rows", C.c_void_p)]
+        ("color_off", C.c_void_p), ("color_rows", C.c_void_p), ("bseg_off", C.c_void_p), ("bseg", C.c_void_p)]
 
 
 class FsoState(C.Structure):
@@ -78,6 +78,10 @@ def oracle_lib():
         L.fso_set_analytic_jacobian.restype = None
         L.fso_solve.argtypes = [vp, vp, vp, i, i]
         L.fso_solve.restype = d
+        L.fso_body_forces.argtypes = [vp, vp, vp, i, vp]
+        L.fso_body_forces.restype = None
+        L.fso_surface_cp.argtypes = [vp, vp, vp, i, vp, vp]
+        L.fso_surface_cp.restype = i
         _olib = L
     return _olib
 
@@ -102,7 +106,8 @@ class Oracle:
         m.nel = info["ntri"] + info["nquad"]
         for name in ("x", "sub_n", "sub_xm", "sub_xg", "el_cent", "bedge_n", "cv_area", "moments", "edge_weight"):
             setattr(m, name, arr(name, np.float64))
-        for name in ("edges", "sub_edge", "sub_elem", "bedge", "bbc", "ia", "iau", "ja", "color_off", "color_rows"):
+        for name in ("edges", "sub_edge", "sub_elem", "bedge", "bbc", "ia", "iau", "ja", "color_off", "color_rows",
+                     "bseg_off", "bseg"):
             setattr(m, name, arr(name, np.int32))
         m.ncolor = info["ncolor"]
         if info["nstencil"] > 0:
@@ -135,6 +140,20 @@ class Oracle:
         r = self.L.fso_solve(C.byref(self.m), C.byref(self.params), C.byref(self.s), ts, multicolor)
         assert r >= 0, "oracle step failed"
         return r
+
+    def body_forces(self, citer):
+        """Compute_Body_Forces of the current state after step `citer`: (lift, drag)."""
+        f = np.zeros(2)
+        self.L.fso_body_forces(C.byref(self.m), C.byref(self.params), C.byref(self.s), citer, f.ctypes.data)
+        return f
+
+    def surface_cp(self, b):
+        """write_surface_cp's nodal values for boundary b: (x, cp) of its unique nodes."""
+        nseg = self._keep["bseg_off"][b + 1] - self._keep["bseg_off"][b]
+        x, cp = np.zeros(2 * nseg + 1), np.zeros(2 * nseg + 1)
+        n = self.L.fso_surface_cp(C.byref(self.m), C.byref(self.params), C.byref(self.s), b, x.ctypes.data,
+                                  cp.ctypes.data)
+        return x[:n], cp[:n]
 
     def __getitem__(self, k):
         return self.a[k]

@@ -68,7 +68,7 @@ def run_case(name, nx, ny, seed, p, nsteps, stencil, step_fields, dump_every=1, 
 def main():
     if not os.path.exists(U.REF_DRIVER):
         raise SystemExit("build the reference harness first: make -C oracle ref")
-    full = ["rms", "R", "dQ", "Q", "phi", "grad", "ev_con", "dt"]
+    full = ["rms", "R", "dQ", "Q", "phi", "grad", "ev_con", "dt", "forces"]
     # tiny meshes: complete step histories through the first-order and linear phases
     run_case("bump17x9_o1", 17, 9, 7, params(order=1, foits=2), 6, False, full, lhs_steps=(1, 3, 6))
     run_case("bump17x9_o3", 17, 9, 7, params(order=3, foits=2, soits=3), 5, True, full, lhs_steps=(3, 5))
@@ -77,10 +77,10 @@ def main():
              lhs_steps=(6,))
     # the CPU-runnable configuration: 97x33 bump channel, M=0.5, k=1, Barth limiter
     run_case("cfg1_97x33", 97, 33, 1, params(order=3, foits=20, soits=20, mach=0.5, alpha=0.0, CFL=100.0, ramp=30,
-                                              subits=20), 2, True, ["rms", "R", "dQ", "Q"])
+                                              subits=20), 2, True, ["rms", "R", "dQ", "Q", "forces"])
     # converged first-order-then-linear unlimited solve on the same mesh (1e-10 residual drop reached by step 174)
     run_case("cfg1_97x33_steady", 97, 33, 1, params(order=1, foits=20, limit=0, mach=0.5, alpha=0.0, CFL=100.0,
-                                                     ramp=30, subits=20), 180, False, ["rms"], dump_every=20,
+                                                     ramp=30, subits=20), 180, False, ["rms", "forces"], dump_every=20,
              keep_steps=[20, 40, 60, 80, 100, 120, 140, 160, 180], extra=["Qfinal"], setup=False)
 
 

@@ -53,10 +53,13 @@ def test_batch_equals_alone(precision):
     b.init_freestream()
     steps, drop, status = b.solve_steady(120, 1e-6)
     print("batch steps", steps, "drop", drop, "status", status, flush=True)
+    forces = b.body_forces()
     for i, (m, p) in enumerate(zip(meshes, params)):
         s = fsm.Solver(m, p, precision)
         s.init_freestream()
         st, dr = s.solve_steady(120, 1e-6)
+        # Compute_Body_Forces of the final state, alone and in the batch
+        np.testing.assert_array_equal(s.body_forces(st), forces[i])
         same = np.array_equal(s.get("Q"), b.get(i, "Q"))
         print("instance %d alone steps %d drop %.3e  batch steps %d drop %.3e  bit-identical %s" %
               (i, st, dr, steps[i], drop[i], same), flush=True)

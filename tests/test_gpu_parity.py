@@ -273,3 +273,62 @@ def test_mixed_converged_state_matches_fp64(order, limit, soits):
           (steps_m, drop_m, ts, rel))
     assert drop_m <= 1e-10
     assert rel <= 1e-10
+
+
+def _oracle_from_device(s, o, info):
+    """The oracle's state and reconstruction fields set to the device's."""
+    nn = info["nn"]
+    o["nQ"][:] = s.get("nQ")
+    o["Q"][:] = s.get("Q")
+    o["grad"][:] = s.get("grad")
+    o["phi"][:] = np.repeat(s.get("phi"), 8)
+    o["hess"][:] = s.get("hess")
+    o["pv"][:] = s.get("pv")
+    assert o["pv"].size == 4 * nn
+
+
+@pytest.mark.parametrize("order,soits,limit", [(1, 0, 1), (3, 2, 1), (3, 2, 2), (1, 0, 0)])
+def test_body_forces_match_oracle(order, soits, limit):
+    """Compute_Body_Forces on the device (per-bedge Gauss-point terms summed
+    in bedge order) equals the oracle's bit for bit from the same fields, in
+    every phase: nodal, limited linear and the k=2 point-value quadratic."""
+    p, m, s, o = _pair(33, 17, 21, order=order, soits=soits, limit=limit, alpha=2.0, subits=8)
+    info = m.info()
+    s.init_freestream()
+    nsteps = p.foits + soits + 3
+    seen = set()
+    for ts in range(1, nsteps + 1):
+        s.solve(ts)
+        _oracle_from_device(s, o, info)
+        fd, fo = s.body_forces(ts), o.body_forces(ts)
+        seen.add(0 if ts <= p.foits else (1 if (order == 1 or ts <= p.foits + soits) else 3))
+        print("ts %d forces device %s oracle %s" % (ts, fd, fo), flush=True)
+        np.testing.assert_array_equal(fd, fo)
+    assert seen == ({0, 1} if order == 1 else {0, 1, 3})
+    # surface Cp of every boundary, and the reference's file format
+    for b in range(info["nb"]):
+        xd, cd = s.surface_cp(b)
+        xo, co = o.surface_cp(b)
+        np.testing.assert_array_equal(xd, xo)
+        np.testing.assert_array_equal(cd, co)
+
+
+def test_surface_cp_file(tmp_path):
+    p, m, s, o = _pair(17, 9, 22, order=1, limit=0)
+    s.init_freestream()
+    for ts in range(1, 4):
+        s.solve(ts)
+    s.write_surface_cp(str(tmp_path))
+    bbc = m.array("bbc")
+    for b in range(m.info()["nb"]):
+        path = tmp_path / ("surface_%d_pressure.dat" % (b + 1))
+        if bbc[b] == 0 or bbc[b] % 2 == 0:
+            assert not path.exists()
+            continue
+        rows = [tuple(map(float, ln.split())) for ln in path.read_text().splitlines()]
+        x, cp = s.surface_cp(b)
+        assert len(rows) == len(x)
+        xs = [r[0] for r in rows]
+        assert xs == sorted(xs)
+        want = sorted(zip(np.float32(x).tolist(), np.float32(cp).tolist()))
+        np.testing.assert_allclose(np.array(rows), np.array(want), atol=1e-6)

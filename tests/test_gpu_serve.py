@@ -73,6 +73,7 @@ def test_serving_results_equal_alone(precision):
         assert info["steps"] == steps
         assert rel <= 1e-10
         assert np.array_equal(Q, Qa)
+        np.testing.assert_array_equal(np.array(info["forces"]), s.body_forces(steps))
         s.close()
 
 

@@ -6,6 +6,10 @@
 * Against mathematics, for what the reference build cannot run here (the k=2
   factorization needs LAPACK): the pseudo-inverse against numpy's pinv, and the
   k=2 derivatives exact for control-volume averages of a quadratic.
+* Body forces: the reference build's values every step of the first-order and
+  linear phases; at k=2 (which the reference build cannot enter) the boundary
+  Gauss-point reconstruction is exact for quadratic fields and reduces to the
+  linear phase's when the second moments vanish.  Surface Cp: closed form.
 * Roe flux: consistency, conservation, supersonic upwinding; LSQ gradient exact
   for linear fields; Gauss-Seidel converges to the dense solution.
 A planted mistake (sign, index, dropped term) fails at least one of these.
@@ -57,6 +61,9 @@ def test_step_history_bitexact(case):
                 np.testing.assert_array_equal(o[f], g["%s_%d" % (f, ts)][1:])
         if "LHS_%d" % ts in g:
             np.testing.assert_array_equal(o["LHS"], g["LHS_%d" % ts])
+        if "forces_%d" % ts in g:
+            # Compute_Body_Forces after the step (main.C:874-876)
+            np.testing.assert_array_equal(o.body_forces(ts), g["forces_%d" % ts])
 
 
 def test_converged_state_matches_reference():
@@ -66,6 +73,7 @@ def test_converged_state_matches_reference():
         dq = o.solve(ts, multicolor=0)
         if "rms_%d" % ts in g:
             assert dq == g["rms_%d" % ts][0]
+            np.testing.assert_array_equal(o.body_forces(ts), g["forces_%d" % ts])
     nn = m.info()["nn"]
     rho, rho_ref = o["Q"][:4 * nn:4], g["Qfinal"][4:4 * nn + 4:4]
     assert np.linalg.norm(rho - rho_ref) / np.linalg.norm(rho_ref) <= 1e-13
@@ -232,3 +240,117 @@ def test_analytic_jacobian_equals_fd_at_equal_states():
     L.fso_set_analytic_jacobian(0)
     scale = np.max(np.abs(out[1]))
     assert np.max(np.abs(out[0] - out[1])) <= 1e-6 * scale
+
+
+def _wall_setup(nx=25, ny=11):
+    p = fsm.Params.make(order=3, foits=1, soits=1, limit=1, reorder=1, mach=0.6, alpha=2.5)
+    m = fsm.Mesh.bump(nx, ny, 0.2, 5)
+    m.prepare(p)
+    o = U.Oracle(m, p)
+    info = m.info()
+    return p, m, o, info
+
+
+def _pressure(q, g=1.4):
+    return (g - 1) * (q[..., 3] - 0.5 * q[..., 0] * ((q[..., 1] / q[..., 0]) ** 2 + (q[..., 2] / q[..., 0]) ** 2))
+
+
+def _wall_bedges(m):
+    be = m.array("bedge").reshape(-1, 5)
+    bbc = m.array("bbc")
+    bc = bbc[be[:, 3]]
+    return (bc != 0) & (bc % 2 == 1)
+
+
+def test_body_forces_k2_quadratic_exact():
+    """At k=2 the boundary reconstruction is pv + first and second moments
+    (reconstruct.C:1840-1852): for an exact quadratic field it returns the
+    field at the Gauss point, so the forces equal the 3-point Gauss rule of the
+    field's pressure (grid_io.C:1462-1582) evaluated independently here."""
+    p, m, o, info = _wall_setup()
+    nn = info["nn"]
+    x = m.array("x")[:2 * nn].reshape(-1, 2)
+    base = np.array([1.0, 0.45, 0.05, 2.3])
+    lin = np.array([[0.02, -0.01], [0.03, 0.01], [-0.02, 0.015], [0.04, -0.03]])
+    quad = np.array([[0.01, -0.02, 0.005], [0.015, 0.01, -0.01], [0.0, 0.02, 0.01], [0.03, 0.01, -0.02]])
+
+    def field(pts):
+        X, Y = pts[..., 0:1], pts[..., 1:2]
+        return base + lin[:, 0] * X + lin[:, 1] * Y + 0.5 * quad[:, 0] * X * X + 0.5 * quad[:, 1] * Y * Y + \
+            quad[:, 2] * X * Y
+
+    qn = field(x)
+    hess = np.zeros((nn, 4, 5))
+    hess[:, :, 0] = lin[:, 0] + quad[:, 0] * x[:, 0:1] + quad[:, 2] * x[:, 1:2]
+    hess[:, :, 1] = lin[:, 1] + quad[:, 1] * x[:, 1:2] + quad[:, 2] * x[:, 0:1]
+    hess[:, :, 2:5] = quad
+    o["pv"][:] = qn.ravel()
+    o["hess"][:] = hess.ravel()
+    o["nQ"][:4 * nn] = qn.ravel() * 1.01   # nodal values must not be used at k=2
+    o["phi"][:] = 1.0
+    k2_ts = p.foits + p.soits + 1
+    got = o.body_forces(k2_ts)
+    wall = _wall_bedges(m)
+    bn = m.array("bedge_n").reshape(-1, 3)
+    xg = m.array("bedge_xg").reshape(-1, 3, 2)
+    P = _pressure(field(xg))                      # (nbe, 3)
+    w = np.array([5 / 9, 8 / 9, 5 / 9])
+    a = np.deg2rad(p.alpha)
+    nx, ny, ln = -bn[:, 0], -bn[:, 1], bn[:, 2]
+    lift = ((P * w).sum(1) * (ny * np.cos(a) - nx * np.sin(a)) * ln * 0.5)[wall].sum()
+    drag = ((P * w).sum(1) * (nx * np.cos(a) + ny * np.sin(a)) * ln * 0.5)[wall].sum()
+    np.testing.assert_allclose(got, [lift, drag], rtol=1e-12)
+    assert abs(lift) > 1e-3 and abs(drag) > 1e-3
+
+
+def test_body_forces_k2_reduces_to_linear():
+    """With vanishing second moments, point value = nodal state and first
+    moments = gradient, the k=2 forces equal the linear phase's (pinned against
+    the reference build above) to the last bit."""
+    p, m, o, info = _wall_setup()
+    nn = info["nn"]
+    rng = np.random.default_rng(3)
+    Q = U.perturbed_state(nn, info["ng"], p, seed=11)
+    o["nQ"][:] = Q
+    g = rng.uniform(-0.05, 0.05, size=(nn, 4, 2))
+    o["grad"][:] = g.ravel()
+    hess = np.zeros((nn, 4, 5))
+    hess[:, :, 0:2] = g
+    o["hess"][:] = hess.ravel()
+    o["pv"][:] = o["nQ"][:4 * nn]
+    phi = rng.uniform(0.3, 1.0, size=nn)
+    o["phi"][:] = np.repeat(phi, 8)
+    lin = o.body_forces(p.foits + 1)
+    k2 = o.body_forces(p.foits + p.soits + 1)
+    np.testing.assert_array_equal(k2, lin)
+    # and the first-order phase reads the nodal state only
+    first = o.body_forces(p.foits)
+    assert not np.array_equal(first, lin)
+
+
+def test_surface_cp_closed_form():
+    """write_surface_cp's nodal branch (grid_io.C:1270-1330): unique wall nodes
+    in segment order, Cp = (P/p_inf - 1) 2/(gamma M^2) from Q."""
+    p, m, o, info = _wall_setup()
+    nn, ng = info["nn"], info["ng"]
+    x = m.array("x").reshape(-1, 2)
+    pinf = 1 / 1.4
+    Pfield = pinf * (1 + 0.1 * x[:, 0] - 0.05 * x[:, 1])
+    rho, u, v = 1.0 + 0.02 * x[:, 0], 0.5, 0.05
+    Q = np.stack([rho, rho * u, rho * v, Pfield / 0.4 + 0.5 * rho * (u * u + v * v)], axis=1)
+    o["Q"][:] = Q.ravel()
+    bseg_off, bseg = m.array("bseg_off"), m.array("bseg").reshape(-1, 2)
+    bbc = m.array("bbc")
+    for b in range(info["nb"]):
+        xs, cp = o.surface_cp(b)
+        if bbc[b] == 0 or bbc[b] % 2 == 0:
+            assert len(xs) == 0
+            continue
+        nodes = []
+        for s in range(bseg_off[b], bseg_off[b + 1]):
+            for n in bseg[s]:
+                if n not in nodes:
+                    nodes.append(n)
+        nodes = np.array(nodes)
+        np.testing.assert_array_equal(xs, x[nodes, 0])
+        np.testing.assert_allclose(cp, (Pfield[nodes] / pinf - 1) * 2 / (1.4 * p.mach ** 2), rtol=1e-11, atol=1e-13)

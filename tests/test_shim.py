@@ -96,3 +96,8 @@ def test_reference_driver_with_device_solve():
           % (n, rel, out["cpu"]["rms_%d" % n][0], out["gpu"]["rms_%d" % n][0]))
     assert out["gpu"]["rms_%d" % n][0] < 1e-9
     assert rel <= 1e-10
+    # Compute_Body_Forces after the last step: the reference's own function on
+    # the CPU build, the device one (the binding replaces it too) on the other
+    fc, fg = out["cpu"]["forces_%d" % n], out["gpu"]["forces_%d" % n]
+    print("body forces CPU %s device %s" % (fc, fg))
+    np.testing.assert_allclose(fg, fc, rtol=1e-9, atol=1e-12)
