@@ -114,7 +114,8 @@ def parse():
     ap.add_argument("--serve-max-steps", type=int, default=2000)
     ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-solve-at-a-time FP64 baseline")
     ap.add_argument("--batch-precisions", default="fp64,mixed")
-    ap.add_argument("--sections", default="config2,batch,serving,cpu",
+    ap.add_argument("--config3-max-steps", type=int, default=3000)
+    ap.add_argument("--sections", default="config2,config3,batch,serving,cpu",
                     help="parts to run (a profiled run picks one; the headline needs config2)")
     return ap.parse_args()
 
@@ -124,6 +125,11 @@ def parse():
 # stalls short of a 1e-10 drop), k=2 with the Barth limiter (its k=2 phase
 # residual is first order, so the limiter does not move the solution)
 WORKLOAD_CONTROLS = dict(CFL=200.0, ramp=30, subits=40)
+# the 600k-node transonic O-mesh: at CFL 200 and 40 sweeps the FP64 path is
+# still at a 4e-8 drop after 3000 steps and the mixed path (analytic
+# approximate Jacobian) reaches a non-physical state; CFL 1000 with 80 sweeps
+# brings both to a 1e-10 drop in about 1460 steps (measured)
+CONFIG3_CONTROLS = dict(CFL=1000.0, ramp=50, subits=80)
 
 
 def workload_params(W, cases):
@@ -283,6 +289,8 @@ def bench_serving(a, precision):
     lat = np.array([r["t_done"] - r["t_submit"] for r in res])
     dev = np.array([r["t_done"] - r["t_admit"] for r in res])
     conv = st == fsm.INST_CONVERGED
+    print("serving %s done: %d requests in %.1f s" % ("mixed" if precision == fsm.FS_PREC_MIXED else "fp64", len(res),
+                                                    wall), file=sys.stderr, flush=True)
     q = lambda v, p: float(np.percentile(v, p)) if len(v) else None
     return {"requests": len(res), "offered_rate_per_s": a.serve_rate, "stream_s": a.serve_seconds, "wall_s": wall,
             "converged": int(conv.sum()), "max_steps": int((st == fsm.INST_MAX_STEPS).sum()),
@@ -291,6 +299,59 @@ def bench_serving(a, precision):
             "p50_admit_to_converged_s": q(dev[conv], 50), "p99_admit_to_converged_s": q(dev[conv], 99),
             "solves_per_s": float(conv.sum()) / wall,
             "nodes_median": float(np.median([r["nn"] for r in res])) if res else None}
+
+
+def bench_config3(a, fsm):
+    """The precision configuration: an O-mesh of 513 wall x 1171 radial nodes
+    around the 12%-thick profile (first cell 1e-3 chord, 30 chords, +-0.1
+    jitter, about 1.2 M triangles), M=0.8 at 1.25 degrees, k=2 with the Barth
+    limiter.  FP64 and mixed steady solves to a 1e-10 drop (hipGraph steps);
+    the mixed path's error against the FP64 path for the converged density
+    and the lift-like wall pressure integrals (Compute_Body_Forces)."""
+    import numpy as np
+    p = fsm.Params.make(nsteps=0, iter=1, jufreq=1, gamma=1.4, alpha=1.25, mach=0.8, method=1, order=3, foits=10,
+                        soits=10, limit=1, reorder=1, **CONFIG3_CONTROLS)
+    t0 = time.perf_counter()
+    mesh = fsm.Mesh.airfoil(513, 1171, 1e-3, 30.0, 0.1, 3)
+    mesh.prepare(p)
+    info = mesh.info()
+    out = {"workload": "O-mesh 513x1171 around the 12%-thick profile, M=0.8, alpha=1.25, k=2, Barth",
+           "nodes": info["nn"], "triangles": info["ntri"], "subedges": info["nsub"], "bedges": info["nbe"],
+           "setup_s": time.perf_counter() - t0, "controls": dict(CONFIG3_CONTROLS, tol=1e-10)}
+    rho, forces = {}, {}
+    evals = info["nsub"] + info["nbe"]
+    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+        s = fsm.Solver(mesh, p, prec)
+        s.use_graph(True)
+        s.init_freestream()
+        t0 = time.perf_counter()
+        try:
+            steps, drop = s.solve_steady(a.config3_max_steps, 1e-10)
+        except fsm.FlowSolverError as e:
+            # a non-physical state: reported, and the section goes on
+            out[name] = {"failed": str(e), "wall_s": time.perf_counter() - t0}
+            print("config3 %s failed: %s" % (name, e), file=sys.stderr, flush=True)
+            s.close()
+            continue
+        wall = time.perf_counter() - t0
+        forces[name] = s.body_forces(steps)
+        rho[name] = s.get("Q")[0:4 * info["nn"]:4]
+        t = s.time_steps(steps + 1, 10)
+        out[name] = {"steps": steps, "drop": drop, "converged": bool(drop <= 1e-10), "wall_s": wall,
+                     "ms_per_step": t["step_ms"], "value": evals / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s",
+                     "lift": float(forces[name][0]), "drag": float(forces[name][1])}
+        s.close()
+        print("config3 %s: %d steps, drop %.2e, %.1f s" % (name, steps, drop, wall), file=sys.stderr, flush=True)
+    if len(rho) < 2:
+        return out
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    out["time_to_solution_speedup"] = out["fp64"]["wall_s"] / out["mixed"]["wall_s"]
+    out["density_l2_diff_mixed_vs_fp64"] = float(np.linalg.norm(rho["mixed"] - rho["fp64"]) /
+                                                 np.linalg.norm(rho["fp64"]))
+    rel = np.abs(forces["mixed"] - forces["fp64"]) / np.abs(forces["fp64"])
+    out["lift_rel_diff_mixed_vs_fp64"] = float(rel[0])
+    out["drag_rel_diff_mixed_vs_fp64"] = float(rel[1])
+    return out
 
 
 def cpu_baseline(U, fsm, p, nx, ny, steps):
@@ -325,6 +386,8 @@ def main():
     if "config2" not in sections:
         line = {"metric": "edge-flux evals/s", "value": None, "unit": "edge-flux evals/s", "n_gpus": 1,
                 "note": "headline section not run (--sections %s)" % a.sections}
+        if "config3" in sections:
+            line["config3"] = bench_config3(a, fsm)
         if "batch" in sections and a.batch > 0:
             line["batch"] = bench_batch(a)
         print(json.dumps(line), flush=True)
@@ -382,8 +445,12 @@ def main():
                      "kernel_ms": {"edge_flux_f64": tm["edge_ms"], "node_assembly": tm["node_ms"],
                                    "gauss_seidel_f32": tm["gs_ms"], "diag_lu_f32": tm["lu_ms"],
                                    "other": tm["other_ms"]}}
+    print("config2 done", file=sys.stderr, flush=True)
+    if "config3" in sections:
+        line["config3"] = bench_config3(a, fsm)
     if "batch" in sections and a.batch > 0:
         line["batch"] = bench_batch(a)
+        print("batch done", file=sys.stderr, flush=True)
     if "serving" in sections and a.serve_seconds > 0:
         sv = {"mixed": bench_serving(a, fsm.FS_PREC_MIXED), "fp64": bench_serving(a, fsm.FS_PREC_F64)}
         sv["controls"] = dict(WORKLOAD_CONTROLS, tol=1e-10, max_steps=a.serve_max_steps)

@@ -332,3 +332,51 @@ def test_surface_cp_file(tmp_path):
         assert xs == sorted(xs)
         want = sorted(zip(np.float32(x).tolist(), np.float32(cp).tolist()))
         np.testing.assert_allclose(np.array(rows), np.array(want), atol=1e-6)
+
+
+def test_transonic_airfoil_fp64_and_mixed_vs_oracle():
+    """The precision configuration in miniature: an O-mesh of 64 x 17 nodes
+    around the 12%-thick profile (first cell 1e-3 chord, 30 chords, +-0.1
+    jitter), M=0.8 at 1.25 degrees, k=2 with the Barth limiter.
+
+    Fixed point: FP64 and mixed solves converged to a 1e-12 residual drop
+    against the oracle's (same drop): density L2 within 1e-10, lift and drag
+    (Compute_Body_Forces) within 1e-9 relative.  At the serving stopping test
+    (1e-10 drop) a state is still a few 1e-10 from the fixed point on this
+    case (the FP64 path 7e-10, measured), so that comparison is printed, and
+    bounded only by 1e-8."""
+    p = _params(order=3, foits=10, soits=10, limit=1, mach=0.8, alpha=1.25, CFL=100.0, ramp=30, subits=20)
+    m = fsm.Mesh.airfoil(64, 17, 1e-3, 30.0, 0.1, 3)
+    m.prepare(p)
+    nn = m.info()["nn"]
+    o = U.Oracle(m, p)
+    o.init_freestream()
+    ts_ref, r0 = p.foits + p.soits + 1, None
+    for ts in range(1, 2001):
+        o.solve(ts, multicolor=0)
+        r = np.sqrt(o.s.last_rms)
+        if ts == ts_ref:
+            r0 = r
+        if r0 and r / r0 <= 1e-12:
+            break
+    f_ref = o.body_forces(ts)
+    rho_ref = o["Q"][0:4 * nn:4]
+    print("oracle: %d steps to a 1e-12 drop, forces %s" % (ts, f_ref), flush=True)
+    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+        for tol in (1e-10, 1e-12):
+            s = fsm.Solver(m, p, prec)
+            s.init_freestream()
+            steps, drop = s.solve_steady(2000, tol)
+            f = s.body_forces(steps)
+            rho = s.get("Q")[0:4 * nn:4]
+            rel = np.linalg.norm(rho - rho_ref) / np.linalg.norm(rho_ref)
+            ferr = np.abs(f - f_ref) / np.abs(f_ref)
+            print("%s to %.0e: %d steps drop %.2e; density L2 rel err vs oracle %.3e; lift/drag rel err %.3e %.3e"
+                  % (name, tol, steps, drop, rel, ferr[0], ferr[1]), flush=True)
+            assert drop <= tol
+            if tol == 1e-12:
+                assert rel <= 1e-10
+                assert np.all(ferr <= 1e-9)
+            else:
+                assert rel <= 1e-8
+            s.close()
