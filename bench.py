@@ -374,6 +374,66 @@ def cpu_baseline(U, fsm, p, nx, ny, steps):
             "sample": "%d k=2-phase pseudo-time steps of the same %dx%d bump channel (%.1f s)" % (steps, nx, ny, dt)}
 
 
+def _cpu_batch_worker(case, steps, barrier, queue):
+    """One batch solve on the oracle in its own process (spawned: no GPU):
+    mesh preparation untimed, then `steps` pseudo-time steps timed after a
+    barrier shared by all workers."""
+    import fsutil as U
+    import workloads as W
+    p = workload_params(W, [case])[0]
+    m = W.make_mesh(case)
+    m.prepare(p)
+    o = U.Oracle(m, p)
+    o.init_freestream()
+    info = m.info()
+    barrier.wait()
+    t0 = time.time()
+    for ts in range(1, steps + 1):
+        o.solve(ts, multicolor=0)
+    queue.put((t0, time.time(), (info["nsub"] + info["nbe"]) * steps, info["ntri"]))
+
+
+def cpu_baseline_all_cores(steps=3):
+    """The oracle on every host core this process may use (at most 16): one
+    independent batch solve per core, the first solves of the batch workload,
+    timed together after a barrier.  Throughput = the sum of the per-core
+    rates (a CPU deployment keeps every core busy with a queue of solves);
+    the span from the first start to the last finish is reported beside it."""
+    import multiprocessing as mp
+    import workloads as W
+    ncores = min(16, len(os.sched_getaffinity(0)))
+    cases = W.batch_cases(ncores)
+    ctx = mp.get_context("spawn")
+    barrier, queue = ctx.Barrier(ncores), ctx.Queue()
+    procs = [ctx.Process(target=_cpu_batch_worker, args=(c, steps, barrier, queue)) for c in cases]
+    for pr in procs:
+        pr.start()
+    res = []
+    while len(res) < len(procs):
+        try:
+            res.append(queue.get(timeout=5))
+        except Exception:
+            if any(pr.exitcode not in (None, 0) for pr in procs):
+                for pr in procs:
+                    if pr.is_alive():
+                        pr.terminate()
+                raise RuntimeError("a CPU baseline worker failed")
+    for pr in procs:
+        pr.join()
+    span = max(r[1] for r in res) - min(r[0] for r in res)
+    rate = sum(r[2] / (r[1] - r[0]) for r in res)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": rate, "unit": "edge-flux evals/s", "cores": ncores, "kind": "port", "cpu": model,
+            "value_over_span": sum(r[2] for r in res) / span,
+            "sample": "%d first-order-phase pseudo-time steps of each of the first %d batch solves (%d triangles "
+                      "in all), one process per core, %.1f s" % (steps, ncores, sum(r[3] for r in res), span)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -457,6 +517,8 @@ def main():
         line["serving"] = sv
     if "cpu" in sections and not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(U, fsm, p, a.nx, a.ny, a.cpu_steps)
+        line["cpu_baseline_all_cores"] = cpu_baseline_all_cores()
+        print("cpu baselines done", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 
