@@ -253,15 +253,19 @@ __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
 // ---------------------------------------------------------------------------
 // K5+K7: edge-centric face fluxes and forward-difference Jacobian blocks
 // (residual.C:112-333 with Reconstruct, reconstruct.C:181-205, and
-// compute_numerical_jacobian_roe, jacobian.C:45-185).  4 lanes per edge:
-// every lane evaluates the face flux of the edge's 1-2 subedges in ascending
-// order, lane c stores flux component c and perturbs state component c, so
-// it owns column c of the edge's two off-diagonal blocks (no atomics).  Each
-// block entry is accumulated over the subedges in the reference's order.
+// compute_numerical_jacobian_roe, jacobian.C:45-185).  2 lanes per edge:
+// both evaluate the face flux of the edge's 1-2 subedges in ascending order;
+// lane 0 stores +f at the L incidence and perturbs the L state (block (R,L)),
+// lane 1 stores -f at the R incidence and perturbs the R state (block
+// (L,R)).  The unperturbed side's primitive state and physical flux are
+// computed once per subedge and reused by the four perturbed evaluations,
+// which are the same operations on the same operands as four full
+// Roe_flux_centered calls.  Each lane owns its block (no atomics); each entry
+// is accumulated over the subedges in the reference's order.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
-  int e = tid >> 2, c = tid & 3;
+  int e = tid >> 1, side = tid & 1;
   if (e >= m.ne) return;
   const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
   if (!ip.active) return;
@@ -280,9 +284,9 @@ __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st)
     qL0[j] = nQ[4 * (size_t)L + j];
     qR0[j] = nQ[4 * (size_t)R + j];
   }
-  double colLR[4], colRL[4];
+  double blk[16];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) { colLR[j] = 0.0; colRL[j] = 0.0; }
+  for (int k = 0; k < 16; ++k) blk[k] = 0.0;
   for (int t = 0; t < 2; ++t) {
     int s = m.edge_sub[2 * e + t];
     if (s < 0) break;
@@ -302,36 +306,46 @@ __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st)
         qr[j] += (phiR * grad[8 * (size_t)R + 2 * j] * dxr0 + phiR * grad[8 * (size_t)R + 2 * j + 1] * dxr1);
       }
     }
-    double f[4];
-    roe_flux(nx, ny, len, gamma, ql, qr, f);
+    const Prim PL = prim(ql, gamma), PR = prim(qr, gamma);
+    double FL[4], FR[4], f[4];
+    euler_flux_prim(PL, FL, nx, ny, len);
+    euler_flux_prim(PR, FR, nx, ny, len);
+    roe_flux_parts(nx, ny, len, gamma, ql, qr, PL, PR, FL, FR, f);
     // R[L] += f and R[R] -= f become additions of +f and -f (bit-identical)
-    const double fcomp = (c == 0) ? f[0] : (c == 1) ? f[1] : (c == 2) ? f[2] : f[3];
-    inc_flux[4 * kL + c] = fcomp;
-    inc_flux[4 * kR + c] = -fcomp;
+    double4* dst = reinterpret_cast<double4*>(inc_flux + 4 * (side ? kR : kL));
+    *dst = side ? make_double4(-f[0], -f[1], -f[2], -f[3]) : make_double4(f[0], f[1], f[2], f[3]);
     if (jupdate) {
-      double qp[4], fp[4];
+      // the side this lane perturbs and the side it keeps
+      const double* qs = side ? qr : ql;
+      const Prim& Pk = side ? PL : PR;
+      const double* Fk = side ? FL : FR;
+      const double* qk = side ? ql : qr;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) qp[j] = (j == c) ? ql[j] + eps : ql[j];
-      roe_flux(nx, ny, len, gamma, qp, qr, fp);
+      for (int i = 0; i < 4; ++i) {
+        double qp[4], fp[4], Fp[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) colRL[j] -= (fp[j] - f[j]) / eps;
+        for (int j = 0; j < 4; ++j) qp[j] = qs[j];
+        qp[i] += eps;
+        const Prim Pp = prim(qp, gamma);
+        euler_flux_prim(Pp, Fp, nx, ny, len);
+        // (qp, qr) for the L side, (ql, qp) for the R side
+        roe_flux_parts(nx, ny, len, gamma, side ? qk : qp, side ? qp : qk, side ? Pk : Pp, side ? Pp : Pk,
+                       side ? Fk : Fp, side ? Fp : Fk, fp);
+        // block (R,L) -= dF/dqL, block (L,R) += dF/dqR; x - d is x + (-d)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) qp[j] = (j == c) ? qr[j] + eps : qr[j];
-      roe_flux(nx, ny, len, gamma, ql, qp, fp);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) colLR[j] += (fp[j] - f[j]) / eps;
+        for (int j = 0; j < 4; ++j) {
+          const double d = (fp[j] - f[j]) / eps;
+          blk[j * 4 + i] += side ? d : -d;
+        }
+      }
     }
   }
   // the edge owns its two off-diagonal blocks: written every FP64 pass
   // (zeros when the instance skips the Jacobian, as build_matrix clears
   // them), never by the mixed path's FP64 residual pass
-  double* blr = st.LHS + 16 * (size_t)m.edge_slot[2 * e];
-  double* brl = st.LHS + 16 * (size_t)m.edge_slot[2 * e + 1];
+  double2* dst = reinterpret_cast<double2*>(st.LHS + 16 * (size_t)m.edge_slot[2 * e + (side ? 0 : 1)]);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    blr[4 * j + c] = colLR[j];
-    brl[4 * j + c] = colRL[j];
-  }
+  for (int k = 0; k < 8; ++k) dst[k] = make_double2(blk[2 * k], blk[2 * k + 1]);
 }
 
 // Residual-only face fluxes of the mixed path's FP64 outer residual, one
@@ -773,7 +787,7 @@ void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
 void launch_edge_flux(const DevMesh& m, const DevState& s, int jac_allowed, hipStream_t st) {
   if (m.ne == 0) return;
   if (jac_allowed)
-    hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(4 * m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
+    hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(2 * m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
   else if (m.nsub > 0)
     hipLaunchKernelGGL(k_sub_flux, dim3(nblocks(m.nsub, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
 }

@@ -23,35 +23,46 @@ namespace fsd {
 
 constexpr int NV = 4;
 
+// Primitive quantities of one state as flux.C computes them (flux.C:37-46,
+// flux.C:356-371): u = rho u / rho, v = rho v / rho, p, H = E + p, h = H/rho.
+// Every function below reads them from here, so a state shared by several
+// flux evaluations (the unperturbed side of a finite-difference Jacobian) is
+// converted once with the same operations.
+struct Prim {
+  double rho, u, v, E, p, H, h;
+};
+FS_HD Prim prim(const double* q, double gamma) {
+  Prim s;
+  s.rho = q[0];
+  s.u = q[1] / q[0];
+  s.v = q[2] / q[0];
+  s.E = q[3];
+  s.p = (gamma - 1.) * (s.E - 0.5 * s.rho * (s.u * s.u + s.v * s.v));
+  s.H = s.E + s.p;
+  s.h = s.H / s.rho;
+  return s;
+}
+
 // physical flux F(q).n*len (flux.C:37-68)
+FS_HD void euler_flux_prim(const Prim& s, double* F, double nx, double ny, double len) {
+  double theta = s.u * nx + s.v * ny;
+  F[0] = s.rho * theta * len;
+  F[1] = s.rho * s.u * theta * len + s.p * nx * len;
+  F[2] = s.rho * s.v * theta * len + s.p * ny * len;
+  F[3] = theta * (s.E + s.p) * len;
+}
 FS_HD void euler_flux(const double* Q, double* F, double nx, double ny, double len, double gamma) {
-  double rho = Q[0];
-  double u = Q[1] / Q[0];
-  double v = Q[2] / Q[0];
-  double E = Q[3];
-  double P = (gamma - 1.0) * (E - 0.5 * rho * (u * u + v * v));
-  double theta = u * nx + v * ny;
-  F[0] = rho * theta * len;
-  F[1] = rho * u * theta * len + P * nx * len;
-  F[2] = rho * v * theta * len + P * ny * len;
-  F[3] = theta * (E + P) * len;
+  euler_flux_prim(prim(Q, gamma), F, nx, ny, len);
 }
 
 // Roe matrix T |Lambda| Tinv for the Roe average of (qL, qR)
 // (flux.C:356-524; MMMult_DtG / MMMult_GtG of linear_algebra.C:657-714)
-FS_HD void roe_matrix(double nx, double ny, double gamma, const double* qleft, const double* qright,
-                      double* Roemat) {
-  double rhoL = qleft[0], uL = qleft[1] / qleft[0], vL = qleft[2] / qleft[0], EL = qleft[3];
-  double pL = (gamma - 1.) * (EL - 0.5 * rhoL * (uL * uL + vL * vL));
-  double HL = EL + pL, hL = HL / rhoL;
-  double rhoR = qright[0], uR = qright[1] / qright[0], vR = qright[2] / qright[0], ER = qright[3];
-  double pR = (gamma - 1.) * (ER - 0.5 * rhoR * (uR * uR + vR * vR));
-  double HR = ER + pR, hR = HR / rhoR;
-  double rhobar = sqrt(rhoL * rhoR);
-  double big_gamma = rhobar / (rhoL + rhobar);
-  double ubar = uL + big_gamma * (uR - uL);
-  double vbar = vL + big_gamma * (vR - vL);
-  double hbar = hL + big_gamma * (hR - hL);
+FS_HD void roe_matrix_prim(double nx, double ny, double gamma, const Prim& L, const Prim& R, double* Roemat) {
+  double rhobar = sqrt(L.rho * R.rho);
+  double big_gamma = rhobar / (L.rho + rhobar);
+  double ubar = L.u + big_gamma * (R.u - L.u);
+  double vbar = L.v + big_gamma * (R.v - L.v);
+  double hbar = L.h + big_gamma * (R.h - L.h);
   double thetabar = ubar * nx + vbar * ny;
   double V2bar = (ubar * ubar + vbar * vbar);
   double cbar = sqrt((gamma - 1.0) * (hbar - 0.5 * V2bar));
@@ -103,13 +114,17 @@ FS_UNROLL
       Roemat[i * 4 + j] = acc;
     }
 }
+FS_HD void roe_matrix(double nx, double ny, double gamma, const double* qleft, const double* qright, double* Roemat) {
+  roe_matrix_prim(nx, ny, gamma, prim(qleft, gamma), prim(qright, gamma), Roemat);
+}
 
-// Roe_flux_centered (flux.C:299-541), RECON_PRIM = 0
-FS_HD void roe_flux(double nx, double ny, double len, double gamma, const double* qleft, const double* qright,
-                    double* flux) {
+// Roe_flux_centered (flux.C:299-541), RECON_PRIM = 0, from the two states,
+// their primitive quantities and their physical fluxes
+FS_HD void roe_flux_parts(double nx, double ny, double len, double gamma, const double* qleft, const double* qright,
+                          const Prim& L, const Prim& R, const double* FL, const double* FR, double* flux) {
   double Rm[16];
-  roe_matrix(nx, ny, gamma, qleft, qright, Rm);
-  double dQ[4], diss[4], FL[4], FR[4];
+  roe_matrix_prim(nx, ny, gamma, L, R, Rm);
+  double dQ[4], diss[4];
 FS_UNROLL
   for (int i = 0; i < 4; i++) dQ[i] = qright[i] - qleft[i];
 FS_UNROLL
@@ -119,10 +134,16 @@ FS_UNROLL
     for (int j = 0; j < 4; j++) acc += Rm[i * 4 + j] * dQ[j];
     diss[i] = acc * len;
   }
-  euler_flux(qleft, FL, nx, ny, len, gamma);
-  euler_flux(qright, FR, nx, ny, len, gamma);
 FS_UNROLL
   for (int i = 0; i < 4; i++) flux[i] = 0.5 * (FL[i] + FR[i] - diss[i]);
+}
+FS_HD void roe_flux(double nx, double ny, double len, double gamma, const double* qleft, const double* qright,
+                    double* flux) {
+  const Prim L = prim(qleft, gamma), R = prim(qright, gamma);
+  double FL[4], FR[4];
+  euler_flux_prim(L, FL, nx, ny, len);
+  euler_flux_prim(R, FR, nx, ny, len);
+  roe_flux_parts(nx, ny, len, gamma, qleft, qright, L, R, FL, FR, flux);
 }
 
 // max(|theta+c|, |theta-c|) of one node's own state (cv_calc.C:819-849)
