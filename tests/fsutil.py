@@ -188,6 +188,16 @@ def read_dump(path):
     return out
 
 
+def golden_mesh(g):
+    """The seeded mesh a golden fixture was written from: the bump channel,
+    or the O-mesh when the fixture records (first cell, radius, jitter)."""
+    nx, ny, seed = (int(v) for v in g["params"][:3])
+    if "airfoil" in g.files:
+        h0, radius, jitter = (float(v) for v in g["airfoil"])
+        return fsm.Mesh.airfoil(nx, ny, h0, radius, jitter, seed)
+    return fsm.Mesh.bump(nx, ny, 0.2, seed)
+
+
 def write_case(dirpath, mesh, params):
     os.makedirs(dirpath, exist_ok=True)
     mesh.write(os.path.join(dirpath, "grid.mesh"), os.path.join(dirpath, "bc.dat"))

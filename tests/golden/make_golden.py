@@ -33,9 +33,12 @@ def params(**kw):
 
 
 def run_case(name, nx, ny, seed, p, nsteps, stencil, step_fields, dump_every=1, keep_steps=None, extra=None,
-             setup=True, lhs_steps=()):
+             setup=True, lhs_steps=(), airfoil=None):
+    """airfoil = (first cell, outer radius, jitter): an O-mesh of nx wall x ny
+    radial nodes instead of the bump channel."""
     with tempfile.TemporaryDirectory() as tmp:
-        mesh = fsm.Mesh.bump(nx, ny, 0.2, seed)
+        mesh = fsm.Mesh.airfoil(nx, ny, airfoil[0], airfoil[1], airfoil[2], seed) if airfoil else \
+            fsm.Mesh.bump(nx, ny, 0.2, seed)
         U.write_case(tmp, mesh, p)
         out = os.path.join(tmp, "dump.bin")
         r = subprocess.run([U.REF_DRIVER, tmp, out, str(nsteps), "1" if stencil else "0", str(dump_every)],
@@ -46,6 +49,8 @@ def run_case(name, nx, ny, seed, p, nsteps, stencil, step_fields, dump_every=1, 
     keep = {"params": np.array([nx, ny, seed, p.order, p.foits, p.soits, p.limit, nsteps], dtype=np.int64),
             "params_f": np.array([p.CFL, p.mach, p.alpha, p.ramp, p.subits], dtype=np.float64),
             "hdr": d["hdr"]}
+    if airfoil:
+        keep["airfoil"] = np.array(airfoil, dtype=np.float64)
     for k in (SETUP_INT + SETUP_DBL) if setup else ["Q0"]:
         keep["s_" + k] = d[k]
     if stencil:
@@ -75,6 +80,10 @@ def main():
     # Venkatakrishnan limiter (limit=2) at the k=1 face points
     run_case("bump17x9_o1_venkat", 17, 9, 7, params(order=1, foits=2, limit=2, mach=0.675), 6, False, full,
              lhs_steps=(6,))
+    # O-mesh around the 12%-thick profile: closed wall, far field, seam of the
+    # periodic direction; limited linear phase at an angle of attack
+    run_case("airfoil32x9_o1", 32, 9, 4, params(order=1, foits=2, mach=0.5, alpha=2.0), 6, False, full,
+             lhs_steps=(6,), airfoil=(2e-2, 8.0, 0.1))
     # the CPU-runnable configuration: 97x33 bump channel, M=0.5, k=1, Barth limiter
     run_case("cfg1_97x33", 97, 33, 1, params(order=3, foits=20, soits=20, mach=0.5, alpha=0.0, CFL=100.0, ramp=30,
                                               subits=20), 2, True, ["rms", "R", "dQ", "Q", "forces"])

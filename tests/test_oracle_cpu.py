@@ -35,14 +35,14 @@ def _setup(g, **over):
               subits=int(subits), method=1, iter=1, jufreq=1, reorder=1, gamma=1.4, nsteps=nsteps)
     kw.update(over)
     p = fsm.Params.make(**kw)
-    m = fsm.Mesh.bump(nx, ny, 0.2, seed)
+    m = U.golden_mesh(g)
     m.prepare(p)
     o = U.Oracle(m, p)
     o.init_freestream()
     return p, m, o, nsteps
 
 
-@pytest.mark.parametrize("case", ["bump17x9_o1", "bump17x9_o3", "bump17x9_o1_venkat", "cfg1_97x33"])
+@pytest.mark.parametrize("case", ["bump17x9_o1", "bump17x9_o3", "bump17x9_o1_venkat", "cfg1_97x33", "airfoil32x9_o1"])
 def test_step_history_bitexact(case):
     g = _load(case)
     p, m, o, nsteps = _setup(g)

@@ -9,7 +9,7 @@ import pytest
 import fsutil as U
 
 fsm = U.fsm
-CASES = ["bump17x9_o1", "bump17x9_o3", "cfg1_97x33"]
+CASES = ["bump17x9_o1", "bump17x9_o3", "cfg1_97x33", "airfoil32x9_o1"]
 
 
 def _load(name):
@@ -21,7 +21,7 @@ def _mesh_for(g):
     cfl, mach, alpha, ramp, subits = g["params_f"]
     p = fsm.Params.make(order=order, foits=foits, soits=soits, limit=limit, mach=mach, alpha=alpha, CFL=cfl,
                         ramp=int(ramp), subits=int(subits), method=1, iter=1, jufreq=1, reorder=1, gamma=1.4)
-    m = fsm.Mesh.bump(nx, ny, 0.2, seed)
+    m = U.golden_mesh(g)
     m.prepare(p)
     return p, m
 
