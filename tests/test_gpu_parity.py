@@ -32,9 +32,12 @@ def _params(**kw):
     return fsm.Params.make(**base)
 
 
-def _pair(nx, ny, seed, **kw):
+def _pair(nx, ny, seed, airfoil=False, **kw):
+    """Device solver and oracle on one seeded mesh: the bump channel of nx x ny
+    nodes, or (airfoil) the O-mesh of nx wall x ny radial nodes around the
+    12%-thick profile."""
     p = _params(**kw)
-    m = fsm.Mesh.bump(nx, ny, 0.2, seed)
+    m = fsm.Mesh.airfoil(nx, ny, 2e-2, 8.0, 0.1, seed) if airfoil else fsm.Mesh.bump(nx, ny, 0.2, seed)
     m.prepare(p)
     s = fsm.Solver(m, p, fsm.FS_PREC_F64)
     o = U.Oracle(m, p)
@@ -168,10 +171,13 @@ def test_roe_wave_form_matches_reference_form():
     assert worst <= 1e-13
 
 
-@pytest.mark.parametrize("order,soits,limit", [(1, 0, 1), (3, 2, 1), (1, 0, 2), (3, 2, 2)])
-def test_step_history(order, soits, limit):
-    """fs_solve against the oracle's solve with the same multicolor sweep order."""
-    p, m, s, o = _pair(33, 17, 2, order=order, soits=soits, limit=limit, subits=10)
+@pytest.mark.parametrize("order,soits,limit,airfoil", [(1, 0, 1, False), (3, 2, 1, False), (1, 0, 2, False),
+                                                       (3, 2, 2, False), (1, 0, 1, True), (3, 2, 1, True)])
+def test_step_history(order, soits, limit, airfoil):
+    """fs_solve against the oracle's solve with the same multicolor sweep order
+    (bump channel, and the O-mesh with its closed wall)."""
+    p, m, s, o = (_pair(40, 11, 4, airfoil=True, order=order, soits=soits, limit=limit, subits=10, alpha=2.0)
+                  if airfoil else _pair(33, 17, 2, order=order, soits=soits, limit=limit, subits=10))
     s.init_freestream()
     o.init_freestream()
     nsteps = p.foits + soits + 4
@@ -287,12 +293,14 @@ def _oracle_from_device(s, o, info):
     assert o["pv"].size == 4 * nn
 
 
-@pytest.mark.parametrize("order,soits,limit", [(1, 0, 1), (3, 2, 1), (3, 2, 2), (1, 0, 0)])
-def test_body_forces_match_oracle(order, soits, limit):
+@pytest.mark.parametrize("order,soits,limit,airfoil", [(1, 0, 1, False), (3, 2, 1, False), (3, 2, 2, False),
+                                                       (1, 0, 0, False), (3, 2, 1, True)])
+def test_body_forces_match_oracle(order, soits, limit, airfoil):
     """Compute_Body_Forces on the device (per-bedge Gauss-point terms summed
     in bedge order) equals the oracle's bit for bit from the same fields, in
     every phase: nodal, limited linear and the k=2 point-value quadratic."""
-    p, m, s, o = _pair(33, 17, 21, order=order, soits=soits, limit=limit, alpha=2.0, subits=8)
+    p, m, s, o = (_pair(40, 11, 21, airfoil=True, order=order, soits=soits, limit=limit, alpha=2.0, subits=8)
+                  if airfoil else _pair(33, 17, 21, order=order, soits=soits, limit=limit, alpha=2.0, subits=8))
     info = m.info()
     s.init_freestream()
     nsteps = p.foits + soits + 3
