@@ -375,7 +375,7 @@ int fs_interior_residual(fs_solver* s, int jupdate, int ts) {
   fsk::InstParams p = mirror_params(s, ts);
   p.jac = jupdate ? 1 : 0;
   if (int rc = s->eng.upload_params_custom(p)) return rc;
-  fsk::launch_edge_flux(s->eng.dm, s->eng.ds, 1, s->eng.stream);
+  fsk::launch_edge_flux(s->eng.dm, s->eng.ds, s->eng.stream);
   fsk::launch_node(s->eng.dm, s->eng.ds, fsk::NODE_INTERIOR | fsk::NODE_JAC, s->eng.stream);
   return finish(s, false);
 }

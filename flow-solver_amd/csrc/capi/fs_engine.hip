@@ -84,9 +84,10 @@ size_t footprint_bytes(const fs::Mesh& g, int prec) {
   size_t b = 0;
   // mesh image
   b += nn * (4 * 4 + 8 * 2 + 8 + 8 * 9 + 4 * 4 + 4 * 3) + ne * 4 * 6 + nsub * (8 * 3 + 4 * 2);
-  b += ninc * (8 * 12) + neinc * (4 + 16) + nbe * (4 * 4 + 8 * 9) + nnz * 8 + (nn + 1) * 4 * 4 + nst * (4 + 40);
+  b += ninc * (8 * 12 + 4) + neinc * (4 + 16) + nbe * (4 * 4 + 8 * 9) + nnz * 8 + (nn + 1) * 4 * 4 + nst * (4 + 40);
   // state
-  b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + ninc * 4 * 8 + nbe * 8 * 22;
+  b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + nbe * 8 * 22;
+  if (prec != FS_PREC_MIXED) b += ninc * 4 * 8;   // per-incidence face fluxes of the FP64 pass
   if (prec == FS_PREC_MIXED)
     b += nnz * 16 * 4 + nn * (16 * 4 + 4 * 4 * 2);
   else
@@ -217,7 +218,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   std::vector<double> sub_n(3 * NS);
   std::vector<int> sub_edge_d(NS);
   std::vector<int> nsub_off(NN + 1), nedge_off(NN + 1), nbe_off(NN + 1), st_off(NN + 1);
-  std::vector<int> sub_inc(2 * NS), inc_other(EINC), nbe_list(BINC), st_nodes(NST);
+  std::vector<int> sub_inc(2 * NS), inc_other(EINC), inc_sother(KINC), nbe_list(BINC), st_nodes(NST);
   std::vector<double> inc_n(4 * KINC), inc_fc(2 * KINC), inc_xg(6 * KINC), inc_gw(2 * EINC), st_Wt(5 * NST);
   std::vector<int> breal(NB), bbc(NB);
   std::vector<double> bn(3 * NB), bxg(6 * NB);
@@ -255,6 +256,11 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       int is = ent & 0x7fffffff;
       size_t K = (size_t)kinc + k;
       sub_inc[2 * (size_t)(so + is) + (ent < 0 ? 1 : 0)] = (int)K;
+      {
+        const int e = g.sub_edge[is];
+        const int other = (ent < 0 ? g.edges[2 * (size_t)e] : g.edges[2 * (size_t)e + 1]) + no;
+        inc_sother[K] = other | (ent < 0 ? (int)0x80000000u : 0);
+      }
       for (int c = 0; c < 3; ++c) inc_n[4 * K + c] = g.sub_n[3 * (size_t)is + c];
       inc_n[4 * K + 3] = 0.0;
       int el = g.sub_elem[is];
@@ -404,6 +410,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = upload(&d.nsub_off, nsub_off)) || (rc = upload(&d.nedge_off, nedge_off)) ||
       (rc = upload(&d.sub_inc, sub_inc)) || (rc = upload(&d.inc_n, inc_n)) || (rc = upload(&d.inc_fc, inc_fc)) ||
       (rc = upload(&d.inc_xg, inc_xg)) || (rc = upload(&d.inc_other, inc_other)) ||
+      (rc = upload(&d.inc_sother, inc_sother)) ||
       (rc = upload(&d.inc_gw, inc_gw)) || (rc = upload(&d.nbe_off, nbe_off)) ||
       (rc = upload(&d.nbe_list, nbe_list)) || (rc = upload(&d.bedge_real, breal)) ||
       (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.bedge_xg, bxg)) || (rc = upload(&d.roff, roff)) ||
@@ -421,7 +428,8 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   if ((rc = alloc(&st.Q, ntot * 4)) || (rc = alloc(&st.nQ, ntot * 4)) || (rc = alloc(&st.dQ, nn * 4)) ||
       (rc = alloc(&st.R, nn * 4)) || (rc = alloc(&st.RHS, nn * 4)) || (rc = alloc(&st.grad, nn * 8)) ||
       (rc = alloc(&st.phi, nn)) || (rc = alloc(&st.hess, nn * 20)) || (rc = alloc(&st.pv, nn * 4)) ||
-      (rc = alloc(&st.ev_con, nn)) || (rc = alloc(&st.dt, nn)) || (rc = alloc(&st.inc_flux, (size_t)KINC * 4)) ||
+      (rc = alloc(&st.ev_con, nn)) || (rc = alloc(&st.dt, nn)) ||
+      (prec != FS_PREC_MIXED && (rc = alloc(&st.inc_flux, (size_t)KINC * 4))) ||
       (rc = alloc(&st.bflux, (size_t)NB * 4)) || (rc = alloc(&st.bjac, (size_t)NB * 16)) ||
       (rc = alloc(&st.bforce, (size_t)NB * 2)) || (rc = alloc(&d_fp_, (size_t)B)) ||
       (rc = alloc(&st.err, (size_t)B)) || (rc = alloc(&st.inst_norms, 2 * (size_t)B)) ||
@@ -558,11 +566,11 @@ int Engine::newton_iteration(int it, int key, bool record) {
   fsk::launch_limiter(dm, s, st);
   fsk::ColorRanges cr{color_off.data(), color_nback.data()};
   if (precision == FS_PREC_MIXED) {
-    // FP64 residual without Jacobian, FP32 analytic Jacobian and sweeps
+    // FP32 analytic Jacobian; the FP64 residual is evaluated inside the node
+    // pass (with ev_con/dt and the diagonal blocks); FP32 sweeps
     if (record) FSE_HIP(hipEventRecord(ev_[0], st));
-    fsk::launch_edge_flux(dm, s, 0, st);
-    if (record) FSE_HIP(hipEventRecord(ev_[1], st));
     fsk::launch_edge_jac32(dm, s, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[1], st));
     fsk::launch_bedge(dm, s, st);
     if (record) FSE_HIP(hipEventRecord(ev_[2], st));
     fsk::launch_node_mixed(dm, s, st);
@@ -575,7 +583,7 @@ int Engine::newton_iteration(int it, int key, bool record) {
     fsk::launch_norms(dm, s, nc, st);
   } else {
     if (record) FSE_HIP(hipEventRecord(ev_[0], st));
-    fsk::launch_edge_flux(dm, s, 1, st);
+    fsk::launch_edge_flux(dm, s, st);
     if (record) FSE_HIP(hipEventRecord(ev_[1], st));
     fsk::launch_bedge(dm, s, st);
     if (record) FSE_HIP(hipEventRecord(ev_[2], st));
@@ -847,9 +855,9 @@ int Engine::autotune(double* seconds) {
   for (int b : blocks) {
     tu.face_block = b;
     if (precision == FS_PREC_MIXED)
-      rc = time_it([&] { fsk::launch_edge_flux(dm, s, 0, st); fsk::launch_edge_jac32(dm, s, st); }, 2, &t);
+      rc = time_it([&] { fsk::launch_edge_jac32(dm, s, st); }, 2, &t);
     else
-      rc = time_it([&] { fsk::launch_edge_flux(dm, s, 1, st); }, 2, &t);
+      rc = time_it([&] { fsk::launch_edge_flux(dm, s, st); }, 2, &t);
     if (rc) return rc;
     if (t < best) { best = t; keepb = b; }
   }

@@ -1,12 +1,14 @@
 // FP64 kernels of the residual-and-implicit-update path (gfx950, wave64).
 //
-// Deterministic by construction: every per-node sum is a gather over the
-// node's incident subedges/edges/bedges in ascending index order, which is
-// the order in which the reference's sequential scatter loops add into a
-// node (residual.C:112-333 then residual.C:429-1193; gradient.C:382-403;
-// cv_calc.C:819-953).  Face fluxes are computed once per subedge by an
-// edge-centric kernel and written to a per-subedge buffer; the node kernel
-// reads them back.  No floating-point atomics anywhere.
+// Deterministic by construction: the per-node sums of the residual, the
+// gradient and ev_con are gathers over the node's incident subedges/edges/
+// bedges in ascending index order, which is the order in which the
+// reference's sequential scatter loops add into a node (residual.C:112-333
+// then residual.C:429-1193; gradient.C:382-403; cv_calc.C:819-953).  Face
+// fluxes are computed once per subedge by an edge-centric kernel and written
+// to a per-incidence buffer; the node kernel reads them back.  Other sums
+// (k=2 derivatives, norms) use fixed lane partitions and butterflies.  No
+// floating-point atomics anywhere.
 //
 // Every element reads its instance's controls (InstParams) and does nothing
 // for an inactive instance, so one launch serves a packed batch of meshes and
@@ -340,51 +342,11 @@ __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st)
       }
     }
   }
-  // the edge owns its two off-diagonal blocks: written every FP64 pass
-  // (zeros when the instance skips the Jacobian, as build_matrix clears
-  // them), never by the mixed path's FP64 residual pass
+  // the edge owns its two off-diagonal blocks: written every pass (zeros
+  // when the instance skips the Jacobian, as build_matrix clears them)
   double2* dst = reinterpret_cast<double2*>(st.LHS + 16 * (size_t)m.edge_slot[2 * e + (side ? 0 : 1)]);
 #pragma unroll
   for (int k = 0; k < 8; ++k) dst[k] = make_double2(blk[2 * k], blk[2 * k + 1]);
-}
-
-// Residual-only face fluxes of the mixed path's FP64 outer residual, one
-// thread per subedge (characteristic-wave Roe flux, same reconstruction):
-// every face is independent, so no thread loops over an edge's subedges.
-__global__ void __launch_bounds__(kMaxBlock) k_sub_flux(DevMesh m, DevState st) {
-  int s = xcd_block() * blockDim.x + threadIdx.x;
-  if (s >= m.nsub) return;
-  int e = m.sub_edge[s];
-  const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
-  if (!ip.active) return;
-  const double* __restrict__ nQ = st.nQ;
-  int L = m.edges[2 * e], R = m.edges[2 * e + 1];
-  int kL = m.sub_inc[2 * (size_t)s], kR = m.sub_inc[2 * (size_t)s + 1];
-  double nx = m.sub_n[3 * (size_t)s], ny = m.sub_n[3 * (size_t)s + 1], len = m.sub_n[3 * (size_t)s + 2];
-  double ql[4], qr[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    ql[j] = nQ[4 * (size_t)L + j];
-    qr[j] = nQ[4 * (size_t)R + j];
-  }
-  if (ip.recon_flux == 1) {
-    const double* __restrict__ grad = st.grad;
-    double fc0 = m.inc_fc[2 * (size_t)kL], fc1 = m.inc_fc[2 * (size_t)kL + 1];
-    double dxl0 = fc0 - m.x[2 * (size_t)L], dxl1 = fc1 - m.x[2 * (size_t)L + 1];
-    double dxr0 = fc0 - m.x[2 * (size_t)R], dxr1 = fc1 - m.x[2 * (size_t)R + 1];
-    double phiL = st.phi[L], phiR = st.phi[R];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      ql[j] += (phiL * grad[8 * (size_t)L + 2 * j] * dxl0 + phiL * grad[8 * (size_t)L + 2 * j + 1] * dxl1);
-      qr[j] += (phiR * grad[8 * (size_t)R + 2 * j] * dxr0 + phiR * grad[8 * (size_t)R + 2 * j + 1] * dxr1);
-    }
-  }
-  double f[4];
-  roe_flux_wave(nx, ny, len, ip.gamma, ql, qr, f);
-  double4* fl = reinterpret_cast<double4*>(st.inc_flux + 4 * (size_t)kL);
-  double4* fr = reinterpret_cast<double4*>(st.inc_flux + 4 * (size_t)kR);
-  *fl = make_double4(f[0], f[1], f[2], f[3]);
-  *fr = make_double4(-f[0], -f[1], -f[2], -f[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -784,12 +746,9 @@ void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
   hipLaunchKernelGGL(k_limiter, dim3(nblocks(16 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
 }
-void launch_edge_flux(const DevMesh& m, const DevState& s, int jac_allowed, hipStream_t st) {
+void launch_edge_flux(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.ne == 0) return;
-  if (jac_allowed)
-    hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(2 * m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
-  else if (m.nsub > 0)
-    hipLaunchKernelGGL(k_sub_flux, dim3(nblocks(m.nsub, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
+  hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(2 * m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
 }
 void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nbe == 0) return;

@@ -75,6 +75,7 @@ struct DevMesh {
   const double* inc_n;      // 4 per subedge incidence: (nx, ny, len, 0)
   const double* inc_fc;     // 2 per subedge incidence: face point (k=1)
   const double* inc_xg;     // 6 per subedge incidence: Gauss points (k=2)
+  const int* inc_sother;    // per subedge incidence: other node | (1<<31 if this node is R)
   const int* inc_other;     // per edge incidence: other node | (1<<31 if this node is R)
   const double* inc_gw;     // 2 per edge incidence: LSQ weights of this side
   const int* nbe_off;       // nn+1
@@ -176,13 +177,12 @@ struct ColorRanges {
   const int* nback;   // ncolor, host: rows of a color visited by backward launches
 };
 
-// Per-instance modes come from DevState::ip.  `jac_allowed` = 0 suppresses the
-// FD Jacobian of the FP64 face pass (the mixed path's FP64 residual).
+// Per-instance modes come from DevState::ip.
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
 void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st);
-void launch_edge_flux(const DevMesh& m, const DevState& s, int jac_allowed, hipStream_t st);
+void launch_edge_flux(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_node(const DevMesh& m, const DevState& s, int flags, hipStream_t st);
 void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st);

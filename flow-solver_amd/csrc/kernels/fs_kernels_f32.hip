@@ -152,9 +152,19 @@ __global__ void __launch_bounds__(kMaxBlock) k_edge_jac32(DevMesh m, DevState st
   }
 }
 
-// Node assembly of the mixed step: FP64 ev_con/dt and residual gather (same
-// order as the FP64 path), FP32 diagonal block V/dt I - sum of transposed
+// Node pass of the mixed step: FP64 ev_con/dt (same order as the FP64 path),
+// the FP64 interior residual, FP32 diagonal block V/dt I - sum of transposed
 // off-diagonals + boundary blocks, FP32 right-hand side -R.
+//
+// The residual is accumulated here rather than through a per-subedge flux
+// buffer: lane v of the node's 4 evaluates the characteristic-wave Roe flux
+// of the node's subedge incidences v, v+4, ... (the flux of the subedge with
+// the reference's L/R orientation and reconstruction, +f where the node is
+// L and -f where it is R), then the 4 partial sums are combined by a fixed
+// butterfly.  Each subedge is evaluated at both of its nodes (twice the
+// flux arithmetic, none of the buffer's 64 bytes of writes and reads per
+// subedge); the sum order depends only on the node's own incidence list, so
+// an instance's residual is the same alone and in a batch.
 __global__ void __launch_bounds__(kMaxBlock) k_node_mixed(DevMesh m, DevState s) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
@@ -167,11 +177,63 @@ __global__ void __launch_bounds__(kMaxBlock) k_node_mixed(DevMesh m, DevState s)
 #pragma unroll
   for (int j = 0; j < 4; ++j) q[j] = s.nQ[4 * (size_t)n + j];
   double Rv = 0.0, ev = 0.0;
+  const int k0 = m.nsub_off[n], k1 = m.nsub_off[n + 1];
 #pragma unroll 4
-  for (int k = m.nsub_off[n]; k < m.nsub_off[n + 1]; ++k) {
+  for (int k = k0; k < k1; ++k) {
     const double* nr = m.inc_n + 4 * (size_t)k;
     ev += (max_ev(q, nr[0], nr[1], p.gamma) * nr[2]);
-    Rv += s.inc_flux[4 * (size_t)k + v];
+  }
+  {
+    const int recon = p.recon_flux;
+    const double* __restrict__ grad = s.grad;
+    double gs[8], phis = 0.0, xs0 = 0.0, xs1 = 0.0;
+    if (recon == 1) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) gs[c] = grad[8 * (size_t)n + c];
+      phis = s.phi[n];
+      xs0 = m.x[2 * (size_t)n];
+      xs1 = m.x[2 * (size_t)n + 1];
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = k0 + v; k < k1; k += 4) {
+      const int ent = m.inc_sother[k];
+      const int o = ent & 0x7fffffff;
+      const bool isR = ent < 0;
+      const double* nr = m.inc_n + 4 * (size_t)k;
+      double qs[4], qo[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        qs[j] = q[j];
+        qo[j] = s.nQ[4 * (size_t)o + j];
+      }
+      if (recon == 1) {
+        // reconstruct.C:181-205 at the subedge's face point, for both ends
+        const double fc0 = m.inc_fc[2 * (size_t)k], fc1 = m.inc_fc[2 * (size_t)k + 1];
+        const double dxs0 = fc0 - xs0, dxs1 = fc1 - xs1;
+        const double dxo0 = fc0 - m.x[2 * (size_t)o], dxo1 = fc1 - m.x[2 * (size_t)o + 1];
+        const double phio = s.phi[o];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          qs[j] += (phis * gs[2 * j] * dxs0 + phis * gs[2 * j + 1] * dxs1);
+          qo[j] += (phio * grad[8 * (size_t)o + 2 * j] * dxo0 + phio * grad[8 * (size_t)o + 2 * j + 1] * dxo1);
+        }
+      }
+      double ql[4], qr[4], f[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        ql[j] = isR ? qo[j] : qs[j];
+        qr[j] = isR ? qs[j] : qo[j];
+      }
+      roe_flux_wave(nr[0], nr[1], nr[2], p.gamma, ql, qr, f);
+      // R[L] += f, R[R] -= f; adding -f is the same operation as subtracting f
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] += isR ? -f[j] : f[j];
+    }
+#pragma unroll
+    for (int off = 1; off <= 2; off <<= 1)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] += __shfl_xor(acc[j], off, 4);
+    Rv = v == 0 ? acc[0] : v == 1 ? acc[1] : v == 2 ? acc[2] : acc[3];
   }
   for (int k = m.nbe_off[n]; k < m.nbe_off[n + 1]; ++k) {
     int b = m.nbe_list[k];
