@@ -386,6 +386,43 @@ def cpu_baseline(U, fsm, p, nx, ny, steps):
             "sample": "%d k=2-phase pseudo-time steps of the same %dx%d bump channel (%.1f s)" % (steps, nx, ny, dt)}
 
 
+def cpu_reference_calibration(U, fsm, nx, ny, steps=2):
+    """The reference build itself (oracle/_ref/ref_driver, compiled from the
+    reference's sources) on the same bump channel, 1 core, against the port
+    on the same steps.  The reference build has no LAPACK, so it cannot enter
+    the k=2 phase: both run `steps` steps of the limited linear phase
+    (order 1 from the first step), which has the k=2 phase's FD Jacobian and
+    20 sequential sweeps plus the gradient and the limiter.  None when the
+    reference build is absent."""
+    import subprocess
+    import tempfile
+    if not os.path.exists(U.REF_DRIVER):
+        return None
+    p = fsm.Params.make(nsteps=steps, iter=1, jufreq=1, CFL=50.0, ramp=20, gamma=1.4, alpha=0.0, mach=0.675,
+                        subits=20, method=1, order=1, foits=0, soits=0, limit=1, reorder=1)
+    mesh = fsm.Mesh.bump(nx, ny, 0.2, 2)
+    with tempfile.TemporaryDirectory() as tmp:
+        U.write_case(tmp, mesh, p)
+        r = subprocess.run([U.REF_DRIVER, tmp, os.path.join(tmp, "dump.bin"), str(steps), "0", "0"],
+                           capture_output=True, text=True, cwd=tmp, timeout=600)
+        if r.returncode != 0:
+            return {"error": r.stderr[-300:]}
+        t_ref = float(U.read_dump(os.path.join(tmp, "dump.bin"))["times"][1])
+    mesh.prepare(p)
+    info = mesh.info()
+    o = U.Oracle(mesh, p)
+    o.init_freestream()
+    t0 = time.perf_counter()
+    for ts in range(1, steps + 1):
+        o.solve(ts, multicolor=0)
+    t_port = time.perf_counter() - t0
+    evals = (info["nsub"] + info["nbe"]) * steps
+    return {"reference_value": evals / t_ref, "port_value": evals / t_port, "unit": "edge-flux evals/s",
+            "cores": 1, "reference_over_port": t_port / t_ref,
+            "sample": "%d limited linear-phase steps of the %dx%d bump channel: reference %.1f s, port %.1f s"
+                      % (steps, nx, ny, t_ref, t_port)}
+
+
 def _cpu_batch_worker(case, steps, barrier, queue):
     """One batch solve on the oracle in its own process (spawned: no GPU):
     mesh preparation untimed, then `steps` pseudo-time steps timed after a
@@ -538,6 +575,7 @@ def main():
     if "cpu" in sections and not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(U, fsm, p, a.nx, a.ny, a.cpu_steps)
         line["cpu_baseline_all_cores"] = cpu_baseline_all_cores()
+        line["cpu_reference_calibration"] = cpu_reference_calibration(U, fsm, a.nx, a.ny)
         print("cpu baselines done", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
