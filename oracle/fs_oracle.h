@@ -34,9 +34,13 @@
  *                           phases); k=2 phase: quadratic exactness of the Gauss-point
  *                           reconstruction and its reduction to the linear phase (tests)
  *   fso_surface_cp          closed form (uniform state gives Cp of the freestream pressure)
- *   fso_limiter_barth/_venkat (k=2 Gauss points), k=2 phase of fso_solve:
- *                           parity unpinned against the reference build (it cannot
- *                           enter the k=2 phase here); pinned by the k=2 invariants
+ *   fso_limiter_barth (k=2 Gauss points): linear preservation (phi = 1 for exact
+ *                           k=2 data of a linear field) and a bowl limited only near
+ *                           its minimum (tests); no reference-build vector
+ *   fso_limiter_venkat (k=2 Gauss points), k=2 phase of fso_solve: parity unpinned
+ *                           against the reference build (it cannot enter the k=2
+ *                           phase here); the k=2 phase of fso_solve is pinned by the
+ *                           invariants of its parts
  */
 #ifndef FS_ORACLE_H
 #define FS_ORACLE_H

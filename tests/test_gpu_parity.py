@@ -388,3 +388,25 @@ def test_transonic_airfoil_fp64_and_mixed_vs_oracle():
             else:
                 assert rel <= 1e-8
             s.close()
+
+
+@pytest.mark.parametrize("limit", [1, 2])
+def test_k2_gauss_limiter_matches_oracle(limit):
+    """The k=2 phase limiter (Barth, Venkatakrishnan) at the Gauss points on
+    the device against the oracle from identical inputs: phi bit for bit."""
+    p, m, s, o = _pair(33, 17, 23, order=3, soits=2, limit=limit)
+    info = m.info()
+    Q = U.perturbed_state(info["nn"], info["ng"], p, seed=9, amp=0.08)
+    _set_state(s, o, Q)
+    k2_ts = p.foits + p.soits + 1
+    s.derivatives()
+    o["hess"][:] = s.get("hess")
+    o["pv"][:] = s.get("pv")
+    s.limiter(k2_ts)
+    o.s.citer = k2_ts
+    U.oracle_call(o, "fso_limiter_venkat" if limit == 2 else "fso_limiter_barth")
+    phi_d, phi_o = s.get("phi"), o["phi"][0::8]
+    print("k=2 limiter %d: min phi %.4f, %d of %d nodes limited" % (limit, phi_d.min(), int((phi_d < 1).sum()),
+                                                                   phi_d.size))
+    assert (phi_d < 1).sum() > 0
+    np.testing.assert_array_equal(phi_d, phi_o)

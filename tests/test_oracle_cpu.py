@@ -6,6 +6,9 @@
 * Against mathematics, for what the reference build cannot run here (the k=2
   factorization needs LAPACK): the pseudo-inverse against numpy's pinv, and the
   k=2 derivatives exact for control-volume averages of a quadratic.
+* k=2 Gauss-point limiter: with exact k=2 data of a linear field every Gauss
+  point lies inside a triangle of its node, so Barth's limiter must return
+  phi = 1 everywhere (linear preservation); a curved field must be limited.
 * Body forces: the reference build's values every step of the first-order and
   linear phases; at k=2 (which the reference build cannot enter) the boundary
   Gauss-point reconstruction is exact for quadratic fields and reduces to the
@@ -354,3 +357,51 @@ def test_surface_cp_closed_form():
         nodes = np.array(nodes)
         np.testing.assert_array_equal(xs, x[nodes, 0])
         np.testing.assert_allclose(cp, (Pfield[nodes] / pinf - 1) * 2 / (1.4 * p.mach ** 2), rtol=1e-11, atol=1e-13)
+
+
+def test_k2_gauss_limiter_preserves_linear_fields():
+    """The k=2 phase limiter (limiter.C:39-497 at the Gauss points of
+    reconstruct.C:1276-1296): a linear field with exact first moments, zero
+    second moments and exact point values reconstructs to the field itself at
+    every Gauss point, which lies inside a triangle of the node, so it never
+    leaves the edge-neighbour extrema: Barth gives phi = 1 exactly.  A bowl
+    (an interior minimum) must be limited at its minimum."""
+    p = fsm.Params.make(order=3, foits=1, soits=1, limit=1, reorder=1, mach=0.5)
+    m = fsm.Mesh.bump(33, 13, 0.2, 6)
+    m.prepare(p)
+    info = m.info()
+    nn = info["nn"]
+    x = m.array("x")[:2 * nn].reshape(-1, 2)
+    k2_ts = p.foits + p.soits + 1
+
+    def limit(q, grad_xy, second):
+        o = U.Oracle(m, p)
+        Q = np.zeros((nn + info["ng"], 4))
+        Q[:nn] = q
+        o["nQ"][:] = Q.ravel()
+        h = np.zeros((nn, 4, 5))
+        h[:, :, 0:2] = grad_xy
+        h[:, :, 2:5] = second
+        o["hess"][:] = h.ravel()
+        o["pv"][:] = q.ravel()
+        o.s.citer = k2_ts
+        U.oracle_call(o, "fso_limiter_barth")
+        return o["phi"][0::8].copy()
+
+    b = np.array([[0.3, -0.2], [1.0, 0.4], [-0.5, 0.8], [0.05, 2.0]])
+    q = 1.0 + x @ b.T
+    phi = limit(q, np.broadcast_to(b, (nn, 4, 2)), 0.0)
+    assert np.all(phi == 1.0), (phi.min(), np.argmin(phi))
+    # a bowl whose minimum lies between nodes: the exact quadratic dips below
+    # every nodal value near the minimum, and only there is it limited
+    tri = m.array("tri").reshape(-1, 3)
+    c = x[tri[len(tri) // 2]].mean(0)
+    r2 = ((x - c) ** 2).sum(1)
+    qb = np.stack([1.0 + r2] * 4, 1)
+    g = np.stack([2 * (x - c)] * 4, 1)
+    sec = np.broadcast_to(np.array([2.0, 2.0, 0.0]), (nn, 4, 3))
+    phib = limit(qb, g, sec)
+    limited = np.nonzero(phib < 1.0)[0]
+    assert len(limited) > 0
+    h = np.sqrt(np.sort(r2)[1])
+    assert np.all(np.sqrt(r2[limited]) <= 3 * h), np.sqrt(r2[limited]) / h
