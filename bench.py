@@ -533,6 +533,7 @@ def main():
                      "traffic": (None if (a.nx, a.ny) != (1025, 257) else measured_traffic("fsk::k_gs_color")),
                      "algorithmic_bytes": gs_bytes / gs_launches, "launches_per_step": gs_launches,
                      "avg_ms": tim["gs_ms"] / gs_launches},
+        "achievable": None,
         "edge_kernel_roofline": {"kernel": "k_edge_flux (Roe flux + FD Jacobian blocks, FP64 VALU)",
                                  "achieved_GBs": edge_gbs, "algorithmic_bytes": abytes, "avg_ms": tim["edge_ms"]},
         "kernel_ms": {"edge_flux": tim["edge_ms"], "node_assembly": tim["node_ms"], "gauss_seidel": tim["gs_ms"],
@@ -540,6 +541,11 @@ def main():
         "jacobian_flux_evals_per_step": 8 * info["nsub"],
         "setup_s": t_setup,
     }
+    # achievable streaming bandwidth of this device (4 GiB, beyond the
+    # last-level cache), beside the published peak the roofline uses
+    rd, cp = fsm.hbm_probe(4 << 30, 5)
+    line["achievable"] = {"read_GBs": rd, "copy_GBs": cp, "gs_frac_of_read": achieved / rd,
+                          "probe": "fs_hbm_probe: streaming float4 read and copy of 4 GiB, 5 passes each"}
     # the mixed-precision path on the same workload and the same timed steps
     sm = fsm.Solver(mesh, p, fsm.FS_PREC_MIXED)
     sm.init_freestream()

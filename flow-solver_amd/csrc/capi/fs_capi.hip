@@ -83,6 +83,45 @@ extern "C" {
 
 const char* fs_last_error(void) { return g_err.c_str(); }
 
+int fs_hbm_probe(int64_t bytes, int reps, double* read_gbs, double* copy_gbs) {
+  if (bytes < (1 << 20) || reps < 1 || !read_gbs || !copy_gbs) return fail(FS_ERR_ARG, "bad arguments");
+  const size_t n16 = (size_t)bytes / 16, nb = n16 * 16;
+  void *a = nullptr, *b = nullptr;
+  float* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FS_OK;
+  auto release = [&] {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(out);
+  };
+  if (hipMalloc(&a, nb) != hipSuccess || hipMalloc(&b, nb) != hipSuccess || hipMalloc((void**)&out, 64) != hipSuccess ||
+      hipMemset(a, 0, nb) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    release();
+    return fail(FS_ERR_HIP, "probe buffers: " + std::string(hipGetErrorString(hipGetLastError())));
+  }
+  float ms_r = 0.f, ms_c = 0.f;
+  fsk::launch_probe_read(a, n16, out, nullptr);   // warm-up
+  fsk::launch_probe_copy(a, b, n16, nullptr);
+  if (hipEventRecord(e0, nullptr) != hipSuccess) rc = FS_ERR_HIP;
+  for (int k = 0; k < reps && !rc; ++k) fsk::launch_probe_read(a, n16, out, nullptr);
+  if (!rc && (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms_r, e0, e1) != hipSuccess))
+    rc = FS_ERR_HIP;
+  if (!rc && hipEventRecord(e0, nullptr) != hipSuccess) rc = FS_ERR_HIP;
+  for (int k = 0; k < reps && !rc; ++k) fsk::launch_probe_copy(a, b, n16, nullptr);
+  if (!rc && (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms_c, e0, e1) != hipSuccess))
+    rc = FS_ERR_HIP;
+  release();
+  if (rc) return fail(rc, "probe launch failed");
+  *read_gbs = (double)reps * nb / (ms_r * 1e-3) / 1e9;
+  *copy_gbs = 2.0 * reps * nb / (ms_c * 1e-3) / 1e9;
+  return FS_OK;
+}
+
 int fs_device_count(int* count) {
   if (!count) return fail(FS_ERR_ARG, "null count");
   int n = 0;

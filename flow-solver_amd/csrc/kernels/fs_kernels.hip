@@ -710,6 +710,36 @@ __global__ void __launch_bounds__(kBlock) k_body_terms(DevMesh m, DevState s, co
   s.bforce[2 * (size_t)b + 1] = temp2 * len * 0.5;
 }
 
+// Streaming probes of the device's achievable HBM bandwidth (no counterpart
+// in the reference): a read of n float4 with 8 loads in flight per thread,
+// and a copy.
+__global__ void __launch_bounds__(kBlock) k_probe_read(const float4* __restrict__ a, size_t n, float* __restrict__ out) {
+  constexpr int U = 8;
+  size_t i = (size_t)blockIdx.x * blockDim.x * U + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x * U;
+  float acc = 0.f;
+  for (; i + (U - 1) * (size_t)blockDim.x < n; i += stride) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = a[i + u * (size_t)blockDim.x];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += v[u].x + v[u].y + v[u].z + v[u].w;
+  }
+  // never true for the zero-filled probe buffer; keeps the loads live
+  if (acc == 1.0f) out[0] = acc;
+}
+__global__ void __launch_bounds__(kBlock) k_probe_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+void launch_probe_read(const void* a, size_t n16, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_probe_read, dim3(8192), dim3(kBlock), 0, st, (const float4*)a, n16, out);
+}
+void launch_probe_copy(const void* a, void* b, size_t n16, hipStream_t st) {
+  hipLaunchKernelGGL(k_probe_copy, dim3(8192), dim3(kBlock), 0, st, (const float4*)a, (float4*)b, n16);
+}
+
 template <int kForm>
 __global__ void __launch_bounds__(kBlock) k_roe_batch(int n, double gamma, const double* __restrict__ nrm,
                                                       const double* __restrict__ qL, const double* __restrict__ qR,

@@ -205,6 +205,9 @@ struct ForceParams {
   double gamma, ca, sa;
 };
 void launch_body_terms(const DevMesh& m, const DevState& s, const ForceParams* fp, hipStream_t st);
+// streaming read / copy of n16 16-byte words (achievable bandwidth probes)
+void launch_probe_read(const void* a, size_t n16, float* out, hipStream_t st);
+void launch_probe_copy(const void* a, void* b, size_t n16, hipStream_t st);
 void launch_roe_batch(int n, int form, double gamma, const double* normals, const double* qL, const double* qR,
                       double* flux, hipStream_t st);
 

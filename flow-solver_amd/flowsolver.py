@@ -45,6 +45,14 @@ class FlowSolverError(RuntimeError):
         self.code = code
 
 
+def hbm_probe(nbytes=4 << 30, reps=5):
+    """Achievable HBM bandwidth (GB/s) of the current device: (streaming read,
+    copy counting both directions)."""
+    r, c = C.c_double(), C.c_double()
+    _check(lib().fs_hbm_probe(nbytes, reps, C.byref(r), C.byref(c)))
+    return r.value, c.value
+
+
 class Params(C.Structure):
     """PARAMS: the 24 run controls of euler.params (params.C:44-217)."""
     _fields_ = [
@@ -114,6 +122,7 @@ def lib():
     sig = {
         "fs_last_error": (C.c_char_p, []),
         "fs_device_count": (i, [C.POINTER(i)]),
+        "fs_hbm_probe": (i, [i64, i, C.POINTER(d), C.POINTER(d)]),
         "fs_params_default": (None, [vp]),
         "fs_params_read": (i, [C.c_char_p, vp]),
         "fs_params_write": (i, [C.c_char_p, vp]),

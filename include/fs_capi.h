@@ -47,6 +47,11 @@ typedef struct fs_solver fs_solver;
 
 const char* fs_last_error(void);
 int fs_device_count(int* count);
+/* Achievable HBM bandwidth of the current device (no counterpart in the
+ * reference): a streaming read and a copy of `bytes` (at least 1 MiB; use
+ * more than the 256 MB last-level cache), `reps` timed passes each, in GB/s
+ * (the copy counts read and write bytes).  Allocates and frees its buffers. */
+int fs_hbm_probe(int64_t bytes, int reps, double* read_gbs, double* copy_gbs);
 
 /* ---- parameters (params.C:27-219 read_parameters) ---- */
 void fs_params_default(fs_params* p);

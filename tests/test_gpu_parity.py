@@ -410,3 +410,12 @@ def test_k2_gauss_limiter_matches_oracle(limit):
                                                                    phi_d.size))
     assert (phi_d < 1).sum() > 0
     np.testing.assert_array_equal(phi_d, phi_o)
+
+
+def test_hbm_probe_in_range():
+    """The bandwidth probe the bench reports beside the roofline: a streaming
+    read and a copy of 1 GiB land between 1 TB/s and the 8 TB/s peak."""
+    rd, cp = fsm.hbm_probe(1 << 30, 3)
+    print("probe: read %.0f GB/s, copy %.0f GB/s" % (rd, cp))
+    assert 1000.0 < rd < 8000.0
+    assert 1000.0 < cp < 8000.0
