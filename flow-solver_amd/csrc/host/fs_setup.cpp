@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace fs {
 
@@ -962,16 +963,18 @@ bool pinv_jacobi(const double* A, int m, int n, double* W) {
 // Weighted k=2 least-squares matrix per node (compute_derivatives.C:859-913:
 // rows w_j*[dx+Mx_j-Mx_i, dy+My_j-My_i, quadratic moment columns], w=1/dist)
 // and its pseudo-inverse with the row weights folded into the columns.
-bool build_pseudo_inverse(Mesh* m, std::string* err) {
-  const int nn = m->nn;
-  m->st_W.assign(5 * m->st_nodes.size(), 0.0);
-  std::vector<double> A, W;
+// Nodes are independent and each writes only its own stencil's entries, so
+// contiguous node ranges run on host threads; the result does not depend on
+// the thread count.
+namespace {
+bool pinv_nodes(Mesh* m, int i0, int i1) {
+  std::vector<double> A, W, wv;
   const double* mo = m->moments.data();
-  for (int i = 0; i < nn; ++i) {
+  for (int i = i0; i < i1; ++i) {
     int o = m->st_off[i], mm = m->st_off[i + 1] - o;
     A.assign((size_t)mm * 5, 0.0);
     W.assign((size_t)mm * 5, 0.0);
-    std::vector<double> wv(mm);
+    wv.assign(mm, 0.0);
     double xi = m->x[2 * i], yi = m->x[2 * i + 1];
     for (int r = 0; r < mm; ++r) {
       int node = m->st_nodes[o + r];
@@ -988,11 +991,34 @@ bool build_pseudo_inverse(Mesh* m, std::string* err) {
       A[r + 3 * mm] = 0.5 * w * ((yn - yi) * (yn - yi) + 2.0 * Mn[1] * (yn - yi) + Mn[3] - Mi[3]);
       A[r + 4 * mm] = w * (Mn[4] + Mn[0] * (yn - yi) + Mn[1] * (xn - xi) + (yn - yi) * (xn - xi) - Mi[4]);
     }
-    if (!pinv_jacobi(A.data(), mm, 5, W.data())) { if (err) *err = "rank-deficient k=2 stencil"; return false; }
+    if (!pinv_jacobi(A.data(), mm, 5, W.data())) return false;
     for (int r = 0; r < 5; ++r)
       for (int j = 0; j < mm; ++j) m->st_W[5 * (size_t)o + (size_t)r * mm + j] = W[(size_t)r * mm + j] * wv[j];
   }
   return true;
+}
+}  // namespace
+
+bool build_pseudo_inverse(Mesh* m, std::string* err) {
+  const int nn = m->nn;
+  m->st_W.assign(5 * m->st_nodes.size(), 0.0);
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nth = nn < 8192 ? 1 : (int)std::min(16u, hw);
+  bool ok = true;
+  if (nth == 1) {
+    ok = pinv_nodes(m, 0, nn);
+  } else {
+    std::vector<std::thread> th;
+    std::vector<char> res(nth, 1);
+    for (int t = 0; t < nth; ++t) {
+      const int i0 = (int)((int64_t)nn * t / nth), i1 = (int)((int64_t)nn * (t + 1) / nth);
+      th.emplace_back([m, i0, i1, t, &res] { res[t] = pinv_nodes(m, i0, i1) ? 1 : 0; });
+    }
+    for (auto& x : th) x.join();
+    for (char r : res) ok = ok && r;
+  }
+  if (!ok && err) *err = "rank-deficient k=2 stencil";
+  return ok;
 }
 
 bool prepare(Mesh* m, const Params& p, std::string* err) {
