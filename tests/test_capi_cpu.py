@@ -94,3 +94,32 @@ def test_schedule_drops_turnaround_repeat():
     # on; that launch is dropped
     s = _schedule(4, 3)
     assert [c for c, _ in s] == [0, 1, 2, 3, 2, 1, 0, 1, 2, 3]
+
+
+@pytest.mark.parametrize("kw,frag", [(dict(order=2), "order"), (dict(order=4), "order"), (dict(limit=3), "limit"),
+                                     (dict(method=0), "method"), (dict(unsteady=1), "unsteady"),
+                                     (dict(jufreq=0), "jufreq"), (dict(gamma=1.0), "gamma")])
+def test_unsupported_params_rejected(kw, frag):
+    """Run controls outside the path are rejected at creation with
+    FS_ERR_UNSUPPORTED (before any device work), naming the field."""
+    base = dict(order=1, foits=2, limit=1, reorder=1, mach=0.5)
+    m = fsm.Mesh.bump(9, 5, 0.1, 1)
+    m.prepare(fsm.Params.make(**base))
+    base.update(kw)
+    with pytest.raises(fsm.FlowSolverError) as e:
+        fsm.Solver(m, fsm.Params.make(**base))
+    assert e.value.code == -7, e.value
+    assert frag in str(e.value)
+
+
+def test_batch_rejects_mixed_sweep_controls():
+    """Instances of one batch share subits, iter and jufreq (one launch per
+    sweep for all of them)."""
+    m1, m2 = fsm.Mesh.bump(9, 5, 0.1, 1), fsm.Mesh.bump(11, 5, 0.1, 2)
+    p1 = fsm.Params.make(order=1, foits=2, limit=1, reorder=1, mach=0.5, subits=10)
+    p2 = fsm.Params.make(order=1, foits=2, limit=1, reorder=1, mach=0.5, subits=12)
+    m1.prepare(p1)
+    m2.prepare(p2)
+    with pytest.raises(fsm.FlowSolverError) as e:
+        fsm.Batch([m1, m2], [p1, p2])
+    assert e.value.code == -1 and "subits" in str(e.value)

@@ -193,3 +193,42 @@ def test_autotune_keeps_results_and_caches(precision, tmp_path, monkeypatch):
     b.close()
     assert t2["from_cache"]
     assert {k: t2[k] for k in t2 if k != "from_cache"} == {k: t1[k] for k in t2 if k != "from_cache"}
+
+
+def test_nonphysical_instance_fails_alone():
+    """A non-physical state (negative density at one node, set from the host)
+    ends that solve with the reference's fatal condition as a status: the
+    single-mesh solver returns FS_ERR_STATE naming the exit it replaces, and
+    in a batch only that instance is marked failed while the others run on
+    and still equal their solves alone."""
+    meshes, params = _build(fsm.FS_PREC_F64)
+    meshes, params = meshes[:3], params[:3]
+    bad = 1
+    info = meshes[bad].info()
+    s = fsm.Solver(meshes[bad], params[bad])
+    s.init_freestream()
+    Q = s.get("Q")
+    Q[4 * (info["nn"] // 2)] = -0.5
+    s.set("Q", Q)
+    with pytest.raises(fsm.FlowSolverError) as e:
+        s.solve(1)
+    print("single solver:", e.value, flush=True)
+    assert e.value.code == -5
+    assert "init_memory.C" in str(e.value) or "cvbc.C" in str(e.value)
+    s.close()
+    b = fsm.Batch(meshes, params)
+    b.init_freestream()
+    b.set(bad, "Q", Q)
+    steps, drop, status = b.solve_steady(80, 1e-6)
+    print("batch status", status, "steps", steps, flush=True)
+    assert status[bad] == fsm.INST_FAILED and steps[bad] == 1
+    for i in range(3):
+        if i == bad:
+            continue
+        assert status[i] != fsm.INST_FAILED
+        a = fsm.Solver(meshes[i], params[i])
+        a.init_freestream()
+        st, dr = a.solve_steady(80, 1e-6)
+        assert st == steps[i]
+        assert np.array_equal(a.get("Q"), b.get(i, "Q"))
+        a.close()
