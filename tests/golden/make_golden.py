@@ -51,6 +51,8 @@ def run_case(name, nx, ny, seed, p, nsteps, stencil, step_fields, dump_every=1, 
             "hdr": d["hdr"]}
     if airfoil:
         keep["airfoil"] = np.array(airfoil, dtype=np.float64)
+    if p.iter != 1 or p.jufreq != 1:
+        keep["newton"] = np.array([p.iter, p.jufreq], dtype=np.int64)
     for k in (SETUP_INT + SETUP_DBL) if setup else ["Q0"]:
         keep["s_" + k] = d[k]
     if stencil:
@@ -80,6 +82,9 @@ def main():
     # Venkatakrishnan limiter (limit=2) at the k=1 face points
     run_case("bump17x9_o1_venkat", 17, 9, 7, params(order=1, foits=2, limit=2, mach=0.675), 6, False, full,
              lhs_steps=(6,))
+    # three Newton iterations per step, the Jacobian updated on the 1st and 3rd
+    run_case("bump17x9_o1_newton", 17, 9, 7, params(order=1, foits=2, iter=3, jufreq=2), 5, False, full,
+             lhs_steps=(5,))
     # O-mesh around the 12%-thick profile: closed wall, far field, seam of the
     # periodic direction; limited linear phase at an angle of attack
     run_case("airfoil32x9_o1", 32, 9, 4, params(order=1, foits=2, mach=0.5, alpha=2.0), 6, False, full,

@@ -34,8 +34,9 @@ def _load(name):
 def _setup(g, **over):
     nx, ny, seed, order, foits, soits, limit, nsteps = (int(v) for v in g["params"])
     cfl, mach, alpha, ramp, subits = g["params_f"]
+    it, ju = (int(v) for v in g["newton"]) if "newton" in g.files else (1, 1)
     kw = dict(order=order, foits=foits, soits=soits, limit=limit, mach=mach, alpha=alpha, CFL=cfl, ramp=int(ramp),
-              subits=int(subits), method=1, iter=1, jufreq=1, reorder=1, gamma=1.4, nsteps=nsteps)
+              subits=int(subits), method=1, iter=it, jufreq=ju, reorder=1, gamma=1.4, nsteps=nsteps)
     kw.update(over)
     p = fsm.Params.make(**kw)
     m = U.golden_mesh(g)
@@ -45,7 +46,8 @@ def _setup(g, **over):
     return p, m, o, nsteps
 
 
-@pytest.mark.parametrize("case", ["bump17x9_o1", "bump17x9_o3", "bump17x9_o1_venkat", "cfg1_97x33", "airfoil32x9_o1"])
+@pytest.mark.parametrize("case", ["bump17x9_o1", "bump17x9_o3", "bump17x9_o1_venkat", "cfg1_97x33", "airfoil32x9_o1",
+                                  "bump17x9_o1_newton"])
 def test_step_history_bitexact(case):
     g = _load(case)
     p, m, o, nsteps = _setup(g)
