@@ -423,10 +423,11 @@ def cpu_reference_calibration(U, fsm, nx, ny, steps=2):
                       % (steps, nx, ny, t_ref, t_port)}
 
 
-def _cpu_batch_worker(case, steps, barrier, queue):
-    """One batch solve on the oracle in its own process (spawned: no GPU):
-    mesh preparation untimed, then `steps` pseudo-time steps timed after a
-    barrier shared by all workers."""
+def _cpu_batch_worker(case, steps, barrier, queue, core):
+    """One batch solve on the oracle in its own process (spawned: no GPU),
+    pinned to one core: mesh preparation untimed, then `steps` pseudo-time
+    steps timed after a barrier shared by all workers."""
+    os.sched_setaffinity(0, {core})
     import fsutil as U
     import workloads as W
     p = workload_params(W, [case])[0]
@@ -444,17 +445,20 @@ def _cpu_batch_worker(case, steps, barrier, queue):
 
 def cpu_baseline_all_cores(steps=3):
     """The oracle on every host core this process may use (at most 16): one
-    independent batch solve per core, the first solves of the batch workload,
+    independent batch solve per core (each process pinned to its core), the
+    first solves of the batch workload,
     timed together after a barrier.  Throughput = the sum of the per-core
     rates (a CPU deployment keeps every core busy with a queue of solves);
     the span from the first start to the last finish is reported beside it."""
     import multiprocessing as mp
     import workloads as W
-    ncores = min(16, len(os.sched_getaffinity(0)))
+    cores = sorted(os.sched_getaffinity(0))[:16]
+    ncores = len(cores)
     cases = W.batch_cases(ncores)
     ctx = mp.get_context("spawn")
     barrier, queue = ctx.Barrier(ncores), ctx.Queue()
-    procs = [ctx.Process(target=_cpu_batch_worker, args=(c, steps, barrier, queue)) for c in cases]
+    procs = [ctx.Process(target=_cpu_batch_worker, args=(c, steps, barrier, queue, core))
+             for c, core in zip(cases, cores)]
     for pr in procs:
         pr.start()
     res = []
@@ -480,7 +484,7 @@ def cpu_baseline_all_cores(steps=3):
     return {"value": rate, "unit": "edge-flux evals/s", "cores": ncores, "kind": "port", "cpu": model,
             "value_over_span": sum(r[2] for r in res) / span,
             "sample": "%d first-order-phase pseudo-time steps of each of the first %d batch solves (%d triangles "
-                      "in all), one process per core, %.1f s" % (steps, ncores, sum(r[3] for r in res), span)}
+                      "in all), one process pinned to each core, %.1f s" % (steps, ncores, sum(r[3] for r in res), span)}
 
 
 def main():
