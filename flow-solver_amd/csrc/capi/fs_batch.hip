@@ -12,9 +12,23 @@
 #include "fs_capi_internal.hpp"
 #include "fs_engine.hpp"
 
+// A batch starts as one engine over all instances.  With re-packing on
+// (fs_batch_set_compaction), during steady solves, once half of the newest
+// engine's instances have stopped, the running ones are re-packed into a
+// smaller engine (their state and step counters copied on the device), so
+// the launches of later steps cover only live work.  The
+// engine an instance stopped in keeps its final fields; `where` maps every
+// instance to the engine holding its current state.  An instance's
+// arithmetic does not depend on the other instances of its engine, so
+// re-packing changes no result.
 struct fs_batch {
-  fse::Engine eng;
+  std::vector<std::unique_ptr<fse::Engine>> eng;
+  std::vector<std::pair<int, int>> where;   // instance -> (engine, index in it)
+  std::vector<const fs::Mesh*> meshes;
+  std::vector<fs_params> params;
+  int precision = FS_PREC_F64;
   bool graph = false;
+  bool compact = false;
   struct Track {
     int status = FS_INST_RUNNING;
     int steps = 0;
@@ -22,6 +36,10 @@ struct fs_batch {
     int err = 0;
   };
   std::vector<Track> track;
+
+  fse::Engine& engine_of(int i) { return *eng[where[i].first]; }
+  fse::Instance& inst(int i) { return eng[where[i].first]->inst[where[i].second]; }
+  int ninst() const { return (int)where.size(); }
 };
 
 namespace {
@@ -29,14 +47,61 @@ int fail(int code, const std::string& msg) { return fse::set_error(code, msg); }
 
 int ts_ref(const fs_params& p) { return (p.order == 3) ? p.foits + p.soits + 1 : p.foits + 1; }
 
+// Re-packs the running instances of the newest engine into a new engine
+// once at most half of its instances still run (and it holds at least 8).
+// Called only before a step: every moved instance then steps in the new
+// engine before any of its fields is read there.
+int maybe_compact(fs_batch* b) {
+  fse::Engine& E = *b->eng.back();
+  const int ne = (int)b->eng.size() - 1;
+  std::vector<int> live;
+  for (int i = 0; i < b->ninst(); ++i)
+    if (b->where[i].first == ne && b->inst(i).active) live.push_back(i);
+  if (!b->compact || E.ninst() < 8 || live.empty() || 2 * (int)live.size() > E.ninst()) return FS_OK;
+  std::vector<const fs::Mesh*> ms;
+  std::vector<fs_params> ps;
+  for (int i : live) {
+    ms.push_back(b->meshes[i]);
+    ps.push_back(b->params[i]);
+  }
+  auto N = std::make_unique<fse::Engine>();
+  int rc = N->create(ms, ps, b->precision);
+  if (rc) return rc;
+  N->dm.tune = E.dm.tune;
+  FSE_HIP(hipStreamSynchronize(E.stream));
+  for (size_t k = 0; k < live.size(); ++k) {
+    fse::Instance& from = b->inst(live[k]);
+    fse::Instance& to = N->inst[k];
+    // state with the ghosts' memory: real nodes, then the instance's ghosts
+    FSE_HIP(hipMemcpyAsync(N->ds.Q + 4 * (size_t)to.noff, E.ds.Q + 4 * (size_t)from.noff,
+                           4 * (size_t)from.nn * sizeof(double), hipMemcpyDeviceToDevice, N->stream));
+    if (from.nbe > 0)
+      FSE_HIP(hipMemcpyAsync(N->ds.Q + 4 * ((size_t)N->dm.nn + to.boff), E.ds.Q + 4 * ((size_t)E.dm.nn + from.boff),
+                             4 * (size_t)from.nbe * sizeof(double), hipMemcpyDeviceToDevice, N->stream));
+    to.ts = from.ts;
+    to.active = true;
+    to.r2 = from.r2;
+    to.dq2 = from.dq2;
+    from.active = false;
+    b->where[live[k]] = {ne + 1, (int)k};
+  }
+  FSE_HIP(hipStreamSynchronize(N->stream));
+  b->eng.push_back(std::move(N));
+  return FS_OK;
+}
+
 // one step of the running instances, then the per-instance convergence
 // bookkeeping of fs_solve_steady
 int batch_step(fs_batch* b, double tol, bool stop_on_tol) {
-  fse::Engine& E = b->eng;
-  int rc = E.step(true, b->graph, false);
-  if (rc) return rc;
-  for (int i = 0; i < E.ninst(); ++i) {
-    fse::Instance& I = E.inst[i];
+  for (auto& E : b->eng) {
+    bool any = false;
+    for (const auto& I : E->inst) any = any || I.active;
+    if (!any) continue;
+    int rc = E->step(true, b->graph, false);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < b->ninst(); ++i) {
+    fse::Instance& I = b->inst(i);
     fs_batch::Track& T = b->track[i];
     if (!I.active) continue;
     T.steps = I.ts;
@@ -74,8 +139,14 @@ int fs_batch_create(int n, const fs_mesh* const* meshes, const fs_params* params
     ms[i] = &meshes[i]->m;
   }
   auto b = std::make_unique<fs_batch>();
-  int rc = b->eng.create(ms, ps, precision);
+  b->eng.push_back(std::make_unique<fse::Engine>());
+  int rc = b->eng[0]->create(ms, ps, precision);
   if (rc) return rc;
+  b->meshes = ms;
+  b->params = ps;
+  b->precision = precision;
+  b->where.resize(n);
+  for (int i = 0; i < n; ++i) b->where[i] = {0, i};
   b->track.assign(n, fs_batch::Track{});
   *out = b.release();
   return FS_OK;
@@ -85,17 +156,23 @@ void fs_batch_destroy(fs_batch* b) { delete b; }
 
 int fs_batch_get_info(const fs_batch* b, fs_batch_info* info) {
   if (!b || !info) return fail(FS_ERR_ARG, "null argument");
-  const fse::Engine& E = b->eng;
-  info->ninst = E.ninst();
+  const fse::Engine& E = *b->eng[0];
+  info->ninst = b->ninst();
   info->nn = E.dm.nn; info->ne = E.dm.ne; info->nsub = E.dm.nsub; info->nbe = E.dm.nbe; info->nnz = E.dm.nnz;
   info->ncolor = E.dm.ncolor;
-  info->device_bytes = (int64_t)E.device_bytes();
+  int64_t bytes = 0;
+  for (const auto& e : b->eng) bytes += (int64_t)e->device_bytes();
+  info->device_bytes = bytes;
   return FS_OK;
 }
 
 int fs_batch_init_freestream(fs_batch* b) {
   if (!b) return fail(FS_ERR_ARG, "null batch");
-  return b->eng.init_freestream(-1);
+  for (int i = 0; i < b->ninst(); ++i) {
+    int rc = b->engine_of(i).init_freestream(b->where[i].second);
+    if (rc) return rc;
+  }
+  return FS_OK;
 }
 
 int fs_batch_use_graph(fs_batch* b, int on) {
@@ -104,23 +181,29 @@ int fs_batch_use_graph(fs_batch* b, int on) {
   return FS_OK;
 }
 
+int fs_batch_set_compaction(fs_batch* b, int on) {
+  if (!b) return fail(FS_ERR_ARG, "null batch");
+  b->compact = on != 0;
+  return FS_OK;
+}
+
 int fs_batch_autotune(fs_batch* b, double* seconds) {
   if (!b) return fail(FS_ERR_ARG, "null batch");
-  return b->eng.autotune(seconds);
+  return b->eng.back()->autotune(seconds);
 }
 
 int fs_batch_get_tuning(const fs_batch* b, int* values, int* from_cache) {
   if (!b || !values) return fail(FS_ERR_ARG, "null argument");
-  b->eng.tuning_values(values);
-  if (from_cache) *from_cache = b->eng.tuned_from_cache() ? 1 : 0;
+  b->eng.back()->tuning_values(values);
+  if (from_cache) *from_cache = b->eng.back()->tuned_from_cache() ? 1 : 0;
   return FS_OK;
 }
 
 int fs_batch_set_active(fs_batch* b, int i, int active) {
-  if (!b || i < 0 || i >= b->eng.ninst()) return fail(FS_ERR_ARG, "instance out of range");
+  if (!b || i < 0 || i >= b->ninst()) return fail(FS_ERR_ARG, "instance out of range");
   fs_batch::Track& T = b->track[i];
   if (T.status == FS_INST_FAILED) return fail(FS_ERR_STATE, "instance failed: " + std::string(fse::device_error_message(T.err)));
-  b->eng.inst[i].active = active != 0;
+  b->inst(i).active = active != 0;
   T.status = active ? FS_INST_RUNNING : FS_INST_PAUSED;
   return FS_OK;
 }
@@ -129,24 +212,24 @@ int fs_batch_step(fs_batch* b, double* r2, double* dq) {
   if (!b) return fail(FS_ERR_ARG, "null batch");
   int rc = batch_step(b, 0.0, false);
   if (rc) return rc;
-  for (int i = 0; i < b->eng.ninst(); ++i) {
-    if (r2) r2[i] = b->eng.inst[i].r2;
-    if (dq) dq[i] = std::sqrt(b->eng.inst[i].dq2);
+  for (int i = 0; i < b->ninst(); ++i) {
+    if (r2) r2[i] = b->inst(i).r2;
+    if (dq) dq[i] = std::sqrt(b->inst(i).dq2);
   }
   return FS_OK;
 }
 
 int fs_batch_solve_steady(fs_batch* b, int max_steps, double tol, int* steps, double* drop, int* status) {
   if (!b || max_steps < 1) return fail(FS_ERR_ARG, "bad arguments");
-  fse::Engine& E = b->eng;
   for (int k = 0; k < max_steps; ++k) {
     bool any = false;
-    for (const auto& I : E.inst) any = any || I.active;
+    for (int i = 0; i < b->ninst(); ++i) any = any || b->inst(i).active;
     if (!any) break;
-    int rc = batch_step(b, tol, true);
+    int rc = maybe_compact(b);
+    if (!rc) rc = batch_step(b, tol, true);
     if (rc) return rc;
   }
-  for (int i = 0; i < E.ninst(); ++i) {
+  for (int i = 0; i < b->ninst(); ++i) {
     fs_batch::Track& T = b->track[i];
     if (T.status == FS_INST_RUNNING) T.status = FS_INST_MAX_STEPS;
     if (steps) steps[i] = T.steps;
@@ -157,7 +240,7 @@ int fs_batch_solve_steady(fs_batch* b, int max_steps, double tol, int* steps, do
 }
 
 int fs_batch_inst_status(const fs_batch* b, int i, int* status, int* steps, double* drop) {
-  if (!b || i < 0 || i >= b->eng.ninst()) return fail(FS_ERR_ARG, "instance out of range");
+  if (!b || i < 0 || i >= b->ninst()) return fail(FS_ERR_ARG, "instance out of range");
   const fs_batch::Track& T = b->track[i];
   if (status) *status = T.status;
   if (steps) *steps = T.steps;
@@ -167,29 +250,36 @@ int fs_batch_inst_status(const fs_batch* b, int i, int* status, int* steps, doub
 
 int fs_batch_get(fs_batch* b, int i, int field, double* host, int64_t n) {
   if (!b || !host) return fail(FS_ERR_ARG, "null argument");
-  return b->eng.get_field(i, field, host, n);
+  if (i < 0 || i >= b->ninst()) return fail(FS_ERR_ARG, "instance out of range");
+  return b->engine_of(i).get_field(b->where[i].second, field, host, n);
 }
 
 int fs_batch_set(fs_batch* b, int i, int field, const double* host, int64_t n) {
   if (!b || !host) return fail(FS_ERR_ARG, "null argument");
-  return b->eng.set_field(i, field, host, n);
+  if (i < 0 || i >= b->ninst()) return fail(FS_ERR_ARG, "instance out of range");
+  return b->engine_of(i).set_field(b->where[i].second, field, host, n);
 }
 
 int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t) {
   if (!b || !t || nsteps < 1) return fail(FS_ERR_ARG, "bad arguments");
-  fse::Engine& E = b->eng;
-  int rc = E.sync();
-  if (rc) return rc;
+  for (auto& e : b->eng) {
+    int rc = e->sync();
+    if (rc) return rc;
+  }
   int64_t evals = 0;
   auto t0 = std::chrono::steady_clock::now();
   for (int k = 0; k < nsteps; ++k) {
-    for (const auto& I : E.inst)
+    for (int i = 0; i < b->ninst(); ++i) {
+      const fse::Instance& I = b->inst(i);
       if (I.active) evals += (int64_t)I.nsub + I.nbe;
-    rc = batch_step(b, 0.0, false);
+    }
+    int rc = batch_step(b, 0.0, false);
     if (rc) return rc;
   }
-  rc = E.sync();
-  if (rc) return rc;
+  for (auto& e : b->eng) {
+    int rc = e->sync();
+    if (rc) return rc;
+  }
   auto t1 = std::chrono::steady_clock::now();
   *t = fs_timing{};
   t->step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / nsteps;
@@ -199,10 +289,20 @@ int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t) {
 
 int fs_batch_body_forces(fs_batch* b, double* forces) {
   if (!b || !forces) return fail(FS_ERR_ARG, "null argument");
-  fse::Engine& E = b->eng;
-  std::vector<int> citer(E.ninst());
-  for (int i = 0; i < E.ninst(); ++i) citer[i] = E.inst[i].ts;
-  return E.body_forces(citer.data(), forces);
+  for (int e = 0; e < (int)b->eng.size(); ++e) {
+    fse::Engine& E = *b->eng[e];
+    std::vector<int> citer(E.ninst());
+    std::vector<double> f(2 * (size_t)E.ninst());
+    for (int k = 0; k < E.ninst(); ++k) citer[k] = E.inst[k].ts;
+    int rc = E.body_forces(citer.data(), f.data());
+    if (rc) return rc;
+    for (int i = 0; i < b->ninst(); ++i)
+      if (b->where[i].first == e) {
+        forces[2 * (size_t)i] = f[2 * (size_t)b->where[i].second];
+        forces[2 * (size_t)i + 1] = f[2 * (size_t)b->where[i].second + 1];
+      }
+  }
+  return FS_OK;
 }
 
 int fs_mesh_footprint(const fs_mesh* m, int precision, int64_t* bytes) {

@@ -151,6 +151,7 @@ def lib():
         "fs_surface_cp": (i, [vp, i, vp, vp, i64, C.POINTER(i64)]),
         "fs_write_surface_cp": (i, [vp, C.c_char_p]),
         "fs_batch_body_forces": (i, [vp, vp]),
+        "fs_batch_set_compaction": (i, [vp, i]),
         "fs_roe_flux_batch": (i, [i, d, vp, vp, vp, vp, vp]),
         "fs_solve_steady": (i, [vp, i, d, C.POINTER(i), C.POINTER(d)]),
         "fs_solver_time_steps": (i, [vp, i, i, vp]),
@@ -467,6 +468,10 @@ class Batch:
 
     def set_active(self, i, on):
         _check(lib().fs_batch_set_active(self._h, i, 1 if on else 0))
+
+    def set_compaction(self, on=True):
+        """Re-pack running instances into a smaller engine during steady solves."""
+        _check(lib().fs_batch_set_compaction(self._h, 1 if on else 0))
 
     def step(self):
         r2, dq = np.zeros(self.n), np.zeros(self.n)

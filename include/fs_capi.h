@@ -260,6 +260,15 @@ int fs_batch_init_freestream(fs_batch* b);
 int fs_batch_use_graph(fs_batch* b, int on);
 int fs_batch_autotune(fs_batch* b, double* seconds);
 int fs_batch_get_tuning(const fs_batch* b, int* values, int* from_cache);
+/* Re-packing during fs_batch_solve_steady (off by default; no counterpart in
+ * the reference): once at most half of the newest engine's instances still
+ * run (and it holds at least 8), the running ones move to a new, smaller
+ * engine on the device, so later launches cover only live work.  Results do
+ * not change; stopped instances keep their final fields.  Each re-pack
+ * builds a new device image on the host, which costs more than it saves on
+ * the 256-solve workload (measured), so it pays only for long tails of a
+ * few running solves. */
+int fs_batch_set_compaction(fs_batch* b, int on);
 /* convergence mask: a masked-off instance is skipped by every kernel */
 int fs_batch_set_active(fs_batch* b, int i, int active);
 /* one pseudo-time step of every active instance (each step counter advances);

@@ -232,3 +232,49 @@ def test_nonphysical_instance_fails_alone():
         assert st == steps[i]
         assert np.array_equal(a.get("Q"), b.get(i, "Q"))
         a.close()
+
+
+def test_compaction_keeps_results():
+    """Steady solves of 10 instances that stop at different steps: with the
+    running instances re-packed into smaller engines as others stop, every
+    instance's steps, drop, final state, residual and body forces equal those
+    of the same batch without re-packing and of the solve alone."""
+    cases = []
+    for k in range(10):
+        nx, ny = 17 + 4 * (k % 4), 9 + 2 * (k % 3)
+        kw = dict(mach=0.3 + 0.04 * k, alpha=-1.0 + 0.5 * k, order=1 if k % 3 else 3, soits=0 if k % 3 else 3,
+                  limit=0 if k % 3 else 1)
+        cases.append(((nx, ny, 0.2, 40 + k), _params(**kw)))
+    meshes = []
+    for (nx, ny, amp, seed), p in cases:
+        m = fsm.Mesh.bump(nx, ny, amp, seed)
+        m.prepare(p)
+        meshes.append(m)
+    params = [p for _, p in cases]
+    runs = {}
+    for on in (True, False):
+        b = fsm.Batch(meshes, params)
+        b.set_compaction(on)
+        b.init_freestream()
+        steps, drop, status = b.solve_steady(200, 1e-8)
+        runs[on] = (b, steps, drop, status, b.body_forces(), b.info()["device_bytes"])
+    bc, steps, drop, status, forces, bytes_on = runs[True]
+    bn, steps_n, drop_n, status_n, forces_n, bytes_off = runs[False]
+    print("steps", steps, "status", status, "device bytes with / without re-packing", bytes_on, bytes_off, flush=True)
+    assert len(set(steps.tolist())) > 3
+    assert bytes_on > bytes_off, "no re-packing happened"
+    np.testing.assert_array_equal(steps, steps_n)
+    np.testing.assert_array_equal(drop, drop_n)
+    np.testing.assert_array_equal(status, status_n)
+    np.testing.assert_array_equal(forces, forces_n)
+    for i in range(10):
+        for f in ("Q", "R", "dQ"):
+            assert np.array_equal(bc.get(i, f), bn.get(i, f)), (i, f)
+    for i in (int(np.argmax(steps)), int(np.argmin(steps))):
+        a = fsm.Solver(meshes[i], params[i])
+        a.init_freestream()
+        st, dr = a.solve_steady(200, 1e-8)
+        assert st == steps[i] and dr == drop[i]
+        assert np.array_equal(a.get("Q"), bc.get(i, "Q"))
+        np.testing.assert_array_equal(a.body_forces(st), forces[i])
+        a.close()
