@@ -205,8 +205,10 @@ int fs_solve_steady(fs_solver* s, int max_steps, double tol, int* steps, double*
  *      pseudo-time step and of the dominant kernel ---- */
 typedef struct {
   double step_ms;        /* mean wall time per pseudo-time step */
-  double edge_ms;        /* mean device time of the edge flux kernel */
-  double node_ms;        /* mean device time of the node assembly kernel */
+  double edge_ms;        /* mean device time of the face kernel: flux + FD Jacobian
+                            (FP64), the FP32 analytic Jacobian (mixed) */
+  double node_ms;        /* mean device time of the node kernel: assembly (FP64),
+                            FP64 residual + assembly (mixed) */
   double gs_ms;          /* mean device time of the Gauss-Seidel sweep launches */
   double lu_ms;          /* mean device time of the diagonal-block factorisation */
   double other_ms;
