@@ -27,7 +27,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + INC]
 
-HOST_SRCS = ["csrc/host/fs_setup.cpp"]
+HOST_SRCS = ["csrc/host/fs_setup.cpp", "csrc/host/fs_curved.cpp"]
 HIP_SRCS = ["csrc/kernels/fs_kernels.hip", "csrc/kernels/fs_kernels_f32.hip", "csrc/capi/fs_engine.hip",
             "csrc/capi/fs_capi.hip", "csrc/capi/fs_batch.hip", "csrc/capi/fs_serve.hip"]
 

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "../host/fs_curved.hpp"
 #include "../host/fs_setup.hpp"
 #include "../kernels/fs_kernels.hpp"
 #include "fs_capi.h"
@@ -212,6 +213,67 @@ int fs_mesh_create_airfoil(int nwall, int nrad, double h0, double rmax, double j
   return FS_OK;
 }
 
+int fs_mesh_create_airfoil_split(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed,
+                                 int upper_bc, int lower_bc, fs_mesh** out) {
+  if (!out || nwall < 8 || nwall % 2 || nrad < 3 || !(h0 > 0) || !(rmax > 1) || jitter < 0 || jitter >= 0.5)
+    return fail(FS_ERR_ARG, "bad airfoil O-mesh arguments");
+  auto wall_ok = [](int bc, bool upper) { return bc == 1 || bc == (upper ? 17 : 19); };
+  if (!wall_ok(upper_bc, true) || !wall_ok(lower_bc, false))
+    return fail(FS_ERR_ARG, "upper/lower wall codes must be 1 (polyline) or 17/19 (NACA 0012 upper/lower)");
+  auto h = std::make_unique<fs_mesh>();
+  fs::make_airfoil_omesh_split(nwall, nrad, h0, rmax, jitter, seed, upper_bc, lower_bc, &h->m);
+  *out = h.release();
+  return FS_OK;
+}
+
+int fs_mesh_create_annulus(int ntheta, int nrad, double r_in, double r_out, double jitter, uint64_t seed, int bc_in,
+                           int bc_out, fs_mesh** out) {
+  if (!out || ntheta < 8 || ntheta % 2 || nrad < 2 || !(r_in > 0) || !(r_out > r_in) || jitter < 0 ||
+      jitter >= 0.5)
+    return fail(FS_ERR_ARG, "bad annulus arguments");
+  // Codes 13-16 are left out: their arclength takes the angle difference
+  // without the branch-cut handling of codes 11/12 and 21/22
+  // (curved_boundaries.C:1063-1093), so a closed ring gets one segment about
+  // 2 pi R long next to the angle pi.
+  auto ring_ok = [](int bc, double r) {
+    if (bc <= 10) return bc >= 0;
+    const double R = (bc == 11 || bc == 12) ? 1.0 : (bc == 21 || bc == 22) ? 300.0 : -1.0;
+    return R == r;
+  };
+  if (!ring_ok(bc_in, r_in) || !ring_ok(bc_out, r_out))
+    return fail(FS_ERR_ARG, "ring codes: 0-10, or 11/12 at radius 1, or 21/22 at radius 300");
+  auto h = std::make_unique<fs_mesh>();
+  fs::make_annulus(ntheta, nrad, r_in, r_out, jitter, seed, bc_in, bc_out, &h->m);
+  *out = h.release();
+  return FS_OK;
+}
+
+int fs_curved_midpoint(int bc, const double* xL, const double* xR, double* xM) {
+  if (!xL || !xR || !xM) return fail(FS_ERR_ARG, "null argument");
+  if (!fs::curved_bc_known(bc)) return fail(FS_ERR_ARG, "not a curved boundary code (11-22)");
+  fs::curved_midpoint(bc, xL, xR, xM);
+  return FS_OK;
+}
+int fs_curved_gauss_points(int bc, const double* xL, const double* xR, double* GP) {
+  if (!xL || !xR || !GP) return fail(FS_ERR_ARG, "null argument");
+  if (!fs::curved_bc_known(bc)) return fail(FS_ERR_ARG, "not a curved boundary code (11-22)");
+  std::string err;
+  if (!fs::curved_gauss_points(bc, xL, xR, GP, &err)) return fail(FS_ERR_MESH, err);
+  return FS_OK;
+}
+int fs_curved_normal(int bc, const double* x, double* nx, double* ny) {
+  if (!x || !nx || !ny) return fail(FS_ERR_ARG, "null argument");
+  if (!fs::curved_bc_known(bc)) return fail(FS_ERR_ARG, "not a curved boundary code (11-22)");
+  fs::curved_normal(bc, x, nx, ny);
+  return FS_OK;
+}
+int fs_curved_arclength(int bc, const double* xL, const double* xR, double* len) {
+  if (!xL || !xR || !len) return fail(FS_ERR_ARG, "null argument");
+  if (!fs::curved_bc_known(bc)) return fail(FS_ERR_ARG, "not a curved boundary code (11-22)");
+  *len = fs::curved_arclength(bc, xL, xR);
+  return FS_OK;
+}
+
 int fs_mesh_prepare(fs_mesh* m, const fs_params* p) {
   if (!m || !p) return fail(FS_ERR_ARG, "null argument");
   if (m->prepared) return fail(FS_ERR_ARG, "mesh already prepared");
@@ -294,6 +356,7 @@ int fs_mesh_get_array(const fs_mesh* m, int which, const void** data, int64_t* c
     case FS_M_BBC: return I(g.bbc);
     case FS_M_BSEG_OFF: return I(g.bseg_off);
     case FS_M_BSEG: return I(g.bseg);
+    case FS_M_BEDGE_GN: return D(g.bedge_gn);
     default: return fail(FS_ERR_ARG, "unknown mesh array id");
   }
 }

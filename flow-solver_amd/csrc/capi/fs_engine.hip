@@ -84,7 +84,7 @@ size_t footprint_bytes(const fs::Mesh& g, int prec) {
   size_t b = 0;
   // mesh image
   b += nn * (4 * 4 + 8 * 2 + 8 + 8 * 9 + 4 * 4 + 4 * 3) + ne * 4 * 6 + nsub * (8 * 3 + 4 * 2);
-  b += ninc * (8 * 12 + 4) + neinc * (4 + 16) + nbe * (4 * 4 + 8 * 9) + nnz * 8 + (nn + 1) * 4 * 4 + nst * (4 + 40);
+  b += ninc * (8 * 12 + 4) + neinc * (4 + 16) + nbe * (4 * 4 + 8 * 15) + nnz * 8 + (nn + 1) * 4 * 4 + nst * (4 + 40);
   // state
   b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + nbe * 8 * 22;
   if (prec != FS_PREC_MIXED) b += ninc * 4 * 8;   // per-incidence face fluxes of the FP64 pass
@@ -221,7 +221,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   std::vector<int> sub_inc(2 * NS), inc_other(EINC), inc_sother(KINC), nbe_list(BINC), st_nodes(NST);
   std::vector<double> inc_n(4 * KINC), inc_fc(2 * KINC), inc_xg(6 * KINC), inc_gw(2 * EINC), st_Wt(5 * NST);
   std::vector<int> breal(NB), bbc(NB);
-  std::vector<double> bn(3 * NB), bxg(6 * NB);
+  std::vector<double> bn(3 * NB), bxg(6 * NB), bgn(6 * NB);
   long long kinc = 0, einc = 0, binc = 0, sto = 0;
   for (int i = 0; i < B; ++i) {
     const fs::Mesh& g = *meshes[i];
@@ -285,6 +285,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       bbc[bo + b] = g.bbc[g.bedge[5 * (size_t)b + 3]];
       for (int c = 0; c < 3; ++c) bn[3 * (size_t)(bo + b) + c] = g.bedge_n[3 * (size_t)b + c];
       for (int c = 0; c < 6; ++c) bxg[6 * (size_t)(bo + b) + c] = g.bedge_xg[6 * (size_t)b + c];
+      for (int c = 0; c < 6; ++c) bgn[6 * (size_t)(bo + b) + c] = g.bedge_gn[6 * (size_t)b + c];
     }
     // pseudo-inverse entry-major: Wt[(o+j)*5 + r] = W[5*o + r*m + j]
     for (int n = 0; n + 1 < (int)g.st_off.size(); ++n) {
@@ -413,7 +414,8 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = upload(&d.inc_sother, inc_sother)) ||
       (rc = upload(&d.inc_gw, inc_gw)) || (rc = upload(&d.nbe_off, nbe_off)) ||
       (rc = upload(&d.nbe_list, nbe_list)) || (rc = upload(&d.bedge_real, breal)) ||
-      (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.bedge_xg, bxg)) || (rc = upload(&d.roff, roff)) ||
+      (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.bedge_xg, bxg)) ||
+      (rc = upload(&d.bedge_gn, bgn)) || (rc = upload(&d.roff, roff)) ||
       (rc = upload(&d.tslot, tslot)) || (rc = upload(&d.ja, ja_d)) || (rc = upload(&d.iau, iau_d)) ||
       (rc = upload(&d.pdeg, pdeg)) || (rc = upload(&d.sbase, sbase)) ||
       (rc = upload(&d.color_rows, color_rows)) || (rc = upload(&d.st_off, st_off)) ||

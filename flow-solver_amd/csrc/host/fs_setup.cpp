@@ -4,6 +4,8 @@
 // the last bit where the same operations are performed.
 #include "fs_setup.hpp"
 
+#include "fs_curved.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -269,6 +271,30 @@ void make_bump_channel(int nx, int ny, double jitter, uint64_t seed, Mesh* m) {
   m->bbc = {1, 0, 1, 0};  // lower wall, outlet, upper wall, inlet
 }
 
+namespace {
+// Triangles and boundary segments of a ring mesh with nw nodes per ring and
+// nrad rings (node r*nw + k): each quad split into two counter-clockwise
+// triangles; boundary 0 the inner ring traversed clockwise, boundary 1 the
+// outer ring counter-clockwise, so the domain lies left of every segment.
+void ring_connect(int nw, int nrad, Mesh* m) {
+  auto id = [nw](int r, int k) { return r * nw + (k % nw); };
+  m->ntri = 2 * nw * (nrad - 1);
+  m->tri.reserve(3 * (size_t)m->ntri);
+  for (int r = 0; r < nrad - 1; ++r)
+    for (int k = 0; k < nw; ++k) {
+      int a = id(r, k), b = id(r, k + 1), c = id(r + 1, k + 1), d = id(r + 1, k);
+      m->tri.insert(m->tri.end(), {a, d, c});
+      m->tri.insert(m->tri.end(), {a, c, b});
+    }
+  m->nb = 2;
+  m->bseg_off = {0};
+  for (int k = 0; k < nw; ++k) m->bseg.insert(m->bseg.end(), {id(0, k + 1), id(0, k)});
+  m->bseg_off.push_back((int)m->bseg.size() / 2);
+  for (int k = 0; k < nw; ++k) m->bseg.insert(m->bseg.end(), {id(nrad - 1, k), id(nrad - 1, k + 1)});
+  m->bseg_off.push_back((int)m->bseg.size() / 2);
+}
+}  // namespace
+
 void make_airfoil_omesh(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed, Mesh* m) {
   *m = Mesh();
   const double PI = 3.14159265358979323846;
@@ -310,22 +336,47 @@ void make_airfoil_omesh(int nwall, int nrad, double h0, double rmax, double jitt
       m->x[2 * n + 1] = (1.0 - sr) * yw + sr * yo;
     }
   }
-  auto id = [nwall](int r, int k) { return r * nwall + (k % nwall); };
-  m->ntri = 2 * nwall * (nrad - 1);
-  m->tri.reserve(3 * (size_t)m->ntri);
-  for (int r = 0; r < nrad - 1; ++r)
-    for (int k = 0; k < nwall; ++k) {
-      int a = id(r, k), b = id(r, k + 1), c = id(r + 1, k + 1), d = id(r + 1, k);
-      m->tri.insert(m->tri.end(), {a, d, c});
-      m->tri.insert(m->tri.end(), {a, c, b});
-    }
-  m->nb = 2;
-  m->bseg_off = {0};
-  for (int k = 0; k < nwall; ++k) m->bseg.insert(m->bseg.end(), {id(0, k + 1), id(0, k)});
-  m->bseg_off.push_back((int)m->bseg.size() / 2);
-  for (int k = 0; k < nwall; ++k) m->bseg.insert(m->bseg.end(), {id(nrad - 1, k), id(nrad - 1, k + 1)});
-  m->bseg_off.push_back((int)m->bseg.size() / 2);
+  ring_connect(nwall, nrad, m);
   m->bbc = {1, 0};  // wall (polyline), farfield
+}
+
+void make_airfoil_omesh_split(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed,
+                              int upper_bc, int lower_bc, Mesh* m) {
+  make_airfoil_omesh(nwall, nrad, h0, rmax, jitter, seed, m);
+  // wall segment k runs from wall node k+1 to k; nodes 0..nwall/2 are the
+  // upper surface (trailing edge to leading edge)
+  m->bseg_off = {0, nwall / 2, nwall, 2 * nwall};
+  m->nb = 3;
+  m->bbc = {upper_bc, lower_bc, 0};
+}
+
+void make_annulus(int ntheta, int nrad, double r_in, double r_out, double jitter, uint64_t seed, int bc_in,
+                  int bc_out, Mesh* m) {
+  *m = Mesh();
+  const double PI = 3.14159265358979323846;
+  m->nn = ntheta * nrad;
+  m->x.assign(2 * (size_t)m->nn, 0.0);
+  Rng rng(seed);
+  const double dth = 2.0 * PI / ntheta;
+  auto ring = [&](int r) { return r == 0 ? r_in : r == nrad - 1 ? r_out : r_in * std::pow(r_out / r_in, r / (nrad - 1.0)); };
+  for (int r = 0; r < nrad; ++r)
+    for (int k = 0; k < ntheta; ++k) {
+      double th = k * dth, rr = ring(r);
+      if (r > 0 && r < nrad - 1 && jitter > 0.0) {
+        const double dr = std::min(ring(r + 1) - rr, rr - ring(r - 1));
+        th += rng.sym(jitter) * dth;
+        rr += rng.sym(jitter) * dr;
+      }
+      double px = rr * std::cos(th), py = rr * std::sin(th);
+      // the two nodes on the x axis lie exactly on it
+      if (k == 0) { px = rr; py = 0.0; }
+      if (2 * k == ntheta) { px = -rr; py = 0.0; }
+      int n = r * ntheta + k;
+      m->x[2 * n] = px;
+      m->x[2 * n + 1] = py;
+    }
+  ring_connect(ntheta, nrad, m);
+  m->bbc = {bc_in, bc_out};
 }
 
 // ------------------------------------------------------------------------
@@ -563,9 +614,35 @@ bool build_geometry(Mesh* m, std::string* err) {
   m->bedge_n.assign(3 * (size_t)m->nbe, 0.0);
   int ib = 0;
   for (int b = 0; b < m->nb; ++b) {
-    if (m->bbc[b] > 10) { if (err) *err = "curved boundaries (bc>10) are not supported"; return false; }
+    const int bc = m->bbc[b];
+    if (curved_bc(bc) && !curved_bc_known(bc)) {
+      if (err) *err = "unknown curved boundary code " + std::to_string(bc) + " (codes 11-22 are defined)";
+      return false;
+    }
     for (int s = m->bseg_off[b]; s < m->bseg_off[b + 1]; ++s) {
       int n0 = m->bseg[2 * s], n1 = m->bseg[2 * s + 1];
+      if (curved_bc(bc)) {
+        // geometry.C:636-690: both ghosts at the curved midpoint, the
+        // normals of the curve at the segment's own nodes
+        const double* xL = &m->x[2 * (size_t)n0];
+        const double* xR = &m->x[2 * (size_t)n1];
+        if (bc >= 17 && bc <= 20 && (xL[0] < 0.0 || xR[0] < 0.0)) {
+          if (err) *err = "NACA 0012 boundary node with x < 0";
+          return false;
+        }
+        double xM[2];
+        curved_midpoint(bc, xL, xR, xM);
+        for (int h = 0; h < 2; ++h) {
+          int real = h == 0 ? n0 : n1;
+          int* be = &m->bedge[5 * (size_t)ib];
+          be[0] = real; be[1] = ib; be[3] = b; be[4] = s - m->bseg_off[b];
+          curved_normal(bc, h == 0 ? xL : xR, &m->bedge_n[3 * ib], &m->bedge_n[3 * ib + 1]);
+          m->x[2 * (nn + ib)] = xM[0];
+          m->x[2 * (nn + ib) + 1] = xM[1];
+          ++ib;
+        }
+        continue;
+      }
       double xmid = (m->x[2 * n0] + m->x[2 * n1]) / 2.;
       double ymid = (m->x[2 * n0 + 1] + m->x[2 * n1 + 1]) / 2.;
       double nx = m->x[2 * n1 + 1] - ymid;
@@ -607,10 +684,51 @@ bool build_geometry(Mesh* m, std::string* err) {
       }
   }
   for (int i = 0; i < m->nbe; ++i) {
+    const int bc = m->bbc[m->bedge[5 * (size_t)i + 3]];
+    if (curved_bc(bc)) {
+      // geometry.C:773-784: unit normal kept, length along the curve from
+      // the node to the curved midpoint
+      const int node = m->bedge[5 * (size_t)i];
+      m->bedge_n[3 * i + 2] = curved_arclength(bc, &m->x[2 * (size_t)node], &m->x[2 * (size_t)(nn + i)]);
+      continue;
+    }
     double nx = m->bedge_n[3 * i], ny = m->bedge_n[3 * i + 1];
     double mag = std::sqrt(nx * nx + ny * ny);
     if (mag > 1.0E-20) { nx /= mag; ny /= mag; }
     m->bedge_n[3 * i] = nx; m->bedge_n[3 * i + 1] = ny; m->bedge_n[3 * i + 2] = mag;
+  }
+  // Fix_Subedge_Data (geometry.C:852-981): the subedge of a curved segment's
+  // edge runs from the element centroid to the curved midpoint
+  {
+    std::vector<int> first_sub(m->ne, -1);
+    for (int s = m->nsub - 1; s >= 0; --s) first_sub[m->sub_edge[s]] = s;
+    for (int i = 0; i < m->nbe; ++i) {
+      const int* be = &m->bedge[5 * (size_t)i];
+      const int b = be[3], bc = m->bbc[b];
+      if (!curved_bc(bc)) continue;
+      const int sg = m->bseg_off[b] + be[4];
+      const int nL = m->bseg[2 * sg], nR = m->bseg[2 * sg + 1];
+      const int lo = std::min(nL, nR), hi = std::max(nL, nR);
+      int e = -1;
+      for (int k = m->nnsn[lo]; k < m->nnsn[lo + 1]; ++k)
+        if (m->edges[2 * m->esn[k] + 1] == hi) { e = m->esn[k]; break; }
+      const int j = e < 0 ? -1 : first_sub[e];
+      if (j < 0) { if (err) *err = "curved segment without an edge"; return false; }
+      const int g = be[2];
+      if (m->sub_elem[j] != g) { if (err) *err = "curved segment and its subedge lie in different elements"; return false; }
+      const double XC0 = m->el_cent[2 * (size_t)g], XC1 = m->el_cent[2 * (size_t)g + 1];
+      const double XM0 = m->x[2 * (size_t)(nn + be[1])], XM1 = m->x[2 * (size_t)(nn + be[1]) + 1];
+      double nx = XC1 - XM1, ny = XM0 - XC0;
+      const double len = std::sqrt(nx * nx + ny * ny);
+      nx /= len;
+      ny /= len;
+      if (nL > nR) { nx = -nx; ny = -ny; }
+      m->sub_n[3 * (size_t)j] = nx;
+      m->sub_n[3 * (size_t)j + 1] = ny;
+      m->sub_n[3 * (size_t)j + 2] = len;
+      m->sub_xm[2 * (size_t)j] = XM0;
+      m->sub_xm[2 * (size_t)j + 1] = XM1;
+    }
   }
   // Gauss points (geometry.C:993-1197)
   const double t1 = std::sqrt(15.0) / 5.0, t2 = 0., t3 = std::sqrt(15.0) / (-5.0);
@@ -628,17 +746,33 @@ bool build_geometry(Mesh* m, std::string* err) {
     o[5] = (ymid + yc) * 0.5 + (yc - ymid) * 0.5 * t3;
   }
   m->bedge_xg.assign(6 * (size_t)m->nbe, 0.0);
+  m->bedge_gn.assign(6 * (size_t)m->nbe, 0.0);
   for (int i = 0; i < m->nbe; ++i) {
     const int* be = &m->bedge[5 * (size_t)i];
     int real = be[0], b = be[3], s = m->bseg_off[b] + be[4];
     int nL = m->bseg[2 * s], nR = m->bseg[2 * s + 1];
+    const int bc = m->bbc[b];
+    double* o = &m->bedge_xg[6 * (size_t)i];
+    double* on = &m->bedge_gn[6 * (size_t)i];
+    if (curved_bc(bc)) {
+      // geometry.C:1160-1194: Gauss points of the arc between the node and
+      // the curved midpoint, each with the curve's normal there
+      const double* xn = &m->x[2 * (size_t)real];
+      const double* xg = &m->x[2 * (size_t)(nn + be[1])];
+      if (!curved_gauss_points(bc, real == nL ? xn : xg, real == nL ? xg : xn, o, err)) return false;
+      for (int k = 0; k < 3; ++k) curved_normal(bc, &o[2 * k], &on[2 * k], &on[2 * k + 1]);
+      continue;
+    }
+    for (int k = 0; k < 3; ++k) {
+      on[2 * k] = m->bedge_n[3 * (size_t)i];
+      on[2 * k + 1] = m->bedge_n[3 * (size_t)i + 1];
+    }
     double xi = m->x[2 * real], yi = m->x[2 * real + 1];
     double xmid = 0.5 * (m->x[2 * nL] + m->x[2 * nR]);
     double ymid = 0.5 * (m->x[2 * nL + 1] + m->x[2 * nR + 1]);
     double xL, xR, yL, yR;
     if (real == nL) { xL = xi; xR = xmid; yL = yi; yR = ymid; }
     else { xL = xmid; xR = xi; yL = ymid; yR = yi; }
-    double* o = &m->bedge_xg[6 * (size_t)i];
     o[0] = (xL + xR) * 0.5 + (xR - xL) * 0.5 * t1;
     o[1] = (yL + yR) * 0.5 + (yR - yL) * 0.5 * t1;
     o[2] = (xL + xR) * 0.5 + (xR - xL) * 0.5 * t2;
@@ -657,16 +791,49 @@ bool build_geometry(Mesh* m, std::string* err) {
     m->cv_area[L] += (qc + qmid) / 2. * nx * len;
     m->cv_area[R] -= (qc + qmid) / 2. * nx * len;
   }
-  for (int b = 0; b < m->nb; ++b)
+  const double wc[3] = {8. / 9., 5. / 9., 5. / 9.};  // weights of the curved Gauss points
+  for (int b = 0; b < m->nb; ++b) {
+    const int bc = m->bbc[b];
     for (int s = m->bseg_off[b]; s < m->bseg_off[b + 1]; ++s) {
       int n0 = m->bseg[2 * s], n1 = m->bseg[2 * s + 1];
+      if (curved_bc(bc)) {
+        // cv_calc.C:130-196: x nx integrated along each half arc
+        const double* xL = &m->x[2 * (size_t)n0];
+        const double* xR = &m->x[2 * (size_t)n1];
+        double xM[2], GP[6];
+        curved_midpoint(bc, xL, xR, xM);
+        for (int h = 0; h < 2; ++h) {
+          const double* a = h == 0 ? xL : xM;
+          const double* c = h == 0 ? xM : xR;
+          if (!curved_gauss_points(bc, a, c, GP, err)) return false;
+          double Ftemp = 0.;
+          for (int k = 0; k < 3; ++k) {
+            double nx, ny;
+            curved_normal(bc, &GP[2 * k], &nx, &ny);
+            Ftemp += (wc[k] * GP[2 * k] * nx);
+          }
+          const double len = curved_arclength(bc, a, c);
+          Ftemp = Ftemp * len * 0.5;
+          m->cv_area[h == 0 ? n0 : n1] += Ftemp;
+        }
+        continue;
+      }
       double ymid = (m->x[2 * n0 + 1] + m->x[2 * n1 + 1]) / 2.;
       double nx = m->x[2 * n1 + 1] - ymid;
       m->cv_area[n0] += (0.75 * m->x[2 * n0] + 0.25 * m->x[2 * n1]) * nx;
       m->cv_area[n1] += (0.75 * m->x[2 * n1] + 0.25 * m->x[2 * n0]) * nx;
     }
+  }
   for (int n = 0; n < nn; ++n)
-    if (!(m->cv_area[n] > 0.0)) { if (err) *err = "non-positive control volume area"; return false; }
+    if (!(m->cv_area[n] > 0.0)) {
+      if (err) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "non-positive control volume area %.6e at node %d (%.6f, %.6f)", m->cv_area[n],
+                      n, m->x[2 * (size_t)n], m->x[2 * (size_t)n + 1]);
+        *err = buf;
+      }
+      return false;
+    }
   // control-volume moments about each node (cv_calc.C:981-1518), boundary
   // integrals at 3 Gauss points, divided by the area
   m->moments.assign(9 * (size_t)nn, 0.0);
@@ -719,6 +886,43 @@ bool build_geometry(Mesh* m, std::string* err) {
     int node = be[0], b = be[3], s = m->bseg_off[b] + be[4];
     int nL = m->bseg[2 * s], nR = m->bseg[2 * s + 1];
     double xi = m->x[2 * node], yi = m->x[2 * node + 1];
+    const int bc = m->bbc[b];
+    if (curved_bc(bc)) {
+      // cv_calc.C:1428-1568: the moments' boundary integrals along the half
+      // arc between the node and the curved midpoint
+      double XM[2], XL[2], XR[2], GP[6];
+      curved_midpoint(bc, &m->x[2 * (size_t)nL], &m->x[2 * (size_t)nR], XM);
+      if (node == nL) { XL[0] = xi; XL[1] = yi; XR[0] = XM[0]; XR[1] = XM[1]; }
+      else { XL[0] = XM[0]; XL[1] = XM[1]; XR[0] = xi; XR[1] = yi; }
+      if (!curved_gauss_points(bc, XL, XR, GP, err)) return false;
+      const double len = curved_arclength(bc, XL, XR);
+      double I[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+      for (int k = 0; k < 3; ++k) {
+        double nx, ny;
+        curved_normal(bc, &GP[2 * k], &nx, &ny);
+        const double dx = GP[2 * k] - xi, dy = GP[2 * k + 1] - yi;
+        I[0] += (wc[k] * dx * dx * nx * len);
+        I[1] += (wc[k] * dx * dy * nx * len);
+        I[2] += (wc[k] * dx * dx * dx * nx * len);
+        I[3] += (wc[k] * dx * dy * dy * nx * len);
+        I[4] += (wc[k] * dx * dx * dy * nx * len);
+        I[5] += (wc[k] * dx * dx * dx * dx * nx * len);
+        I[6] += (wc[k] * dx * dy * dy * dy * nx * len);
+        I[7] += (wc[k] * dx * dx * dx * dy * nx * len);
+        I[8] += (wc[k] * dx * dx * dy * dy * nx * len);
+      }
+      I[0] = 0.5 * I[0] * 0.5;
+      I[1] = I[1] * 0.5;
+      I[2] = (1. / 3.) * I[2] * 0.5;
+      I[3] = I[3] * 0.5;
+      I[4] = 0.5 * I[4] * 0.5;
+      I[5] = (0.25) * I[5] * 0.5;
+      I[6] = I[6] * 0.5;
+      I[7] = (1. / 3.) * I[7] * 0.5;
+      I[8] = 0.5 * I[8] * 0.5;
+      for (int k = 0; k < 9; ++k) m->moments[9 * (size_t)node + k] += I[k];
+      continue;
+    }
     double xmid = 0.5 * (m->x[2 * nL] + m->x[2 * nR]);
     double ymid = 0.5 * (m->x[2 * nL + 1] + m->x[2 * nR + 1]);
     double len = m->bedge_n[3 * i + 2], nx = m->bedge_n[3 * i];

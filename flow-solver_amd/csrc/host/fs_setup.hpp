@@ -73,7 +73,9 @@ struct Mesh {
   int nbe = 0;
   std::vector<int> bedge;                       // 5*nbe: real,ghost,gelem,b,s
   std::vector<double> bedge_n;                  // 3*nbe
-  std::vector<double> bedge_xg;                 // 6*nbe
+  std::vector<double> bedge_xg;                 // 6*nbe Gauss points of the half segment
+  std::vector<double> bedge_gn;                 // 6*nbe unit normal at each Gauss point
+                                                //  (the bedge normal on a straight wall)
   std::vector<double> cv_area;                  // nn
   std::vector<double> moments;                  // 9*nn
   std::vector<double> edge_weight;              // 4*ne
@@ -115,6 +117,19 @@ void make_bump_channel(int nx, int ny, double jitter, uint64_t seed, Mesh* m);
 // radius rmax chords, seeded jitter of interior nodes.
 void make_airfoil_omesh(int nwall, int nrad, double h0, double rmax,
                         double jitter, uint64_t seed, Mesh* m);
+// The same O-mesh (nwall even) with the wall cut at the leading and trailing
+// edges into an upper and a lower boundary with their own codes (1 for
+// polyline walls, 17 and 19 for the analytic NACA 0012 surfaces); boundaries
+// upper, lower, farfield.
+void make_airfoil_omesh_split(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed,
+                              int upper_bc, int lower_bc, Mesh* m);
+// Annulus r_in <= r <= r_out around the origin: ntheta (even) nodes per
+// ring on nrad geometrically spaced rings, nodes at angles 0 and pi exactly on
+// the x axis, interior nodes jittered; boundaries inner (bc_in), outer
+// (bc_out).  With the circle codes 11/12 (radius 1) and 21/22 (radius 300)
+// the rings are walls or farfields on the analytic circles.
+void make_annulus(int ntheta, int nrad, double r_in, double r_out, double jitter, uint64_t seed, int bc_in,
+                  int bc_out, Mesh* m);
 
 bool write_grid(const std::string& path, const Mesh& m);
 bool read_grid(const std::string& path, Mesh* m, std::string* err);

@@ -665,7 +665,10 @@ __global__ void __launch_bounds__(kBlock) k_body_terms(DevMesh m, DevState s, co
   }
   const ForceParams f = fp[inst_of(m, m.inst_bedge, b)];
   const int real = m.bedge_real[b];
-  const double nx = -m.bedge_n[3 * (size_t)b], ny = -m.bedge_n[3 * (size_t)b + 1];
+  // straight wall: the bedge normal at every Gauss point, points stored
+  // (+sqrt(3/5), 0, -sqrt(3/5)); curved wall (grid_io.C:1612-1660): the
+  // curve's normal at each point, points stored (0, +sqrt(3/5), -sqrt(3/5))
+  const bool curved = bc > 10;
   const double len = m.bedge_n[3 * (size_t)b + 2];
   const double x0 = m.x[2 * (size_t)real], x1 = m.x[2 * (size_t)real + 1];
   const double phi = s.phi[real], sigma = phi;
@@ -685,10 +688,11 @@ __global__ void __launch_bounds__(kBlock) k_body_terms(DevMesh m, DevState s, co
       c2[j][2] = h[4];
     }
   }
-  const double w[3] = {5. / 9., 8. / 9., 5. / 9.};
+  const double w[3] = {curved ? 8. / 9. : 5. / 9., curved ? 5. / 9. : 8. / 9., 5. / 9.};
   double temp1 = 0., temp2 = 0.;
 #pragma unroll
   for (int g = 0; g < 3; ++g) {
+    const double nx = -m.bedge_gn[6 * (size_t)b + 2 * g], ny = -m.bedge_gn[6 * (size_t)b + 2 * g + 1];
     const double dx0 = m.bedge_xg[6 * (size_t)b + 2 * g] - x0;
     const double dx1 = m.bedge_xg[6 * (size_t)b + 2 * g + 1] - x1;
     double q[4];

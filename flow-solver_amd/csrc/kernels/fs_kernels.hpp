@@ -83,7 +83,8 @@ struct DevMesh {
   const int* bedge_real;    // nbe
   const int* bedge_bc;      // nbe
   const double* bedge_n;    // 3*nbe
-  const double* bedge_xg;   // 6*nbe: Gauss points of the bedge's half segment
+  const double* bedge_xg;   // 6*nbe: Gauss points of the bedge's half segment (half arc on a curved wall)
+  const double* bedge_gn;   // 6*nbe: unit normal at each Gauss point
   // Block storage.  Rows are visited in color order (position t = index in
   // color_rows).  Off-diagonal blocks are sliced: positions are cut into
   // slices of 16, and off-diagonal block e (the reference's nsn order) of the

@@ -24,7 +24,7 @@ FS_PREC_MIXED = 1
 _MESH_ARRAYS = [
     "x", "tri", "nn_map", "ncsn", "csn", "nnsn", "nsn", "esn", "edges", "c2e", "sub_edge", "sub_elem", "sub_n",
     "sub_xm", "sub_xg", "el_cent", "bedge", "bedge_n", "bedge_xg", "cv_area", "moments", "edge_weight", "ia", "iau",
-    "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg",
+    "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg", "bedge_gn",
 ]
 _FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS", "LHS32"]
 _TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block"]
@@ -43,6 +43,39 @@ class FlowSolverError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("fs error %d: %s" % (code, msg))
         self.code = code
+
+
+def _pt(v):
+    a = (C.c_double * 2)(float(v[0]), float(v[1]))
+    return a
+
+
+def curved_midpoint(bc, xL, xR):
+    """curved_boundary_midpoint (curved_boundaries.C:91)."""
+    out = (C.c_double * 2)()
+    _check(lib().fs_curved_midpoint(bc, _pt(xL), _pt(xR), out))
+    return (out[0], out[1])
+
+
+def curved_gauss_points(bc, xL, xR):
+    """curved_boundary_gauss_points (curved_boundaries.C:345): 3 (x, y) points."""
+    out = (C.c_double * 6)()
+    _check(lib().fs_curved_gauss_points(bc, _pt(xL), _pt(xR), out))
+    return [(out[2 * k], out[2 * k + 1]) for k in range(3)]
+
+
+def curved_normal(bc, x):
+    """curved_boundary_normal_vector (curved_boundaries.C:701)."""
+    nx, ny = C.c_double(), C.c_double()
+    _check(lib().fs_curved_normal(bc, _pt(x), C.byref(nx), C.byref(ny)))
+    return (nx.value, ny.value)
+
+
+def curved_arclength(bc, xL, xR):
+    """curved_boundary_arclength (curved_boundaries.C:1009)."""
+    ln = C.c_double()
+    _check(lib().fs_curved_arclength(bc, _pt(xL), _pt(xR), C.byref(ln)))
+    return ln.value
 
 
 def hbm_probe(nbytes=4 << 30, reps=5):
@@ -130,6 +163,12 @@ def lib():
         "fs_mesh_write": (i, [vp, C.c_char_p, C.c_char_p]),
         "fs_mesh_create_bump": (i, [i, i, d, C.c_uint64, C.POINTER(vp)]),
         "fs_mesh_create_airfoil": (i, [i, i, d, d, d, C.c_uint64, C.POINTER(vp)]),
+        "fs_mesh_create_airfoil_split": (i, [i, i, d, d, d, C.c_uint64, i, i, C.POINTER(vp)]),
+        "fs_mesh_create_annulus": (i, [i, i, d, d, d, C.c_uint64, i, i, C.POINTER(vp)]),
+        "fs_curved_midpoint": (i, [i, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
+        "fs_curved_gauss_points": (i, [i, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
+        "fs_curved_normal": (i, [i, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
+        "fs_curved_arclength": (i, [i, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
         "fs_mesh_prepare": (i, [vp, vp]),
         "fs_mesh_get_info": (i, [vp, vp]),
         "fs_mesh_get_array": (i, [vp, i, C.POINTER(vp), C.POINTER(i64), C.POINTER(i)]),
@@ -234,6 +273,21 @@ class Mesh:
     def airfoil(cls, nwall, nrad, h0=1e-3, rmax=30.0, jitter=0.1, seed=3):
         h = C.c_void_p()
         _check(lib().fs_mesh_create_airfoil(nwall, nrad, h0, rmax, jitter, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def airfoil_split(cls, nwall, nrad, upper_bc=17, lower_bc=19, h0=1e-3, rmax=30.0, jitter=0.1, seed=3):
+        """O-mesh with separate upper/lower walls: codes 17/19 are the analytic
+        NACA 0012 surfaces, 1/1 polyline walls."""
+        h = C.c_void_p()
+        _check(lib().fs_mesh_create_airfoil_split(nwall, nrad, h0, rmax, jitter, seed, upper_bc, lower_bc,
+                                                  C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def annulus(cls, ntheta, nrad, r_in=1.0, r_out=3.0, bc_in=11, bc_out=16, jitter=0.1, seed=5):
+        h = C.c_void_p()
+        _check(lib().fs_mesh_create_annulus(ntheta, nrad, r_in, r_out, jitter, seed, bc_in, bc_out, C.byref(h)))
         return cls(h.value)
 
     @classmethod

@@ -68,6 +68,39 @@ int fs_mesh_create_bump(int nx, int ny, double jitter, uint64_t seed, fs_mesh** 
 int fs_mesh_create(int nn, const double* x, int ntri, const int* tri, int nquad, const int* quad, int nb,
                    const int* bseg_off, const int* bseg, const int* bbc, fs_mesh** out);
 int fs_mesh_create_airfoil(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed, fs_mesh** out);
+/* The same O-mesh (nwall even) with its wall cut at the leading and trailing
+ * edges into boundaries upper (upper_bc), lower (lower_bc), farfield (0):
+ * codes 1/1 give polyline walls, 17/19 the analytic NACA 0012 surfaces
+ * (curved_boundaries.C:28-56). */
+int fs_mesh_create_airfoil_split(int nwall, int nrad, double h0, double rmax, double jitter, uint64_t seed,
+                                 int upper_bc, int lower_bc, fs_mesh** out);
+/* Annulus r_in <= r <= r_out: ntheta (even) nodes per ring on nrad
+ * geometrically spaced rings, nodes at angles 0 and pi exactly on the x
+ * axis (the reference's circle code assumes no segment crosses it,
+ * curved_boundaries.C:152-154); boundaries inner (bc_in) and outer (bc_out).  Ring codes: 0-10, 11/12 at
+ * radius 1, 21/22 at radius 300.  Codes 13-16 cannot close a ring: the
+ * reference's arclength for them has no branch-cut handling
+ * (curved_boundaries.C:1063-1093), so the segment next to the angle pi would
+ * be about 2 pi R long. */
+int fs_mesh_create_annulus(int ntheta, int nrad, double r_in, double r_out, double jitter, uint64_t seed, int bc_in,
+                           int bc_out, fs_mesh** out);
+
+/* ---- analytic curved walls, boundary codes 11-22 (curved_boundaries.C:28-44;
+ *      include/curved_boundaries.h:21-33).  Points are (x, y).  The NACA
+ *      arclength is the reference's integral evaluated by adaptive
+ *      Gauss-Legendre quadrature after x = t^2 (the reference uses GSL QAGS
+ *      at 1e-8 tolerance): NACA results agree with the reference's to that
+ *      tolerance, circle results follow its formulas operation by
+ *      operation. ---- */
+/* curved_boundaries.C:91 curved_boundary_midpoint */
+int fs_curved_midpoint(int bc, const double* xL, const double* xR, double* xM);
+/* curved_boundaries.C:345 curved_boundary_gauss_points: GP[6] = the arc's
+ * Gauss points (middle, +sqrt(3/5), -sqrt(3/5)) */
+int fs_curved_gauss_points(int bc, const double* xL, const double* xR, double* GP);
+/* curved_boundaries.C:701 curved_boundary_normal_vector */
+int fs_curved_normal(int bc, const double* x, double* nx, double* ny);
+/* curved_boundaries.C:1009 curved_boundary_arclength */
+int fs_curved_arclength(int bc, const double* xL, const double* xR, double* len);
 /* Preprocessing of main.C:149-559: Cuthill_McKee (reorder.C:58), metrics2D
  * (metrics.C:31), CSN/NSN/ESN/c2e maps (maps.C:35-517), Build_Edge_Data and
  * Build_Boundary_Edge_Data (geometry.C:119,407), Generate_Gaussian_Data_For_Edge
@@ -109,7 +142,7 @@ typedef enum {
   FS_M_EL_CENT,        /* f64 2*nel */
   FS_M_BEDGE,          /* i32 5*nbe */
   FS_M_BEDGE_N,        /* f64 3*nbe */
-  FS_M_BEDGE_XG,       /* f64 6*nbe */
+  FS_M_BEDGE_XG,       /* f64 6*nbe: Gauss points of each bedge's half segment or half arc */
   FS_M_CV_AREA,        /* f64 nn */
   FS_M_MOMENTS,        /* f64 9*nn */
   FS_M_EDGE_WEIGHT,    /* f64 4*ne */
@@ -119,6 +152,8 @@ typedef enum {
   FS_M_ST_W,           /* f64 5 per stencil entry */
   FS_M_BBC,            /* i32 nb */
   FS_M_BSEG_OFF, FS_M_BSEG, /* i32 */
+  FS_M_BEDGE_GN,       /* f64 6*nbe: unit normal at each bedge Gauss point
+                          (the curve's on a curved wall, the bedge normal otherwise) */
   FS_M_COUNT
 } fs_mesh_array;
 /* Borrowed host pointer to a preprocessed array; dtype 0 = int32, 1 = f64. */

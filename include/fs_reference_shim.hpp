@@ -417,7 +417,7 @@ double linear_solve(G* grid, S* smem, P p) {
 }
 
 // grid_io.C:1390-1697: lift (Forces[0]) and drag (Forces[1]) over the walls
-// (odd bc, straight boundaries) from the GRID's state after step grid->citer
+// (odd bc; straight and curved) from the GRID's state after step grid->citer
 template <class G, class P>
 void Compute_Body_Forces(G* grid, P p, double* Forces) {
   Twin& t = detail::twin(grid, detail::to_fs(p));
@@ -437,6 +437,25 @@ void write_surface_cp(G* grid, P p) {
   Twin& t = detail::twin(grid, detail::to_fs(p));
   detail::put(t, FS_F_Q, grid->Q, 4, true);
   detail::check(fs_write_surface_cp(t.solver, "."), "write_surface_cp");
+}
+
+// include/curved_boundaries.h:25-33 (boundary codes 11-22): the analytic
+// curved walls on the host; the GRID argument is unused, as in the reference
+template <class G>
+void curved_boundary_midpoint(G*, int bc, double* xL, double* xR, double* xM) {
+  detail::check(fs_curved_midpoint(bc, xL, xR, xM), "curved_boundary_midpoint");
+}
+template <class G>
+void curved_boundary_gauss_points(G*, int bc, double* xL, double* xR, double* GP) {
+  detail::check(fs_curved_gauss_points(bc, xL, xR, GP), "curved_boundary_gauss_points");
+}
+template <class G>
+void curved_boundary_normal_vector(G*, int bc, double* x, double* nx, double* ny) {
+  detail::check(fs_curved_normal(bc, x, nx, ny), "curved_boundary_normal_vector");
+}
+template <class G>
+void curved_boundary_arclength(G*, int bc, double* xL, double* xR, double* arclength) {
+  detail::check(fs_curved_arclength(bc, xL, xR, arclength), "curved_boundary_arclength");
 }
 
 // releases the device twin of a GRID

@@ -832,9 +832,29 @@ void fso_body_forces(const fso_mesh* m, const fso_params* p, const fso_state* s,
     int nL, nR;
     double nx, ny, len, xi, yi, xmid, ymid, xL, xR, yL, yR, temp1 = 0., temp2 = 0.;
     if (bc == 0 || (bc % 2 == 0)) continue;
+    len = m->bedge_n[3 * i + 2];
+    if (bc > 10) {
+      /* grid_io.C:1612-1660: curved wall, the curve's Gauss points and
+         normals (from the mesh's geometry), weights 8/9, 5/9, 5/9 */
+      const double wc[3] = {8. / 9., 5. / 9., 5. / 9.};
+      for (g = 0; g < 3; g++) {
+        double q[4], P;
+        const double* gp = &m->bedge_xg[6 * i + 2 * g];
+        nx = -m->bedge_gn[6 * i + 2 * g];
+        ny = -m->bedge_gn[6 * i + 2 * g + 1];
+        recon_gauss_boundary(m, p, s, citer, real, gp, q);
+        P = pressure_of(p->gamma, q);
+        temp1 += (wc[g] * (P * ny * cos(angle) - P * nx * sin(angle)));
+        temp2 += (wc[g] * (P * nx * cos(angle) + P * ny * sin(angle)));
+      }
+      temp1 = temp1 * len * 0.5;
+      temp2 = temp2 * len * 0.5;
+      forces[0] += temp1;
+      forces[1] += temp2;
+      continue;
+    }
     nx = -m->bedge_n[3 * i];
     ny = -m->bedge_n[3 * i + 1];
-    len = m->bedge_n[3 * i + 2];
     nL = m->bseg[2 * seg];
     nR = m->bseg[2 * seg + 1];
     xi = m->x[2 * real];

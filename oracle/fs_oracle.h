@@ -32,7 +32,11 @@
  *                           no reference-build vector exists for the k=2 phase
  *   fso_body_forces         reference build (forces per step, first-order and linear
  *                           phases); k=2 phase: quadratic exactness of the Gauss-point
- *                           reconstruction and its reduction to the linear phase (tests)
+ *                           reconstruction and its reduction to the linear phase (tests);
+ *                           curved walls (codes 11-22, Gauss points and normals read from
+ *                           the mesh): no net force from a uniform state on a closed
+ *                           curved wall (the reference's curved code needs GSL and does
+ *                           not build here)
  *   fso_surface_cp          closed form (uniform state gives Cp of the freestream pressure)
  *   fso_limiter_barth (k=2 Gauss points): linear preservation (phi = 1 for exact
  *                           k=2 data of a linear field) and a bowl limited only near
@@ -74,6 +78,8 @@ typedef struct {
   const int* color_rows;    /* nn */
   const int* bseg_off;      /* nb+1: segments of boundary b */
   const int* bseg;          /* 2 per segment (n0, n1) */
+  const double* bedge_xg;   /* 6*nbe: Gauss points of the bedge (read for curved walls only) */
+  const double* bedge_gn;   /* 6*nbe: unit normal at each Gauss point (curved walls only) */
 } fso_mesh;
 
 typedef struct {

@@ -37,7 +37,8 @@ class FsoMesh(C.Structure):
         ("bedge", C.c_void_p), ("bedge_n", C.c_void_p), ("bbc", C.c_void_p), ("cv_area", C.c_void_p),
         ("moments", C.c_void_p), ("edge_weight", C.c_void_p), ("ia", C.c_void_p), ("iau", C.c_void_p),
         ("ja", C.c_void_p), ("st_off", C.c_void_p), ("st_nodes", C.c_void_p), ("ncolor", C.c_int),
-        ("color_off", C.c_void_p), ("color_rows", C.c_void_p), ("bseg_off", C.c_void_p), ("bseg", C.c_void_p)]
+        ("color_off", C.c_void_p), ("color_rows", C.c_void_p), ("bseg_off", C.c_void_p), ("bseg", C.c_void_p),
+        ("bedge_xg", C.c_void_p), ("bedge_gn", C.c_void_p)]
 
 
 class FsoState(C.Structure):
@@ -104,7 +105,8 @@ class Oracle:
         m = FsoMesh()
         m.nn, m.ng, m.ne, m.nsub, m.nbe, m.nnz, m.nb = (info[k] for k in ("nn", "ng", "ne", "nsub", "nbe", "nnz", "nb"))
         m.nel = info["ntri"] + info["nquad"]
-        for name in ("x", "sub_n", "sub_xm", "sub_xg", "el_cent", "bedge_n", "cv_area", "moments", "edge_weight"):
+        for name in ("x", "sub_n", "sub_xm", "sub_xg", "el_cent", "bedge_n", "cv_area", "moments", "edge_weight",
+                     "bedge_xg", "bedge_gn"):
             setattr(m, name, arr(name, np.float64))
         for name in ("edges", "sub_edge", "sub_elem", "bedge", "bbc", "ia", "iau", "ja", "color_off", "color_rows",
                      "bseg_off", "bseg"):

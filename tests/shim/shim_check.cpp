@@ -9,6 +9,10 @@
 //                                 mesh; then one reference call sequence through the
 //                                 loop-level wrappers.  Exit 0 when every compared array
 //                                 is identical bit for bit.
+//   shim_check curved bc xL yL xR yR
+//                                 the curved_boundary_* functions on one segment: prints
+//                                 the midpoint, the 3 Gauss points, the normal at (xL, yL)
+//                                 and the arclength (%.17g)
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -185,11 +189,29 @@ static int solve_mode(int nx, int ny, int seed, int order, int limit, int steps)
   return ok ? 0 : 1;
 }
 
+struct NoGrid {};  // the curved functions take a GRID* they do not read
+
+static int curved_mode(int bc, double xl, double yl, double xr, double yr) {
+  double xL[2] = {xl, yl}, xR[2] = {xr, yr}, xM[2], GP[6], nx, ny, len;
+  NoGrid* g = nullptr;
+  fsref::curved_boundary_midpoint(g, bc, xL, xR, xM);
+  fsref::curved_boundary_gauss_points(g, bc, xL, xR, GP);
+  fsref::curved_boundary_normal_vector(g, bc, xL, &nx, &ny);
+  fsref::curved_boundary_arclength(g, bc, xL, xR, &len);
+  std::printf("%.17g %.17g\n", xM[0], xM[1]);
+  for (int k = 0; k < 3; ++k) std::printf("%.17g %.17g\n", GP[2 * k], GP[2 * k + 1]);
+  std::printf("%.17g %.17g\n%.17g\n", nx, ny, len);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc == 4 && !std::strcmp(argv[1], "roe")) return roe_mode(argv[2], argv[3]);
   if (argc == 8 && !std::strcmp(argv[1], "solve"))
     return solve_mode(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]),
                       std::atoi(argv[6]), std::atoi(argv[7]));
-  std::printf("usage: shim_check roe <in> <out> | solve nx ny seed order limit steps\n");
+  if (argc == 7 && !std::strcmp(argv[1], "curved"))
+    return curved_mode(std::atoi(argv[2]), std::atof(argv[3]), std::atof(argv[4]), std::atof(argv[5]),
+                       std::atof(argv[6]));
+  std::printf("usage: shim_check roe <in> <out> | solve nx ny seed order limit steps | curved bc xL yL xR yR\n");
   return 2;
 }

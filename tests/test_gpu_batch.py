@@ -21,7 +21,7 @@ def _params(**kw):
 
 
 def _cases():
-    """(mesh factory, params): uneven sizes, both orders, both geometries."""
+    """(mesh factory, params): uneven sizes, both orders, straight and curved walls."""
     return [
         (lambda: fsm.Mesh.bump(33, 17, 0.2, 1), _params(mach=0.5, order=1)),
         (lambda: fsm.Mesh.bump(41, 13, 0.2, 2), _params(mach=0.6, alpha=1.0, order=3, soits=4, limit=1)),
@@ -31,6 +31,11 @@ def _cases():
         # 3 colors where the others have 4: its backward launches of color 2
         # must be skipped for it to see its own sweep order
         (lambda: fsm.Mesh.bump(3, 3, 0.0, 1), _params(mach=0.4, order=1)),
+        # analytic curved walls: NACA 0012 surfaces, cylinder in a circular farfield
+        (lambda: fsm.Mesh.airfoil_split(40, 9, upper_bc=17, lower_bc=19, h0=2e-2, rmax=8.0, jitter=0.1, seed=6),
+         _params(mach=0.5, alpha=1.5, order=3, soits=2, limit=1)),
+        (lambda: fsm.Mesh.annulus(32, 9, r_in=1.0, r_out=300.0, bc_in=11, bc_out=22, jitter=0.1, seed=7),
+         _params(mach=0.3, alpha=0.5, order=1)),
     ]
 
 

@@ -32,12 +32,26 @@ def _params(**kw):
     return fsm.Params.make(**base)
 
 
-def _pair(nx, ny, seed, airfoil=False, **kw):
-    """Device solver and oracle on one seeded mesh: the bump channel of nx x ny
-    nodes, or (airfoil) the O-mesh of nx wall x ny radial nodes around the
-    12%-thick profile."""
+def _mesh(kind, nx, ny, seed):
+    """bump: channel of nx x ny nodes; airfoil: O-mesh of nx wall x ny radial
+    nodes around the 12%-thick profile (polyline wall); naca: the same with the
+    analytic NACA 0012 upper/lower surfaces (codes 17/19); cylinder: unit-circle
+    wall (code 11) in the R=300 farfield circle (code 22)."""
+    if kind == "bump":
+        return fsm.Mesh.bump(nx, ny, 0.2, seed)
+    if kind == "airfoil":
+        return fsm.Mesh.airfoil(nx, ny, 2e-2, 8.0, 0.1, seed)
+    if kind == "naca":
+        return fsm.Mesh.airfoil_split(nx, ny, upper_bc=17, lower_bc=19, h0=2e-2, rmax=8.0, jitter=0.1, seed=seed)
+    if kind == "cylinder":
+        return fsm.Mesh.annulus(nx, ny, r_in=1.0, r_out=300.0, bc_in=11, bc_out=22, jitter=0.1, seed=seed)
+    raise ValueError(kind)
+
+
+def _pair(nx, ny, seed, airfoil=False, kind=None, **kw):
+    """Device solver and oracle on one seeded mesh (see _mesh; airfoil=True is kind "airfoil")."""
     p = _params(**kw)
-    m = fsm.Mesh.airfoil(nx, ny, 2e-2, 8.0, 0.1, seed) if airfoil else fsm.Mesh.bump(nx, ny, 0.2, seed)
+    m = _mesh(kind or ("airfoil" if airfoil else "bump"), nx, ny, seed)
     m.prepare(p)
     s = fsm.Solver(m, p, fsm.FS_PREC_F64)
     o = U.Oracle(m, p)
@@ -174,13 +188,17 @@ def test_roe_wave_form_matches_reference_form():
 @pytest.mark.parametrize("order,soits,limit,airfoil,newton", [
     (1, 0, 1, False, (1, 1)), (3, 2, 1, False, (1, 1)), (1, 0, 2, False, (1, 1)), (3, 2, 2, False, (1, 1)),
     (1, 0, 1, True, (1, 1)), (3, 2, 1, True, (1, 1)),
+    # analytic curved walls (curved_boundaries.C)
+    (1, 0, 1, "naca", (1, 1)), (3, 2, 1, "naca", (1, 1)), (1, 0, 1, "cylinder", (1, 1)),
+    (3, 2, 1, "cylinder", (1, 1)),
     # 3 Newton iterations per step with the Jacobian updated every 2nd
     # (solve.C:328-338): iterations without an update keep only V/dt
     (1, 0, 1, False, (3, 2)), (3, 2, 1, False, (2, 2))])
 def test_step_history(order, soits, limit, airfoil, newton):
     """fs_solve against the oracle's solve with the same multicolor sweep order
-    (bump channel, and the O-mesh with its closed wall; several Newton
-    iterations per step with Jacobian reuse).
+    (bump channel, the O-mesh with its closed polyline wall, the O-mesh with
+    the analytic NACA surfaces, the cylinder in a circular farfield; several
+    Newton iterations per step with Jacobian reuse).
 
     dQ bar: 1e-10 with one Newton iteration per step.  With several, the
     sweeps' last-bit differences (1e-17 relative, from the summation order of
@@ -191,9 +209,13 @@ def test_step_history(order, soits, limit, airfoil, newton):
     identical start by test_newton_iterations_match_oracle."""
     it, ju = newton
     dq_bar = 1e-10 if it == 1 else 1e-6
-    p, m, s, o = (_pair(40, 11, 4, airfoil=True, order=order, soits=soits, limit=limit, subits=10, alpha=2.0)
-                  if airfoil else _pair(33, 17, 2, order=order, soits=soits, limit=limit, subits=10, iter=it,
-                                        jufreq=ju))
+    if airfoil in ("naca", "cylinder"):
+        p, m, s, o = _pair(40, 11, 4, kind=airfoil, order=order, soits=soits, limit=limit, subits=10, alpha=2.0,
+                           mach=0.3 if airfoil == "cylinder" else 0.5)
+    elif airfoil:
+        p, m, s, o = _pair(40, 11, 4, airfoil=True, order=order, soits=soits, limit=limit, subits=10, alpha=2.0)
+    else:
+        p, m, s, o = _pair(33, 17, 2, order=order, soits=soits, limit=limit, subits=10, iter=it, jufreq=ju)
     s.init_freestream()
     o.init_freestream()
     nsteps = p.foits + soits + 4
@@ -310,13 +332,20 @@ def _oracle_from_device(s, o, info):
 
 
 @pytest.mark.parametrize("order,soits,limit,airfoil", [(1, 0, 1, False), (3, 2, 1, False), (3, 2, 2, False),
-                                                       (1, 0, 0, False), (3, 2, 1, True)])
+                                                       (1, 0, 0, False), (3, 2, 1, True), (3, 2, 1, "naca"),
+                                                       (3, 2, 1, "cylinder")])
 def test_body_forces_match_oracle(order, soits, limit, airfoil):
     """Compute_Body_Forces on the device (per-bedge Gauss-point terms summed
     in bedge order) equals the oracle's bit for bit from the same fields, in
-    every phase: nodal, limited linear and the k=2 point-value quadratic."""
-    p, m, s, o = (_pair(40, 11, 21, airfoil=True, order=order, soits=soits, limit=limit, alpha=2.0, subits=8)
-                  if airfoil else _pair(33, 17, 21, order=order, soits=soits, limit=limit, alpha=2.0, subits=8))
+    every phase: nodal, limited linear and the k=2 point-value quadratic; on
+    curved walls with the curve's Gauss points and normals."""
+    if airfoil in ("naca", "cylinder"):
+        p, m, s, o = _pair(40, 11, 21, kind=airfoil, order=order, soits=soits, limit=limit, alpha=2.0, subits=8,
+                           mach=0.3 if airfoil == "cylinder" else 0.5)
+    elif airfoil:
+        p, m, s, o = _pair(40, 11, 21, airfoil=True, order=order, soits=soits, limit=limit, alpha=2.0, subits=8)
+    else:
+        p, m, s, o = _pair(33, 17, 21, order=order, soits=soits, limit=limit, alpha=2.0, subits=8)
     info = m.info()
     s.init_freestream()
     nsteps = p.foits + soits + 3
@@ -358,10 +387,13 @@ def test_surface_cp_file(tmp_path):
         np.testing.assert_allclose(np.array(rows), np.array(want), atol=1e-6)
 
 
-def test_transonic_airfoil_fp64_and_mixed_vs_oracle():
+@pytest.mark.parametrize("kind", ["airfoil", "naca"])
+def test_transonic_airfoil_fp64_and_mixed_vs_oracle(kind):
     """The precision configuration in miniature: an O-mesh of 64 x 17 nodes
     around the 12%-thick profile (first cell 1e-3 chord, 30 chords, +-0.1
-    jitter), M=0.8 at 1.25 degrees, k=2 with the Barth limiter.
+    jitter), M=0.8 at 1.25 degrees, k=2 with the Barth limiter; the wall a
+    polyline (airfoil) or the analytic NACA 0012 surfaces (naca, codes 17/19,
+    128 wall nodes).
 
     Fixed point: FP64 and mixed solves converged to a 1e-12 residual drop
     against the oracle's (same drop): density L2 within 1e-10, lift and drag
@@ -370,7 +402,11 @@ def test_transonic_airfoil_fp64_and_mixed_vs_oracle():
     case (the FP64 path 7e-10, measured), so that comparison is printed, and
     bounded only by 1e-8."""
     p = _params(order=3, foits=10, soits=10, limit=1, mach=0.8, alpha=1.25, CFL=100.0, ramp=30, subits=20)
-    m = fsm.Mesh.airfoil(64, 17, 1e-3, 30.0, 0.1, 3)
+    # the curved wall at 128 wall nodes: at 64, the arc's bulge at the leading
+    # edge (1.8e-3) exceeds the first cell (1e-3) and the multicolor and
+    # sequential sweeps both stop at a non-physical state with these controls
+    m = (fsm.Mesh.airfoil(64, 17, 1e-3, 30.0, 0.1, 3) if kind == "airfoil" else
+         fsm.Mesh.airfoil_split(128, 17, upper_bc=17, lower_bc=19, h0=1e-3, rmax=30.0, jitter=0.1, seed=3))
     m.prepare(p)
     nn = m.info()["nn"]
     o = U.Oracle(m, p)

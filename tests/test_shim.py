@@ -2,7 +2,8 @@
 driven by tests/shim/shim_check (built by __graft_entry__.build()).
 
 CPU: the host Roe_flux_centered of the shared per-face header equals the
-oracle's restatement of flux.C:299-541 bit for bit.
+oracle's restatement of flux.C:299-541 bit for bit; the curved_boundary_*
+functions (curved_boundaries.h:25-33) return what the C-ABI returns.
 GPU: a GRID in the reference layout advanced by fsref::solve, and one
 reference call sequence through the loop-level wrappers, give the C-ABI
 solver's arrays bit for bit."""
@@ -101,3 +102,18 @@ def test_reference_driver_with_device_solve():
     fc, fg = out["cpu"]["forces_%d" % n], out["gpu"]["forces_%d" % n]
     print("body forces CPU %s device %s" % (fc, fg))
     np.testing.assert_allclose(fg, fc, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("bc,xL,xR", [(17, (0.1, 0.04683), (0.2, 0.05738)), (19, (0.6, -0.0456), (0.5, -0.0529)),
+                                      (11, (1.0, 0.0), (0.9238795325112867, 0.3826834323650898)),
+                                      (22, (300.0, 0.0), (0.0, 300.0))])
+def test_curved_boundary_functions(bc, xL, xR):
+    _need_shim()
+    out = subprocess.run([SHIM, "curved", str(bc), repr(xL[0]), repr(xL[1]), repr(xR[0]), repr(xR[1])],
+                         capture_output=True, text=True, check=True).stdout.split()
+    v = [float(t) for t in out]
+    assert tuple(v[0:2]) == U.fsm.curved_midpoint(bc, xL, xR)
+    gp = U.fsm.curved_gauss_points(bc, xL, xR)
+    assert [tuple(v[2 + 2 * k:4 + 2 * k]) for k in range(3)] == gp
+    assert tuple(v[8:10]) == U.fsm.curved_normal(bc, xL)
+    assert v[10] == U.fsm.curved_arclength(bc, xL, xR)
