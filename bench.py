@@ -127,6 +127,7 @@ def parse():
     ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-solve-at-a-time FP64 baseline")
     ap.add_argument("--batch-precisions", default="fp64,mixed")
     ap.add_argument("--config3-max-steps", type=int, default=3000)
+    ap.add_argument("--no-config3-curved", action="store_true", help="skip the analytic-wall variant of config 3")
     ap.add_argument("--sections", default="config2,config3,batch,serving,cpu",
                     help="parts to run (a profiled run picks one; the headline needs config2)")
     return ap.parse_args()
@@ -313,24 +314,11 @@ def bench_serving(a, precision):
             "nodes_median": float(np.median([r["nn"] for r in res])) if res else None}
 
 
-def bench_config3(a, fsm):
-    """The precision configuration: an O-mesh of 513 wall x 1171 radial nodes
-    around the 12%-thick profile (first cell 1e-3 chord, 30 chords, +-0.1
-    jitter, about 1.2 M triangles), M=0.8 at 1.25 degrees, k=2 with the Barth
-    limiter.  FP64 and mixed steady solves to a 1e-10 drop (hipGraph steps);
-    the mixed path's error against the FP64 path for the converged density
-    and the lift-like wall pressure integrals (Compute_Body_Forces)."""
+def _config3_solves(a, fsm, mesh, p, info):
+    """FP64 and mixed steady solves of one config-3 mesh to a 1e-10 drop
+    (hipGraph steps); the mixed path's error against the FP64 path."""
     import numpy as np
-    p = fsm.Params.make(nsteps=0, iter=1, jufreq=1, gamma=1.4, alpha=1.25, mach=0.8, method=1, order=3, foits=10,
-                        soits=10, limit=1, reorder=1, **CONFIG3_CONTROLS)
-    t0 = time.perf_counter()
-    mesh = fsm.Mesh.airfoil(513, 1171, 1e-3, 30.0, 0.1, 3)
-    mesh.prepare(p)
-    info = mesh.info()
-    out = {"workload": "O-mesh 513x1171 around the 12%-thick profile, M=0.8, alpha=1.25, k=2, Barth",
-           "nodes": info["nn"], "triangles": info["ntri"], "subedges": info["nsub"], "bedges": info["nbe"],
-           "setup_s": time.perf_counter() - t0, "controls": dict(CONFIG3_CONTROLS, tol=1e-10)}
-    rho, forces = {}, {}
+    out, rho, forces = {}, {}, {}
     evals = info["nsub"] + info["nbe"]
     for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
         s = fsm.Solver(mesh, p, prec)
@@ -363,6 +351,43 @@ def bench_config3(a, fsm):
     rel = np.abs(forces["mixed"] - forces["fp64"]) / np.abs(forces["fp64"])
     out["lift_rel_diff_mixed_vs_fp64"] = float(rel[0])
     out["drag_rel_diff_mixed_vs_fp64"] = float(rel[1])
+    return out
+
+
+def bench_config3(a, fsm):
+    """The precision configuration: an O-mesh of 513 wall x 1171 radial nodes
+    around the 12%-thick profile (first cell 1e-3 chord, 30 chords, +-0.1
+    jitter, about 1.2 M triangles), M=0.8 at 1.25 degrees, k=2 with the Barth
+    limiter, with a polyline wall.  FP64 and mixed steady solves to a 1e-10
+    drop; the mixed path's error against the FP64 path for the converged
+    density and the lift-like wall pressure integrals (Compute_Body_Forces).
+    Then the same on the analytic NACA 0012 surfaces (codes 17/19, 512 wall
+    nodes: the upper/lower split needs a leading-edge node), and the
+    geometry's effect on lift and drag."""
+    p = fsm.Params.make(nsteps=0, iter=1, jufreq=1, gamma=1.4, alpha=1.25, mach=0.8, method=1, order=3, foits=10,
+                        soits=10, limit=1, reorder=1, **CONFIG3_CONTROLS)
+    t0 = time.perf_counter()
+    mesh = fsm.Mesh.airfoil(513, 1171, 1e-3, 30.0, 0.1, 3)
+    mesh.prepare(p)
+    info = mesh.info()
+    out = {"workload": "O-mesh 513x1171 around the 12%-thick profile, M=0.8, alpha=1.25, k=2, Barth",
+           "nodes": info["nn"], "triangles": info["ntri"], "subedges": info["nsub"], "bedges": info["nbe"],
+           "setup_s": time.perf_counter() - t0, "controls": dict(CONFIG3_CONTROLS, tol=1e-10)}
+    out.update(_config3_solves(a, fsm, mesh, p, info))
+    del mesh
+    if a.no_config3_curved:
+        return out
+    t0 = time.perf_counter()
+    mesh = fsm.Mesh.airfoil_split(512, 1171, upper_bc=17, lower_bc=19, h0=1e-3, rmax=30.0, jitter=0.1, seed=3)
+    mesh.prepare(p)
+    info = mesh.info()
+    cv = {"workload": "the same O-mesh with 512 wall nodes on the analytic NACA 0012 surfaces (codes 17/19)",
+          "nodes": info["nn"], "triangles": info["ntri"], "setup_s": time.perf_counter() - t0}
+    cv.update(_config3_solves(a, fsm, mesh, p, info))
+    if "lift" in cv.get("fp64", {}) and "lift" in out.get("fp64", {}):
+        cv["lift_curved_vs_polyline"] = cv["fp64"]["lift"] / out["fp64"]["lift"]
+        cv["drag_curved_vs_polyline"] = cv["fp64"]["drag"] / out["fp64"]["drag"]
+    out["curved"] = cv
     return out
 
 

@@ -38,13 +38,15 @@
  *                           curved wall (the reference's curved code needs GSL and does
  *                           not build here)
  *   fso_surface_cp          closed form (uniform state gives Cp of the freestream pressure)
- *   fso_limiter_barth (k=2 Gauss points): linear preservation (phi = 1 for exact
- *                           k=2 data of a linear field) and a bowl limited only near
- *                           its minimum (tests); no reference-build vector
- *   fso_limiter_venkat (k=2 Gauss points), k=2 phase of fso_solve: parity unpinned
- *                           against the reference build (it cannot enter the k=2
- *                           phase here); the k=2 phase of fso_solve is pinned by the
- *                           invariants of its parts
+ *   fso_limiter_barth, fso_limiter_venkat (k=2 Gauss points): from CV averages of
+ *                           quadratic fields, phi equals the per-point function at the
+ *                           exact field values of every Gauss point (tests); Barth also
+ *                           by linear preservation and a bowl limited only near its
+ *                           minimum; no reference-build vector (it cannot enter the k=2
+ *                           phase here)
+ *   k=2 phase of fso_solve  no reference-build vector; pinned by the invariants of its
+ *                           parts (first-order residual and Jacobian as in the reference's
+ *                           k=2 phase, derivatives, limiters, body forces above)
  */
 #ifndef FS_ORACLE_H
 #define FS_ORACLE_H
