@@ -89,20 +89,36 @@ double arclen_density_t(double t) {
   const double p = 0.08907 - t * (0.07560 + t2 * (0.42192 - t2 * (0.51174 - t2 * 0.24864)));
   return 2.0 * std::sqrt(t2 + p * p);
 }
+// the reference's integrand in x, sqrt(1 + y'(x)^2) (curved_boundaries.C:58-65)
+double arclen_density_x(double x) {
+  const double yp = naca_slope(x);
+  return std::sqrt(1.0 + yp * yp);
+}
+template <double (*F)(double)>
 double panel(double a, double b) {
   const GaussLegendre& g = gauss_legendre();
   const double m = 0.5 * (a + b), h = 0.5 * (b - a);
   double s = 0.0;
-  for (int i = 0; i < kGL; ++i) s += g.w[i] * arclen_density_t(m + h * g.x[i]);
+  for (int i = 0; i < kGL; ++i) s += g.w[i] * F(m + h * g.x[i]);
   return s * h;
 }
-// interval halving until the two halves agree with the whole to a few ulps
-// (the integrand is positive, so panel sums never cancel)
-double adapt(double a, double b, double whole, int depth) {
+// Interval halving until the two halves agree with the whole to 1e-14 of
+// the panel or to its share of 1e-15 of the whole integral.  The integrand
+// is positive, so panel sums never cancel, and the 1e-14 floor lies well
+// above the rounding of a 10-point sum (about 1e-15), so rounding alone never
+// drives the recursion; the depth cap bounds it regardless.
+template <double (*F)(double)>
+double adapt(double a, double b, double whole, double tol, int depth) {
   const double m = 0.5 * (a + b);
-  const double l = panel(a, m), r = panel(m, b), lr = l + r;
-  if (depth >= 40 || std::fabs(lr - whole) <= 4e-15 * lr) return lr;
-  return adapt(a, m, l, depth + 1) + adapt(m, b, r, depth + 1);
+  const double l = panel<F>(a, m), r = panel<F>(m, b), lr = l + r;
+  const double diff = std::fabs(lr - whole);
+  if (depth >= 24 || diff <= tol || diff <= 1e-14 * lr) return lr;
+  return adapt<F>(a, m, l, 0.5 * tol, depth + 1) + adapt<F>(m, b, r, 0.5 * tol, depth + 1);
+}
+template <double (*F)(double)>
+double integrate(double a, double b) {
+  const double whole = panel<F>(a, b);
+  return adapt<F>(a, b, whole, 1e-15 * whole, 0);
 }
 
 }  // namespace
@@ -115,8 +131,12 @@ double naca_y(double x, int bc) {
 
 double naca_arclength(double xa, double xb) {
   if (!(xb > xa)) return 0.0;
-  const double ta = std::sqrt(xa), tb = std::sqrt(xb);
-  return adapt(ta, tb, panel(ta, tb), 0);
+  // Away from the leading edge (the interval at least twice its length from
+  // x = 0) the integrand in x is smooth on the interval and is integrated as
+  // it is: there sqrt(xb) - sqrt(xa) would lose digits to cancellation.
+  // Otherwise in t = sqrt(x), where the singularity is gone.
+  if (xa >= 2.0 * (xb - xa)) return integrate<arclen_density_x>(xa, xb);
+  return integrate<arclen_density_t>(std::sqrt(xa), std::sqrt(xb));
 }
 
 void curved_midpoint(int bc, const double* xL, const double* xR, double* xM) {

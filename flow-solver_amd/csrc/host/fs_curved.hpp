@@ -15,10 +15,10 @@
 //
 // The reference integrates the NACA arclength with GSL's QAGS at 1e-8
 // absolute and relative tolerance.  GSL is not part of this build: the
-// arclength here is the same integral after the substitution x = t^2, which
-// removes the 1/sqrt(x) singularity at the leading edge, evaluated by
-// adaptive Gauss-Legendre quadrature to about 1e-15 relative.  Results agree
-// with the reference's to its quadrature tolerance, not bit for bit.
+// arclength here is the same integral, near the leading edge after the
+// substitution x = t^2 (which removes the 1/sqrt(x) singularity), evaluated
+// by adaptive Gauss-Legendre quadrature to about 1e-15 relative.  Results
+// agree with the reference's to its quadrature tolerance, not bit for bit.
 #pragma once
 #include <string>
 

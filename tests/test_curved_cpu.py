@@ -65,6 +65,22 @@ def test_naca_arclength_matches_qags(a, b):
     assert ours == rev
 
 
+def test_naca_arclength_random_intervals():
+    """Many intervals, from a few 1e-12 chord to the whole chord: each
+    matches QAGS, and the sum over a partition equals the whole."""
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        a = rng.uniform(0, 1) ** 3
+        b = min(1.0, a + 10 ** rng.uniform(-12, 0))
+        ours = fsm.curved_arclength(17, (a, naca_y(a)), (b, naca_y(b)))
+        ref = qags(a, b)
+        assert abs(ours - ref) <= 1e-13 * ref + 1e-17, (a, b, ours, ref)
+    cuts = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, 63)), [1.0]])
+    parts = sum(fsm.curved_arclength(17, (p0, naca_y(p0)), (p1, naca_y(p1))) for p0, p1 in zip(cuts[:-1], cuts[1:]))
+    whole = fsm.curved_arclength(17, (0.0, 0.0), (1.0, naca_y(1.0)))
+    assert abs(parts - whole) <= 1e-14 * whole
+
+
 def test_naca_gauss_points_at_arclength_fractions():
     beta = [0.5, (1 + math.sqrt(0.6)) / 2, (1 - math.sqrt(0.6)) / 2]
     for bc, sgn in ((17, 1.0), (19, -1.0)):
