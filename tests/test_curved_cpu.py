@@ -277,3 +277,18 @@ def test_uniform_state_no_force_on_closed_curved_wall():
     assert max(abs(v) for v in out[(1, 1)]) <= 1e-15
     # measured 6.6e-13 with the weights 8/9, 5/9, 5/9; 4e-4 with 5/9, 8/9, 5/9
     assert max(abs(v) for v in out[(17, 19)]) <= 1e-11
+
+
+def test_curved_codes_through_the_grid_files(tmp_path):
+    """A mesh with curved codes written in the reference's grid and bc.dat
+    formats and read back prepares to the same curved geometry."""
+    m = fsm.Mesh.airfoil_split(64, 9, upper_bc=17, lower_bc=19, h0=2e-2, rmax=8.0, jitter=0.1, seed=3)
+    g, bc = str(tmp_path / "naca.mesh"), str(tmp_path / "bc.dat")
+    m.write(g, bc)
+    assert open(bc).read().split() == ["17", "19", "0"]
+    m2 = fsm.Mesh.read(g, bc)
+    p = _params()
+    m.prepare(p)
+    m2.prepare(p)
+    for name in ("x", "bedge_n", "bedge_xg", "bedge_gn", "sub_n", "sub_xm", "cv_area", "moments"):
+        np.testing.assert_array_equal(m.array(name), m2.array(name), err_msg=name)
