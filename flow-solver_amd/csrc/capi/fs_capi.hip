@@ -357,6 +357,7 @@ int fs_mesh_get_array(const fs_mesh* m, int which, const void** data, int64_t* c
     case FS_M_BSEG_OFF: return I(g.bseg_off);
     case FS_M_BSEG: return I(g.bseg);
     case FS_M_BEDGE_GN: return D(g.bedge_gn);
+    case FS_M_QUAD: return I(g.quad);
     default: return fail(FS_ERR_ARG, "unknown mesh array id");
   }
 }

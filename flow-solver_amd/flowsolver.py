@@ -24,7 +24,7 @@ FS_PREC_MIXED = 1
 _MESH_ARRAYS = [
     "x", "tri", "nn_map", "ncsn", "csn", "nnsn", "nsn", "esn", "edges", "c2e", "sub_edge", "sub_elem", "sub_n",
     "sub_xm", "sub_xg", "el_cent", "bedge", "bedge_n", "bedge_xg", "cv_area", "moments", "edge_weight", "ia", "iau",
-    "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg", "bedge_gn",
+    "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg", "bedge_gn", "quad",
 ]
 _FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS", "LHS32"]
 _TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block"]
@@ -273,6 +273,24 @@ class Mesh:
     def airfoil(cls, nwall, nrad, h0=1e-3, rmax=30.0, jitter=0.1, seed=3):
         h = C.c_void_p()
         _check(lib().fs_mesh_create_airfoil(nwall, nrad, h0, rmax, jitter, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, x, tri, quad, bseg_off, bseg, bbc):
+        """A mesh from 0-based arrays: node coordinates (nn x 2), triangles
+        (ntri x 3), quads (nquad x 4), boundary segment offsets (nb + 1),
+        segments (nseg x 2) and boundary codes (nb): the grid file's contents."""
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
+        quad = np.ascontiguousarray(quad, dtype=np.int32).reshape(-1)
+        bseg_off = np.ascontiguousarray(bseg_off, dtype=np.int32)
+        bseg = np.ascontiguousarray(bseg, dtype=np.int32).reshape(-1)
+        bbc = np.ascontiguousarray(bbc, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().fs_mesh_create(len(x) // 2, x.ctypes.data, len(tri) // 3, tri.ctypes.data, len(quad) // 4,
+                                    quad.ctypes.data, len(bbc), bseg_off.ctypes.data, bseg.ctypes.data,
+                                    bbc.ctypes.data, C.byref(h)))
         return cls(h.value)
 
     @classmethod

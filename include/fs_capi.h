@@ -154,6 +154,7 @@ typedef enum {
   FS_M_BSEG_OFF, FS_M_BSEG, /* i32 */
   FS_M_BEDGE_GN,       /* f64 6*nbe: unit normal at each bedge Gauss point
                           (the curve's on a curved wall, the bedge normal otherwise) */
+  FS_M_QUAD,           /* i32 4*nquad */
   FS_M_COUNT
 } fs_mesh_array;
 /* Borrowed host pointer to a preprocessed array; dtype 0 = int32, 1 = f64. */

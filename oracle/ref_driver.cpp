@@ -122,6 +122,7 @@ int main(int argc, char** argv) {
   if (grid.nn_map) dumpi("nn_map", grid.nn_map, nn + 1);
   dumpd("x", grid.x, (long)(nn + ng + 1) * 2);
   dumpi("c2n_tri", grid.c2n[Tri], (long)(grid.num_elem[Tri] + 1) * 3);
+  if (grid.num_elem[Quad] > 0) dumpi("c2n_quad", grid.c2n[Quad], (long)(grid.num_elem[Quad] + 1) * 4);
   dumpi("ncsn", grid.ncsn, nn + 2);
   dumpi("csn", grid.csn, grid.ncsn[nn + 1]);
   dumpi("nnsn", grid.nnsn, nn + 2);
@@ -129,6 +130,7 @@ int main(int argc, char** argv) {
   dumpi("esn", grid.esn, grid.nnsn[nn + 1]);
   dumpi("edges", grid.edges, (long)(grid.nedges + 1) * 2);
   dumpi("c2e_tri", grid.c2e[Tri], (long)(grid.num_elem[Tri] + 1) * 3);
+  if (grid.num_elem[Quad] > 0) dumpi("c2e_quad", grid.c2e[Quad], (long)(grid.num_elem[Quad] + 1) * 4);
   dumpi("subedges", grid.subedges, (long)(grid.nsubedges + 1) * 2);
   dumpd("xn_subedges", grid.xn_subedges, (long)(grid.nsubedges + 1) * 3);
   dumpd("xm_subedges", grid.xm_subedges, (long)(grid.nsubedges + 1) * 2);

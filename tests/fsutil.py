@@ -190,10 +190,32 @@ def read_dump(path):
     return out
 
 
+def bump_mixed(nx, ny, seed, jitter=0.2):
+    """The seeded bump channel with mixed elements: the cells with
+    (i + 2 j) % 3 == 0 stay quadrilaterals (counter-clockwise), the others are
+    split into two triangles as in the all-triangle channel.  Triangles come
+    first in the element numbering, as in the grid file."""
+    b = fsm.Mesh.bump(nx, ny, jitter, seed)
+    x = b.array("x")
+    tri, quad = [], []
+    for j in range(ny - 1):
+        for i in range(nx - 1):
+            a, bb, c, d = j * nx + i, j * nx + i + 1, (j + 1) * nx + i + 1, (j + 1) * nx + i
+            if (i + 2 * j) % 3 == 0:
+                quad.append((a, bb, c, d))
+            else:
+                tri += [(a, bb, c), (a, c, d)]
+    return fsm.Mesh.from_arrays(x, np.array(tri), np.array(quad), b.array("bseg_off"), b.array("bseg"),
+                                b.array("bbc"))
+
+
 def golden_mesh(g):
-    """The seeded mesh a golden fixture was written from: the bump channel,
-    or the O-mesh when the fixture records (first cell, radius, jitter)."""
+    """The seeded mesh a golden fixture was written from: the bump channel
+    (with quadrilaterals when the fixture is marked "mixed"), or the O-mesh
+    when the fixture records (first cell, radius, jitter)."""
     nx, ny, seed = (int(v) for v in g["params"][:3])
+    if "mixed" in g.files:
+        return bump_mixed(nx, ny, seed)
     if "airfoil" in g.files:
         h0, radius, jitter = (float(v) for v in g["airfoil"])
         return fsm.Mesh.airfoil(nx, ny, h0, radius, jitter, seed)

@@ -33,12 +33,17 @@ def params(**kw):
 
 
 def run_case(name, nx, ny, seed, p, nsteps, stencil, step_fields, dump_every=1, keep_steps=None, extra=None,
-             setup=True, lhs_steps=(), airfoil=None):
+             setup=True, lhs_steps=(), airfoil=None, mixed=False):
     """airfoil = (first cell, outer radius, jitter): an O-mesh of nx wall x ny
-    radial nodes instead of the bump channel."""
+    radial nodes instead of the bump channel; mixed: the bump channel with
+    quadrilaterals (fsutil.bump_mixed)."""
     with tempfile.TemporaryDirectory() as tmp:
-        mesh = fsm.Mesh.airfoil(nx, ny, airfoil[0], airfoil[1], airfoil[2], seed) if airfoil else \
-            fsm.Mesh.bump(nx, ny, 0.2, seed)
+        if airfoil:
+            mesh = fsm.Mesh.airfoil(nx, ny, airfoil[0], airfoil[1], airfoil[2], seed)
+        elif mixed:
+            mesh = U.bump_mixed(nx, ny, seed)
+        else:
+            mesh = fsm.Mesh.bump(nx, ny, 0.2, seed)
         U.write_case(tmp, mesh, p)
         out = os.path.join(tmp, "dump.bin")
         r = subprocess.run([U.REF_DRIVER, tmp, out, str(nsteps), "1" if stencil else "0", str(dump_every)],
@@ -51,10 +56,14 @@ def run_case(name, nx, ny, seed, p, nsteps, stencil, step_fields, dump_every=1, 
             "hdr": d["hdr"]}
     if airfoil:
         keep["airfoil"] = np.array(airfoil, dtype=np.float64)
+    if mixed:
+        keep["mixed"] = np.array([1], dtype=np.int64)
     if p.iter != 1 or p.jufreq != 1:
         keep["newton"] = np.array([p.iter, p.jufreq], dtype=np.int64)
     for k in (SETUP_INT + SETUP_DBL) if setup else ["Q0"]:
         keep["s_" + k] = d[k]
+    if setup and "c2n_quad" in d:
+        keep["s_c2n_quad"] = d["c2n_quad"]
     if stencil:
         for k in ("gnnsn1", "gnnsn", "gnsn"):
             keep["s_" + k] = d[k]
@@ -89,6 +98,11 @@ def main():
     # periodic direction; limited linear phase at an angle of attack
     run_case("airfoil32x9_o1", 32, 9, 4, params(order=1, foits=2, mach=0.5, alpha=2.0), 6, False, full,
              lhs_steps=(6,), airfoil=(2e-2, 8.0, 0.1))
+    # mixed elements: the bump channel with a third of its cells kept as
+    # quadrilaterals (4 subedges each), first-order and limited linear phases
+    run_case("bumpmix17x9_o1", 17, 9, 7, params(order=1, foits=2), 6, False, full, lhs_steps=(6,), mixed=True)
+    run_case("bumpmix17x9_o3", 17, 9, 7, params(order=3, foits=2, soits=3), 5, True, full, lhs_steps=(5,),
+             mixed=True)
     # the CPU-runnable configuration: 97x33 bump channel, M=0.5, k=1, Barth limiter
     run_case("cfg1_97x33", 97, 33, 1, params(order=3, foits=20, soits=20, mach=0.5, alpha=0.0, CFL=100.0, ramp=30,
                                               subits=20), 2, True, ["rms", "R", "dQ", "Q", "forces"])

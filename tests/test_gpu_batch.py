@@ -36,6 +36,8 @@ def _cases():
          _params(mach=0.5, alpha=1.5, order=3, soits=2, limit=1)),
         (lambda: fsm.Mesh.annulus(32, 9, r_in=1.0, r_out=300.0, bc_in=11, bc_out=22, jitter=0.1, seed=7),
          _params(mach=0.3, alpha=0.5, order=1)),
+        # triangles and quadrilaterals
+        (lambda: U.bump_mixed(29, 11, 8), _params(mach=0.5, order=3, soits=2, limit=1)),
     ]
 
 

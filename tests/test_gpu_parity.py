@@ -36,7 +36,8 @@ def _mesh(kind, nx, ny, seed):
     """bump: channel of nx x ny nodes; airfoil: O-mesh of nx wall x ny radial
     nodes around the 12%-thick profile (polyline wall); naca: the same with the
     analytic NACA 0012 upper/lower surfaces (codes 17/19); cylinder: unit-circle
-    wall (code 11) in the R=300 farfield circle (code 22)."""
+    wall (code 11) in the R=300 farfield circle (code 22); mixed: the bump
+    channel with a third of its cells quadrilaterals."""
     if kind == "bump":
         return fsm.Mesh.bump(nx, ny, 0.2, seed)
     if kind == "airfoil":
@@ -45,6 +46,8 @@ def _mesh(kind, nx, ny, seed):
         return fsm.Mesh.airfoil_split(nx, ny, upper_bc=17, lower_bc=19, h0=2e-2, rmax=8.0, jitter=0.1, seed=seed)
     if kind == "cylinder":
         return fsm.Mesh.annulus(nx, ny, r_in=1.0, r_out=300.0, bc_in=11, bc_out=22, jitter=0.1, seed=seed)
+    if kind == "mixed":
+        return U.bump_mixed(nx, ny, seed)
     raise ValueError(kind)
 
 
@@ -191,6 +194,8 @@ def test_roe_wave_form_matches_reference_form():
     # analytic curved walls (curved_boundaries.C)
     (1, 0, 1, "naca", (1, 1)), (3, 2, 1, "naca", (1, 1)), (1, 0, 1, "cylinder", (1, 1)),
     (3, 2, 1, "cylinder", (1, 1)),
+    # triangles and quadrilaterals
+    (1, 0, 1, "mixed", (1, 1)), (3, 2, 1, "mixed", (1, 1)),
     # 3 Newton iterations per step with the Jacobian updated every 2nd
     # (solve.C:328-338): iterations without an update keep only V/dt
     (1, 0, 1, False, (3, 2)), (3, 2, 1, False, (2, 2))])
@@ -209,7 +214,9 @@ def test_step_history(order, soits, limit, airfoil, newton):
     identical start by test_newton_iterations_match_oracle."""
     it, ju = newton
     dq_bar = 1e-10 if it == 1 else 1e-6
-    if airfoil in ("naca", "cylinder"):
+    if airfoil == "mixed":
+        p, m, s, o = _pair(33, 17, 2, kind="mixed", order=order, soits=soits, limit=limit, subits=10)
+    elif airfoil in ("naca", "cylinder"):
         p, m, s, o = _pair(40, 11, 4, kind=airfoil, order=order, soits=soits, limit=limit, subits=10, alpha=2.0,
                            mach=0.3 if airfoil == "cylinder" else 0.5)
     elif airfoil:

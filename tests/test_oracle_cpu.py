@@ -49,7 +49,7 @@ def _setup(g, **over):
 
 
 @pytest.mark.parametrize("case", ["bump17x9_o1", "bump17x9_o3", "bump17x9_o1_venkat", "cfg1_97x33", "airfoil32x9_o1",
-                                  "bump17x9_o1_newton"])
+                                  "bump17x9_o1_newton", "bumpmix17x9_o1", "bumpmix17x9_o3"])
 def test_step_history_bitexact(case):
     g = _load(case)
     p, m, o, nsteps = _setup(g)

@@ -9,7 +9,7 @@ import pytest
 import fsutil as U
 
 fsm = U.fsm
-CASES = ["bump17x9_o1", "bump17x9_o3", "cfg1_97x33", "airfoil32x9_o1"]
+CASES = ["bump17x9_o1", "bump17x9_o3", "cfg1_97x33", "airfoil32x9_o1", "bumpmix17x9_o1"]
 
 
 def _load(name):
@@ -38,6 +38,8 @@ def test_maps_bitexact(case):
     np.testing.assert_array_equal(m.array("edges"), g["s_edges"][2:] - 1)
     np.testing.assert_array_equal(m.array("csn"), g["s_csn"] - 1)
     np.testing.assert_array_equal(m.array("tri"), g["s_c2n_tri"][3:] - 1)
+    if "s_c2n_quad" in g.files:
+        np.testing.assert_array_equal(m.array("quad"), g["s_c2n_quad"][4:] - 1)
     se = g["s_subedges"].reshape(-1, 2)[1:]
     np.testing.assert_array_equal(m.array("sub_edge"), se[:, 0] - 1)
     np.testing.assert_array_equal(m.array("sub_elem"), se[:, 1] - 1)
@@ -60,7 +62,7 @@ def test_geometry_bitexact(case):
         assert np.array_equal(a, b), (mine, np.max(np.abs(a - b)))
 
 
-@pytest.mark.parametrize("case", ["bump17x9_o3", "cfg1_97x33"])
+@pytest.mark.parametrize("case", ["bump17x9_o3", "cfg1_97x33", "bumpmix17x9_o3"])
 def test_k2_stencils_bitexact(case):
     g = _load(case)
     p, m = _mesh_for(g)
