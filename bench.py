@@ -124,6 +124,7 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="length of the request stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.5, help="offered requests per second")
     ap.add_argument("--serve-max-steps", type=int, default=2000)
+    ap.add_argument("--serve-precisions", default="mixed,fp64", help="streams to run, in this order")
     ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-solve-at-a-time FP64 baseline")
     ap.add_argument("--batch-precisions", default="fp64,mixed")
     ap.add_argument("--config3-max-steps", type=int, default=3000)
@@ -277,6 +278,7 @@ def bench_serving(a, precision):
     cases = W.serving_cases(len(arrivals))
     params = workload_params(W, cases)
     meshes = [W.make_mesh(c) for c in cases]
+    print("serving: %d request meshes built" % len(meshes), file=sys.stderr, flush=True)
     srv = fsm.Server(precision=precision, max_steps=a.serve_max_steps, tol=1e-10, prep_threads=16)
     res = []
 
@@ -291,20 +293,30 @@ def bench_serving(a, precision):
     th = threading.Thread(target=collect)
     th.start()
     w0 = time.perf_counter()
-    for arr, m, p in zip(arrivals, meshes, params):
+    name = "mixed" if precision == fsm.FS_PREC_MIXED else "fp64"
+    last = 0.0
+    for k, (arr, m, p) in enumerate(zip(arrivals, meshes, params)):
         d = arr - (time.perf_counter() - w0)
         if d > 0:
             time.sleep(d)
         srv.submit(m, p)
-    th.join()
+        now = time.perf_counter() - w0
+        if now - last >= 60.0:
+            last = now
+            print("serving %s: %d of %d submitted, %d finished at %.0f s" % (name, k + 1, len(cases), len(res), now),
+                  file=sys.stderr, flush=True)
+    while th.is_alive():
+        th.join(60.0)
+        if th.is_alive():
+            print("serving %s: draining, %d of %d finished at %.0f s" %
+                  (name, len(res), len(cases), time.perf_counter() - w0), file=sys.stderr, flush=True)
     wall = time.perf_counter() - w0
     srv.close()
     st = np.array([r["status"] for r in res])
     lat = np.array([r["t_done"] - r["t_submit"] for r in res])
     dev = np.array([r["t_done"] - r["t_admit"] for r in res])
     conv = st == fsm.INST_CONVERGED
-    print("serving %s done: %d requests in %.1f s" % ("mixed" if precision == fsm.FS_PREC_MIXED else "fp64", len(res),
-                                                    wall), file=sys.stderr, flush=True)
+    print("serving %s done: %d requests in %.1f s" % (name, len(res), wall), file=sys.stderr, flush=True)
     q = lambda v, p: float(np.percentile(v, p)) if len(v) else None
     return {"requests": len(res), "offered_rate_per_s": a.serve_rate, "stream_s": a.serve_seconds, "wall_s": wall,
             "converged": int(conv.sum()), "max_steps": int((st == fsm.INST_MAX_STEPS).sum()),
@@ -605,7 +617,8 @@ def main():
         line["batch"] = bench_batch(a)
         print("batch done", file=sys.stderr, flush=True)
     if "serving" in sections and a.serve_seconds > 0:
-        sv = {"mixed": bench_serving(a, fsm.FS_PREC_MIXED), "fp64": bench_serving(a, fsm.FS_PREC_F64)}
+        precs = {"mixed": fsm.FS_PREC_MIXED, "fp64": fsm.FS_PREC_F64}
+        sv = {k: bench_serving(a, precs[k]) for k in a.serve_precisions.split(",") if k}
         sv["controls"] = dict(WORKLOAD_CONTROLS, tol=1e-10, max_steps=a.serve_max_steps)
         line["serving"] = sv
     if "cpu" in sections and not a.no_cpu_baseline:
