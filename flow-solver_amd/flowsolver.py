@@ -303,7 +303,9 @@ class Mesh:
         return cls(h.value)
 
     @classmethod
-    def annulus(cls, ntheta, nrad, r_in=1.0, r_out=3.0, bc_in=11, bc_out=16, jitter=0.1, seed=5):
+    def annulus(cls, ntheta, nrad, r_in=1.0, r_out=300.0, bc_in=11, bc_out=22, jitter=0.1, seed=5):
+        """Annulus of geometrically spaced rings: by default the unit-circle
+        wall (code 11) inside the R=300 farfield circle (code 22)."""
         h = C.c_void_p()
         _check(lib().fs_mesh_create_annulus(ntheta, nrad, r_in, r_out, jitter, seed, bc_in, bc_out, C.byref(h)))
         return cls(h.value)
