@@ -151,6 +151,15 @@ def workload_params(W, cases):
     return [W.case_params(c, limit=(0 if c["k"] == 1 else 1), **WORKLOAD_CONTROLS) for c in cases]
 
 
+def outcome_groups(fsm, cases, status):
+    """Solve outcomes by mesh kind and reconstruction order."""
+    groups = {}
+    for c, st in zip(cases, status):
+        g = groups.setdefault("%s_k%d" % (c["kind"], c["k"]), {"converged": 0, "max_steps": 0, "failed": 0})
+        g[{fsm.INST_CONVERGED: "converged", fsm.INST_MAX_STEPS: "max_steps"}.get(int(st), "failed")] += 1
+    return groups
+
+
 def bench_batch(a):
     import numpy as np
     import workloads as W
@@ -197,7 +206,10 @@ def bench_batch(a):
                              "steps_median": float(np.median(steps[conv])) if conv.any() else None,
                              "instance_steps_total": int(steps.sum()),
                              "steps_per_instance": [int(v) for v in steps],
-                             "solves_per_s": float(conv.sum()) / wall}
+                             "solves_per_s": float(conv.sum()) / wall,
+                             "outcomes": outcome_groups(fsm, cases, status),
+                             "unconverged_mach_min": min([cases[i]["mach"] for i in range(a.batch) if not conv[i]],
+                                                         default=None)}
             b.close()
         out[name] = res
     if "fp64" not in out or "mixed" not in out:
@@ -295,11 +307,12 @@ def bench_serving(a, precision):
     w0 = time.perf_counter()
     name = "mixed" if precision == fsm.FS_PREC_MIXED else "fp64"
     last = 0.0
+    rid_case = {}
     for k, (arr, m, p) in enumerate(zip(arrivals, meshes, params)):
         d = arr - (time.perf_counter() - w0)
         if d > 0:
             time.sleep(d)
-        srv.submit(m, p)
+        rid_case[srv.submit(m, p)] = k
         now = time.perf_counter() - w0
         if now - last >= 60.0:
             last = now
@@ -318,7 +331,12 @@ def bench_serving(a, precision):
     conv = st == fsm.INST_CONVERGED
     print("serving %s done: %d requests in %.1f s" % (name, len(res), wall), file=sys.stderr, flush=True)
     q = lambda v, p: float(np.percentile(v, p)) if len(v) else None
+    # outcomes by mesh kind and reconstruction order, and the Mach numbers of
+    # the requests that did not converge
+    groups = outcome_groups(fsm, [cases[rid_case[r["id"]]] for r in res], [r["status"] for r in res])
+    bad = [cases[rid_case[r["id"]]]["mach"] for r in res if r["status"] != fsm.INST_CONVERGED]
     return {"requests": len(res), "offered_rate_per_s": a.serve_rate, "stream_s": a.serve_seconds, "wall_s": wall,
+            "outcomes": groups, "unconverged_mach_min": min(bad) if bad else None,
             "converged": int(conv.sum()), "max_steps": int((st == fsm.INST_MAX_STEPS).sum()),
             "failed": int((st == fsm.INST_FAILED).sum()),
             "p50_time_to_converged_s": q(lat[conv], 50), "p99_time_to_converged_s": q(lat[conv], 99),
