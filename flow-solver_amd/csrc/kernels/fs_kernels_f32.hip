@@ -1,13 +1,20 @@
-// FP32 inner kernels of the mixed-precision path (gfx950, wave64).
+// Kernels of the mixed-precision path (gfx950, wave64).
 //
-// The outer loop stays FP64: the residual (face fluxes, reconstruction,
-// limiter), its norms, the convergence test and the state update use the FP64
-// kernels.  Inside it the Jacobian blocks and the Gauss-Seidel sweeps run in
-// FP32: the analytic Roe Jacobian of compute_approximate_jacobian_roe
-// (jacobian.C:301-658) at the same reconstructed face states the reference
-// differentiates around, FP32 block storage (64 B per 4x4 block) and FP32
-// sweeps.  The fixed point is the FP64 residual's, so converged states agree
-// with the FP64 path to the FP64 stopping tolerance.
+// The outer loop stays FP64: the node kernel k_node_mixed evaluates the FP64
+// residual (characteristic-wave Roe flux at the FP64 face states), ev_con and
+// dt; the limiter, gradient, derivatives, norms, convergence test and state
+// update are the FP64 kernels.  Inside it the Jacobian blocks and the
+// Gauss-Seidel sweeps run in FP32: the analytic Roe Jacobian of
+// compute_approximate_jacobian_roe (jacobian.C:301-658) at the same
+// reconstructed face states the reference differentiates around, FP32 block
+// storage (64 B per 4x4 block), diagonal blocks inverted in FP64 with
+// pivoting and stored FP32, and FP32 sweeps.  The fixed point is the FP64
+// residual's, so converged states agree with the FP64 path to the FP64
+// stopping tolerance.
+//
+// Contraction (FMA) is on in this file, for the mixed path's FP64 residual
+// too: unlike the FP64 path's kernels it is not bound to the reference's
+// operation order, and contraction moves its fluxes by last-bit amounts only.
 #include "fs_device.hpp"
 #include "fs_kernels.hpp"
 
