@@ -123,3 +123,21 @@ def test_batch_rejects_mixed_sweep_controls():
     with pytest.raises(fsm.FlowSolverError) as e:
         fsm.Batch([m1, m2], [p1, p2])
     assert e.value.code == -1 and "subits" in str(e.value)
+
+
+def test_mesh_from_arrays_validates_input():
+    """fs_mesh_create (the grid file's contents from arrays) rejects node
+    indices out of range in triangles, quadrilaterals and segments, and
+    a mesh with both element types prepares."""
+    x = [(0, 0), (1, 0), (2, 0), (0, 1), (1, 1), (2, 1)]
+    tri = [(0, 1, 4), (0, 4, 3)]
+    quad = [(1, 2, 5, 4)]
+    seg = [(0, 1), (1, 2), (2, 5), (5, 4), (4, 3)]
+    seg_closed = seg + [(3, 0)]
+    m = fsm.Mesh.from_arrays(x, tri, quad, [0, 6], seg_closed, [1])
+    m.prepare(fsm.Params.make(order=1, subits=4))
+    info = m.info()
+    assert (info["ntri"], info["nquad"], info["nsub"]) == (2, 1, 10)
+    for bad_tri, bad_quad, bad_seg in (([(0, 1, 9)], quad, seg), (tri, [(1, 2, 5, 6)], seg), (tri, quad, [(0, 7)])):
+        with pytest.raises(fsm.FlowSolverError):
+            fsm.Mesh.from_arrays(x, bad_tri, bad_quad, [0, len(bad_seg)], bad_seg, [1])
