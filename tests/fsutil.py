@@ -209,6 +209,28 @@ def bump_mixed(nx, ny, seed, jitter=0.2):
                                 b.array("bbc"))
 
 
+def annulus_quads(ntheta, nrad, r_in=1.0, r_out=300.0, bc_in=11, bc_out=22):
+    """Quadrilateral annulus on geometrically spaced rings (no jitter), nodes
+    at angles 0 and pi on the x axis; boundaries inner, outer (domain on the
+    left of every segment)."""
+    th = 2.0 * np.pi * np.arange(ntheta) / ntheta
+    rr = r_in * (r_out / r_in) ** (np.arange(nrad) / (nrad - 1.0))
+    rr[-1] = r_out
+    x = np.zeros((nrad * ntheta, 2))
+    for r in range(nrad):
+        x[r * ntheta:(r + 1) * ntheta, 0] = rr[r] * np.cos(th)
+        x[r * ntheta:(r + 1) * ntheta, 1] = rr[r] * np.sin(th)
+        x[r * ntheta, :] = (rr[r], 0.0)
+        x[r * ntheta + ntheta // 2, :] = (-rr[r], 0.0)
+    idx = lambda r, k: r * ntheta + (k % ntheta)
+    quad = [(idx(r, k), idx(r + 1, k), idx(r + 1, k + 1), idx(r, k + 1)) for r in range(nrad - 1)
+            for k in range(ntheta)]
+    seg = [(idx(0, k + 1), idx(0, k)) for k in range(ntheta)] + \
+          [(idx(nrad - 1, k), idx(nrad - 1, k + 1)) for k in range(ntheta)]
+    return fsm.Mesh.from_arrays(x, np.zeros((0, 3)), np.array(quad), [0, ntheta, 2 * ntheta], np.array(seg),
+                                [bc_in, bc_out])
+
+
 def golden_mesh(g):
     """The seeded mesh a golden fixture was written from: the bump channel
     (with quadrilaterals when the fixture is marked "mixed"), or the O-mesh

@@ -292,3 +292,20 @@ def test_curved_codes_through_the_grid_files(tmp_path):
     m2.prepare(p)
     for name in ("x", "bedge_n", "bedge_xg", "bedge_gn", "sub_n", "sub_xm", "cv_area", "moments"):
         np.testing.assert_array_equal(m.array(name), m2.array(name), err_msg=name)
+
+
+@pytest.mark.parametrize("ntheta", [64, 128])
+def test_quad_annulus_of_circles_domain_integrals(ntheta):
+    """Quadrilateral cells on the curved walls: the boundary subedge of a
+    curved segment then starts at a quadrilateral's centroid.  The same
+    closed-form domain integrals as on the triangle annulus."""
+    m = U.annulus_quads(ntheta, 13)
+    m.prepare(_params())
+    assert m.info()["ntri"] == 0 and m.info()["nquad"] == 12 * ntheta
+    d = _domain_sums(m)
+    R4 = math.pi * (300.0 ** 4 - 1.0) / 4
+    exact = dict(one=math.pi * (300.0 ** 2 - 1.0), x=0.0, y=0.0, xx=R4, yy=R4, xy=0.0)
+    scale = dict(one=300.0 ** 2, x=300.0 ** 3, y=300.0 ** 3, xx=300.0 ** 4, yy=300.0 ** 4, xy=300.0 ** 4)
+    for k in exact:
+        print(k, d[k], exact[k], (d[k] - exact[k]) / scale[k])
+        assert abs(d[k] - exact[k]) <= 1e-12 * scale[k], k

@@ -37,7 +37,8 @@ def _mesh(kind, nx, ny, seed):
     nodes around the 12%-thick profile (polyline wall); naca: the same with the
     analytic NACA 0012 upper/lower surfaces (codes 17/19); cylinder: unit-circle
     wall (code 11) in the R=300 farfield circle (code 22); mixed: the bump
-    channel with a third of its cells quadrilaterals."""
+    channel with a third of its cells quadrilaterals; cylquad: the cylinder on
+    quadrilaterals only."""
     if kind == "bump":
         return fsm.Mesh.bump(nx, ny, 0.2, seed)
     if kind == "airfoil":
@@ -48,6 +49,8 @@ def _mesh(kind, nx, ny, seed):
         return fsm.Mesh.annulus(nx, ny, r_in=1.0, r_out=300.0, bc_in=11, bc_out=22, jitter=0.1, seed=seed)
     if kind == "mixed":
         return U.bump_mixed(nx, ny, seed)
+    if kind == "cylquad":
+        return U.annulus_quads(nx, ny)
     raise ValueError(kind)
 
 
@@ -194,8 +197,8 @@ def test_roe_wave_form_matches_reference_form():
     # analytic curved walls (curved_boundaries.C)
     (1, 0, 1, "naca", (1, 1)), (3, 2, 1, "naca", (1, 1)), (1, 0, 1, "cylinder", (1, 1)),
     (3, 2, 1, "cylinder", (1, 1)),
-    # triangles and quadrilaterals
-    (1, 0, 1, "mixed", (1, 1)), (3, 2, 1, "mixed", (1, 1)),
+    # triangles and quadrilaterals; quadrilaterals on curved walls
+    (1, 0, 1, "mixed", (1, 1)), (3, 2, 1, "mixed", (1, 1)), (3, 2, 1, "cylquad", (1, 1)),
     # 3 Newton iterations per step with the Jacobian updated every 2nd
     # (solve.C:328-338): iterations without an update keep only V/dt
     (1, 0, 1, False, (3, 2)), (3, 2, 1, False, (2, 2))])
@@ -216,9 +219,9 @@ def test_step_history(order, soits, limit, airfoil, newton):
     dq_bar = 1e-10 if it == 1 else 1e-6
     if airfoil == "mixed":
         p, m, s, o = _pair(33, 17, 2, kind="mixed", order=order, soits=soits, limit=limit, subits=10)
-    elif airfoil in ("naca", "cylinder"):
+    elif airfoil in ("naca", "cylinder", "cylquad"):
         p, m, s, o = _pair(40, 11, 4, kind=airfoil, order=order, soits=soits, limit=limit, subits=10, alpha=2.0,
-                           mach=0.3 if airfoil == "cylinder" else 0.5)
+                           mach=0.5 if airfoil == "naca" else 0.3)
     elif airfoil:
         p, m, s, o = _pair(40, 11, 4, airfoil=True, order=order, soits=soits, limit=limit, subits=10, alpha=2.0)
     else:
@@ -340,15 +343,15 @@ def _oracle_from_device(s, o, info):
 
 @pytest.mark.parametrize("order,soits,limit,airfoil", [(1, 0, 1, False), (3, 2, 1, False), (3, 2, 2, False),
                                                        (1, 0, 0, False), (3, 2, 1, True), (3, 2, 1, "naca"),
-                                                       (3, 2, 1, "cylinder")])
+                                                       (3, 2, 1, "cylinder"), (3, 2, 1, "cylquad")])
 def test_body_forces_match_oracle(order, soits, limit, airfoil):
     """Compute_Body_Forces on the device (per-bedge Gauss-point terms summed
     in bedge order) equals the oracle's bit for bit from the same fields, in
     every phase: nodal, limited linear and the k=2 point-value quadratic; on
     curved walls with the curve's Gauss points and normals."""
-    if airfoil in ("naca", "cylinder"):
+    if airfoil in ("naca", "cylinder", "cylquad"):
         p, m, s, o = _pair(40, 11, 21, kind=airfoil, order=order, soits=soits, limit=limit, alpha=2.0, subits=8,
-                           mach=0.3 if airfoil == "cylinder" else 0.5)
+                           mach=0.5 if airfoil == "naca" else 0.3)
     elif airfoil:
         p, m, s, o = _pair(40, 11, 21, airfoil=True, order=order, soits=soits, limit=limit, alpha=2.0, subits=8)
     else:
