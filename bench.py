@@ -126,6 +126,8 @@ def parse():
     ap.add_argument("--serve-max-steps", type=int, default=2000)
     ap.add_argument("--serve-precisions", default="mixed,fp64", help="streams to run, in this order")
     ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-solve-at-a-time FP64 baseline")
+    ap.add_argument("--batch-plain-steady", action="store_true",
+                    help="also run every batch solve alone to its stopping test in FP64 (about 90 s more)")
     ap.add_argument("--batch-precisions", default="fp64,mixed")
     ap.add_argument("--config3-max-steps", type=int, default=3000)
     ap.add_argument("--no-config3-curved", action="store_true", help="skip the analytic-wall variant of config 3")
@@ -236,6 +238,27 @@ def bench_batch(a):
         t_plain = sum(n * ms for n, ms in zip(st, plain["ms_per_step_per_instance"])) * 1e-3
         out["plain_fp64_time_to_solution_s_estimated"] = t_plain
         out["time_to_solution_speedup_vs_plain_fp64"] = t_plain / out["mixed"]["steady"]["wall_s"]
+    if a.batch_plain_steady and a.batch_max_steps > 0 and "steady" in out["mixed"]:
+        # the plain path measured: every solve alone, one after the other, to
+        # its stopping test (hipGraph steps)
+        t0, nconv = time.perf_counter(), 0
+        for i, (m, p) in enumerate(zip(meshes, params)):
+            s = fsm.Solver(m, p, fsm.FS_PREC_F64)
+            s.use_graph(True)
+            s.init_freestream()
+            try:
+                _, dr = s.solve_steady(a.batch_max_steps, 1e-10)
+                nconv += int(dr <= 1e-10)
+            except fsm.FlowSolverError:
+                pass
+            s.close()
+            if (i + 1) % 32 == 0:
+                print("plain FP64 solves: %d of %d at %.0f s" % (i + 1, len(meshes), time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
+        t_meas = time.perf_counter() - t0
+        out["plain_fp64_time_to_solution_s_measured"] = t_meas
+        out["plain_fp64_converged"] = nconv
+        out["time_to_solution_speedup_vs_plain_fp64_measured"] = t_meas / out["mixed"]["steady"]["wall_s"]
     for k in ("fp64", "mixed"):
         if k in out and "steady" in out[k]:
             out[k]["steady"].pop("steps_per_instance", None)
