@@ -29,7 +29,7 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-I" + 
 
 HOST_SRCS = ["csrc/host/fs_setup.cpp", "csrc/host/fs_curved.cpp"]
 HIP_SRCS = ["csrc/kernels/fs_kernels.hip", "csrc/kernels/fs_kernels_f32.hip", "csrc/capi/fs_engine.hip",
-            "csrc/capi/fs_capi.hip", "csrc/capi/fs_batch.hip", "csrc/capi/fs_serve.hip"]
+            "csrc/capi/fs_capi.hip", "csrc/capi/fs_batch.hip", "csrc/capi/fs_serve.hip", "csrc/capi/fs_norms.hip"]
 
 
 def hipcc():

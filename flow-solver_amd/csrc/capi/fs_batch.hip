@@ -112,6 +112,12 @@ int batch_step(fs_batch* b, double tol, bool stop_on_tol) {
       continue;
     }
     double rn = std::sqrt(I.r2);
+    if (!(rn == rn)) {
+      // a non-finite residual without a device flag: failed, as in serving
+      T.status = FS_INST_FAILED;
+      I.active = false;
+      continue;
+    }
     int tr = ts_ref(I.prm);
     if (I.ts == tr || (I.ts > tr && T.r0 < 0)) T.r0 = rn;
     if (T.r0 > 0) {

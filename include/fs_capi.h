@@ -218,6 +218,13 @@ int fs_roe_flux_batch(int n, double gamma, const double* d_normals, const double
  * the mixed path's FP64 residual uses (same flux to a few ulps of |F|). */
 int fs_roe_flux_batch_form(int n, int form, double gamma, const double* d_normals, const double* d_qL,
                            const double* d_qR, double* d_flux, void* stream);
+/* solve.C:785-793 and solve.C:885-893, the norm loops of solve(): over nn
+ * nodes of device arrays d_R and d_dQ (4 values per node),
+ * d_out[0] = sum R^2 and d_out[1] = sum dQ^2 (d_out a device pointer to 2
+ * doubles).  Summed in the solver's fixed order (chunks of 1024 nodes), so
+ * the same arrays give the same bits as fs_solve's r2.  Allocates its chunk
+ * partials and waits for `stream` before returning. */
+int fs_state_norms(int64_t nn, const double* d_R, const double* d_dQ, double* d_out, void* stream);
 
 /* grid_io.C:1390 Compute_Body_Forces: forces[0] (lift) and forces[1] (drag)
  * integrated over the walls (odd bc <= 10) from the resident state after

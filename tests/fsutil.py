@@ -273,3 +273,42 @@ def perturbed_state(nn, ng, params, seed, amp=0.05):
     rng = np.random.default_rng(seed)
     Q = np.tile(q, nn + ng) * (1.0 + amp * rng.uniform(-1.0, 1.0, 4 * (nn + ng)))
     return Q
+
+
+def flux_scale(mesh, Q, gamma):
+    """Per node and variable, the sum over the node's faces (subedges and
+    boundary edges) of len * (|F(q_a).n| + |F(q_b).n|) from the two nodal
+    states of the face: the size of the flux terms the residual at that node
+    is summed from, which normalises a residual difference where the terms
+    cancel (a residual near zero at a nearly converged state)."""
+    info = mesh.info()
+    nn = info["nn"]
+    q = np.asarray(Q[:4 * (nn + info["ng"])]).reshape(-1, 4)
+
+    def absflux(s, nx, ny):
+        rho = s[:, 0]
+        u, v = s[:, 1] / rho, s[:, 2] / rho
+        p = (gamma - 1.0) * (s[:, 3] - 0.5 * rho * (u * u + v * v))
+        th = u * nx + v * ny
+        return np.abs(np.stack([rho * th, s[:, 1] * th + p * nx, s[:, 2] * th + p * ny, (s[:, 3] + p) * th], 1))
+
+    edges = mesh.array("edges").reshape(-1, 2)
+    se = mesh.array("sub_edge")
+    sn = mesh.array("sub_n").reshape(-1, 3)
+    a, b = edges[se, 0], edges[se, 1]
+    f = (absflux(q[a], sn[:, 0], sn[:, 1]) + absflux(q[b], sn[:, 0], sn[:, 1])) * sn[:, 2:3]
+    scale = np.zeros((nn, 4))
+    np.add.at(scale, a, f)
+    np.add.at(scale, b, f)
+    bed = mesh.array("bedge").reshape(-1, 5)
+    bn = mesh.array("bedge_n").reshape(-1, 3)
+    real = bed[:, 0]
+    ghost = nn + np.arange(bed.shape[0])
+    fb = (absflux(q[real], bn[:, 0], bn[:, 1]) + absflux(q[ghost], bn[:, 0], bn[:, 1])) * bn[:, 2:3]
+    np.add.at(scale, real, fb)
+    return scale.ravel()
+
+
+def residual_err(R, Rref, scale):
+    """max over nodes and variables of |R - Rref| / scale (flux_scale)."""
+    return float(np.max(np.abs(np.asarray(R) - np.asarray(Rref)) / (np.asarray(scale) + 1e-300)))

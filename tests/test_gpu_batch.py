@@ -26,6 +26,8 @@ def _cases():
         (lambda: fsm.Mesh.bump(33, 17, 0.2, 1), _params(mach=0.5, order=1)),
         (lambda: fsm.Mesh.bump(41, 13, 0.2, 2), _params(mach=0.6, alpha=1.0, order=3, soits=4, limit=1)),
         (lambda: fsm.Mesh.airfoil(48, 10, 2e-2, 8.0, 0.05, 3), _params(mach=0.4, alpha=2.0, order=1)),
+        # 3,201 nodes: four norm chunks, at a non-zero position in the pack
+        (lambda: fsm.Mesh.bump(97, 33, 0.2, 1), _params(mach=0.5, alpha=0.5, order=3, soits=3, limit=1)),
         (lambda: fsm.Mesh.bump(17, 9, 0.1, 4), _params(mach=0.3, alpha=-1.0, order=3, soits=2, limit=0)),
         (lambda: fsm.Mesh.bump(25, 11, 0.2, 5), _params(mach=0.55, order=1, limit=1, foits=3)),
         # 3 colors where the others have 4: its backward launches of color 2
