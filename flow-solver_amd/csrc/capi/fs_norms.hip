@@ -9,9 +9,13 @@
 // bits as each other and as fs_solve's r2.
 //
 // fs_state_norms allocates its chunk partials per call and waits for the
-// stream before freeing them.
+// stream before freeing them.  fs_state_norms_noalloc keeps the partials in
+// the file-scope array `scratch` below (kNoAllocMaxChunks chunks), allocates
+// nothing and only enqueues work, so it can run under stream capture or in a
+// loop that must not allocate.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "../kernels/fs_kernels.hpp"
@@ -21,6 +25,15 @@
 namespace {
 
 constexpr int kThreads = 256;
+// 65,536 chunks of 1,024 nodes: up to 67,108,864 nodes, 1 MiB of partials
+constexpr int64_t kNoAllocMaxChunks = 65536;
+constexpr int kMaxDevices = 64;
+
+// Chunk partials of fs_state_norms_noalloc, one copy per device for the life
+// of the process.  Host code never passes `scratch` itself to a kernel: the
+// name is a host-side stand-in, and the device address comes from
+// hipGetSymbolAddress (scratch_on_current_device).
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxChunks];
 
 // chunk ch: sum of R^2 and dQ^2 over the 4 components of nodes
 // [ch*C, min(nn, (ch+1)*C)), C = kNormChunkNodes
@@ -88,6 +101,28 @@ int check_args(int64_t nn, const double* R, const double* dQ, double* out) {
   return FS_OK;
 }
 
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fse::set_error(FS_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fse::set_error(FS_ERR_HIP, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fse::set_error(FS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -105,6 +140,21 @@ int fs_state_norms(int64_t nn, const double* d_R, const double* d_dQ, double* d_
   (void)hipFree(part);
   if (e != hipSuccess) return fse::set_error(FS_ERR_HIP, std::string("norm launch: ") + hipGetErrorString(e));
   if (s != hipSuccess) return fse::set_error(FS_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(s));
+  return FS_OK;
+}
+
+int fs_state_norms_noalloc(int64_t nn, const double* d_R, const double* d_dQ, double* d_out, void* stream) {
+  int rc = check_args(nn, d_R, d_dQ, d_out);
+  if (rc) return rc;
+  if ((nn + fsk::kNormChunkNodes - 1) / fsk::kNormChunkNodes > kNoAllocMaxChunks)
+    return fse::set_error(FS_ERR_ARG, "fs_state_norms_noalloc: at most " +
+                                          std::to_string(kNoAllocMaxChunks * fsk::kNormChunkNodes) +
+                                          " nodes (the fixed partials buffer); use fs_state_norms");
+  double* part = nullptr;
+  rc = scratch_on_current_device(&part);
+  if (rc) return rc;
+  enqueue(nn, d_R, d_dQ, part, d_out, (hipStream_t)stream);
+  FSE_HIP(hipGetLastError());
   return FS_OK;
 }
 

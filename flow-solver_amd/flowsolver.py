@@ -228,6 +228,7 @@ def lib():
         "fs_mesh_create": (i, [i, vp, i, vp, i, vp, i, vp, vp, vp, C.POINTER(vp)]),
         "fs_roe_flux_batch_form": (i, [i, i, d, vp, vp, vp, vp, vp]),
         "fs_state_norms": (i, [i64, vp, vp, vp, vp]),
+        "fs_state_norms_noalloc": (i, [i64, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -225,6 +225,15 @@ int fs_roe_flux_batch_form(int n, int form, double gamma, const double* d_normal
  * the same arrays give the same bits as fs_solve's r2.  Allocates its chunk
  * partials and waits for `stream` before returning. */
 int fs_state_norms(int64_t nn, const double* d_R, const double* d_dQ, double* d_out, void* stream);
+/* No counterpart in the reference.  fs_state_norms for callers that cannot
+ * allocate device memory (code under HIP stream capture, serving loops that
+ * must not call hipMalloc): same arguments and the same result bits, but the
+ * chunk partials live in a fixed per-device buffer of the library (1 MiB,
+ * held until the process ends), so nn is at most 67,108,864 (FS_ERR_ARG
+ * beyond); nothing is allocated and the call only enqueues work on `stream`.
+ * Every call on one device shares that buffer: calls that overlap on one
+ * device (on different streams or threads) race. */
+int fs_state_norms_noalloc(int64_t nn, const double* d_R, const double* d_dQ, double* d_out, void* stream);
 
 /* grid_io.C:1390 Compute_Body_Forces: forces[0] (lift) and forces[1] (drag)
  * integrated over the walls (odd bc <= 10) from the resident state after
