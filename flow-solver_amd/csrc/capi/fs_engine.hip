@@ -89,7 +89,7 @@ size_t footprint_bytes(const fs::Mesh& g, int prec) {
   b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + nbe * 8 * 22;
   if (prec != FS_PREC_MIXED) b += ninc * 4 * 8;   // per-incidence face fluxes of the FP64 pass
   if (prec == FS_PREC_MIXED)
-    b += nnz * 16 * 4 + nn * (16 * 4 + 4 * 4 * 2);
+    b += nnz * 16 * 4 + nn * (16 * 4 + 4 * 4 * 2) + neinc * (6 * 8 + 4 * 8 + 4);
   else
     b += nnz * 16 * 8 + nn * 16 * 8;
   b += (nn / fsk::kNormChunkNodes + 2) * (16 + 12);
@@ -220,6 +220,9 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   std::vector<int> nsub_off(NN + 1), nedge_off(NN + 1), nbe_off(NN + 1), st_off(NN + 1);
   std::vector<int> sub_inc(2 * NS), inc_other(EINC), inc_sother(KINC), nbe_list(BINC), st_nodes(NST);
   std::vector<double> inc_n(4 * KINC), inc_fc(2 * KINC), inc_xg(6 * KINC), inc_gw(2 * EINC), st_Wt(5 * NST);
+  const bool mixed = prec == FS_PREC_MIXED;
+  std::vector<double> einc_geo(mixed ? 6 * EINC : 0), einc_fc(mixed ? 4 * EINC : 0);
+  std::vector<int> einc_slot(mixed ? EINC : 0);
   std::vector<int> breal(NB), bbc(NB);
   std::vector<double> bn(3 * NB), bxg(6 * NB), bgn(6 * NB);
   long long kinc = 0, einc = 0, binc = 0, sto = 0;
@@ -277,6 +280,22 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       inc_other[K] = other | (isR ? (int)0x80000000u : 0);
       inc_gw[2 * K] = g.edge_weight[4 * (size_t)e + (isR ? 2 : 0)];
       inc_gw[2 * K + 1] = g.edge_weight[4 * (size_t)e + (isR ? 3 : 1)];
+      if (mixed) {
+        // the edge's subedges: normal and length, face point (reconstruct.C:181-186)
+        for (int t = 0, j = g.esub_off[e]; t < 2; ++t, ++j) {
+          double* geo = &einc_geo[6 * K + 3 * t];
+          double* fc = &einc_fc[4 * K + 2 * t];
+          if (j < g.esub_off[e + 1]) {
+            const int sb = g.esub_list[j], el = g.sub_elem[sb];
+            for (int c = 0; c < 3; ++c) geo[c] = g.sub_n[3 * (size_t)sb + c];
+            fc[0] = (g.el_cent[2 * (size_t)el] + g.sub_xm[2 * (size_t)sb]) * 0.5;
+            fc[1] = (g.el_cent[2 * (size_t)el + 1] + g.sub_xm[2 * (size_t)sb + 1]) * 0.5;
+          } else {
+            geo[0] = geo[1] = geo[2] = 0.0;
+            fc[0] = fc[1] = 0.0;
+          }
+        }
+      }
     }
     for (size_t k = 0; k < g.nbe_list.size(); ++k) nbe_list[(size_t)binc + k] = g.nbe_list[k] + bo;
     for (int b = 0; b < g.nbe; ++b) {
@@ -386,6 +405,16 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       edge_slot[2 * (size_t)(I.eoff + e)] = I.slot_dev[g.edge_slot_lr[e]];
       edge_slot[2 * (size_t)(I.eoff + e) + 1] = I.slot_dev[g.edge_slot_rl[e]];
     }
+    if (mixed) {
+      // block (this node, other node) of each edge incidence: (L,R) seen from
+      // L, (R,L) seen from R
+      const size_t e0 = (size_t)nedge_off[I.noff];
+      for (size_t k = 0; k < g.nedge_list.size(); ++k) {
+        const int ent = g.nedge_list[k];
+        const int e = ent & 0x7fffffff;
+        einc_slot[e0 + k] = edge_slot[2 * (size_t)(I.eoff + e) + (ent < 0 ? 1 : 0)];
+      }
+    }
   }
   roff[NN] = (int)tslot.size();
   d.nnz = (int)NSLOT;
@@ -413,6 +442,8 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = upload(&d.inc_xg, inc_xg)) || (rc = upload(&d.inc_other, inc_other)) ||
       (rc = upload(&d.inc_sother, inc_sother)) ||
       (rc = upload(&d.inc_gw, inc_gw)) || (rc = upload(&d.nbe_off, nbe_off)) ||
+      (rc = upload(&d.einc_geo, einc_geo)) || (rc = upload(&d.einc_fc, einc_fc)) ||
+      (rc = upload(&d.einc_slot, einc_slot)) ||
       (rc = upload(&d.nbe_list, nbe_list)) || (rc = upload(&d.bedge_real, breal)) ||
       (rc = upload(&d.bedge_bc, bbc)) || (rc = upload(&d.bedge_n, bn)) || (rc = upload(&d.bedge_xg, bxg)) ||
       (rc = upload(&d.bedge_gn, bgn)) || (rc = upload(&d.roff, roff)) ||
@@ -568,14 +599,13 @@ int Engine::newton_iteration(int it, int key, bool record) {
   fsk::launch_limiter(dm, s, st);
   fsk::ColorRanges cr{color_off.data(), color_nback.data()};
   if (precision == FS_PREC_MIXED) {
-    // FP32 analytic Jacobian; the FP64 residual is evaluated inside the node
-    // pass (with ev_con/dt and the diagonal blocks); FP32 sweeps
+    // boundary fluxes and blocks, then one node pass for the FP64 residual,
+    // ev_con/dt and every FP32 Jacobian block; FP32 sweeps
     if (record) FSE_HIP(hipEventRecord(ev_[0], st));
-    fsk::launch_edge_jac32(dm, s, st);
-    if (record) FSE_HIP(hipEventRecord(ev_[1], st));
     fsk::launch_bedge(dm, s, st);
+    if (record) FSE_HIP(hipEventRecord(ev_[1], st));
     if (record) FSE_HIP(hipEventRecord(ev_[2], st));
-    fsk::launch_node_mixed(dm, s, st);
+    fsk::launch_resid_jac32(dm, s, st);
     if (record) FSE_HIP(hipEventRecord(ev_[3], st));
     fsk::launch_diag_lu32(dm, s, st);
     if (record) FSE_HIP(hipEventRecord(ev_[5], st));
@@ -845,25 +875,24 @@ int Engine::autotune(double* seconds) {
   for (int b : blocks) {
     tu.node_block = b;
     if (precision == FS_PREC_MIXED)
-      rc = time_it([&] { fsk::launch_node_mixed(dm, s, st); }, 2, &t);
+      rc = time_it([&] { fsk::launch_resid_jac32(dm, s, st); }, 2, &t);
     else
       rc = time_it([&] { fsk::launch_node(dm, s, nflags, st); }, 2, &t);
     if (rc) return rc;
     if (t < best) { best = t; keepb = b; }
   }
   tu.node_block = keepb;
-  keepb = tu.face_block;
-  best = 1e30f;
-  for (int b : blocks) {
-    tu.face_block = b;
-    if (precision == FS_PREC_MIXED)
-      rc = time_it([&] { fsk::launch_edge_jac32(dm, s, st); }, 2, &t);
-    else
+  if (precision != FS_PREC_MIXED) {
+    keepb = tu.face_block;
+    best = 1e30f;
+    for (int b : blocks) {
+      tu.face_block = b;
       rc = time_it([&] { fsk::launch_edge_flux(dm, s, st); }, 2, &t);
-    if (rc) return rc;
-    if (t < best) { best = t; keepb = b; }
+      if (rc) return rc;
+      if (t < best) { best = t; keepb = b; }
+    }
+    tu.face_block = keepb;
   }
-  tu.face_block = keepb;
   keepb = tu.cell_block;
   best = 1e30f;
   for (int b : blocks) {

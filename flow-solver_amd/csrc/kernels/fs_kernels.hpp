@@ -78,6 +78,13 @@ struct DevMesh {
   const int* inc_sother;    // per subedge incidence: other node | (1<<31 if this node is R)
   const int* inc_other;     // per edge incidence: other node | (1<<31 if this node is R)
   const double* inc_gw;     // 2 per edge incidence: LSQ weights of this side
+  // Mixed path only (null otherwise), per edge incidence: the edge's two
+  // subedges' (nx, ny, len) in the edge's L->R orientation (len = 0 where a
+  // boundary edge has one), their face points, and the block slot of
+  // (this node, other node).
+  const double* einc_geo;   // 6 per edge incidence
+  const double* einc_fc;    // 4 per edge incidence
+  const int* einc_slot;     // 1 per edge incidence
   const int* nbe_off;       // nn+1
   const int* nbe_list;
   const int* bedge_real;    // nbe
@@ -192,8 +199,9 @@ void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipStream_t st);
 // mixed precision
-void launch_edge_jac32(const DevMesh& m, const DevState& s, hipStream_t st);
-void launch_node_mixed(const DevMesh& m, const DevState& s, hipStream_t st);
+// FP64 residual, ev_con/dt, FP32 analytic Jacobian blocks (off-diagonal and
+// diagonal) and right-hand sides in one node pass
+void launch_resid_jac32(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st);
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st);

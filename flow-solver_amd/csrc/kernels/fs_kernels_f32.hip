@@ -1,16 +1,16 @@
 // Kernels of the mixed-precision path (gfx950, wave64).
 //
-// The outer loop stays FP64: the node kernel k_node_mixed evaluates the FP64
-// residual (characteristic-wave Roe flux at the FP64 face states), ev_con and
-// dt; the limiter, gradient, derivatives, norms, convergence test and state
-// update are the FP64 kernels.  Inside it the Jacobian blocks and the
-// Gauss-Seidel sweeps run in FP32: the analytic Roe Jacobian of
-// compute_approximate_jacobian_roe (jacobian.C:301-658) at the same
-// reconstructed face states the reference differentiates around, FP32 block
-// storage (64 B per 4x4 block), diagonal blocks inverted in FP64 with
-// pivoting and stored FP32, and FP32 sweeps.  The fixed point is the FP64
-// residual's, so converged states agree with the FP64 path to the FP64
-// stopping tolerance.
+// The outer loop stays FP64: the node pass k_resid_jac32 evaluates the FP64
+// residual (Roe flux with the dissipation in closed form, at the FP64 face
+// states), ev_con and dt; the limiter, gradient, derivatives, norms,
+// convergence test and state update are the FP64 kernels.  Inside it the
+// Jacobian blocks and the Gauss-Seidel sweeps run in FP32: the analytic Roe
+// Jacobian of compute_approximate_jacobian_roe (jacobian.C:301-658) at the
+// same reconstructed face states the reference differentiates around, built
+// in the same node pass from the flux's Roe quantities, FP32 block storage
+// (64 B per 4x4 block), diagonal blocks inverted in FP64 with pivoting and
+// stored FP32, and FP32 sweeps.  The fixed point is the FP64 residual's, so
+// converged states agree with the FP64 path to the FP64 stopping tolerance.
 //
 // Contraction (FMA) is on in this file, for the mixed path's FP64 residual
 // too: unlike the FP64 path's kernels it is not bound to the reference's
@@ -30,55 +30,31 @@ constexpr int kBlock = 256;
 inline int nblocks(int n, int b = kBlock) { return (n + b - 1) / b; }
 }  // namespace
 
-// Roe matrix T|Lambda|Tinv in FP32 with shared reciprocals
-__device__ __forceinline__ void roe_matrix_f32(float nx, float ny, float g, const float* qL, const float* qR,
-                                               float* Rm) {
-  float irL = 1.0f / qL[0], irR = 1.0f / qR[0];
-  float uL = qL[1] * irL, vL = qL[2] * irL, uR = qR[1] * irR, vR = qR[2] * irR;
-  float gm1 = g - 1.0f;
-  float pL = gm1 * (qL[3] - 0.5f * qL[0] * (uL * uL + vL * vL));
-  float pR = gm1 * (qR[3] - 0.5f * qR[0] * (uR * uR + vR * vR));
-  float hL = (qL[3] + pL) * irL, hR = (qR[3] + pR) * irR;
-  float rb = sqrtf(qL[0] * qR[0]);
-  float bg = rb / (qL[0] + rb);
-  float ub = uL + bg * (uR - uL), vb = vL + bg * (vR - vL), hb = hL + bg * (hR - hL);
-  float th = ub * nx + vb * ny;
-  float V2 = ub * ub + vb * vb;
-  float c = sqrtf(gm1 * (hb - 0.5f * V2));
-  float ic = 1.0f / c, ic2 = ic * ic;
-  float lam[4] = {fabsf(th), fabsf(th), fabsf(th + c), fabsf(th - c)};
-  float T[16], Ti[16];
-  T[0] = 1.0f; T[1] = 0.0f; T[2] = ic2; T[3] = ic2;
-  T[4] = ub; T[5] = -ny; T[6] = ub * ic2 + nx * ic; T[7] = ub * ic2 - nx * ic;
-  T[8] = vb; T[9] = nx; T[10] = vb * ic2 + ny * ic; T[11] = vb * ic2 - ny * ic;
-  T[12] = 0.5f * V2; T[13] = -ub * ny + vb * nx;
-  float igm1 = 1.0f / gm1;
-  T[14] = igm1 + th * ic + 0.5f * V2 * ic2;
-  T[15] = igm1 - th * ic + 0.5f * V2 * ic2;
-  Ti[0] = 1.0f - gm1 * 0.5f * V2 * ic2; Ti[1] = gm1 * ub * ic2; Ti[2] = gm1 * vb * ic2; Ti[3] = -gm1 * ic2;
-  Ti[4] = ub * ny - vb * nx; Ti[5] = -ny; Ti[6] = nx; Ti[7] = 0.0f;
-  Ti[8] = 0.5f * (-c * th + gm1 * 0.5f * V2); Ti[9] = 0.5f * (c * nx - gm1 * ub);
-  Ti[10] = 0.5f * (c * ny - gm1 * vb); Ti[11] = 0.5f * gm1;
-  Ti[12] = 0.5f * (c * th + gm1 * 0.5f * V2); Ti[13] = -0.5f * (c * nx + gm1 * ub);
-  Ti[14] = -0.5f * (c * ny + gm1 * vb); Ti[15] = 0.5f * gm1;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      float a = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) a += T[i * 4 + k] * (lam[k] * Ti[k * 4 + j]);
-      Rm[i * 4 + j] = a;
-    }
+// Exchange within a lane quad through DPP quad permutes (no LDS round trip):
+// xor 1 swaps lanes (0,1) and (2,3), xor 2 swaps (0,2) and (1,3).  Every lane
+// of the quad must be active.
+template <int kXor>
+__device__ __forceinline__ int quad_xor_i(int x) {
+  constexpr int ctrl = kXor == 1 ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
+  return __builtin_amdgcn_update_dpp(0, x, ctrl, 0xF, 0xF, false);
+}
+template <int kXor>
+__device__ __forceinline__ float quad_xor(float x) {
+  return __int_as_float(quad_xor_i<kXor>(__float_as_int(x)));
+}
+template <int kXor>
+__device__ __forceinline__ double quad_xor(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = quad_xor_i<kXor>((int)(b & 0xffffffffLL)), hi = quad_xor_i<kXor>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// Euler flux Jacobian dF.n/dq (jacobian.C:561-582)
-__device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, const float* q, float* d) {
-  float ir = 1.0f / q[0];
-  float u = q[1] * ir, v = q[2] * ir, E = q[3];
-  float th = u * nx + v * ny;
-  float ph = 0.5f * (g - 1.0f) * (u * u + v * v);
-  float gE = g * E * ir;
+// Euler flux Jacobian dF.n/dq (jacobian.C:561-582) in FP32 from the state's
+// velocity and total energy per mass (e = E/rho)
+__device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, float u, float v, float e, float* d) {
+  const float th = u * nx + v * ny;
+  const float ph = 0.5f * (g - 1.0f) * (u * u + v * v);
+  const float gE = g * e;
   d[0] = 0.0f; d[1] = nx; d[2] = ny; d[3] = 0.0f;
   d[4] = ph * nx - u * th; d[5] = (2.0f - g) * u * nx + th; d[6] = u * ny - (g - 1.0f) * v * nx; d[7] = (g - 1.0f) * nx;
   d[8] = ph * ny - v * th; d[9] = v * nx - (g - 1.0f) * u * ny; d[10] = th + (2.0f - g) * v * ny; d[11] = (g - 1.0f) * ny;
@@ -86,185 +62,194 @@ __device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, c
   d[14] = (1.0f - g) * v * th + ny * (gE - ph); d[15] = g * th;
 }
 
-// Edge-centric FP32 analytic Jacobian: for each of the edge's subedges,
-// dF/dqL = 0.5(A(qL) + |A_roe|) len and dF/dqR = 0.5(A(qR) - |A_roe|) len at
-// the (limited) reconstructed face states; block (L,R) += dF/dqR and block
-// (R,L) -= dF/dqL (jacobian.C:612-658).  Zeros when jupdate is off.
-__global__ void __launch_bounds__(kMaxBlock) k_edge_jac32(DevMesh m, DevState st) {
-  int e = xcd_block() * blockDim.x + threadIdx.x;
-  if (e >= m.ne) return;
-  const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
-  if (!ip.active) return;
-  const int recon = ip.recon_flux, jupdate = ip.jac;
-  const float gamma = (float)ip.gamma;
-  const double* __restrict__ nQ = st.nQ;
-  const double* __restrict__ grad = st.grad;
-  const double* __restrict__ phi = st.phi;
-  float* __restrict__ LHS32 = st.LHS32;
-  float offLR[16], offRL[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) { offLR[k] = 0.0f; offRL[k] = 0.0f; }
-  if (jupdate) {
-    int L = m.edges[2 * e], R = m.edges[2 * e + 1];
-    float qL0[4], qR0[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      qL0[j] = (float)nQ[4 * (size_t)L + j];
-      qR0[j] = (float)nQ[4 * (size_t)R + j];
-    }
-    float gL[8], gR[8], phL = 0.f, phR = 0.f, xL0 = 0.f, xL1 = 0.f, xR0 = 0.f, xR1 = 0.f;
-    if (recon == 1) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) { gL[k] = (float)grad[8 * (size_t)L + k]; gR[k] = (float)grad[8 * (size_t)R + k]; }
-      phL = (float)phi[L];
-      phR = (float)phi[R];
-      xL0 = (float)m.x[2 * (size_t)L]; xL1 = (float)m.x[2 * (size_t)L + 1];
-      xR0 = (float)m.x[2 * (size_t)R]; xR1 = (float)m.x[2 * (size_t)R + 1];
-    }
-    for (int t = 0; t < 2; ++t) {
-      int s = m.edge_sub[2 * e + t];
-      if (s < 0) break;
-      float nx = (float)m.sub_n[3 * (size_t)s], ny = (float)m.sub_n[3 * (size_t)s + 1];
-      float len = (float)m.sub_n[3 * (size_t)s + 2];
-      float ql[4], qr[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) { ql[j] = qL0[j]; qr[j] = qR0[j]; }
-      if (recon == 1) {
-        size_t ki = (size_t)m.sub_inc[2 * (size_t)s];
-        float fc0 = (float)m.inc_fc[2 * ki], fc1 = (float)m.inc_fc[2 * ki + 1];
-        float dl0 = fc0 - xL0, dl1 = fc1 - xL1, dr0 = fc0 - xR0, dr1 = fc1 - xR1;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          ql[j] += phL * (gL[2 * j] * dl0 + gL[2 * j + 1] * dl1);
-          qr[j] += phR * (gR[2 * j] * dr0 + gR[2 * j + 1] * dr1);
-        }
-      }
-      float Rm[16], A[16];
-      roe_matrix_f32(nx, ny, gamma, ql, qr, Rm);
-      float hl = 0.5f * len;
-      flux_jacobian_f32(nx, ny, gamma, ql, A);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) offRL[k] -= hl * (A[k] + Rm[k]);
-      flux_jacobian_f32(nx, ny, gamma, qr, A);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) offLR[k] += hl * (A[k] - Rm[k]);
-    }
-  }
-  float4* blr = reinterpret_cast<float4*>(LHS32 + 16 * (size_t)m.edge_slot[2 * e]);
-  float4* brl = reinterpret_cast<float4*>(LHS32 + 16 * (size_t)m.edge_slot[2 * e + 1]);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    blr[k] = make_float4(offLR[4 * k], offLR[4 * k + 1], offLR[4 * k + 2], offLR[4 * k + 3]);
-    brl[k] = make_float4(offRL[4 * k], offRL[4 * k + 1], offRL[4 * k + 2], offRL[4 * k + 3]);
-  }
-}
-
-// Node pass of the mixed step: FP64 ev_con/dt (same order as the FP64 path),
-// the FP64 interior residual, FP32 diagonal block V/dt I - sum of transposed
-// off-diagonals + boundary blocks, FP32 right-hand side -R.
-//
-// The residual is accumulated here rather than through a per-subedge flux
-// buffer: lane v of the node's 4 evaluates the characteristic-wave Roe flux
-// of the node's subedge incidences v, v+4, ... (the flux of the subedge with
-// the reference's L/R orientation and reconstruction, +f where the node is
-// L and -f where it is R), then the 4 partial sums are combined by a fixed
-// butterfly.  Each subedge is evaluated at both of its nodes (twice the
-// flux arithmetic, none of the buffer's 64 bytes of writes and reads per
-// subedge); the sum order depends only on the node's own incidence list, so
-// an instance's residual is the same alone and in a batch.
-__global__ void __launch_bounds__(kMaxBlock) k_node_mixed(DevMesh m, DevState s) {
-  int tid = xcd_block() * blockDim.x + threadIdx.x;
-  int n = tid >> 2, v = tid & 3;
+// Residual and Jacobian pass of the mixed step, one pass per node (4 lanes
+// per node), nothing re-read from an earlier pass but the boundary edges'
+// fluxes and blocks:
+//   * FP64 residual R = sum over the node's faces of the flux out of its
+//     control volume (residual.C:112-333 scatter, gathered): each end node
+//     evaluates the subedge's flux with its own state on the left and its
+//     outward normal, which is the reference's f and -f up to rounding (the
+//     two ends' fluxes cancel to a few ulps, not bit for bit); the sum order
+//     is a fixed function of the node's own edge list (alone and in a batch
+//     alike);
+//   * ev_con (cv_calc.C:790-965) and dt = cfl V / ev_con (init_memory.C:227);
+//   * FP32 analytic Jacobian (compute_approximate_jacobian_roe,
+//     jacobian.C:301-658) at the same face states: with the outward normal
+//     n_o, A = dF.n_o/dq and M = |A_roe| (built in FP32 from the FP64 flux's
+//     Roe quantities; |A_roe| does not change with the normal's sign),
+//       diagonal block of n      += len/2 (A(q_n) + M)
+//       block (n, other)         += len/2 (A(q_other) - M)
+//     so the row's off-diagonal blocks are written here once and the diagonal
+//     needs no second read of them;  V/dt I + boundary blocks on the diagonal.
+// Lane v of the node's quad handles subedge (v & 1) of edge incidences
+// v >> 1, 2 + (v >> 1), ...; the two subedges of an edge meet in lanes v and
+// v ^ 1, which add their block contributions and write the edge's block.
+__global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s) {
+  const int tid = xcd_block() * blockDim.x + threadIdx.x;
+  const int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
   const int inst = inst_of(m, m.inst_node, n);
   const InstParams& p = s.ip[inst];
   if (!p.active) return;
-  const int jac = p.jac;
-  double q[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) q[j] = s.nQ[4 * (size_t)n + j];
-  double Rv = 0.0, ev = 0.0;
-  const int k0 = m.nsub_off[n], k1 = m.nsub_off[n + 1];
-#pragma unroll 4
-  for (int k = k0; k < k1; ++k) {
-    const double* nr = m.inc_n + 4 * (size_t)k;
-    ev += (max_ev(q, nr[0], nr[1], p.gamma) * nr[2]);
-  }
+  const int jac = p.jac, recon = p.recon_flux;
+  const double g = p.gamma, g1 = p.gamma - 1.0;
+  const float gf = (float)p.gamma, gm1 = (float)(p.gamma - 1.0);
+  const double* __restrict__ nQ = s.nQ;
+  const double* __restrict__ grad = s.grad;
+  PrimW Pn;
   {
-    const int recon = p.recon_flux;
-    const double* __restrict__ grad = s.grad;
-    double gs[8], phis = 0.0, xs0 = 0.0, xs1 = 0.0;
-    if (recon == 1) {
+    double qn[4];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) gs[c] = grad[8 * (size_t)n + c];
-      phis = s.phi[n];
-      xs0 = m.x[2 * (size_t)n];
-      xs1 = m.x[2 * (size_t)n + 1];
+    for (int j = 0; j < 4; ++j) qn[j] = nQ[4 * (size_t)n + j];
+    Pn = prim_w(qn, g1);
+  }
+  const double cn = sqrt((g * Pn.p) * Pn.ir);   // the node's own sound speed (cv_calc.C:819-849)
+  double acc[4] = {0.0, 0.0, 0.0, 0.0}, ev = 0.0;
+  float dg[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) dg[c] = 0.0f;
+  const int sub = v & 1;
+  const int k0 = m.nedge_off[n], k1 = m.nedge_off[n + 1];
+  // the next face's index, geometry and neighbour state are loaded one
+  // iteration ahead, so their latency overlaps the current face's arithmetic
+  int k = k0 + (v >> 1);
+  int ent_n = 0;
+  double geo_n[3] = {0.0, 0.0, 0.0}, qb_n[4] = {1.0, 0.0, 0.0, 1.0};
+  if (k < k1) {
+    ent_n = m.inc_other[k];
+    const double* geo = m.einc_geo + 6 * (size_t)k + 3 * sub;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) geo_n[c] = geo[c];
+    const size_t on = (size_t)(ent_n & 0x7fffffff);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) qb_n[j] = nQ[4 * on + j];
+  }
+  for (; k < k1; k += 2) {
+    const int ent = ent_n;
+    const int o = ent & 0x7fffffff;
+    const bool isR = ent < 0;
+    const double nx = geo_n[0], ny = geo_n[1], len = geo_n[2];
+    double qb[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) qb[j] = qb_n[j];
+    if (k + 2 < k1) {
+      ent_n = m.inc_other[k + 2];
+      const double* geo = m.einc_geo + 6 * (size_t)(k + 2) + 3 * sub;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) geo_n[c] = geo[c];
+      const size_t on = (size_t)(ent_n & 0x7fffffff);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) qb_n[j] = nQ[4 * on + j];
     }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = k0 + v; k < k1; k += 4) {
-      const int ent = m.inc_sother[k];
-      const int o = ent & 0x7fffffff;
-      const bool isR = ent < 0;
-      const double* nr = m.inc_n + 4 * (size_t)k;
-      double qs[4], qo[4];
+    PrimW Pb = prim_w(qb, g1);
+    // this lane's contribution to block (n, other)
+    float off[16];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        qs[j] = q[j];
-        qo[j] = s.nQ[4 * (size_t)o + j];
-      }
+    for (int c = 0; c < 16; ++c) off[c] = 0.0f;
+    if (len != 0.0) {
+      double qa[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) qa[j] = nQ[4 * (size_t)n + j];
+      PrimW Pa = Pn;
       if (recon == 1) {
         // reconstruct.C:181-205 at the subedge's face point, for both ends
-        const double fc0 = m.inc_fc[2 * (size_t)k], fc1 = m.inc_fc[2 * (size_t)k + 1];
-        const double dxs0 = fc0 - xs0, dxs1 = fc1 - xs1;
-        const double dxo0 = fc0 - m.x[2 * (size_t)o], dxo1 = fc1 - m.x[2 * (size_t)o + 1];
-        const double phio = s.phi[o];
+        const double* fc = m.einc_fc + 4 * (size_t)k + 2 * sub;
+        const double phia = s.phi[n], phib = s.phi[o];
+        const double dxa0 = fc[0] - m.x[2 * (size_t)n], dxa1 = fc[1] - m.x[2 * (size_t)n + 1];
+        const double dxb0 = fc[0] - m.x[2 * (size_t)o], dxb1 = fc[1] - m.x[2 * (size_t)o + 1];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          qs[j] += (phis * gs[2 * j] * dxs0 + phis * gs[2 * j + 1] * dxs1);
-          qo[j] += (phio * grad[8 * (size_t)o + 2 * j] * dxo0 + phio * grad[8 * (size_t)o + 2 * j + 1] * dxo1);
+          qa[j] += phia * grad[8 * (size_t)n + 2 * j] * dxa0 + phia * grad[8 * (size_t)n + 2 * j + 1] * dxa1;
+          qb[j] += phib * grad[8 * (size_t)o + 2 * j] * dxb0 + phib * grad[8 * (size_t)o + 2 * j + 1] * dxb1;
         }
+        Pa = prim_w(qa, g1);
+        Pb = prim_w(qb, g1);
       }
-      double ql[4], qr[4], f[4];
+      // eigenvalue bound of the node's own state on this face
+      ev += (fabs(Pn.u * nx + Pn.v * ny) + cn) * len;
+      // the flux out of the node's control volume through this subedge:
+      // Roe_flux_centered with the node's face state on the left and the
+      // outward normal (the subedge normal, reversed where the node is the
+      // subedge's R): R[L] += f and R[R] -= f of the reference's scatter
+      const double ox = isR ? -nx : nx, oy = isR ? -ny : ny;
+      double f[4];
+      RoeWave w;
+      roe_flux_wave_prim(ox, oy, len, g1, qa, qb, Pa, Pb, f, &w);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        ql[j] = isR ? qo[j] : qs[j];
-        qr[j] = isR ? qs[j] : qo[j];
+      for (int j = 0; j < 4; ++j) acc[j] += f[j];
+      if (jac) {
+        // len/2 M = len/2 |A_roe| in FP32 from the face's Roe quantities
+        const float hl = (float)(0.5 * len);
+        const float fnx = (float)ox, fny = (float)oy;
+        const float ub = (float)w.ub, vb = (float)w.vb, hb = (float)w.hb, th = (float)w.th, hV2 = (float)w.hV2;
+        const float hat = hl * (float)w.at, k1 = hl * (float)w.k1, k2 = hl * (float)w.k2, k4 = hl * (float)w.k4;
+        const float P[4] = {gm1 * hV2, -gm1 * ub, -gm1 * vb, gm1};
+        const float U[4] = {-th, fnx, fny, 0.0f};
+        const float ra[4] = {1.0f, ub, vb, hb}, rv[4] = {0.0f, fnx, fny, th};
+        float w1[4], w2[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          w1[j] = k1 * P[j] + k2 * U[j];
+          w2[j] = k2 * P[j] + k4 * U[j];
+        }
+        float hM[16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) hM[4 * i + j] = (i == j ? hat : 0.0f) + ra[i] * w1[j] + rv[i] * w2[j];
+        float A[16];
+        flux_jacobian_f32(fnx, fny, gf, (float)Pa.u, (float)Pa.v, (float)(qa[3] * Pa.ir), A);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) dg[c] += hl * A[c] + hM[c];
+        flux_jacobian_f32(fnx, fny, gf, (float)Pb.u, (float)Pb.v, (float)(qb[3] * Pb.ir), A);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) off[c] = hl * A[c] - hM[c];
       }
-      roe_flux_wave(nr[0], nr[1], nr[2], p.gamma, ql, qr, f);
-      // R[L] += f, R[R] -= f; adding -f is the same operation as subtracting f
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[j] += isR ? -f[j] : f[j];
     }
+    // block (n, other) from the edge's two subedges in lanes v and v ^ 1;
+    // lane v writes rows 2 (v & 1) and 2 (v & 1) + 1
 #pragma unroll
-    for (int off = 1; off <= 2; off <<= 1)
+    for (int c = 0; c < 16; ++c) off[c] += quad_xor<1>(off[c]);
+    float row[8];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[j] += __shfl_xor(acc[j], off, 4);
-    Rv = v == 0 ? acc[0] : v == 1 ? acc[1] : v == 2 ? acc[2] : acc[3];
+    for (int j = 0; j < 8; ++j) row[j] = sub ? off[8 + j] : off[j];
+    float4* blk = reinterpret_cast<float4*>(s.LHS32 + 16 * (size_t)m.einc_slot[k]) + 2 * sub;
+    blk[0] = make_float4(row[0], row[1], row[2], row[3]);
+    blk[1] = make_float4(row[4], row[5], row[6], row[7]);
   }
+  // node sums: fixed butterfly over the quad
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] += quad_xor<1>(acc[j]);
+  ev += quad_xor<1>(ev);
+#pragma unroll
+  for (int c = 0; c < 16; ++c) dg[c] += quad_xor<1>(dg[c]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] += quad_xor<2>(acc[j]);
+  ev += quad_xor<2>(ev);
+#pragma unroll
+  for (int c = 0; c < 16; ++c) dg[c] += quad_xor<2>(dg[c]);
+  double Rv = v == 0 ? acc[0] : v == 1 ? acc[1] : v == 2 ? acc[2] : acc[3];
   for (int k = m.nbe_off[n]; k < m.nbe_off[n + 1]; ++k) {
-    int b = m.nbe_list[k];
-    double nx = m.bedge_n[3 * (size_t)b], ny = m.bedge_n[3 * (size_t)b + 1], len = m.bedge_n[3 * (size_t)b + 2];
-    ev += (max_ev(q, nx, ny, p.gamma) * len);
+    const int b = m.nbe_list[k];
+    const double bx = m.bedge_n[3 * (size_t)b], by = m.bedge_n[3 * (size_t)b + 1], bl = m.bedge_n[3 * (size_t)b + 2];
+    ev += (fabs(Pn.u * bx + Pn.v * by) + cn) * bl;
     Rv += s.bflux[4 * (size_t)b + v];
   }
-  double V = m.cv_area[n];
-  double dtn = p.cfl * (V / ev);
+  const double V = m.cv_area[n];
+  const double dtn = p.cfl * (V / ev);
   if (v == 0) {
     if (!(dtn > 0.0) || !isfinite(dtn) || !isfinite(ev)) atomicOr(s.err + inst, 2);
     s.ev_con[n] = ev;
     s.dt[n] = dtn;
   }
-  double vd = V / dtn;
+  const double vd = V / dtn;
   double D[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) D[c] = (c == v) ? vd : 0.0;
   if (jac) {
-#pragma unroll 4
-    for (int q = m.roff[n]; q < m.roff[n + 1]; ++q) {
-      float4 B = *reinterpret_cast<const float4*>(s.LHS32 + 16 * (size_t)m.tslot[q] + 4 * v);
-      D[0] -= B.x; D[1] -= B.y; D[2] -= B.z; D[3] -= B.w;
-    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      D[c] += (double)(v == 0 ? dg[c] : v == 1 ? dg[4 + c] : v == 2 ? dg[8 + c] : dg[12 + c]);
     for (int k = m.nbe_off[n]; k < m.nbe_off[n + 1]; ++k) {
       const double* B = s.bjac + 16 * (size_t)m.nbe_list[k] + 4 * v;
 #pragma unroll
@@ -401,13 +386,9 @@ __global__ void __launch_bounds__(kBlock) k_update_mixed(DevMesh m, const InstPa
   nQ[i] += d;
 }
 
-void launch_edge_jac32(const DevMesh& m, const DevState& s, hipStream_t st) {
-  if (m.ne == 0) return;
-  hipLaunchKernelGGL(k_edge_jac32, dim3(nblocks(m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
-}
-void launch_node_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {
+void launch_resid_jac32(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_node_mixed, dim3(nblocks(4 * m.nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st, m, s);
+  hipLaunchKernelGGL(k_resid_jac32, dim3(nblocks(4 * m.nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st, m, s);
 }
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;

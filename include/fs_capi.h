@@ -164,9 +164,10 @@ void fs_mesh_destroy(fs_mesh* m);
 /* ---- solver on one GPU ----
  * precision FS_PREC_F64: everything in FP64, the reference's algorithms
  * (finite-difference Jacobian, Gauss-Seidel sweeps) on multicolor order.
- * precision FS_PREC_MIXED: FP32 flux, reconstruction, limiter, analytic
- * Jacobian (jacobian.C:301-658) and sweeps inside an FP64 defect-correction
- * loop (FP64 residual, norms and update). */
+ * precision FS_PREC_MIXED: the analytic Jacobian (jacobian.C:301-658) and
+ * the Gauss-Seidel sweeps in FP32 inside an FP64 defect-correction loop: the
+ * residual (flux, reconstruction, limiter), the norms and the update stay
+ * FP64, so the converged state is the FP64 residual's. */
 enum { FS_PREC_F64 = 0, FS_PREC_MIXED = 1 };
 int fs_solver_create(const fs_mesh* m, const fs_params* p, int precision, fs_solver** out);
 void fs_solver_destroy(fs_solver* s);
@@ -258,9 +259,9 @@ int fs_solve_steady(fs_solver* s, int max_steps, double tol, int* steps, double*
 typedef struct {
   double step_ms;        /* mean wall time per pseudo-time step */
   double edge_ms;        /* mean device time of the face kernel: flux + FD Jacobian
-                            (FP64), the FP32 analytic Jacobian (mixed) */
-  double node_ms;        /* mean device time of the node kernel: assembly (FP64),
-                            FP64 residual + assembly (mixed) */
+                            (FP64); the boundary-edge kernel (mixed) */
+  double node_ms;        /* mean device time of the node kernel: assembly (FP64);
+                            FP64 residual + FP32 Jacobian blocks + assembly (mixed) */
   double gs_ms;          /* mean device time of the Gauss-Seidel sweep launches */
   double lu_ms;          /* mean device time of the diagonal-block factorisation */
   double other_ms;
