@@ -1,27 +1,34 @@
 """Benchmark: edge-flux evaluations per second of the residual-and-implicit-
-update path on one MI355X.
+update path on one MI355X, mixed precision against the plain FP64 path, and
+steady-solve latency of the serving loop.
 
-Workload (default): bump channel 1025x257 nodes (524,288 triangles, seeded
-+-0.2 jitter), M=0.675, k=2 (order 3) with the Barth limiter, FP64, the
-reference's implicit pseudo-time step (finite-difference Jacobian, 20
-symmetric Gauss-Seidel sweeps).  Warmup steps walk through the first-order and
-linear phases into the k=2 phase; the timed steps are k=2-phase steps.
+Headline workload: 256 independent steady solves packed into one device
+engine (bump channels and airfoil O-meshes, 5k-200k triangles log-uniform,
+M ~ U[0.3, 0.8], alpha ~ U[-2, 4] deg, k in {1, 2}; seeded), every kernel of a
+pseudo-time step one launch over all instances, mixed precision (FP64
+residual, norms and update; FP32 analytic Jacobian and Gauss-Seidel sweeps).
+Warm-up steps walk every instance into its final phase (linear reconstruction
+for k=1, the k=2 phase for k=2); the timed steps are final-phase steps, timed
+with events on the engine's stream, which also give each kernel's time.
 
-One step = one pseudo-time step (one Newton iteration): residual + Jacobian +
-linear sweeps + update over the whole mesh.  Edge-flux evals per step =
-subedges + boundary edges (the residual's Roe evaluations; the 8 extra
-Jacobian evaluations per subedge are reported separately).
+One step = one pseudo-time step (one Newton iteration) of every instance:
+residual + Jacobian + 40 symmetric Gauss-Seidel sweeps + update.  Edge-flux
+evals per step = subedges + boundary edges of all instances (the residual's
+Roe evaluations).
 
 Also measured (extra objects of the same line):
-  * "mixed": the same workload and steps on the mixed-precision path;
-  * "batch": 256 independent solves packed into one launch per kernel
-    (bump channels and airfoil O-meshes, 5k-200k triangles log-uniform,
-    M ~ U[0.3,0.8], alpha ~ U[-2,4] deg, k in {1,2}; seeded), FP64 and mixed:
-    throughput over full steps, then every solve run to a 1e-10 drop under
-    per-instance convergence masks and the converged states compared;
+  * "fp64": the same batch and steps on the plain FP64 path (the reference's
+    finite-difference Jacobian and FP64 sweeps);
+  * "steady": every batch solve run to a 1e-10 drop under per-instance
+    convergence masks, both precisions, converged states compared;
+  * "plain_fp64_sequential": the same solves one at a time on the FP64 path;
+  * "config2": the 1025x257 bump channel at M=0.675, k=2, FP64 and mixed;
+  * "config3": the 600k-node transonic O-mesh, FP64 and mixed to 1e-10;
   * "serving": a seeded Poisson stream of steady-solve requests (10k-2M
-    triangles) through the C++ request loop, p50/p99 time-to-converged and
-    solves/s, mixed and FP64 on the same stream.
+    triangles) through the C++ request loop, p50/p99 time-to-converged with
+    unconverged requests counted as censored, mixed and FP64;
+  * CPU baselines: the oracle on the host cores (one batch solve per core),
+    the oracle on config 2 (1 core), and the reference build itself.
 
 Prints one JSON line.
 """
@@ -51,16 +58,21 @@ def algorithmic_bytes_edge_kernel(info, recon):
     return b
 
 
-def algorithmic_bytes_mixed_node(info):
-    """Compulsory HBM bytes of one launch of the mixed path's node pass: the
-    residual at the k=2 phase's nodal face states (per subedge 2 node ids and
-    the normal, 32 B; per node the state read and R written, 64 B), plus the
-    pass's own assembly: ev_con and dt (16 B), cv_area (8 B), FP64 RHS and
-    FP32 RHS (48 B) and the FP32 diagonal block (64 B) per node, both FP32
-    off-diagonal blocks of each edge read (128 B), and per boundary edge its
-    flux and Jacobian block (160 B)."""
-    return (info["nsub"] * 32 + info["nn"] * (64 + 16 + 8 + 48 + 64) + info["ne"] * 128
-            + info["nbe"] * 160)
+def algorithmic_bytes_resid_jac32(info, recon):
+    """Compulsory HBM bytes of one launch of the mixed path's node pass
+    k_resid_jac32 (FP64 residual, ev_con/dt, FP32 Jacobian blocks):
+    per node the state read (32 B), R, RHS, RHS32, ev_con and dt written
+    (32 + 32 + 16 + 8 + 8 B), cv_area read (8 B) and the FP32 diagonal block
+    written (64 B); per subedge its normal and length (24 B); per edge its two
+    node ids and two block slots (16 B) and its two FP32 off-diagonal blocks
+    written (128 B); per boundary edge its normal, flux and block read
+    (24 + 32 + 128 B).  With linear reconstruction (recon 1) also the
+    gradient, limiter value and coordinates per node (64 + 8 + 16 B) and the
+    face point per subedge (16 B)."""
+    b = info["nn"] * 200 + info["nsub"] * 24 + info["ne"] * 144 + info["nbe"] * 184
+    if recon == 1:
+        b += info["nn"] * 88 + info["nsub"] * 16
+    return b
 
 
 def gs_schedule(ncolor, subits):
@@ -98,10 +110,47 @@ def algorithmic_bytes_gauss_seidel(mesh, subits, word):
     return total, len(sched)
 
 
-def measured_traffic(kernel):
-    """HBM bytes per dispatch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary (profiles/pmc_traffic_bump1025x257.md gives the commands)."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_bump1025x257.json")
+def algorithmic_bytes_gs_batch(meshes, subits, word):
+    """The same for the sweeps of a packed batch, launch by launch as the
+    engine issues them: one launch per global color of the shared schedule
+    (over the largest color count), each over that color's rows of every
+    instance; a backward launch of color c skips the instances whose last
+    color is c (they repeat no turnaround), and an empty launch is not
+    issued.  Returns (total bytes per linear solve, launches)."""
+    import numpy as np
+    per = []
+    for m in meshes:
+        info = m.info()
+        deg = np.diff(m.array("ia")) - 1
+        co, rows = m.array("color_off"), m.array("color_rows")
+        nn = info["nn"]
+        cols = []
+        for c in range(info["ncolor"]):
+            r = rows[co[c]:co[c + 1]]
+            nr, nb = len(r), int(deg[r].sum())
+            cols.append(nb * (16 * word + 4) + nr * (16 * word + 12 * word + 12) + (nn - nr) * 4 * word)
+        per.append((info["ncolor"], cols))
+    ncmax = max(nc for nc, _ in per)
+    total, launches, last = 0, 0, -1
+    for it in range(subits):
+        for cc in range(ncmax):
+            c = cc if it % 2 == 0 else ncmax - 1 - cc
+            if last == c:
+                continue
+            last = c
+            bwd = it % 2 == 1
+            b = sum(cols[c] for nc, cols in per if c < nc and not (bwd and nc - 1 == c))
+            if b == 0:
+                continue
+            total += b
+            launches += 1
+    return total, launches
+
+
+def measured_traffic(kernel, name="pmc_traffic_bump1025x257.json"):
+    """HBM bytes per dispatch from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary under profiles/ (its .md twin gives the commands)."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             return json.load(f)["kernels"][kernel]["traffic_bytes"]
@@ -112,14 +161,15 @@ def measured_traffic(kernel):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of the headline batch")
+    ap.add_argument("--warmup", type=int, default=20, help="warm-up steps (at least the final-phase start)")
+    ap.add_argument("--config2-steps", type=int, default=20)
+    ap.add_argument("--config2-warmup", type=int, default=6)
     ap.add_argument("--nx", type=int, default=1025)
     ap.add_argument("--ny", type=int, default=257)
     ap.add_argument("--cpu-steps", type=int, default=2, help="k=2-phase steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch", type=int, default=256, help="instances of the batch section (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=256, help="instances of the headline batch")
     ap.add_argument("--batch-max-steps", type=int, default=1500, help="steady batch solve limit (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=30.0, help="length of the request stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.5, help="offered requests per second")
@@ -128,11 +178,11 @@ def parse():
     ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-solve-at-a-time FP64 baseline")
     ap.add_argument("--batch-plain-steady", action="store_true",
                     help="also run every batch solve alone to its stopping test in FP64 (about 90 s more)")
-    ap.add_argument("--batch-precisions", default="fp64,mixed")
     ap.add_argument("--config3-max-steps", type=int, default=3000)
     ap.add_argument("--no-config3-curved", action="store_true", help="skip the analytic-wall variant of config 3")
-    ap.add_argument("--sections", default="config2,config3,batch,serving,cpu",
-                    help="parts to run (a profiled run picks one; the headline needs config2)")
+    ap.add_argument("--sections", default="fp64,steady,plain,config2,config3,serving,cpu",
+                    help="extras beside the headline (a profiled run picks few): fp64, steady, plain, config2, "
+                         "config3, serving, cpu")
     return ap.parse_args()
 
 
@@ -162,108 +212,83 @@ def outcome_groups(fsm, cases, status):
     return groups
 
 
-def bench_batch(a):
-    import numpy as np
+def build_batch(a):
+    """The 256-solve workload: seeded cases, PARAMS, prepared meshes."""
     import workloads as W
-    fsm = W.fsm
     cases = W.batch_cases(a.batch)
     params = workload_params(W, cases)
     t0 = time.perf_counter()
     meshes = W.build_cases(cases, params, workers=16)
-    t_setup = time.perf_counter() - t0
+    setup = time.perf_counter() - t0
     infos = [m.info() for m in meshes]
-    out = {"instances": a.batch, "triangles": sum(i["ntri"] for i in infos), "nodes": sum(i["nn"] for i in infos),
-           "subedges": sum(i["nsub"] for i in infos), "bedges": sum(i["nbe"] for i in infos),
-           "k2_instances": sum(1 for c in cases if c["k"] == 2), "setup_s": t_setup,
-           "controls": dict(WORKLOAD_CONTROLS, tol=1e-10), "graph": True}
-    warm = max(p.foits + p.soits for p in params)
-    rho = {}
-    wanted = a.batch_precisions.split(",")
-    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
-        if name not in wanted:
-            continue
-        b = fsm.Batch(meshes, params, prec)
-        b.use_graph(True)
-        b.init_freestream()
+    desc = {"instances": a.batch, "triangles": sum(i["ntri"] for i in infos), "nodes": sum(i["nn"] for i in infos),
+            "subedges": sum(i["nsub"] for i in infos), "bedges": sum(i["nbe"] for i in infos),
+            "edges": sum(i["ne"] for i in infos), "k2_instances": sum(1 for c in cases if c["k"] == 2),
+            "setup_s": setup, "controls": dict(WORKLOAD_CONTROLS, tol=1e-10)}
+    return cases, params, meshes, infos, desc
+
+
+def batch_throughput(a, fsm, meshes, params, infos, prec):
+    """One engine over every instance: freestream start, one step, launch
+    shapes tuned on the engine's data (results unchanged), hipGraph warm-up
+    steps into every instance's final phase, `a.steps` hipGraph steps timed on
+    the host clock, then `a.steps` steps launched one kernel at a time with
+    events on the engine's stream around each kernel group (the headline
+    region: step time and per-kernel time come from the same steps)."""
+    warm = max(a.warmup, max(p.foits + p.soits for p in params))
+    b = fsm.Batch(meshes, params, prec)
+    b.use_graph(True)
+    b.init_freestream()
+    b.step()
+    tuning = b.autotune()
+    for _ in range(warm - 1):
         b.step()
-        tuning = b.autotune()   # launch shapes for this batch (results unchanged)
-        for _ in range(warm - 1):
-            b.step()
-        t = b.time_steps(a.batch_steps)
-        res = {"value": t["flux_evals"] / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s",
-               "ms_per_step": t["step_ms"], "device_bytes": b.info()["device_bytes"], "tuning": tuning}
-        b.close()
-        if a.batch_max_steps > 0:
-            b = fsm.Batch(meshes, params, prec)
-            b.use_graph(True)
-            b.init_freestream()
-            t0 = time.perf_counter()
-            steps, drop, status = b.solve_steady(a.batch_max_steps, 1e-10)
-            wall = time.perf_counter() - t0
-            conv = status == fsm.INST_CONVERGED
-            rho[name] = {i: b.get(i, "Q")[0:4 * infos[i]["nn"]:4] for i in range(a.batch) if conv[i]}
-            res["steady"] = {"wall_s": wall, "converged": int(conv.sum()),
-                             "failed": int((status == fsm.INST_FAILED).sum()),
-                             "max_steps": int((status == fsm.INST_MAX_STEPS).sum()),
-                             "steps_median": float(np.median(steps[conv])) if conv.any() else None,
-                             "instance_steps_total": int(steps.sum()),
-                             "steps_per_instance": [int(v) for v in steps],
-                             "solves_per_s": float(conv.sum()) / wall,
-                             "outcomes": outcome_groups(fsm, cases, status),
-                             "unconverged_mach_min": min([cases[i]["mach"] for i in range(a.batch) if not conv[i]],
-                                                         default=None)}
-            b.close()
-        out[name] = res
-    if "fp64" not in out or "mixed" not in out:
-        return out
-    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
-    if a.no_batch_plain:
-        return out
-    # the plain FP64 GPU path: the same solves one at a time (no batching),
-    # each timed alone over steps of its final phase
-    plain = plain_sequential(fsm, meshes, params, fsm.FS_PREC_F64, warm, 4)
-    out["plain_fp64_sequential"] = plain
-    out["speedup_mixed_batched_vs_plain_fp64"] = out["mixed"]["value"] / plain["value"]
-    if rho:
-        both = sorted(set(rho["fp64"]) & set(rho["mixed"]))
-        diffs = [float(np.linalg.norm(rho["mixed"][i] - rho["fp64"][i]) / np.linalg.norm(rho["fp64"][i]))
-                 for i in both]
-        out["converged_both"] = len(both)
-        out["max_density_l2_diff_mixed_vs_fp64"] = max(diffs) if diffs else None
-        out["time_to_solution_speedup"] = out["fp64"]["steady"]["wall_s"] / out["mixed"]["steady"]["wall_s"]
-        # an instance's FP64 trajectory is the same alone and batched, so the
-        # plain path needs the same steps: its time to solution is the sum of
-        # steps x alone step time
-        st = out["fp64"]["steady"]["steps_per_instance"]
-        t_plain = sum(n * ms for n, ms in zip(st, plain["ms_per_step_per_instance"])) * 1e-3
-        out["plain_fp64_time_to_solution_s_estimated"] = t_plain
-        out["time_to_solution_speedup_vs_plain_fp64"] = t_plain / out["mixed"]["steady"]["wall_s"]
-    if a.batch_plain_steady and a.batch_max_steps > 0 and "steady" in out["mixed"]:
-        # the plain path measured: every solve alone, one after the other, to
-        # its stopping test (hipGraph steps)
-        t0, nconv = time.perf_counter(), 0
-        for i, (m, p) in enumerate(zip(meshes, params)):
-            s = fsm.Solver(m, p, fsm.FS_PREC_F64)
-            s.use_graph(True)
-            s.init_freestream()
-            try:
-                _, dr = s.solve_steady(a.batch_max_steps, 1e-10)
-                nconv += int(dr <= 1e-10)
-            except fsm.FlowSolverError:
-                pass
-            s.close()
-            if (i + 1) % 32 == 0:
-                print("plain FP64 solves: %d of %d at %.0f s" % (i + 1, len(meshes), time.perf_counter() - t0),
-                      file=sys.stderr, flush=True)
-        t_meas = time.perf_counter() - t0
-        out["plain_fp64_time_to_solution_s_measured"] = t_meas
-        out["plain_fp64_converged"] = nconv
-        out["time_to_solution_speedup_vs_plain_fp64_measured"] = t_meas / out["mixed"]["steady"]["wall_s"]
-    for k in ("fp64", "mixed"):
-        if k in out and "steady" in out[k]:
-            out[k]["steady"].pop("steps_per_instance", None)
-    out["plain_fp64_sequential"].pop("ms_per_step_per_instance", None)
+    tg = b.time_steps(a.steps)
+    b.use_graph(False)
+    t = b.time_steps(a.steps)
+    out = {"value": t["flux_evals"] / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s",
+           "ms_per_step": t["step_ms"], "flux_evals_per_step": t["flux_evals"], "warmup": warm,
+           "graph_ms_per_step": tg["step_ms"], "graph_value": tg["flux_evals"] / (tg["step_ms"] * 1e-3),
+           "device_bytes": b.info()["device_bytes"], "tuning": tuning,
+           "kernel_ms": {"boundary" if prec == fsm.FS_PREC_MIXED else "edge_flux_fd_jacobian": t["edge_ms"],
+                         "residual_jacobian_node_pass" if prec == fsm.FS_PREC_MIXED else "node_assembly": t["node_ms"],
+                         "gauss_seidel": t["gs_ms"], "diag_factor": t["lu_ms"], "other": t["other_ms"]}}
+    b.close()
+    word = 4 if prec == fsm.FS_PREC_MIXED else 8
+    gs_bytes, launches = algorithmic_bytes_gs_batch(meshes, params[0].subits, word)
+    avg = t["gs_ms"] / launches
+    out["gauss_seidel"] = {"launches_per_step": launches, "algorithmic_bytes_per_launch": gs_bytes / launches,
+                           "avg_ms": avg, "achieved_GBs": gs_bytes / launches / (avg * 1e-3) / 1e9}
+    if prec == fsm.FS_PREC_MIXED:
+        rb = sum(algorithmic_bytes_resid_jac32(i, 0 if p.order == 3 else 1) for i, p in zip(infos, params))
+        out["residual_kernel"] = {
+            "kernel": "k_resid_jac32 (FP64 residual, ev_con/dt, FP32 Jacobian blocks and right-hand sides, one "
+                      "node pass)", "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
+            "achieved_GBs": rb / (t["node_ms"] * 1e-3) / 1e9, "frac": rb / (t["node_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "edge_flux_evals_per_s": t["flux_evals"] / (t["node_ms"] * 1e-3)}
     return out
+
+
+def batch_steady(a, fsm, cases, meshes, params, infos, prec):
+    """Every solve to a 1e-10 drop under per-instance convergence masks
+    (hipGraph steps); returns the outcome and the converged densities."""
+    import numpy as np
+    b = fsm.Batch(meshes, params, prec)
+    b.use_graph(True)
+    b.init_freestream()
+    t0 = time.perf_counter()
+    steps, drop, status = b.solve_steady(a.batch_max_steps, 1e-10)
+    wall = time.perf_counter() - t0
+    conv = status == fsm.INST_CONVERGED
+    rho = {i: b.get(i, "Q")[0:4 * infos[i]["nn"]:4] for i in range(len(meshes)) if conv[i]}
+    b.close()
+    res = {"wall_s": wall, "converged": int(conv.sum()), "failed": int((status == fsm.INST_FAILED).sum()),
+           "max_steps": int((status == fsm.INST_MAX_STEPS).sum()),
+           "steps_median": float(np.median(steps[conv])) if conv.any() else None,
+           "instance_steps_total": int(steps.sum()), "solves_per_s": float(conv.sum()) / wall,
+           "outcomes": outcome_groups(fsm, cases, status),
+           "unconverged_mach_min": min([cases[i]["mach"] for i in range(len(cases)) if not conv[i]], default=None)}
+    return res, rho, steps
 
 
 def plain_sequential(fsm, meshes, params, prec, warm, steps):
@@ -354,6 +379,16 @@ def bench_serving(a, precision):
     conv = st == fsm.INST_CONVERGED
     print("serving %s done: %d requests in %.1f s" % (name, len(res), wall), file=sys.stderr, flush=True)
     q = lambda v, p: float(np.percentile(v, p)) if len(v) else None
+
+    def censored(v, ok, p):
+        """p-th percentile (nearest rank) of the time to converged over every
+        request, an unconverged request counting as never converged: None
+        when the rank falls on one of those."""
+        if not len(v):
+            return None
+        w = np.sort(np.where(ok, v, np.inf))
+        r = w[max(0, int(np.ceil(p / 100.0 * len(w))) - 1)]
+        return float(r) if np.isfinite(r) else None
     # outcomes by mesh kind and reconstruction order, and the Mach numbers of
     # the requests that did not converge
     groups = outcome_groups(fsm, [cases[rid_case[r["id"]]] for r in res], [r["status"] for r in res])
@@ -362,8 +397,12 @@ def bench_serving(a, precision):
             "outcomes": groups, "unconverged_mach_min": min(bad) if bad else None,
             "converged": int(conv.sum()), "max_steps": int((st == fsm.INST_MAX_STEPS).sum()),
             "failed": int((st == fsm.INST_FAILED).sum()),
-            "p50_time_to_converged_s": q(lat[conv], 50), "p99_time_to_converged_s": q(lat[conv], 99),
-            "p50_admit_to_converged_s": q(dev[conv], 50), "p99_admit_to_converged_s": q(dev[conv], 99),
+            "converged_fraction": float(conv.mean()) if len(conv) else None,
+            "p50_time_to_converged_s": censored(lat, conv, 50), "p99_time_to_converged_s": censored(lat, conv, 99),
+            "p50_admit_to_converged_s": censored(dev, conv, 50), "p99_admit_to_converged_s": censored(dev, conv, 99),
+            "latency_note": "percentiles over all requests, an unconverged one counted as never converging (null "
+                            "when the rank falls on one); *_converged_only over the converged requests",
+            "p50_converged_only_s": q(lat[conv], 50), "p99_converged_only_s": q(lat[conv], 99),
             "solves_per_s": float(conv.sum()) / wall,
             "nodes_median": float(np.median([r["nn"] for r in res])) if res else None}
 
@@ -502,33 +541,35 @@ def cpu_reference_calibration(U, fsm, nx, ny, steps=2):
                       % (steps, nx, ny, t_ref, t_port)}
 
 
-def _cpu_batch_worker(case, steps, barrier, queue, core):
+def _cpu_batch_worker(case, first, steps, barrier, queue, core):
     """One batch solve on the oracle in its own process (spawned: no GPU),
-    pinned to one core: mesh preparation untimed, then `steps` pseudo-time
-    steps timed after a barrier shared by all workers."""
+    pinned to one core: mesh preparation untimed, the state set to the
+    freestream perturbed by 5% (seeded), then `steps` pseudo-time steps of
+    the final phase (from step `first`) timed after a barrier shared by all
+    workers."""
     os.sched_setaffinity(0, {core})
     import fsutil as U
     import workloads as W
     p = workload_params(W, [case])[0]
     m = W.make_mesh(case)
     m.prepare(p)
-    o = U.Oracle(m, p)
-    o.init_freestream()
     info = m.info()
+    o = U.Oracle(m, p)
+    o["Q"][:] = U.perturbed_state(info["nn"], info["ng"], p, seed=case["seed"])
     barrier.wait()
     t0 = time.time()
-    for ts in range(1, steps + 1):
+    for ts in range(first, first + steps):
         o.solve(ts, multicolor=0)
     queue.put((t0, time.time(), (info["nsub"] + info["nbe"]) * steps, info["ntri"]))
 
 
-def cpu_baseline_all_cores(steps=3):
+def cpu_baseline_all_cores(first, steps=2):
     """The oracle on every host core this process may use (at most 16): one
-    independent batch solve per core (each process pinned to its core), the
-    first solves of the batch workload,
-    timed together after a barrier.  Throughput = the sum of the per-core
-    rates (a CPU deployment keeps every core busy with a queue of solves);
-    the span from the first start to the last finish is reported beside it."""
+    independent solve of the headline batch per core (each process pinned to
+    its core), its final-phase steps timed together after a barrier.
+    Throughput = the sum of the per-core rates (a CPU deployment keeps every
+    core busy with a queue of solves); the span from the first start to the
+    last finish is reported beside it."""
     import multiprocessing as mp
     import workloads as W
     cores = sorted(os.sched_getaffinity(0))[:16]
@@ -536,7 +577,7 @@ def cpu_baseline_all_cores(steps=3):
     cases = W.batch_cases(ncores)
     ctx = mp.get_context("spawn")
     barrier, queue = ctx.Barrier(ncores), ctx.Queue()
-    procs = [ctx.Process(target=_cpu_batch_worker, args=(c, steps, barrier, queue, core))
+    procs = [ctx.Process(target=_cpu_batch_worker, args=(c, first, steps, barrier, queue, core))
              for c, core in zip(cases, cores)]
     for pr in procs:
         pr.start()
@@ -562,8 +603,59 @@ def cpu_baseline_all_cores(steps=3):
         pass
     return {"value": rate, "unit": "edge-flux evals/s", "cores": ncores, "kind": "port", "cpu": model,
             "value_over_span": sum(r[2] for r in res) / span,
-            "sample": "%d first-order-phase pseudo-time steps of each of the first %d batch solves (%d triangles "
-                      "in all), one process pinned to each core, %.1f s" % (steps, ncores, sum(r[3] for r in res), span)}
+            "sample": "%d final-phase pseudo-time steps (from step %d, the timed phase of the GPU run) of each of the "
+                      "first %d solves of the same 256-solve workload (%d triangles in all), one process pinned to "
+                      "each core, %.1f s" % (steps, first, ncores, sum(r[3] for r in res), span)}
+
+
+def bench_config2(a, fsm):
+    """The 1025x257 bump channel at M=0.675, k=2, Barth: FP64 and mixed steps
+    of the k=2 phase, with kernel times from events."""
+    p = fsm.Params.make(nsteps=0, iter=1, jufreq=1, CFL=50.0, ramp=20, gamma=1.4, alpha=0.0, mach=0.675, subits=20,
+                        method=1, order=3, foits=2, soits=2, limit=1, reorder=1)
+    t0 = time.perf_counter()
+    mesh = fsm.Mesh.bump(a.nx, a.ny, 0.2, 2)
+    mesh.prepare(p)
+    t_setup = time.perf_counter() - t0
+    info = mesh.info()
+    evals = info["nsub"] + info["nbe"]
+    warm = max(a.config2_warmup, p.foits + p.soits)
+    out = {"workload": "bump channel %dx%d, M=0.675, k=2 (order 3), Barth limiter" % (a.nx, a.ny),
+           "nodes": info["nn"], "subedges": info["nsub"], "bedges": info["nbe"], "lhs_blocks": info["nnz"],
+           "subits": p.subits, "steps": a.config2_steps, "warmup": warm, "setup_s": t_setup,
+           "jacobian_flux_evals_per_step_fp64": 8 * info["nsub"]}
+    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+        s = fsm.Solver(mesh, p, prec)
+        s.init_freestream()
+        for ts in range(1, warm + 1):
+            s.solve(ts)
+        t = s.time_steps(warm + 1, a.config2_steps)
+        s.close()
+        word = 8 if prec == fsm.FS_PREC_F64 else 4
+        gs_bytes, gs_launches = algorithmic_bytes_gauss_seidel(mesh, p.subits, word)
+        r = {"value": evals / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s", "ms_per_step": t["step_ms"],
+             "gauss_seidel": {"launches_per_step": gs_launches, "avg_ms": t["gs_ms"] / gs_launches,
+                              "algorithmic_bytes_per_launch": gs_bytes / gs_launches,
+                              "achieved_GBs": gs_bytes / (t["gs_ms"] * 1e-3) / 1e9}}
+        if prec == fsm.FS_PREC_F64:
+            ab = algorithmic_bytes_edge_kernel(info, recon=0)
+            r["kernel_ms"] = {"edge_flux_fd_jacobian": t["edge_ms"], "node_assembly": t["node_ms"],
+                              "gauss_seidel": t["gs_ms"], "diag_lu": t["lu_ms"], "other": t["other_ms"]}
+            r["edge_kernel"] = {"kernel": "k_edge_flux (Roe flux + FD Jacobian blocks, FP64 VALU)",
+                                "avg_ms": t["edge_ms"], "algorithmic_bytes": ab,
+                                "achieved_GBs": ab / (t["edge_ms"] * 1e-3) / 1e9}
+            r["gauss_seidel"]["traffic"] = measured_traffic("fsk::k_gs_color")
+        else:
+            rb = algorithmic_bytes_resid_jac32(info, 0)
+            r["kernel_ms"] = {"boundary": t["edge_ms"], "residual_jacobian_node_pass": t["node_ms"],
+                              "gauss_seidel": t["gs_ms"], "diag_inverse": t["lu_ms"], "other": t["other_ms"]}
+            r["residual_kernel"] = {"kernel": "k_resid_jac32", "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
+                                    "achieved_GBs": rb / (t["node_ms"] * 1e-3) / 1e9,
+                                    "frac": rb / (t["node_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                    "traffic": measured_traffic("fsk::k_resid_jac32(fsk::DevMesh, fsk::DevState)")}
+        out[name] = r
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out, mesh, p
 
 
 def main():
@@ -571,100 +663,94 @@ def main():
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     import fsutil as U
 
     fsm = U.fsm
-    sections = set(a.sections.split(","))
-    if "config2" not in sections:
-        line = {"metric": "edge-flux evals/s", "value": None, "unit": "edge-flux evals/s", "n_gpus": 1,
-                "note": "headline section not run (--sections %s)" % a.sections}
-        if "config3" in sections:
-            line["config3"] = bench_config3(a, fsm)
-        if "batch" in sections and a.batch > 0:
-            line["batch"] = bench_batch(a)
-        print(json.dumps(line), flush=True)
-        return
-    p = fsm.Params.make(nsteps=a.warmup + a.steps, iter=1, jufreq=1, CFL=50.0, ramp=20, gamma=1.4, alpha=0.0,
-                        mach=0.675, subits=20, method=1, order=3, foits=2, soits=2, limit=1, reorder=1)
-    t0 = time.perf_counter()
-    mesh = fsm.Mesh.bump(a.nx, a.ny, 0.2, 2)
-    mesh.prepare(p)
-    t_setup = time.perf_counter() - t0
-    info = mesh.info()
-    solver = fsm.Solver(mesh, p, fsm.FS_PREC_F64)
-    solver.init_freestream()
-    warm = max(a.warmup, p.foits + p.soits)
-    for ts in range(1, warm + 1):
-        solver.solve(ts)
-    tim = solver.time_steps(warm + 1, a.steps)
-    ms = tim["step_ms"]
-    evals = info["nsub"] + info["nbe"]
-    value = evals / (ms * 1e-3)
-    abytes = algorithmic_bytes_edge_kernel(info, recon=0)
-    edge_gbs = abytes / (tim["edge_ms"] * 1e-3) / 1e9
-    gs_bytes, gs_launches = algorithmic_bytes_gauss_seidel(mesh, p.subits, 8)
-    achieved = gs_bytes / (tim["gs_ms"] * 1e-3) / 1e9
+    sections = set(a.sections.split(",")) - {""}
+    cases, params, meshes, infos, desc = build_batch(a)
+    print("batch built: %d instances, %d nodes (%.1f s)" % (len(meshes), desc["nodes"], desc["setup_s"]),
+          file=sys.stderr, flush=True)
+    mixed = batch_throughput(a, fsm, meshes, params, infos, fsm.FS_PREC_MIXED)
+    gs = mixed["gauss_seidel"]
+    print("headline (mixed batch): %.3f ms/step" % mixed["ms_per_step"], file=sys.stderr, flush=True)
     line = {
-        "metric": "edge-flux evals/s", "value": value, "unit": "edge-flux evals/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": warm, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "bump channel %dx%d, M=0.675, k=2 (order 3), Barth limiter, FP64 implicit step"
-                               % (a.nx, a.ny), "nodes": info["nn"], "subedges": info["nsub"],
-                   "bedges": info["nbe"], "lhs_blocks": info["nnz"], "precision": "fp64", "subits": p.subits},
-        "roofline": {"bound": "hbm", "kernel": "k_gs_color (multicolor block Gauss-Seidel sweep, FP64)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (None if (a.nx, a.ny) != (1025, 257) else measured_traffic("fsk::k_gs_color")),
-                     "algorithmic_bytes": gs_bytes / gs_launches, "launches_per_step": gs_launches,
-                     "avg_ms": tim["gs_ms"] / gs_launches},
-        "achievable": None,
-        "edge_kernel_roofline": {"kernel": "k_edge_flux (Roe flux + FD Jacobian blocks, FP64 VALU)",
-                                 "achieved_GBs": edge_gbs, "algorithmic_bytes": abytes, "avg_ms": tim["edge_ms"]},
-        "kernel_ms": {"edge_flux": tim["edge_ms"], "node_assembly": tim["node_ms"], "gauss_seidel": tim["gs_ms"],
-                      "diag_lu": tim["lu_ms"], "other": tim["other_ms"]},
-        "jacobian_flux_evals_per_step": 8 * info["nsub"],
-        "setup_s": t_setup,
+        "metric": "edge-flux evals/s + steady-solve p50/p99 latency, mixed vs FP64, 1 GPU",
+        "value": mixed["value"], "unit": "edge-flux evals/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": mixed["warmup"], "ms_per_step": mixed["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic (seeded meshes and flow conditions, freestream start)",
+        "config": dict({"workload": "256 independent steady solves in one engine (bump channels and airfoil "
+                                    "O-meshes, 5k-200k triangles log-uniform, M~U[0.3,0.8], alpha~U[-2,4] deg, "
+                                    "k in {1,2}), mixed precision (FP64 residual, norms and update; FP32 analytic "
+                                    "Jacobian and Gauss-Seidel sweeps), one launch per kernel per pseudo-time step, "
+                                    "final-phase steps",
+                        "precision": "mixed"}, **desc),
+        "roofline": {"bound": "hbm", "kernel": "k_gs_color32 (multicolor block Gauss-Seidel sweep, FP32 blocks; "
+                                               "%.0f%% of the step)" % (100 * mixed["kernel_ms"]["gauss_seidel"] /
+                                                                        mixed["ms_per_step"]),
+                     "achieved": gs["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gs["achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("void fsk::k_gs_color32<6>(fsk::DevMesh, fsk::InstParams const*, "
+                                                 "float const*, float const*, float const*, float*, int, int)",
+                                                 "pmc_traffic_batch256_mixed.json"),
+                     "algorithmic_bytes": gs["algorithmic_bytes_per_launch"],
+                     "launches_per_step": gs["launches_per_step"], "avg_ms": gs["avg_ms"]},
+        "residual_kernel": mixed["residual_kernel"],
+        "mixed": mixed,
     }
-    # achievable streaming bandwidth of this device (4 GiB, beyond the
-    # last-level cache), beside the published peak the roofline uses
     rd, cp = fsm.hbm_probe(4 << 30, 5)
-    line["achievable"] = {"read_GBs": rd, "copy_GBs": cp, "gs_frac_of_read": achieved / rd,
+    line["achievable"] = {"read_GBs": rd, "copy_GBs": cp, "gs_frac_of_read": gs["achieved_GBs"] / rd,
                           "probe": "fs_hbm_probe: streaming float4 read and copy of 4 GiB, 5 passes each"}
-    # the mixed-precision path on the same workload and the same timed steps
-    sm = fsm.Solver(mesh, p, fsm.FS_PREC_MIXED)
-    sm.init_freestream()
-    for ts in range(1, warm + 1):
-        sm.solve(ts)
-    tm = sm.time_steps(warm + 1, a.steps)
-    sm.close()
-    gs32_bytes, _ = algorithmic_bytes_gauss_seidel(mesh, p.subits, 4)
-    res_bytes = algorithmic_bytes_mixed_node(info)
-    line["mixed"] = {"value": evals / (tm["step_ms"] * 1e-3), "ms_per_step": tm["step_ms"],
-                     "gauss_seidel_f32_GBs": gs32_bytes / (tm["gs_ms"] * 1e-3) / 1e9,
-                     "speedup_vs_fp64": ms / tm["step_ms"], "dtype": "f64 residual/norms/update, f32 Jacobian+sweeps",
-                     "kernel_ms": {"jacobian_f32": tm["edge_ms"], "residual_node_f64": tm["node_ms"],
-                                   "gauss_seidel_f32": tm["gs_ms"], "diag_lu_f32": tm["lu_ms"],
-                                   "other": tm["other_ms"]},
-                     "residual_kernel": {
-                         "kernel": "k_node_mixed (FP64 residual accumulated per node, ev_con/dt, FP32 diagonal "
-                                   "blocks and right-hand side)",
-                         "avg_ms": tm["node_ms"], "edge_flux_evals_per_s": evals / (tm["node_ms"] * 1e-3),
-                         "algorithmic_bytes": res_bytes,
-                         "achieved_GBs": res_bytes / (tm["node_ms"] * 1e-3) / 1e9,
-                         "frac": res_bytes / (tm["node_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}}
-    print("config2 done", file=sys.stderr, flush=True)
+    if "fp64" in sections:
+        f = batch_throughput(a, fsm, meshes, params, infos, fsm.FS_PREC_F64)
+        line["fp64"] = f
+        line["speedup_mixed_vs_fp64"] = mixed["value"] / f["value"]
+        print("fp64 batch: %.3f ms/step" % f["ms_per_step"], file=sys.stderr, flush=True)
+    rho, steps = {}, {}
+    if "steady" in sections and a.batch_max_steps > 0:
+        st = {}
+        for prec, name in ((fsm.FS_PREC_MIXED, "mixed"), (fsm.FS_PREC_F64, "fp64")):
+            st[name], rho[name], steps[name] = batch_steady(a, fsm, cases, meshes, params, infos, prec)
+            print("steady %s: %d converged in %.1f s" % (name, st[name]["converged"], st[name]["wall_s"]),
+                  file=sys.stderr, flush=True)
+        both = sorted(set(rho["fp64"]) & set(rho["mixed"]))
+        diffs = [float(np.linalg.norm(rho["mixed"][i] - rho["fp64"][i]) / np.linalg.norm(rho["fp64"][i]))
+                 for i in both]
+        st["converged_both"] = len(both)
+        st["max_density_l2_diff_mixed_vs_fp64"] = max(diffs) if diffs else None
+        st["time_to_solution_speedup"] = st["fp64"]["wall_s"] / st["mixed"]["wall_s"]
+        line["steady"] = st
+    if "plain" in sections:
+        # the plain FP64 GPU path: the same solves one at a time (no
+        # batching), each timed alone over steps of its final phase
+        warm = max(p.foits + p.soits for p in params)
+        plain = plain_sequential(fsm, meshes, params, fsm.FS_PREC_F64, warm, 4)
+        line["speedup_mixed_batched_vs_plain_fp64"] = mixed["value"] / plain["value"]
+        if "fp64" in steps:
+            # an instance's FP64 trajectory is the same alone and batched, so
+            # the plain path needs the same steps
+            t_plain = sum(n * ms for n, ms in zip(steps["fp64"], plain["ms_per_step_per_instance"])) * 1e-3
+            plain["time_to_solution_s_estimated"] = t_plain
+            line["time_to_solution_speedup_vs_plain_fp64"] = t_plain / line["steady"]["mixed"]["wall_s"]
+        plain.pop("ms_per_step_per_instance", None)
+        line["plain_fp64_sequential"] = plain
+        print("plain fp64 done", file=sys.stderr, flush=True)
+    del meshes
+    if "config2" in sections:
+        line["config2"], mesh2, p2 = bench_config2(a, fsm)
+        print("config2 done", file=sys.stderr, flush=True)
     if "config3" in sections:
         line["config3"] = bench_config3(a, fsm)
-    if "batch" in sections and a.batch > 0:
-        line["batch"] = bench_batch(a)
-        print("batch done", file=sys.stderr, flush=True)
     if "serving" in sections and a.serve_seconds > 0:
         precs = {"mixed": fsm.FS_PREC_MIXED, "fp64": fsm.FS_PREC_F64}
         sv = {k: bench_serving(a, precs[k]) for k in a.serve_precisions.split(",") if k}
         sv["controls"] = dict(WORKLOAD_CONTROLS, tol=1e-10, max_steps=a.serve_max_steps)
         line["serving"] = sv
     if "cpu" in sections and not a.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(U, fsm, p, a.nx, a.ny, a.cpu_steps)
-        line["cpu_baseline_all_cores"] = cpu_baseline_all_cores()
+        line["cpu_baseline"] = cpu_baseline_all_cores(mixed["warmup"] + 1)
+        if "config2" in sections:
+            line["cpu_baseline_config2"] = cpu_baseline(U, fsm, p2, a.nx, a.ny, a.cpu_steps)
         line["cpu_reference_calibration"] = cpu_reference_calibration(U, fsm, a.nx, a.ny)
         print("cpu baselines done", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)

@@ -92,12 +92,12 @@ int maybe_compact(fs_batch* b) {
 
 // one step of the running instances, then the per-instance convergence
 // bookkeeping of fs_solve_steady
-int batch_step(fs_batch* b, double tol, bool stop_on_tol) {
+int batch_step(fs_batch* b, double tol, bool stop_on_tol, bool record = false) {
   for (auto& E : b->eng) {
     bool any = false;
     for (const auto& I : E->inst) any = any || I.active;
     if (!any) continue;
-    int rc = E->step(true, b->graph, false);
+    int rc = E->step(true, b->graph && !record, record);
     if (rc) return rc;
   }
   for (int i = 0; i < b->ninst(); ++i) {
@@ -271,7 +271,11 @@ int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t) {
   for (auto& e : b->eng) {
     int rc = e->sync();
     if (rc) return rc;
+    e->t_edge = e->t_node = e->t_gs = e->t_lu = 0;
   }
+  // without graphs the kernel groups are timed with events on each engine's
+  // stream (fs_timing's kernel fields); with graphs only the step
+  const bool record = !b->graph;
   int64_t evals = 0;
   auto t0 = std::chrono::steady_clock::now();
   for (int k = 0; k < nsteps; ++k) {
@@ -279,7 +283,7 @@ int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t) {
       const fse::Instance& I = b->inst(i);
       if (I.active) evals += (int64_t)I.nsub + I.nbe;
     }
-    int rc = batch_step(b, 0.0, false);
+    int rc = batch_step(b, 0.0, false, record);
     if (rc) return rc;
   }
   for (auto& e : b->eng) {
@@ -290,6 +294,15 @@ int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t) {
   *t = fs_timing{};
   t->step_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / nsteps;
   t->flux_evals = evals / nsteps;
+  if (record) {
+    for (auto& e : b->eng) {
+      t->edge_ms += e->t_edge / nsteps;
+      t->node_ms += e->t_node / nsteps;
+      t->gs_ms += e->t_gs / nsteps;
+      t->lu_ms += e->t_lu / nsteps;
+    }
+    t->other_ms = t->step_ms - t->edge_ms - t->node_ms - t->gs_ms - t->lu_ms;
+  }
   return FS_OK;
 }
 

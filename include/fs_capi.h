@@ -339,7 +339,10 @@ int fs_batch_set(fs_batch* b, int i, int field, const double* host, int64_t n);
 /* grid_io.C:1390 Compute_Body_Forces of every instance from its state after
  * its last step: forces[2*i] lift, forces[2*i+1] drag */
 int fs_batch_body_forces(fs_batch* b, double* forces);
-/* nsteps timed pseudo-time steps of the active instances (step counters advance) */
+/* nsteps timed pseudo-time steps of the active instances (step counters
+ * advance).  Without graphs (fs_batch_use_graph off) the kernel fields of
+ * fs_timing are filled from events on the engines' streams; with graphs only
+ * step_ms and flux_evals. */
 int fs_batch_time_steps(fs_batch* b, int nsteps, fs_timing* t);
 /* device bytes a mesh costs inside a batch/solver of the given precision */
 int fs_mesh_footprint(const fs_mesh* m, int precision, int64_t* bytes);
