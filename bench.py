@@ -179,6 +179,11 @@ def parse():
     ap.add_argument("--batch-plain-steady", action="store_true",
                     help="also run every batch solve alone to its stopping test in FP64 (about 90 s more)")
     ap.add_argument("--config3-max-steps", type=int, default=3000)
+    ap.add_argument("--config3-steps", type=int, default=0,
+                    help="instead of steady solves, this many k=2-phase steps per precision (polyline wall only, "
+                         "one kernel launch at a time; for kernel traces)")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the extras named in --sections (profiles of one section)")
     ap.add_argument("--no-config3-curved", action="store_true", help="skip the analytic-wall variant of config 3")
     ap.add_argument("--sections", default="fp64,steady,plain,config2,config3,serving,cpu",
                     help="extras beside the headline (a profiled run picks few): fp64, steady, plain, config2, "
@@ -246,7 +251,7 @@ def batch_throughput(a, fsm, meshes, params, infos, prec):
     tg = b.time_steps(a.steps)
     b.use_graph(False)
     t = b.time_steps(a.steps)
-    out = {"value": t["flux_evals"] / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s",
+    out = {"value": t["flux_evals"] / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s", "batch": len(meshes),
            "ms_per_step": t["step_ms"], "flux_evals_per_step": t["flux_evals"], "warmup": warm,
            "graph_ms_per_step": tg["step_ms"], "graph_value": tg["flux_evals"] / (tg["step_ms"] * 1e-3),
            "device_bytes": b.info()["device_bytes"], "tuning": tuning,
@@ -265,7 +270,9 @@ def batch_throughput(a, fsm, meshes, params, infos, prec):
             "kernel": "k_resid_jac32 (FP64 residual, ev_con/dt, FP32 Jacobian blocks and right-hand sides, one "
                       "node pass)", "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
             "achieved_GBs": rb / (t["node_ms"] * 1e-3) / 1e9, "frac": rb / (t["node_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "edge_flux_evals_per_s": t["flux_evals"] / (t["node_ms"] * 1e-3)}
+            "edge_flux_evals_per_s": t["flux_evals"] / (t["node_ms"] * 1e-3),
+            "traffic": measured_traffic("fsk::k_resid_jac32", "pmc_traffic_batch256_mixed.json")
+            if len(meshes) == 256 else None}
     return out
 
 
@@ -466,6 +473,22 @@ def bench_config3(a, fsm):
     out = {"workload": "O-mesh 513x1171 around the 12%-thick profile, M=0.8, alpha=1.25, k=2, Barth",
            "nodes": info["nn"], "triangles": info["ntri"], "subedges": info["nsub"], "bedges": info["nbe"],
            "setup_s": time.perf_counter() - t0, "controls": dict(CONFIG3_CONTROLS, tol=1e-10)}
+    if a.config3_steps > 0:
+        # short run: the first k=2-phase steps, launched one kernel at a time
+        # (no hipGraph), with kernel times from events
+        warm = p.foits + p.soits
+        for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+            s = fsm.Solver(mesh, p, prec)
+            s.init_freestream()
+            for ts in range(1, warm + 1):
+                s.solve(ts)
+            t = s.time_steps(warm + 1, a.config3_steps)
+            s.close()
+            out[name] = {"ms_per_step": t["step_ms"], "value": t["flux_evals"] / (t["step_ms"] * 1e-3),
+                         "unit": "edge-flux evals/s", "kernel_ms": {k: t[k] for k in ("edge_ms", "node_ms", "gs_ms",
+                                                                                       "lu_ms", "other_ms")}}
+        out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+        return out
     out.update(_config3_solves(a, fsm, mesh, p, info))
     del mesh
     if a.no_config3_curved:
@@ -652,7 +675,7 @@ def bench_config2(a, fsm):
             r["residual_kernel"] = {"kernel": "k_resid_jac32", "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
                                     "achieved_GBs": rb / (t["node_ms"] * 1e-3) / 1e9,
                                     "frac": rb / (t["node_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                    "traffic": measured_traffic("fsk::k_resid_jac32(fsk::DevMesh, fsk::DevState)")}
+                                    "traffic": measured_traffic("fsk::k_resid_jac32")}
         out[name] = r
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     return out, mesh, p
@@ -668,6 +691,15 @@ def main():
 
     fsm = U.fsm
     sections = set(a.sections.split(",")) - {""}
+    if a.skip_headline:
+        line = {"metric": "edge-flux evals/s + steady-solve p50/p99 latency, mixed vs FP64, 1 GPU", "value": None,
+                "note": "headline not run (--skip-headline)"}
+        if "config2" in sections:
+            line["config2"], _, _ = bench_config2(a, fsm)
+        if "config3" in sections:
+            line["config3"] = bench_config3(a, fsm)
+        print(json.dumps(line), flush=True)
+        return
     cases, params, meshes, infos, desc = build_batch(a)
     print("batch built: %d instances, %d nodes (%.1f s)" % (len(meshes), desc["nodes"], desc["setup_s"]),
           file=sys.stderr, flush=True)
@@ -691,9 +723,8 @@ def main():
                                                                         mixed["ms_per_step"]),
                      "achieved": gs["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gs["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("void fsk::k_gs_color32<6>(fsk::DevMesh, fsk::InstParams const*, "
-                                                 "float const*, float const*, float const*, float*, int, int)",
-                                                 "pmc_traffic_batch256_mixed.json"),
+                     "traffic": measured_traffic("fsk::k_gs_color32<%d>" % mixed["tuning"]["gs32_pre"],
+                                                 "pmc_traffic_batch256_mixed.json") if a.batch == 256 else None,
                      "algorithmic_bytes": gs["algorithmic_bytes_per_launch"],
                      "launches_per_step": gs["launches_per_step"], "avg_ms": gs["avg_ms"]},
         "residual_kernel": mixed["residual_kernel"],
