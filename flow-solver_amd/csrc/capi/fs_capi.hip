@@ -441,6 +441,24 @@ int fs_build_matrix(fs_solver* s, int ts) {
   return finish(s, true);
 }
 
+int fs_eigenvalue_contribution(fs_solver* s) {
+  if (!s) return fail(FS_ERR_ARG, "null solver");
+  if (int rc = s->eng.upload_params_custom(mirror_params(s, s->eng.inst[0].ts))) return rc;
+  fsk::launch_node(s->eng.dm, s->eng.ds, fsk::NODE_EVCON | fsk::NODE_EVONLY, s->eng.stream);
+  return finish(s, true);
+}
+
+int fs_set_boundary_values(fs_solver* s) {
+  if (!s) return fail(FS_ERR_ARG, "null solver");
+  // the boundary pass with the Jacobian off: ghost states (and the
+  // boundary fluxes, which build_boundary_residuals reads next)
+  fsk::InstParams p = mirror_params(s, s->eng.inst[0].ts);
+  p.jac = 0;
+  if (int rc = s->eng.upload_params_custom(p)) return rc;
+  fsk::launch_bedge(s->eng.dm, s->eng.ds, s->eng.stream);
+  return finish(s, true);
+}
+
 int fs_gradient(fs_solver* s) {
   if (!s) return fail(FS_ERR_ARG, "null solver");
   fsk::InstParams p = mirror_params(s, s->eng.inst[0].ts);

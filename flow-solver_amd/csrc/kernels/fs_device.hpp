@@ -102,93 +102,6 @@ __device__ __forceinline__ void roe_flux_wave(double nx, double ny, double len, 
   roe_flux_wave_prim(nx, ny, len, g1, qL, qR, prim_w(qL, g1), prim_w(qR, g1), flux, nullptr);
 }
 
-// Characteristic ghost state from the interior state and the previous ghost
-// (cvbc.C:63-475).  Returns false where the reference would exit().
-// The freestream (uinf, vinf, pinf) is formed on the host exactly as
-// cvbc.C:62-69 does, so the device never re-evaluates cos/sin.
-__device__ __forceinline__ bool ghost_state(int bc, double nx, double ny, double gamma, double uinf, double vinf,
-                                            double pinf, const double* qi, double* qg) {
-  const double g = gamma;
-  const double rhoinf = 1.0;
-  double rhoi = qi[0], ui = qi[1] / rhoi, vi = qi[2] / rhoi;
-  double pi = (g - 1.) * (qi[3] - 0.5 * rhoi * (ui * ui + vi * vi));
-  double ci = sqrt((g * pi) / rhoi);
-  double rhob = qg[0], ub = qg[1] / rhob, vb = qg[2] / rhob;
-  double pb = (g - 1.) * (qg[3] - 0.5 * rhob * (ub * ub + vb * vb));
-  double cb = sqrt((g * pb) / rhob);
-  double rho0 = 0.5 * (rhoi + rhob), c0 = 0.5 * (ci + cb);
-  double theta = ui * nx + vi * ny;
-  double ev1 = theta, ev3 = theta + ci, ev4 = theta - ci;
-  if (bc == 0 || (bc % 2 == 0)) {
-    if (ev1 <= 0.0 && ev4 <= 0.0) {
-      if (ev3 >= 0.0) {
-        pb = 0.5 * (pinf + pi - rho0 * c0 * ((uinf - ui) * nx + (vinf - vi) * ny));
-        rhob = rhoinf + (pb - pinf) / (c0 * c0);
-        ub = uinf + (pb - pinf) / (rho0 * c0) * nx;
-        vb = vinf + (pb - pinf) / (rho0 * c0) * ny;
-      } else if (ev3 < 0.0) {
-        rhob = rhoinf; ub = uinf; vb = vinf; pb = pinf;
-      } else {
-        return false;
-      }
-    } else if (ev1 >= 0.0 && ev3 >= 0.0) {
-      if (ev4 <= 0.0) {
-        pb = pinf;
-        rhob = rhoi + (pb - pi) / (c0 * c0);
-        ub = ui - (pb - pi) / (rho0 * c0) * nx;
-        vb = vi - (pb - pi) / (rho0 * c0) * ny;
-      } else if (ev4 > 0.0) {
-        rhob = rhoi; ub = ui; vb = vi; pb = pi;
-      } else {
-        return false;
-      }
-    } else {
-      return false;
-    }
-  } else {
-    pb = pi + rho0 * c0 * theta;
-    rhob = rhoi + (pb - pi) / (c0 * c0);
-    ub = ui - (pb - pi) / (rho0 * c0) * nx;
-    vb = vi - (pb - pi) / (rho0 * c0) * ny;
-  }
-  if (rhob < 0.0) {
-    rhob = rhoi;
-    ub = (1. - 2. * nx * nx) * ui - 2. * nx * ny * vi;
-    vb = (1. - 2. * ny * ny) * vi - 2. * nx * ny * ui;
-    pb = pi;
-  }
-  qg[0] = rhob;
-  qg[1] = rhob * ub;
-  qg[2] = rhob * vb;
-  qg[3] = pb / (g - 1.0) + 0.5 * rhob * (ub * ub + vb * vb);
-  return true;
-}
-
-// diagonal boundary Jacobian 0.5*(A(qL) + |A_roe|)*len (jacobian.C:681-962)
-__device__ __forceinline__ void boundary_jacobian(double nx, double ny, double len, double g, const double* QL,
-                                                  const double* QR, double* d) {
-  double Rm[16];
-  roe_matrix(nx, ny, g, QL, QR, Rm);
-  double rhoL = QL[0], uL = QL[1] / QL[0], vL = QL[2] / QL[0], EL = QL[3];
-  double theta = uL * nx + vL * ny;
-  double phi = 0.5 * (g - 1.) * (uL * uL + vL * vL);
-  d[0] = 0.; d[1] = nx; d[2] = ny; d[3] = 0.;
-  d[4] = phi * nx - uL * theta;
-  d[5] = (2. - g) * uL * nx + theta;
-  d[6] = uL * ny - (g - 1.) * vL * nx;
-  d[7] = (g - 1.) * nx;
-  d[8] = phi * ny - vL * theta;
-  d[9] = vL * nx - (g - 1.) * uL * ny;
-  d[10] = theta + (2. - g) * vL * ny;
-  d[11] = (g - 1.) * ny;
-  d[12] = (2. * phi - g * EL / rhoL) * theta;
-  d[13] = (1. - g) * uL * theta + nx * (g * EL / rhoL - phi);
-  d[14] = (1. - g) * vL * theta + ny * (g * EL / rhoL - phi);
-  d[15] = g * theta;
-#pragma unroll
-  for (int i = 0; i < 16; i++) d[i] = 0.5 * (d[i] + Rm[i]) * len;
-}
-
 // unpivoted 4x4 LU and substitution (linear_algebra.C:571-638)
 __device__ __forceinline__ void lu4(double* A) {
 #pragma unroll

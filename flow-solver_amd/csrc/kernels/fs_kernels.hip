@@ -429,6 +429,13 @@ __global__ void __launch_bounds__(kMaxBlock) k_node(DevMesh m, DevState s, int f
       if (boundary) Rv += s.bflux[4 * (size_t)b + v];
     }
   }
+  if (flags & NODE_EVONLY) {
+    if (v == 0) {
+      if (!isfinite(ev)) atomicOr(s.err + inst, 2);
+      s.ev_con[n] = ev;
+    }
+    return;
+  }
   double D[4];
   double* Dg = s.LHS + 16 * (size_t)m.iau[n] + 4 * v;
   if (evcon) {

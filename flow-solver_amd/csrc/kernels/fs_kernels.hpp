@@ -156,7 +156,9 @@ __device__ __forceinline__ int inst_of(const DevMesh& m, const int* map, int idx
   return m.ninst == 1 ? 0 : map[idx];
 }
 
-enum NodeFlags { NODE_EVCON = 1, NODE_INTERIOR = 2, NODE_BOUNDARY = 4, NODE_JAC = 8, NODE_RHS = 16 };
+// NODE_EVONLY with NODE_EVCON: ev_con alone (cv_calc_eigenvalue_contribution),
+// neither dt nor the V/dt diagonal
+enum NodeFlags { NODE_EVCON = 1, NODE_INTERIOR = 2, NODE_BOUNDARY = 4, NODE_JAC = 8, NODE_RHS = 16, NODE_EVONLY = 32 };
 
 // Deterministic per-instance norms: each instance's nodes are cut into
 // fixed-size chunks counted from the instance's first node, so the sums do

@@ -228,6 +228,8 @@ def lib():
         "fs_mesh_create": (i, [i, vp, i, vp, i, vp, i, vp, vp, vp, C.POINTER(vp)]),
         "fs_roe_flux_batch_form": (i, [i, i, d, vp, vp, vp, vp, vp]),
         "fs_state_norms": (i, [i64, vp, vp, vp, vp]),
+        "fs_eigenvalue_contribution": (i, [vp]),
+        "fs_set_boundary_values": (i, [vp]),
         "fs_state_norms_noalloc": (i, [i64, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -420,6 +422,14 @@ class Solver:
 
     def gradient(self):
         _check(lib().fs_gradient(self._h))
+
+    def eigenvalue_contribution(self):
+        """cv_calc_eigenvalue_contribution (cv_calc.C:790): ev_con of nQ."""
+        _check(lib().fs_eigenvalue_contribution(self._h))
+
+    def set_boundary_values(self):
+        """Set_Boundary_Values (cvbc.C:37): the ghost states of nQ."""
+        _check(lib().fs_set_boundary_values(self._h))
 
     def derivatives(self):
         _check(lib().fs_derivatives(self._h))

@@ -200,6 +200,14 @@ int fs_solver_get(fs_solver* s, int field, double* host, int64_t n);
  * ts is the pseudo-time step (grid->citer in the reference). */
 int fs_build_matrix(fs_solver* s, int ts);                      /* init_memory.C:159 build_matrix */
 int fs_gradient(fs_solver* s);                                  /* gradient.C:245 Compute_Gradient_LeastSquares */
+/* cv_calc.C:790 cv_calc_eigenvalue_contribution: ev_con of the resident nQ
+ * (sum over each node's faces of max|theta +- c| len, in the reference's
+ * ascending face order); FS_ERR_STATE where the reference would exit */
+int fs_eigenvalue_contribution(fs_solver* s);
+/* cvbc.C:37 Set_Boundary_Values: the ghost states of the resident nQ from
+ * their real nodes and their previous values (ghosts have memory);
+ * FS_ERR_STATE where the reference would exit (cvbc.C:330-382) */
+int fs_set_boundary_values(fs_solver* s);
 int fs_derivatives(fs_solver* s);                               /* compute_derivatives.C:40 Compute_Derivatives(.,3) */
 /* limiter.C:39 compute_limiter_barth, or limiter.C:511 compute_limiter_venkatakrishnan
  * when the solver's PARAMS has limit = 2 */

@@ -1,9 +1,12 @@
 /* Per-face FP64 math of the residual path, compiled for the host and for
  * gfx950 from the same source: the Euler flux, the Roe dissipation matrix,
- * Roe_flux_centered and the node eigenvalue bound.
+ * Roe_flux_centered, the node eigenvalue bound, the characteristic ghost
+ * state, the flux Jacobian blocks (analytic, finite-difference, boundary)
+ * and the face-state extrapolation.
  *
  * Operation order follows the reference term by term (flux.C:37-68,
- * flux.C:299-541, cv_calc.C:819-849); compiled without floating-point
+ * flux.C:299-541, cv_calc.C:819-849, cvbc.C:63-475, jacobian.C:45-962,
+ * reconstruct.C:181-205); compiled without floating-point
  * contraction (hipcc -ffp-contract=off; g++ on x86-64 without an FMA target)
  * a host call and a device call give the reference's result bit for bit.
  */
@@ -154,6 +157,153 @@ FS_HD double max_ev(const double* q, double nx, double ny, double gamma) {
   double theta = u * nx + v * ny;
   double tpc = fabs(theta + c), tmc = fabs(theta - c);
   return tpc >= tmc ? tpc : tmc;
+}
+
+// Characteristic ghost state from the interior state and the previous ghost
+// (cvbc.C:63-475).  Returns false where the reference would exit().
+// The freestream (uinf, vinf, pinf) is formed on the host exactly as
+// cvbc.C:62-69 does, so the device never re-evaluates cos/sin.
+FS_HD bool ghost_state(int bc, double nx, double ny, double gamma, double uinf, double vinf,
+                                            double pinf, const double* qi, double* qg) {
+  const double g = gamma;
+  const double rhoinf = 1.0;
+  double rhoi = qi[0], ui = qi[1] / rhoi, vi = qi[2] / rhoi;
+  double pi = (g - 1.) * (qi[3] - 0.5 * rhoi * (ui * ui + vi * vi));
+  double ci = sqrt((g * pi) / rhoi);
+  double rhob = qg[0], ub = qg[1] / rhob, vb = qg[2] / rhob;
+  double pb = (g - 1.) * (qg[3] - 0.5 * rhob * (ub * ub + vb * vb));
+  double cb = sqrt((g * pb) / rhob);
+  double rho0 = 0.5 * (rhoi + rhob), c0 = 0.5 * (ci + cb);
+  double theta = ui * nx + vi * ny;
+  double ev1 = theta, ev3 = theta + ci, ev4 = theta - ci;
+  if (bc == 0 || (bc % 2 == 0)) {
+    if (ev1 <= 0.0 && ev4 <= 0.0) {
+      if (ev3 >= 0.0) {
+        pb = 0.5 * (pinf + pi - rho0 * c0 * ((uinf - ui) * nx + (vinf - vi) * ny));
+        rhob = rhoinf + (pb - pinf) / (c0 * c0);
+        ub = uinf + (pb - pinf) / (rho0 * c0) * nx;
+        vb = vinf + (pb - pinf) / (rho0 * c0) * ny;
+      } else if (ev3 < 0.0) {
+        rhob = rhoinf; ub = uinf; vb = vinf; pb = pinf;
+      } else {
+        return false;
+      }
+    } else if (ev1 >= 0.0 && ev3 >= 0.0) {
+      if (ev4 <= 0.0) {
+        pb = pinf;
+        rhob = rhoi + (pb - pi) / (c0 * c0);
+        ub = ui - (pb - pi) / (rho0 * c0) * nx;
+        vb = vi - (pb - pi) / (rho0 * c0) * ny;
+      } else if (ev4 > 0.0) {
+        rhob = rhoi; ub = ui; vb = vi; pb = pi;
+      } else {
+        return false;
+      }
+    } else {
+      return false;
+    }
+  } else {
+    pb = pi + rho0 * c0 * theta;
+    rhob = rhoi + (pb - pi) / (c0 * c0);
+    ub = ui - (pb - pi) / (rho0 * c0) * nx;
+    vb = vi - (pb - pi) / (rho0 * c0) * ny;
+  }
+  if (rhob < 0.0) {
+    rhob = rhoi;
+    ub = (1. - 2. * nx * nx) * ui - 2. * nx * ny * vi;
+    vb = (1. - 2. * ny * ny) * vi - 2. * nx * ny * ui;
+    pb = pi;
+  }
+  qg[0] = rhob;
+  qg[1] = rhob * ub;
+  qg[2] = rhob * vb;
+  qg[3] = pb / (g - 1.0) + 0.5 * rhob * (ub * ub + vb * vb);
+  return true;
+}
+
+// Euler flux Jacobian dF.n/dq of one state, in the operation order of
+// jacobian.C:561-582 (also jacobian.C:581-599 for the right state and
+// jacobian.C:892-909 at a boundary edge)
+FS_HD void flux_jacobian(double nx, double ny, double g, const double* Q, double* d) {
+  double rhoL = Q[0], uL = Q[1] / Q[0], vL = Q[2] / Q[0], EL = Q[3];
+  double theta = uL * nx + vL * ny;
+  double phi = 0.5 * (g - 1.) * (uL * uL + vL * vL);
+  d[0] = 0.; d[1] = nx; d[2] = ny; d[3] = 0.;
+  d[4] = phi * nx - uL * theta;
+  d[5] = (2. - g) * uL * nx + theta;
+  d[6] = uL * ny - (g - 1.) * vL * nx;
+  d[7] = (g - 1.) * nx;
+  d[8] = phi * ny - vL * theta;
+  d[9] = vL * nx - (g - 1.) * uL * ny;
+  d[10] = theta + (2. - g) * vL * ny;
+  d[11] = (g - 1.) * ny;
+  d[12] = (2. * phi - g * EL / rhoL) * theta;
+  d[13] = (1. - g) * uL * theta + nx * (g * EL / rhoL - phi);
+  d[14] = (1. - g) * vL * theta + ny * (g * EL / rhoL - phi);
+  d[15] = g * theta;
+}
+
+// diagonal boundary Jacobian 0.5*(A(qL) + |A_roe|)*len (jacobian.C:681-962)
+FS_HD void boundary_jacobian(double nx, double ny, double len, double g, const double* QL, const double* QR,
+                             double* d) {
+  double Rm[16];
+  roe_matrix(nx, ny, g, QL, QR, Rm);
+  flux_jacobian(nx, ny, g, QL, d);
+FS_UNROLL
+  for (int i = 0; i < 16; i++) d[i] = 0.5 * (d[i] + Rm[i]) * len;
+}
+
+// compute_approximate_jacobian_roe's blocks (jacobian.C:301-612):
+// dL = 0.5 (A(qL) + |A_roe|) len, dR = 0.5 (A(qR) - |A_roe|) len, row-major
+FS_HD void analytic_jacobian(double nx, double ny, double len, double g, const double* QL, const double* QR,
+                             double* dL, double* dR) {
+  double Rm[16];
+  roe_matrix(nx, ny, g, QL, QR, Rm);
+  flux_jacobian(nx, ny, g, QL, dL);
+  flux_jacobian(nx, ny, g, QR, dR);
+FS_UNROLL
+  for (int i = 0; i < 16; i++) {
+    dL[i] = 0.5 * (dL[i] + Rm[i]) * len;
+    dR[i] = 0.5 * (dR[i] - Rm[i]) * len;
+  }
+}
+
+// compute_numerical_jacobian_roe's blocks (jacobian.C:64-106): forward
+// differences of Roe_flux_centered with eps = 1e-8 in each component of qL
+// (dL) and of qR (dR) around `flux` = Roe_flux_centered(qL, qR), row-major
+FS_HD void fd_jacobian(double nx, double ny, double len, double g, const double* QL, const double* QR,
+                       const double* flux, double* dL, double* dR) {
+  const double eps = 1.0e-08;
+  double Qp[4], fp[4];
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 4; j++) Qp[j] = QL[j];
+    Qp[i] += eps;
+    roe_flux(nx, ny, len, g, Qp, QR, fp);
+    for (int j = 0; j < 4; j++) dL[4 * j + i] = (fp[j] - flux[j]) / eps;
+  }
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 4; j++) Qp[j] = QR[j];
+    Qp[i] += eps;
+    roe_flux(nx, ny, len, g, QL, Qp, fp);
+    for (int j = 0; j < 4; j++) dR[4 * j + i] = (fp[j] - flux[j]) / eps;
+  }
+}
+
+// Face-state extrapolation of one node (reconstruct.C:181-205, 1276-1296):
+// linear, Q += phi (dQ/dx dx + dQ/dy dy) with grad[var][x,y]; quadratic,
+// Q = point value + phi (Q_x dx + Q_y dy) + sigma (Q_xx dx^2/2 + Q_yy dy^2/2
+// + Q_xy dx dy) with hess[var * stride + (x, y, xx, yy, xy)]
+FS_HD void extrapolate_linear(double* Q, double phi, const double* grad, double dx0, double dx1) {
+  for (int j = 0; j < 4; j++) Q[j] += (phi * grad[j * 2 + 0] * dx0 + phi * grad[j * 2 + 1] * dx1);
+}
+FS_HD void extrapolate_quadratic(double* Q, const double* pv, double phi, double sigma, const double* hess, int stride,
+                                 double dx0, double dx1) {
+  for (int j = 0; j < 4; j++) Q[j] = pv[j];
+  for (int j = 0; j < 4; j++) {
+    const double* h = hess + j * stride;
+    Q[j] += (phi * h[0] * dx0 + phi * h[1] * dx1 + sigma * h[2] * dx0 * dx0 * 0.5 + sigma * h[3] * dx1 * dx1 * 0.5 +
+             sigma * h[4] * dx0 * dx1);
+  }
 }
 
 }  // namespace fsd

@@ -5,7 +5,7 @@
  * those functions with the same names and signatures:
  *
  *   solve                              include/solve.h:21           (solve.C:44)
- *   build_matrix                       include/init_memory.h:22     (init_memory.C:159)
+ *   build_matrix                       include/init_memory.h:24     (init_memory.C:159)
  *   build_interior_residuals           include/residual.h:21        (residual.C:40)
  *   build_boundary_residuals           include/residual.h:23        (residual.C:359)
  *   compute_limiter_barth              include/limiter.h:21         (limiter.C:39)
@@ -14,6 +14,18 @@
  *   Compute_Derivatives                include/compute_derivatives.h:19 (compute_derivatives.C:40)
  *   linear_solve                       include/linear_algebra.h:23  (linear_algebra.C:122)
  *   Roe_flux_centered                  include/flux.h:26            (flux.C:299)
+ *   Reconstruct                        include/reconstruct.h:21     (reconstruct.C:38)
+ *   Reconstruct_Gauss                  include/reconstruct.h:25     (reconstruct.C:1109)
+ *   Reconstruct_Gauss_Boundary         include/reconstruct.h:27     (reconstruct.C:1708)
+ *   compute_numerical_jacobian_roe     include/jacobian.h:21        (jacobian.C:45)
+ *   compute_approximate_jacobian_roe   include/jacobian.h:28        (jacobian.C:301)
+ *   compute_boundary_approximate_jacobian_roe include/jacobian.h:32 (jacobian.C:681)
+ *   compute_boundary_numerical_jacobian_roe include/jacobian.h:25 (jacobian.C:205)
+ *   Get_Boundary_Value                 include/cvbc.h:24            (cvbc.C:502)
+ *   Set_Boundary_Values                include/cvbc.h:22            (cvbc.C:37)
+ *   cv_calc_eigenvalue_contribution    include/cv_calc.h:28         (cv_calc.C:790)
+ *   Compute_Edge_Weights               include/gradient.h:23        (gradient.C:923)
+ *   Build_CRS                          include/init_memory.h:22     (init_memory.C:36)
  *
  * as templates over the caller's GRID / SYS_MEM / PARAMS types, reading the
  * fields by the reference's names and layouts (include/grid.h:224-306,
@@ -43,6 +55,15 @@
  * Per-node values only: phi receives the node scalar in its [0] and [1]
  * entries (the ones the path consumes); hess entries 5..8 of each variable
  * are not produced (the k=2 path uses 0..4).
+ *
+ * The per-face functions (Roe_flux_centered, Reconstruct*, the four
+ * Jacobian functions, Get_Boundary_Value) run on the host from the shared
+ * per-face header the kernels are built from (include/fs_face.hpp) and read
+ * or update the caller's own arrays, as the reference's do; Set_Boundary_Values and
+ * cv_calc_eigenvalue_contribution are device passes over the twin;
+ * Compute_Edge_Weights and Build_CRS copy the host preparation's arrays.
+ * Reconstruction orders 0, 1 (k=1) and 3 (k=2) are supported; order 2 and 4
+ * stop as an unsupported configuration.
  */
 #ifndef FS_REFERENCE_SHIM_HPP
 #define FS_REFERENCE_SHIM_HPP
@@ -98,6 +119,16 @@ struct GRID {
   double* ev_con = nullptr;
   double* dt = nullptr;
   double cfl = 0.;
+  // fields of the per-face functions and the setup copies (1-based, slot 0
+  // unused, as include/grid.h:224-306 lays them out)
+  int* edges = nullptr;          // (nedges+1)*2
+  int* subedges = nullptr;       // (nsubedges+1)*2: edge, element
+  double* xm_subedges = nullptr; // (nsubedges+1)*2
+  double* el_cent = nullptr;     // (elements+1)*2
+  int* bedges = nullptr;         // (nbedges+1)*5: real node, ghost, element, boundary, segment
+  double* xn_bedges = nullptr;   // (nbedges+1)*3: outward unit normal, length
+  int* node_order = nullptr;     // nn+nn_ghost+1
+  double* edge_weight = nullptr; // (nedges+1)*4
 };
 
 struct SYS_MEM {
@@ -456,6 +487,308 @@ void curved_boundary_normal_vector(G*, int bc, double* x, double* nx, double* ny
 template <class G>
 void curved_boundary_arclength(G*, int bc, double* xL, double* xR, double* arclength) {
   detail::check(fs_curved_arclength(bc, xL, xR, arclength), "curved_boundary_arclength");
+}
+
+namespace detail {
+[[noreturn]] inline void unsupported(const char* where, const char* what) {
+  std::printf("FATAL ERROR: %s: %s\n", where, what);
+  std::fflush(stdout);
+  std::exit(1);
+}
+// Reconstruction order of `node` at pseudo-time step `ts` (reconstruct.C:72-
+// 171, 1161-1188): 0 nodal, 1 linear, 3 quadratic; -1 for order 0 (no
+// reconstruction).
+template <class G, class P>
+int recon_order(const G* g, const P& p, int ts, int node, const char* where) {
+  if (p.order == 0) return -1;
+  if (p.order == 1) return ts <= p.foits ? 0 : 1;
+  if (p.order != 3) unsupported(where, "only orders 0, 1 and 3 are on this path");
+  if (ts <= p.foits) return 0;
+  if (ts <= p.foits + p.soits) return 1;
+  const int r = g->node_order[node];
+  if (r != 1 && r != 3) unsupported(where, "node order must be 1 or 3");
+  return r;
+}
+// Q of `node` extrapolated by `dx` at order r (reconstruct.C:181-205,
+// 222-254; PV_EXT: the quadratic starts from the point value)
+template <class G>
+void extrapolate(const G* g, int node, int r, const double* dx, double* Q) {
+  if (r == 1) {
+    fsd::extrapolate_linear(Q, g->phi[node * 8], g->grad + 8 * (size_t)node, dx[0], dx[1]);
+  } else if (r == 3) {
+    fsd::extrapolate_quadratic(Q, g->point_value + 4 * (size_t)node, g->phi[node * 8], g->phi[node * 8 + 1],
+                               g->hess + 36 * (size_t)node, 9, dx[0], dx[1]);
+  }
+}
+// slot of block (row, col) in the reference's CRS (init_memory.C:36-138);
+// -1 when absent (the reference's search then writes nothing)
+template <class S>
+int block_slot(const S* s, int row, int col) {
+  for (int i = s->ia[row]; i < s->ia[row + 1]; i++)
+    if (s->ja[i] == col) return i;
+  return -1;
+}
+// the reference's four block updates of one interior face (jacobian.C:
+// 135-182): diag L += dL, diag R -= dR, (L,R) += dR, (R,L) -= dL
+template <class S>
+void scatter_blocks(S* s, int nodeL, int nodeR, const double* dL, const double* dR) {
+  double* LHS = s->LHS;
+  for (int k = 0; k < 16; k++) LHS[(size_t)s->iau[nodeL] * 16 + k] += dL[k];
+  for (int k = 0; k < 16; k++) LHS[(size_t)s->iau[nodeR] * 16 + k] -= dR[k];
+  for (int i = s->ia[nodeL]; i < s->ia[nodeL + 1]; i++)
+    if (s->ja[i] == nodeR)
+      for (int k = 0; k < 16; k++) LHS[(size_t)i * 16 + k] += dR[k];
+  for (int i = s->ia[nodeR]; i < s->ia[nodeR + 1]; i++)
+    if (s->ja[i] == nodeL)
+      for (int k = 0; k < 16; k++) LHS[(size_t)i * 16 + k] -= dL[k];
+}
+inline void warn_nonfinite(const double* d, const char* which, int nodeL, int nodeR) {
+  for (int k = 0; k < 16; k++)
+    if (!std::isfinite(d[k]))
+      std::printf("Problem DETECTED in %s[%d][%d]=%f  between nodeL=%d and nodeR=%d.\n", which, k / 4, k % 4,
+                  (float)d[k], nodeL, nodeR);
+}
+}  // namespace detail
+
+// reconstruct.C:38-446: limited face states of subedge `isedge` (1-based)
+// at its face point for the phase of step ts; QL/QR start from nQ
+template <class G, class P>
+void Reconstruct(int isedge, int nodeL, int nodeR, double QL[4], double QR[4], G* grid, P p, int ts) {
+  for (int j = 0; j < 4; j++) {
+    QL[j] = grid->nQ[nodeL * 4 + j];
+    QR[j] = grid->nQ[nodeR * 4 + j];
+  }
+  const int rL = detail::recon_order(grid, p, ts, nodeL, "Reconstruct");
+  if (rL < 0) return;
+  const int rR = detail::recon_order(grid, p, ts, nodeR, "Reconstruct");
+  // face point: midway between the element centroid and the edge midpoint
+  const int gelem = grid->subedges[isedge * 2 + 1];
+  const double fc0 = (grid->el_cent[gelem * 2 + 0] + grid->xm_subedges[isedge * 2 + 0]) * 0.5;
+  const double fc1 = (grid->el_cent[gelem * 2 + 1] + grid->xm_subedges[isedge * 2 + 1]) * 0.5;
+  double dx[2] = {fc0 - grid->x[nodeL * 2 + 0], fc1 - grid->x[nodeL * 2 + 1]};
+  detail::extrapolate(grid, nodeL, rL, dx, QL);
+  dx[0] = fc0 - grid->x[nodeR * 2 + 0];
+  dx[1] = fc1 - grid->x[nodeR * 2 + 1];
+  detail::extrapolate(grid, nodeR, rR, dx, QR);
+}
+
+// reconstruct.C:1109-1399: limited states of both ends at the point GP, in
+// the phase of grid->citer (as the reference reads it; ts is unused there
+// too)
+template <class G, class P>
+void Reconstruct_Gauss(int isedge, int nodeL, int nodeR, double GP[2], double QL[4], double QR[4], G* grid, P p,
+                       int ts) {
+  (void)isedge;
+  (void)ts;
+  for (int j = 0; j < 4; j++) {
+    QL[j] = grid->nQ[nodeL * 4 + j];
+    QR[j] = grid->nQ[nodeR * 4 + j];
+  }
+  const int rL = detail::recon_order(grid, p, grid->citer, nodeL, "Reconstruct_Gauss");
+  if (rL <= 0) return;
+  const int rR = detail::recon_order(grid, p, grid->citer, nodeR, "Reconstruct_Gauss");
+  double dx[2] = {GP[0] - grid->x[nodeL * 2 + 0], GP[1] - grid->x[nodeL * 2 + 1]};
+  detail::extrapolate(grid, nodeL, rL, dx, QL);
+  dx[0] = GP[0] - grid->x[nodeR * 2 + 0];
+  dx[1] = GP[1] - grid->x[nodeR * 2 + 1];
+  detail::extrapolate(grid, nodeR, rR, dx, QR);
+}
+
+// reconstruct.C:1708-1890: the limited state of a boundary node at GP in the
+// phase of grid->citer
+template <class G, class P>
+void Reconstruct_Gauss_Boundary(int nodeL, double GP[2], double QL[4], G* grid, P p) {
+  for (int j = 0; j < 4; j++) QL[j] = grid->nQ[nodeL * 4 + j];
+  const int r = detail::recon_order(grid, p, grid->citer, nodeL, "Reconstruct_Gauss_Boundary");
+  if (r <= 0) return;
+  const double dx[2] = {GP[0] - grid->x[nodeL * 2 + 0], GP[1] - grid->x[nodeL * 2 + 1]};
+  detail::extrapolate(grid, nodeL, r, dx, QL);
+}
+
+// jacobian.C:45-185: the forward-difference flux Jacobians of one interior
+// face (flux = Roe_flux_centered(QL, QR)) added into smem->LHS
+template <class G, class S, class P>
+void compute_numerical_jacobian_roe(G* grid, S* smem, P p, int nodeL, int nodeR, double QL[4], double QR[4],
+                                    double flux[4], double nx, double ny, double len) {
+  (void)grid;
+  double dL[16], dR[16];
+  fsd::fd_jacobian(nx, ny, len, p.gamma, QL, QR, flux, dL, dR);
+  detail::warn_nonfinite(dL, "dfdqL", nodeL, nodeR);
+  detail::warn_nonfinite(dR, "dfdqR", nodeL, nodeR);
+  detail::scatter_blocks(smem, nodeL, nodeR, dL, dR);
+}
+
+// jacobian.C:301-658: the analytic approximate Jacobians of one interior
+// face, 0.5 (A(Q) +- |A_roe|) len, added into smem->LHS (flux is unused)
+template <class G, class S, class P>
+void compute_approximate_jacobian_roe(G* grid, S* smem, P p, int nodeL, int nodeR, double QL[4], double QR[4],
+                                      double flux[4], double nx, double ny, double len) {
+  (void)grid;
+  (void)flux;
+  double dL[16], dR[16];
+  fsd::analytic_jacobian(nx, ny, len, p.gamma, QL, QR, dL, dR);
+  detail::warn_nonfinite(dL, "dfdqL", nodeL, nodeR);
+  detail::warn_nonfinite(dR, "dfdqR", nodeL, nodeR);
+  detail::scatter_blocks(smem, nodeL, nodeR, dL, dR);
+}
+
+// jacobian.C:681-962: 0.5 (A(QL) + |A_roe|) len of boundary edge `ibedge`
+// (1-based) added to its real node's diagonal block (QR: the ghost state)
+template <class G, class S, class P>
+void compute_boundary_approximate_jacobian_roe(G* grid, S* smem, P p, int ibedge, double QL[4], double QR[4],
+                                               double flux[4], double nx, double ny, double len) {
+  (void)flux;
+  double d[16];
+  fsd::boundary_jacobian(nx, ny, len, p.gamma, QL, QR, d);
+  const int n = grid->bedges[ibedge * 5 + 0];
+  for (int k = 0; k < 16; k++)
+    if (!std::isfinite(d[k]))
+      std::printf("Problem DETECTED in dfdqL[%d][%d]=%f  on bedge=%d - node = %d\n", k / 4, k % 4, (float)d[k],
+                  ibedge, n);
+  for (int k = 0; k < 16; k++) smem->LHS[(size_t)smem->iau[n] * 16 + k] += d[k];
+}
+
+// cvbc.C:502-891: the ghost state Qb of boundary edge `isubedge` (1-based)
+// for the interior state Qi, with the edge's ghost in grid->nQ as the
+// previous boundary state (its memory term)
+template <class G, class P>
+void Get_Boundary_Value(G* grid, P p, int isubedge, double Qi[4], double Qb[4]) {
+  const int ghost = grid->bedges[isubedge * 5 + 1];
+  const int b = grid->bedges[isubedge * 5 + 3];
+  const int bc = grid->bbc[b];
+  const double nx = grid->xn_bedges[isubedge * 3 + 0], ny = grid->xn_bedges[isubedge * 3 + 1];
+  const double angle = p.alpha * M_PI / 180.;
+  const double uinf = p.mach * cos(angle), vinf = p.mach * sin(angle), pinf = 1. / p.gamma;
+  if (bc < 0 && bc % 2 != 0) detail::unsupported("Get_Boundary_Value", "an invalid boundary condition was set");
+  for (int j = 0; j < 4; j++) Qb[j] = grid->nQ[((size_t)grid->nn + ghost) * 4 + j];
+  if (!fsd::ghost_state(bc, nx, ny, p.gamma, uinf, vinf, pinf, Qi, Qb))
+    detail::unsupported("Get_Boundary_Value", "the eigenvalues do not correspond to a valid inflow/outflow state");
+}
+
+// jacobian.C:205-277: forward differences (eps 1e-8) of the boundary flux
+// Roe_flux_centered(QL, Get_Boundary_Value(QL)) added to the real node's
+// diagonal block; flux is the unperturbed boundary flux
+template <class G, class S, class P>
+void compute_boundary_numerical_jacobian_roe(G* grid, S* smem, P p, int ibedge, double QL[4], double flux[4],
+                                             double nx, double ny, double len) {
+  const double eps = 1.0e-08;
+  const int n = grid->bedges[ibedge * 5 + 0];
+  double d[16], Qp[4], Qb[4], fp[4];
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 4; j++) Qp[j] = QL[j];
+    Qp[i] += eps;
+    Get_Boundary_Value(grid, p, ibedge, Qp, Qb);
+    fsd::roe_flux(nx, ny, len, p.gamma, Qp, Qb, fp);
+    for (int j = 0; j < 4; j++) d[4 * j + i] = (fp[j] - flux[j]) / eps;
+  }
+  for (int k = 0; k < 16; k++)
+    if (!std::isfinite(d[k]))
+      std::printf("Problem DETECTED in dfdqL[%d][%d]=%f on bedge %d, real node = %d.\n", k / 4, k % 4, (float)d[k],
+                  ibedge, n);
+  for (int k = 0; k < 16; k++) smem->LHS[(size_t)smem->iau[n] * 16 + k] += d[k];
+}
+
+// cvbc.C:37-481: the ghost states of nQ (with their memory) on the device
+template <class G, class P>
+void Set_Boundary_Values(G* grid, P p) {
+  Twin& t = detail::twin(grid, detail::to_fs(p));
+  detail::put(t, FS_F_NQ, grid->nQ, 4, true);
+  detail::check(fs_set_boundary_values(t.solver), "Set_Boundary_Values");
+  detail::get(t, FS_F_NQ, grid->nQ, 4, true);
+}
+
+// cv_calc.C:790-965: ev_con of nQ on the device
+template <class G, class P>
+void cv_calc_eigenvalue_contribution(G* grid, P p) {
+  Twin& t = detail::twin(grid, detail::to_fs(p));
+  detail::lazy(grid->ev_con, (size_t)grid->nn + 1);
+  detail::put(t, FS_F_NQ, grid->nQ, 4, true);
+  detail::check(fs_eigenvalue_contribution(t.solver), "cv_calc_eigenvalue_contribution");
+  detail::get(t, FS_F_EVCON, grid->ev_con, 1, false);
+}
+
+namespace detail {
+// a prepared host mesh of the GRID in its current numbering: the GRID's
+// twin when it has one, else a temporary (released by the caller)
+template <class G>
+fs_mesh* prepared_mesh(G* g, const fs_params& p, bool* temporary) {
+  if (Twin* t = find_twin(g)) {
+    *temporary = false;
+    return t->mesh;
+  }
+  std::vector<int> tri, quad, bseg, boff(1, 0), bbc;
+  for (int e = 1; e <= g->num_elem[kTri]; ++e)
+    for (int k = 0; k < 3; ++k) tri.push_back(g->c2n[kTri][e * 3 + k] - 1);
+  for (int e = 1; e <= g->num_elem[kQuad]; ++e)
+    for (int k = 0; k < 4; ++k) quad.push_back(g->c2n[kQuad][e * 4 + k] - 1);
+  for (int b = 1; b <= g->nb; ++b) {
+    for (int s = 1; s <= g->nbs[b]; ++s) {
+      bseg.push_back(g->bs[b][s][0] - 1);
+      bseg.push_back(g->bs[b][s][1] - 1);
+    }
+    boff.push_back((int)bseg.size() / 2);
+    bbc.push_back(g->bbc[b]);
+  }
+  std::vector<double> x(2 * (size_t)g->nn);
+  for (int n = 0; n < g->nn; ++n) {
+    x[2 * n] = g->x[2 * (n + 1)];
+    x[2 * n + 1] = g->x[2 * (n + 1) + 1];
+  }
+  fs_mesh* m = nullptr;
+  check(fs_mesh_create(g->nn, x.data(), g->num_elem[kTri], tri.data(), g->num_elem[kQuad], quad.data(), g->nb,
+                       boff.data(), bseg.data(), bbc.data(), &m),
+        "building the host mesh");
+  fs_params q = p;
+  q.reorder = 0;
+  check(fs_mesh_prepare(m, &q), "preparing the host mesh");
+  *temporary = true;
+  return m;
+}
+inline const void* mesh_array(fs_mesh* m, int which, int64_t* count) {
+  const void* d = nullptr;
+  int dtype = 0;
+  check(fs_mesh_get_array(m, which, &d, count, &dtype), "mesh array");
+  return d;
+}
+}  // namespace detail
+
+// gradient.C:923-1121: the LSQ edge weights (4 per edge) of the GRID's
+// edges, from the host preparation (the same closed form, bit for bit)
+template <class G>
+void Compute_Edge_Weights(G* grid) {
+  fs_params p;
+  fs_params_default(&p);
+  bool temporary = false;
+  fs_mesh* m = detail::prepared_mesh(grid, p, &temporary);
+  int64_t cnt = 0;
+  const double* w = static_cast<const double*>(detail::mesh_array(m, FS_M_EDGE_WEIGHT, &cnt));
+  detail::lazy(grid->edge_weight, (size_t)cnt + 4);
+  std::memcpy(grid->edge_weight + 4, w, (size_t)cnt * sizeof(double));
+  if (temporary) fs_mesh_destroy(m);
+}
+
+// init_memory.C:36-138: the block-CSR pattern (ia, iau, ja; the diagonal
+// last in each row), allocated with malloc where null as the reference does
+template <class G, class P, class S>
+void Build_CRS(G* grid, P p, S* smem) {
+  bool temporary = false;
+  fs_mesh* m = detail::prepared_mesh(grid, detail::to_fs(p), &temporary);
+  int64_t nia = 0, niau = 0, nja = 0;
+  const int* ia = static_cast<const int*>(detail::mesh_array(m, FS_M_IA, &nia));
+  const int* iau = static_cast<const int*>(detail::mesh_array(m, FS_M_IAU, &niau));
+  const int* ja = static_cast<const int*>(detail::mesh_array(m, FS_M_JA, &nja));
+  const int nn = grid->nn;
+  if (!smem->ia) smem->ia = static_cast<int*>(std::malloc((size_t)(nn + 2) * sizeof(int)));
+  if (!smem->iau) smem->iau = static_cast<int*>(std::malloc((size_t)(nn + 1) * sizeof(int)));
+  if (!smem->ja) smem->ja = static_cast<int*>(std::malloc((size_t)nja * sizeof(int)));
+  if (!smem->ia || !smem->iau || !smem->ja) detail::unsupported("Build_CRS", "MEMORY ERROR: COULD NOT ALLOCATE");
+  // 1-based rows, 0-based slots, 1-based column nodes
+  smem->ia[0] = 0;
+  for (int n = 1; n <= nn + 1; n++) smem->ia[n] = ia[n - 1];
+  smem->iau[0] = 0;
+  for (int n = 1; n <= nn; n++) smem->iau[n] = iau[n - 1];
+  for (int64_t k = 0; k < nja; k++) smem->ja[k] = ja[k] + 1;
+  if (temporary) fs_mesh_destroy(m);
 }
 
 // releases the device twin of a GRID
