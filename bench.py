@@ -7,6 +7,8 @@ engine (bump channels and airfoil O-meshes, 5k-200k triangles log-uniform,
 M ~ U[0.3, 0.8], alpha ~ U[-2, 4] deg, k in {1, 2}; seeded), every kernel of a
 pseudo-time step one launch over all instances, mixed precision (FP64
 residual, norms and update; FP32 analytic Jacobian and Gauss-Seidel sweeps).
+Batch and serving solves run with recovery from non-physical states, an
+opt-in control the reference lacks (see WORKLOAD_CONTROLS).
 Warm-up steps walk every instance into its final phase (linear reconstruction
 for k=1, the k=2 phase for k=2); the timed steps are final-phase steps, timed
 with events on the engine's stream, which also give each kernel's time.
@@ -43,6 +45,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, os.path.join(ROOT, "flow-solver_amd"))
 
 HBM_PEAK_GBS = 8000.0
+_T0 = time.time()
+
+
+def progress(msg):
+    """A progress line on stderr with the seconds since the bench started."""
+    print("[%6.1f s] %s" % (time.time() - _T0, msg), file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes_edge_kernel(info, recon):
@@ -170,9 +178,12 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2, help="k=2-phase steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=256, help="instances of the headline batch")
-    ap.add_argument("--batch-max-steps", type=int, default=1500, help="steady batch solve limit (0: skip)")
-    ap.add_argument("--serve-seconds", type=float, default=30.0, help="length of the request stream (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=0.5, help="offered requests per second")
+    ap.add_argument("--batch-max-steps", type=int, default=2500, help="steady batch solve limit (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=55.0, help="length of the request stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="offered requests per second")
+    ap.add_argument("--serve-drain", type=float, default=100.0,
+                    help="seconds after the last arrival before unfinished requests are cancelled (counted as "
+                         "never converged)")
     ap.add_argument("--serve-max-steps", type=int, default=2000)
     ap.add_argument("--serve-precisions", default="mixed,fp64", help="streams to run, in this order")
     ap.add_argument("--no-batch-plain", action="store_true", help="skip the one-solve-at-a-time FP64 baseline")
@@ -192,10 +203,23 @@ def parse():
 
 
 # solve controls of the batch and serving workloads: implicit ramped CFL, 40
-# sweeps, first-order start-up; k=1 unlimited (the Barth-limited k=1 iteration
-# stalls short of a 1e-10 drop), k=2 with the Barth limiter (its k=2 phase
-# residual is first order, so the limiter does not move the solution)
+# sweeps, first-order start-up; k=1 unlimited (the limited k=1 iteration,
+# Barth's or Venkatakrishnan's, cycles near a 1e-1 drop), k=2 with the Barth
+# limiter (its k=2-phase residual is first order, so the limiter does not
+# move the solution).  Recovery, an opt-in control the reference does not
+# have: a solve that goes non-physical returns to its checkpoint with half
+# its CFL target and a restarted ramp (at most RECOVERY[0] times,
+# checkpointed every RECOVERY[1] steps); the first restore of a k=1 solve
+# also switches on the Barth limiter, frozen after RECOVERY[2] limited steps
+# (unlimited k=1 goes non-physical on transonic channels that choke at any
+# CFL target).  A solve that needed this fallback converges to the steady
+# state of the frozen-limiter discretization, which depends on its iterate
+# path: mixed and FP64 results are compared with and without it.
 WORKLOAD_CONTROLS = dict(CFL=200.0, ramp=30, subits=40)
+RECOVERY = (4, 10, 50)
+# serving: a request's step limit grows with the square root of its node
+# count (the steps to converge do, measured on the batch)
+SERVE_STEPS_PER_SQRT_NODE = 6.0
 # the 600k-node transonic O-mesh: at CFL 200 and 40 sweeps the FP64 path is
 # still at a 4e-8 drop after 3000 steps and the mixed path (analytic
 # approximate Jacobian) reaches a non-physical state.  CFL 1000 brings both
@@ -206,6 +230,16 @@ CONFIG3_CONTROLS = dict(CFL=1000.0, ramp=50, subits=160)
 
 def workload_params(W, cases):
     return [W.case_params(c, limit=(0 if c["k"] == 1 else 1), **WORKLOAD_CONTROLS) for c in cases]
+
+
+def workload_controls():
+    return dict(WORKLOAD_CONTROLS, tol=1e-10, limiter="k=1 none, k=2 Barth", max_recoveries=RECOVERY[0],
+                checkpoint_every=RECOVERY[1], limiter_fallback_freeze=RECOVERY[2])
+
+
+def apply_controls(b):
+    """The workloads' opt-in recovery policy on a batch."""
+    b.set_recovery(*RECOVERY)
 
 
 def outcome_groups(fsm, cases, status):
@@ -229,7 +263,7 @@ def build_batch(a):
     desc = {"instances": a.batch, "triangles": sum(i["ntri"] for i in infos), "nodes": sum(i["nn"] for i in infos),
             "subedges": sum(i["nsub"] for i in infos), "bedges": sum(i["nbe"] for i in infos),
             "edges": sum(i["ne"] for i in infos), "k2_instances": sum(1 for c in cases if c["k"] == 2),
-            "setup_s": setup, "controls": dict(WORKLOAD_CONTROLS, tol=1e-10)}
+            "setup_s": setup, "controls": workload_controls()}
     return cases, params, meshes, infos, desc
 
 
@@ -244,6 +278,7 @@ def batch_throughput(a, fsm, meshes, params, infos, prec):
     b = fsm.Batch(meshes, params, prec)
     b.use_graph(True)
     b.init_freestream()
+    apply_controls(b)
     b.step()
     tuning = b.autotune()
     for _ in range(warm - 1):
@@ -283,19 +318,27 @@ def batch_steady(a, fsm, cases, meshes, params, infos, prec):
     b = fsm.Batch(meshes, params, prec)
     b.use_graph(True)
     b.init_freestream()
+    apply_controls(b)
     t0 = time.perf_counter()
     steps, drop, status = b.solve_steady(a.batch_max_steps, 1e-10)
     wall = time.perf_counter() - t0
     conv = status == fsm.INST_CONVERGED
+    rec = np.array([b.recoveries(i)[0] for i in range(len(meshes))])
+    fallback = np.array([b.recoveries(i)[1] for i in range(len(meshes))])
     rho = {i: b.get(i, "Q")[0:4 * infos[i]["nn"]:4] for i in range(len(meshes)) if conv[i]}
+    fell = {i for i in range(len(meshes)) if fallback[i]}
     b.close()
     res = {"wall_s": wall, "converged": int(conv.sum()), "failed": int((status == fsm.INST_FAILED).sum()),
            "max_steps": int((status == fsm.INST_MAX_STEPS).sum()),
            "steps_median": float(np.median(steps[conv])) if conv.any() else None,
            "instance_steps_total": int(steps.sum()), "solves_per_s": float(conv.sum()) / wall,
+           "converged_fraction": float(conv.mean()),
+           "recovered_solves": int((rec > 0).sum()), "restores": int(rec.sum()),
+           "converged_after_restore": int((conv & (rec > 0)).sum()),
+           "limiter_fallbacks": int(fallback.sum()), "converged_with_fallback": int((conv & fallback).sum()),
            "outcomes": outcome_groups(fsm, cases, status),
            "unconverged_mach_min": min([cases[i]["mach"] for i in range(len(cases)) if not conv[i]], default=None)}
-    return res, rho, steps
+    return res, rho, steps, fell
 
 
 def plain_sequential(fsm, meshes, params, prec, warm, steps):
@@ -330,7 +373,9 @@ def plain_sequential(fsm, meshes, params, prec, warm, steps):
 def bench_serving(a, precision):
     """Poisson arrivals at a fixed offered rate; each request's mesh is built
     before the clock starts and handed to the server at its arrival time (the
-    server prepares it on its workers).  Latency = done - submit."""
+    server prepares it on its workers).  Latency = done - submit.  Requests
+    still unfinished `serve_drain` seconds after the last arrival are
+    cancelled and count as never converged."""
     import threading
     import numpy as np
     import workloads as W
@@ -345,8 +390,10 @@ def bench_serving(a, precision):
     cases = W.serving_cases(len(arrivals))
     params = workload_params(W, cases)
     meshes = [W.make_mesh(c) for c in cases]
-    print("serving: %d request meshes built" % len(meshes), file=sys.stderr, flush=True)
-    srv = fsm.Server(precision=precision, max_steps=a.serve_max_steps, tol=1e-10, prep_threads=16)
+    progress("serving: %d request meshes built" % len(meshes))
+    srv = fsm.Server(precision=precision, max_steps=a.serve_max_steps, tol=1e-10, prep_threads=16,
+                     max_recoveries=RECOVERY[0], checkpoint_every=RECOVERY[1], limiter_fallback=RECOVERY[2],
+                     max_steps_per_sqrt_node=SERVE_STEPS_PER_SQRT_NODE)
     res = []
 
     def collect():
@@ -371,20 +418,34 @@ def bench_serving(a, precision):
         now = time.perf_counter() - w0
         if now - last >= 60.0:
             last = now
-            print("serving %s: %d of %d submitted, %d finished at %.0f s" % (name, k + 1, len(cases), len(res), now),
-                  file=sys.stderr, flush=True)
+            progress("serving %s: %d of %d submitted, %d finished at %.0f s" % (name, k + 1, len(cases), len(res), now))
+    t_end = time.perf_counter() - w0
+    cancelled = 0
     while th.is_alive():
-        th.join(60.0)
+        left = a.serve_drain - (time.perf_counter() - w0 - t_end)
+        if left <= 0:
+            # the drain deadline: what is still queued or running is cancelled
+            done_ids = {r["id"] for r in list(res)}
+            for rid in rid_case:
+                if rid not in done_ids:
+                    try:
+                        srv.cancel(rid)
+                        cancelled += 1
+                    except fsm.FlowSolverError:
+                        pass   # finished (and released) meanwhile
+            th.join()
+            break
+        th.join(min(60.0, left))
         if th.is_alive():
-            print("serving %s: draining, %d of %d finished at %.0f s" %
-                  (name, len(res), len(cases), time.perf_counter() - w0), file=sys.stderr, flush=True)
+            progress("serving %s: draining, %d of %d finished at %.0f s" %
+                     (name, len(res), len(cases), time.perf_counter() - w0))
     wall = time.perf_counter() - w0
     srv.close()
     st = np.array([r["status"] for r in res])
     lat = np.array([r["t_done"] - r["t_submit"] for r in res])
     dev = np.array([r["t_done"] - r["t_admit"] for r in res])
     conv = st == fsm.INST_CONVERGED
-    print("serving %s done: %d requests in %.1f s" % (name, len(res), wall), file=sys.stderr, flush=True)
+    progress("serving %s done: %d requests in %.1f s" % (name, len(res), wall))
     q = lambda v, p: float(np.percentile(v, p)) if len(v) else None
 
     def censored(v, ok, p):
@@ -401,10 +462,16 @@ def bench_serving(a, precision):
     groups = outcome_groups(fsm, [cases[rid_case[r["id"]]] for r in res], [r["status"] for r in res])
     bad = [cases[rid_case[r["id"]]]["mach"] for r in res if r["status"] != fsm.INST_CONVERGED]
     return {"requests": len(res), "offered_rate_per_s": a.serve_rate, "stream_s": a.serve_seconds, "wall_s": wall,
+            "drain_deadline_s": a.serve_drain,
+            "cancelled_at_deadline": int((st == fsm.INST_CANCELLED).sum()),
             "outcomes": groups, "unconverged_mach_min": min(bad) if bad else None,
             "converged": int(conv.sum()), "max_steps": int((st == fsm.INST_MAX_STEPS).sum()),
             "failed": int((st == fsm.INST_FAILED).sum()),
             "converged_fraction": float(conv.mean()) if len(conv) else None,
+            "recovered_requests": int(sum(1 for r in res if r["recoveries"] > 0)),
+            "restores": int(sum(r["recoveries"] for r in res)),
+            "limiter_fallbacks": int(sum(r["limiter_fallback"] for r in res)),
+            "step_limit_max": int(max(r["max_steps"] for r in res)) if res else None,
             "p50_time_to_converged_s": censored(lat, conv, 50), "p99_time_to_converged_s": censored(lat, conv, 99),
             "p50_admit_to_converged_s": censored(dev, conv, 50), "p99_admit_to_converged_s": censored(dev, conv, 99),
             "latency_note": "percentiles over all requests, an unconverged one counted as never converging (null "
@@ -430,7 +497,7 @@ def _config3_solves(a, fsm, mesh, p, info):
         except fsm.FlowSolverError as e:
             # a non-physical state: reported, and the section goes on
             out[name] = {"failed": str(e), "wall_s": time.perf_counter() - t0}
-            print("config3 %s failed: %s" % (name, e), file=sys.stderr, flush=True)
+            progress("config3 %s failed: %s" % (name, e))
             s.close()
             continue
         wall = time.perf_counter() - t0
@@ -441,7 +508,7 @@ def _config3_solves(a, fsm, mesh, p, info):
                      "ms_per_step": t["step_ms"], "value": evals / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s",
                      "lift": float(forces[name][0]), "drag": float(forces[name][1])}
         s.close()
-        print("config3 %s: %d steps, drop %.2e, %.1f s" % (name, steps, drop, wall), file=sys.stderr, flush=True)
+        progress("config3 %s: %d steps, drop %.2e, %.1f s" % (name, steps, drop, wall))
     if len(rho) < 2:
         return out
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
@@ -701,11 +768,10 @@ def main():
         print(json.dumps(line), flush=True)
         return
     cases, params, meshes, infos, desc = build_batch(a)
-    print("batch built: %d instances, %d nodes (%.1f s)" % (len(meshes), desc["nodes"], desc["setup_s"]),
-          file=sys.stderr, flush=True)
+    progress("batch built: %d instances, %d nodes (%.1f s)" % (len(meshes), desc["nodes"], desc["setup_s"]))
     mixed = batch_throughput(a, fsm, meshes, params, infos, fsm.FS_PREC_MIXED)
     gs = mixed["gauss_seidel"]
-    print("headline (mixed batch): %.3f ms/step" % mixed["ms_per_step"], file=sys.stderr, flush=True)
+    progress("headline (mixed batch): %.3f ms/step" % mixed["ms_per_step"])
     line = {
         "metric": "edge-flux evals/s + steady-solve p50/p99 latency, mixed vs FP64, 1 GPU",
         "value": mixed["value"], "unit": "edge-flux evals/s", "n_gpus": 1, "steps": a.steps,
@@ -737,19 +803,27 @@ def main():
         f = batch_throughput(a, fsm, meshes, params, infos, fsm.FS_PREC_F64)
         line["fp64"] = f
         line["speedup_mixed_vs_fp64"] = mixed["value"] / f["value"]
-        print("fp64 batch: %.3f ms/step" % f["ms_per_step"], file=sys.stderr, flush=True)
+        progress("fp64 batch: %.3f ms/step" % f["ms_per_step"])
     rho, steps = {}, {}
     if "steady" in sections and a.batch_max_steps > 0:
-        st = {}
+        st, fell = {}, set()
         for prec, name in ((fsm.FS_PREC_MIXED, "mixed"), (fsm.FS_PREC_F64, "fp64")):
-            st[name], rho[name], steps[name] = batch_steady(a, fsm, cases, meshes, params, infos, prec)
-            print("steady %s: %d converged in %.1f s" % (name, st[name]["converged"], st[name]["wall_s"]),
-                  file=sys.stderr, flush=True)
+            st[name], rho[name], steps[name], f = batch_steady(a, fsm, cases, meshes, params, infos, prec)
+            fell |= f
+            progress("steady %s: %d converged in %.1f s" % (name, st[name]["converged"], st[name]["wall_s"]))
         both = sorted(set(rho["fp64"]) & set(rho["mixed"]))
-        diffs = [float(np.linalg.norm(rho["mixed"][i] - rho["fp64"][i]) / np.linalg.norm(rho["fp64"][i]))
-                 for i in both]
+        diff = {i: float(np.linalg.norm(rho["mixed"][i] - rho["fp64"][i]) / np.linalg.norm(rho["fp64"][i]))
+                for i in both}
+        plain_ids = [i for i in both if i not in fell]
+        fb_ids = [i for i in both if i in fell]
         st["converged_both"] = len(both)
-        st["max_density_l2_diff_mixed_vs_fp64"] = max(diffs) if diffs else None
+        # solves on their own discretization: the mixed path's error against FP64
+        st["max_density_l2_diff_mixed_vs_fp64"] = max(diff[i] for i in plain_ids) if plain_ids else None
+        st["compared"] = len(plain_ids)
+        # solves the limiter fallback touched in either precision: frozen
+        # limiter values differ between the two iterate paths
+        st["limiter_fallback_solves_compared"] = len(fb_ids)
+        st["max_density_l2_diff_limiter_fallback"] = max(diff[i] for i in fb_ids) if fb_ids else None
         st["time_to_solution_speedup"] = st["fp64"]["wall_s"] / st["mixed"]["wall_s"]
         line["steady"] = st
     if "plain" in sections:
@@ -766,24 +840,25 @@ def main():
             line["time_to_solution_speedup_vs_plain_fp64"] = t_plain / line["steady"]["mixed"]["wall_s"]
         plain.pop("ms_per_step_per_instance", None)
         line["plain_fp64_sequential"] = plain
-        print("plain fp64 done", file=sys.stderr, flush=True)
+        progress("plain fp64 done")
     del meshes
     if "config2" in sections:
         line["config2"], mesh2, p2 = bench_config2(a, fsm)
-        print("config2 done", file=sys.stderr, flush=True)
+        progress("config2 done")
     if "config3" in sections:
         line["config3"] = bench_config3(a, fsm)
     if "serving" in sections and a.serve_seconds > 0:
         precs = {"mixed": fsm.FS_PREC_MIXED, "fp64": fsm.FS_PREC_F64}
         sv = {k: bench_serving(a, precs[k]) for k in a.serve_precisions.split(",") if k}
-        sv["controls"] = dict(WORKLOAD_CONTROLS, tol=1e-10, max_steps=a.serve_max_steps)
+        sv["controls"] = dict(workload_controls(), max_steps=a.serve_max_steps,
+                              max_steps_per_sqrt_node=SERVE_STEPS_PER_SQRT_NODE)
         line["serving"] = sv
     if "cpu" in sections and not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_all_cores(mixed["warmup"] + 1)
         if "config2" in sections:
             line["cpu_baseline_config2"] = cpu_baseline(U, fsm, p2, a.nx, a.ny, a.cpu_steps)
         line["cpu_reference_calibration"] = cpu_reference_calibration(U, fsm, a.nx, a.ny)
-        print("cpu baselines done", file=sys.stderr, flush=True)
+        progress("cpu baselines done")
     print(json.dumps(line), flush=True)
 
 

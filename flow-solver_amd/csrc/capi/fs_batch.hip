@@ -29,11 +29,13 @@ struct fs_batch {
   int precision = FS_PREC_F64;
   bool graph = false;
   bool compact = false;
+  fse::RecoveryPolicy rec;
   struct Track {
     int status = FS_INST_RUNNING;
-    int steps = 0;
+    int steps = 0;            // steps taken, those a recovery rolled back included
     double r0 = -1.0, drop = 1.0;
     int err = 0;
+    fse::RecoveryTrack rt;
   };
   std::vector<Track> track;
 
@@ -66,6 +68,7 @@ int maybe_compact(fs_batch* b) {
   }
   auto N = std::make_unique<fse::Engine>();
   int rc = N->create(ms, ps, b->precision);
+  if (!rc && E.checkpoints_enabled()) rc = N->enable_checkpoints();
   if (rc) return rc;
   N->dm.tune = E.dm.tune;
   FSE_HIP(hipStreamSynchronize(E.stream));
@@ -78,10 +81,18 @@ int maybe_compact(fs_batch* b) {
     if (from.nbe > 0)
       FSE_HIP(hipMemcpyAsync(N->ds.Q + 4 * ((size_t)N->dm.nn + to.boff), E.ds.Q + 4 * ((size_t)E.dm.nn + from.boff),
                              4 * (size_t)from.nbe * sizeof(double), hipMemcpyDeviceToDevice, N->stream));
+    // limiter values, which a frozen limiter keeps from step to step
+    FSE_HIP(hipMemcpyAsync(N->ds.phi + to.noff, E.ds.phi + from.noff, (size_t)from.nn * sizeof(double),
+                           hipMemcpyDeviceToDevice, N->stream));
     to.ts = from.ts;
+    to.freeze = from.freeze;
     to.active = true;
     to.r2 = from.r2;
     to.dq2 = from.dq2;
+    if (E.checkpoints_enabled()) {
+      rc = N->adopt_checkpoint((int)k, E, b->where[live[k]].second);
+      if (rc) return rc;
+    }
     from.active = false;
     b->where[live[k]] = {ne + 1, (int)k};
   }
@@ -91,7 +102,8 @@ int maybe_compact(fs_batch* b) {
 }
 
 // one step of the running instances, then the per-instance convergence
-// bookkeeping of fs_solve_steady
+// bookkeeping of fs_solve_steady (and, with the recovery policy on, the
+// restores and checkpoints it asks for)
 int batch_step(fs_batch* b, double tol, bool stop_on_tol, bool record = false) {
   for (auto& E : b->eng) {
     bool any = false;
@@ -100,21 +112,21 @@ int batch_step(fs_batch* b, double tol, bool stop_on_tol, bool record = false) {
     int rc = E->step(true, b->graph && !record, record);
     if (rc) return rc;
   }
+  std::vector<std::vector<int>> mode(b->eng.size());
+  for (size_t e = 0; e < b->eng.size(); ++e) mode[e].assign(b->eng[e]->ninst(), 0);
   for (int i = 0; i < b->ninst(); ++i) {
     fse::Instance& I = b->inst(i);
     fs_batch::Track& T = b->track[i];
     if (!I.active) continue;
-    T.steps = I.ts;
-    if (I.err) {
+    ++T.steps;
+    const double rn = std::sqrt(I.r2);
+    // a non-finite residual without a device flag fails too, as in serving
+    const bool failed = I.err != 0 || !(rn == rn);
+    const auto& w = b->where[i];
+    if (b->eng[w.first]->recover_or_checkpoint(w.second, failed, rn, b->rec, T.rt, mode[w.first])) continue;
+    if (failed) {
       T.status = FS_INST_FAILED;
       T.err = I.err;
-      I.active = false;
-      continue;
-    }
-    double rn = std::sqrt(I.r2);
-    if (!(rn == rn)) {
-      // a non-finite residual without a device flag: failed, as in serving
-      T.status = FS_INST_FAILED;
       I.active = false;
       continue;
     }
@@ -127,6 +139,11 @@ int batch_step(fs_batch* b, double tol, bool stop_on_tol, bool record = false) {
         I.active = false;
       }
     }
+  }
+  for (size_t e = 0; e < b->eng.size(); ++e) {
+    if (!b->eng[e]->checkpoints_enabled()) continue;
+    int rc = b->eng[e]->checkpoint_pass(mode[e]);
+    if (rc) return rc;
   }
   return FS_OK;
 }
@@ -190,6 +207,32 @@ int fs_batch_use_graph(fs_batch* b, int on) {
 int fs_batch_set_compaction(fs_batch* b, int on) {
   if (!b) return fail(FS_ERR_ARG, "null batch");
   b->compact = on != 0;
+  return FS_OK;
+}
+
+int fs_batch_set_recovery(fs_batch* b, int max_recoveries, int checkpoint_every, int limiter_fallback) {
+  if (!b || max_recoveries < 0 || checkpoint_every < 1 || limiter_fallback < 0) return fail(FS_ERR_ARG, "bad arguments");
+  b->rec.max_recoveries = max_recoveries;
+  b->rec.every = checkpoint_every;
+  b->rec.limiter_fallback = limiter_fallback;
+  if (max_recoveries > 0)
+    for (auto& e : b->eng) {
+      int rc = e->enable_checkpoints();
+      if (rc) return rc;
+    }
+  return FS_OK;
+}
+
+int fs_batch_set_limiter_freeze(fs_batch* b, int steps) {
+  if (!b || steps < 0) return fail(FS_ERR_ARG, "bad arguments");
+  for (int i = 0; i < b->ninst(); ++i) b->inst(i).freeze = steps;
+  return FS_OK;
+}
+
+int fs_batch_inst_recovery(const fs_batch* b, int i, int* restores, int* limiter_fallback) {
+  if (!b || i < 0 || i >= b->ninst()) return fail(FS_ERR_ARG, "instance out of range");
+  if (restores) *restores = b->track[i].rt.recoveries;
+  if (limiter_fallback) *limiter_fallback = b->track[i].rt.limiter_fallback ? 1 : 0;
   return FS_OK;
 }
 

@@ -40,6 +40,13 @@ double cfl_for(const fs_params& p, int citer) {
   return p.CFL;
 }
 
+double cfl_for_inst(const Instance& I) {
+  if (I.cfl_scale == 1.0 && I.ramp_from == 0) return cfl_for(I.prm, I.ts);
+  fs_params q = I.prm;
+  q.CFL = std::max(1.0, I.prm.CFL * I.cfl_scale);
+  return cfl_for(q, I.ts - I.ramp_from);
+}
+
 // order 1: first-order fluxes for foits steps, then limited linear
 // reconstruction.  order 3: first order, then soits linear steps, then the
 // k=2 phase whose face states are nodal (the reference's residual at k=2 is
@@ -77,7 +84,7 @@ bool supported(const fs_params& p, std::string* why) {
   return true;
 }
 
-size_t footprint_bytes(const fs::Mesh& g, int prec) {
+size_t footprint_bytes(const fs::Mesh& g, int prec, bool checkpoints) {
   // block slots: the sliced layout pads each 16-row slice to its largest row
   size_t nn = g.nn, ng = g.ng, ne = g.ne, nsub = g.nsub, nbe = g.nbe, nnz = g.ja.size() + g.nn / 2;
   size_t ninc = g.nsub_list.size(), neinc = g.nedge_list.size(), nst = g.st_nodes.size();
@@ -93,6 +100,7 @@ size_t footprint_bytes(const fs::Mesh& g, int prec) {
   else
     b += nnz * 16 * 8 + nn * 16 * 8;
   b += (nn / fsk::kNormChunkNodes + 2) * (16 + 12);
+  if (checkpoints) b += (nn + ng) * 4 * 8 * 2 + 4;
   return b;
 }
 
@@ -154,6 +162,7 @@ Engine::~Engine() {
   if (h_err_) (void)hipHostFree(h_err_);
   if (h_fp_) (void)hipHostFree(h_fp_);
   if (h_bforce_) (void)hipHostFree(h_bforce_);
+  if (h_rmode_) (void)hipHostFree(h_rmode_);
   for (auto& b : bufs_) (void)hipFree(b.p);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -543,12 +552,17 @@ fsk::InstParams Engine::params_for(int i, int it) const {
   p.uinf = I.uinf;
   p.vinf = I.vinf;
   p.pinf = I.pinf;
-  p.cfl = cfl_for(I.prm, I.ts);
+  p.cfl = cfl_for_inst(I);
   Phase ph = phase_for(I.prm, I.ts);
   p.recon_flux = ph.recon_flux;
   p.grad = ph.gradient ? 1 : 0;
   p.k2 = ph.k2 ? 1 : 0;
   p.lim = ph.lim;
+  if (I.freeze > 0 && (ph.lim & fsk::LIM_ON)) {
+    // the limiter runs from step start + 1 (solve.C:440-471)
+    const int start = I.prm.order == 3 ? I.prm.foits + I.prm.soits : I.prm.foits;
+    if (I.ts - start > I.freeze) p.lim |= fsk::LIM_FROZEN;
+  }
   // solve.C:335-338: update on the first iteration and every jufreq-th
   p.jac = (it == 1 || (it - 1) % jufreq == 0) ? 1 : 0;
   p.active = I.active ? 1 : 0;
@@ -796,6 +810,123 @@ int Engine::check_errors(bool norms) {
 
 int Engine::sync() {
   FSE_HIP(hipStreamSynchronize(stream));
+  return FS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// recovery checkpoints
+// ---------------------------------------------------------------------------
+int Engine::enable_checkpoints() {
+  if (qsave_) return FS_OK;
+  if (dm.ng < dm.nbe) return set_error(FS_ERR_STATE, "checkpoints: fewer ghosts than boundary edges");
+  int rc = alloc(&qsave_, 4 * ((size_t)dm.nn + dm.ng));
+  if (!rc) rc = alloc(&qphase_, 4 * ((size_t)dm.nn + dm.ng));
+  if (!rc) rc = alloc(&d_rmode_, (size_t)ninst());
+  if (rc) return rc;
+  FSE_HIP(hipHostMalloc((void**)&h_rmode_, (size_t)ninst() * sizeof(int)));
+  for (auto& I : inst) I.saved_ts = I.phase_ts = -1;   // the first healthy step saves
+  return FS_OK;
+}
+
+int Engine::checkpoint_pass(const std::vector<int>& mode) {
+  if (!qsave_) return set_error(FS_ERR_STATE, "checkpoints are not enabled");
+  if ((int)mode.size() != ninst()) return set_error(FS_ERR_ARG, "checkpoint modes: one per instance");
+  bool any = false;
+  for (int m : mode) any = any || m != 0;
+  if (!any) return FS_OK;
+  // the staging buffer may still feed the previous pass's copy
+  FSE_HIP(hipStreamSynchronize(stream));
+  std::copy(mode.begin(), mode.end(), h_rmode_);
+  FSE_HIP(hipMemcpyAsync(d_rmode_, h_rmode_, (size_t)ninst() * sizeof(int), hipMemcpyHostToDevice, stream));
+  fsk::launch_checkpoint(dm, ds.Q, qsave_, qphase_, d_rmode_, stream);
+  FSE_HIP(hipGetLastError());
+  return FS_OK;
+}
+
+namespace {
+int phase_id(const fs_params& p, int ts) {
+  const Phase ph = phase_for(p, ts);
+  return ph.recon_flux | (ph.k2 ? 4 : 0) | (ph.lim << 3);
+}
+}  // namespace
+
+bool Engine::recover_or_checkpoint(int i, bool failed, double rn, const RecoveryPolicy& pol, RecoveryTrack& t,
+                                   std::vector<int>& mode) {
+  if (pol.max_recoveries <= 0 || !qsave_) return false;
+  Instance& I = inst[i];
+  if (failed) {
+    if (t.recoveries >= pol.max_recoveries) return false;
+    // the limiter fallback: an unlimited solve in a phase that would limit
+    // restarts that phase from its start with the limiter on
+    fs_params lim = I.prm;
+    lim.limit = 1;
+    const bool fallback = pol.limiter_fallback > 0 && I.prm.limit == 0 && I.phase_ts >= 0 &&
+                          (phase_for(lim, I.ts).lim & fsk::LIM_ON) && phase_id(I.prm, I.phase_ts + 1) == phase_id(I.prm, I.ts);
+    int to = -1;
+    if (fallback) {
+      mode[i] = fsk::CKPT_RESTORE_PHASE;
+      to = I.phase_ts;
+      I.prm.limit = 1;
+      I.freeze = pol.limiter_fallback;
+      t.limiter_fallback = true;
+      I.saved_ts = -1;   // the latest checkpoint is later than the restart
+    } else if (I.saved_ts >= 0) {
+      mode[i] = fsk::CKPT_RESTORE;
+      to = I.saved_ts;
+    } else if (I.phase_ts >= 0) {
+      mode[i] = fsk::CKPT_RESTORE_PHASE;
+      to = I.phase_ts;
+    } else {
+      return false;
+    }
+    ++t.recoveries;
+    I.err = 0;
+    I.ts = to;
+    I.ramp_from = to;
+    I.cfl_scale *= 0.5;
+    t.since_save = 0;
+    t.phase = -1;
+    return true;
+  }
+  const int pid = phase_id(I.prm, I.ts);
+  if (pid != t.phase) {
+    t.phase = pid;
+    t.r_best = rn;
+  }
+  ++t.since_save;
+  if ((I.saved_ts < 0 || t.since_save >= std::max(1, pol.every)) && rn <= 10.0 * t.r_best) {
+    mode[i] |= fsk::CKPT_SAVE;
+    I.saved_ts = I.ts;
+    t.since_save = 0;
+  }
+  // the last step of a phase: the state the next phase starts from
+  if (phase_id(I.prm, I.ts + 1) != pid) {
+    mode[i] |= fsk::CKPT_SAVE_PHASE;
+    I.phase_ts = I.ts;
+  }
+  t.r_best = std::min(t.r_best, rn);
+  return false;
+}
+
+int Engine::adopt_checkpoint(int i, Engine& from, int j) {
+  if (!qsave_ || !from.qsave_) return set_error(FS_ERR_STATE, "checkpoints are not enabled");
+  const Instance& F = from.inst[j];
+  Instance& T = inst[i];
+  for (int slot = 0; slot < 2; ++slot) {
+    double* dst = slot ? qphase_ : qsave_;
+    const double* src = slot ? from.qphase_ : from.qsave_;
+    FSE_HIP(hipMemcpyAsync(dst + 4 * (size_t)T.noff, src + 4 * (size_t)F.noff, 4 * (size_t)F.nn * sizeof(double),
+                           hipMemcpyDeviceToDevice, stream));
+    if (F.nbe > 0)
+      FSE_HIP(hipMemcpyAsync(dst + 4 * ((size_t)dm.nn + T.boff), src + 4 * ((size_t)from.dm.nn + F.boff),
+                             4 * (size_t)F.nbe * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  }
+  T.saved_ts = F.saved_ts;
+  T.phase_ts = F.phase_ts;
+  T.cfl_scale = F.cfl_scale;
+  T.ramp_from = F.ramp_from;
+  T.prm.limit = F.prm.limit;
+  T.freeze = F.freeze;
   return FS_OK;
 }
 

@@ -52,10 +52,48 @@ struct Instance {
   std::vector<int> slot_dev;   // reference block slot -> packed device slot
   int err = 0;                 // device error flags of the last step
   double r2 = 0, dq2 = 0;      // sum R^2 and sum dQ^2 of the last step
+  // recovery (never touched unless the caller enables it): the CFL target is
+  // scaled by cfl_scale and its ramp restarts after step ramp_from; the
+  // latest checkpoint holds the state after step saved_ts (-1: none)
+  double cfl_scale = 1.0;
+  int ramp_from = 0;
+  int saved_ts = -1;
+  int phase_ts = -1;   // the phase checkpoint holds the state after this step
+  // limiter freeze (opt-in, not in the reference): once the limited phase
+  // has run more than this many steps the limiter values are kept (0: the
+  // limiter runs every step, as the reference's)
+  int freeze = 0;
+};
+
+// CFL of instance I's next step: the reference's ramp (init_memory.C:200-216)
+// when no recovery has touched I, else the ramp restarted after step
+// ramp_from towards cfl_scale * CFL (at least 1)
+double cfl_for_inst(const Instance& I);
+
+// Recovery of steady solves from non-physical states.  It is not part of
+// the reference (which exits, cvbc.C:330-382, init_memory.C:257-268) and
+// changes the iterate path of a solve it acts on (never its discretization
+// or stopping test): off unless max_recoveries > 0.
+struct RecoveryPolicy {
+  int max_recoveries = 0;   // restores of one solve before it counts as failed
+  int every = 10;           // steps between checkpoints of a healthy solve
+  // > 0: the first failure of a solve running without a limiter in a phase
+  // that would limit returns it to the start of that phase (the phase
+  // checkpoint) with the Barth limiter switched on, frozen after this many
+  // limited steps
+  int limiter_fallback = 0;
+};
+// per-solve bookkeeping of the policy
+struct RecoveryTrack {
+  int recoveries = 0;
+  bool limiter_fallback = false;  // the fallback switched the limiter on
+  int since_save = 0;
+  int phase = -1;               // phase of the last step (a change resets r_best)
+  double r_best = 0.0;          // smallest residual norm in the current phase
 };
 
 // Device bytes that packing `m` into an engine of precision `prec` costs.
-size_t footprint_bytes(const fs::Mesh& m, int prec);
+size_t footprint_bytes(const fs::Mesh& m, int prec, bool checkpoints = false);
 
 class Engine {
  public:
@@ -113,6 +151,31 @@ class Engine {
 
   size_t device_bytes() const { return bytes_; }
 
+  // Checkpoints for recovery: allocates two saved states (4 doubles per real
+  // node and ghost each: the latest checkpoint and the start of the current
+  // phase); each instance's first checkpoint is taken after its first
+  // healthy step, its phase checkpoint after the last step of each phase.
+  int enable_checkpoints();
+  bool checkpoints_enabled() const { return qsave_ != nullptr; }
+  // After a finished step of instance i (failed: device error flag or a
+  // non-finite residual; rn: its residual norm): with the policy on, a failed
+  // step returns the instance to a checkpoint (step counter included) with
+  // half the CFL target and a restarted ramp: its latest checkpoint, or, on
+  // the limiter fallback, the start of its phase.  A healthy step is
+  // checkpointed after the first healthy step and then every `every` steps,
+  // unless its residual is above ten times the best of its phase, and the
+  // last step of a phase also saves the phase checkpoint.  Marks the copies
+  // in mode[i] (fsk::CheckpointMode bits); returns whether the instance was
+  // restored.  A failed step that is not restored (policy off, or out of
+  // recoveries) is left to the caller.
+  bool recover_or_checkpoint(int i, bool failed, double rn, const RecoveryPolicy& pol, RecoveryTrack& t,
+                             std::vector<int>& mode);
+  // enqueues the copies that mode marks (one launch over the engine)
+  int checkpoint_pass(const std::vector<int>& mode);
+  // copies instance i's checkpoint and recovery controls (and the limiter
+  // switch of the fallback) from engine `from`'s instance j (re-packing)
+  int adopt_checkpoint(int i, Engine& from, int j);
+
   // Times every candidate launch shape of the tunable kernels on this
   // engine's own data and keeps the fastest (coordinate search; the engine
   // must have run a step, and the next step recomputes everything the trial
@@ -155,6 +218,10 @@ class Engine {
   fsk::ForceParams* d_fp_ = nullptr;  // ninst
   fsk::ForceParams* h_fp_ = nullptr;  // pinned staging, ninst
   double* h_bforce_ = nullptr;        // pinned, 2 * nbe
+  double* qsave_ = nullptr;           // checkpointed Q with ghosts, 4 * (nn + ng)
+  double* qphase_ = nullptr;          // Q with ghosts at the start of the phase, 4 * (nn + ng)
+  int* d_rmode_ = nullptr;            // ninst
+  int* h_rmode_ = nullptr;            // pinned staging, ninst
   hipEvent_t ev_[8] = {};
   std::map<int, hipGraphExec_t> graphs_;
 

@@ -38,7 +38,9 @@ struct Request {
   fs_params prm{};
   double t_submit = 0, t_admit = -1, t_done = -1;
   int status = FS_INST_RUNNING;
-  int steps = 0;
+  int steps = 0;            // steps taken, those a recovery rolled back included
+  int limit = 0;            // step limit
+  fse::RecoveryTrack rt;
   double r0 = -1.0, drop = 1.0;
   int64_t bytes = 0;
   int nn = 0, ng = 0;
@@ -124,9 +126,12 @@ struct fs_server {
         finish_locked(r, r->cancel ? FS_INST_CANCELLED : FS_INST_FAILED, r->cancel ? "cancelled" : err);
         continue;
       }
-      r->bytes = (int64_t)fse::footprint_bytes(r->mesh->m, cfg.precision);
+      r->bytes = (int64_t)fse::footprint_bytes(r->mesh->m, cfg.precision, cfg.max_recoveries > 0);
       r->nn = r->mesh->m.nn;
       r->ng = r->mesh->m.ng;
+      r->limit = cfg.max_steps;
+      if (cfg.max_steps_per_sqrt_node > 0)
+        r->limit = std::max(cfg.max_steps, (int)std::ceil(cfg.max_steps_per_sqrt_node * std::sqrt((double)r->nn)));
       ready.push_back(r);
       cv_gpu.notify_all();
     }
@@ -184,6 +189,8 @@ struct fs_server {
     g->eng = std::make_unique<fse::Engine>();
     int rc = g->eng->create(ms, ps, cfg.precision);
     if (!rc) rc = g->eng->init_freestream(-1);
+    if (!rc && cfg.max_recoveries > 0) rc = g->eng->enable_checkpoints();
+    for (auto& I : g->eng->inst) I.freeze = cfg.limiter_freeze;
     // the engine's step counters start at 0: each solve runs its own schedule
     if (!rc) rc = g->eng->sync();
     double t = now();
@@ -197,50 +204,71 @@ struct fs_server {
     groups.push_back(std::move(g));
   }
 
-  // Bookkeeping after a finished step of engine g: stopping test per solve
-  // (as fs_solve_steady), final state copied out, finished solves masked off.
+  // Bookkeeping after a finished step of engine g: the recovery policy's
+  // restores and checkpoints, the stopping test per solve (as
+  // fs_solve_steady), final states copied out and the body forces of the
+  // finished solves from one pass over the engine, finished solves masked off.
   void retire_finished(Group* g) {
     fse::Engine& E = *g->eng;
+    const fse::RecoveryPolicy pol{cfg.max_recoveries, cfg.checkpoint_every, cfg.limiter_fallback};
+    std::vector<int> mode(E.ninst(), 0);
+    std::vector<int> fin_inst, fin_status;
+    std::vector<std::string> fin_err;
     for (int i = 0; i < E.ninst(); ++i) {
       fse::Instance& I = E.inst[i];
       Request* r = g->reqs[i];
       if (!I.active || !r) continue;
-      r->steps = I.ts;
+      ++r->steps;
+      const double rn = std::sqrt(I.r2);
+      const bool failed = I.err != 0 || !(rn == rn);
+      if (E.recover_or_checkpoint(i, failed, rn, pol, r->rt, mode)) continue;
       int status = FS_INST_RUNNING;
       std::string err;
-      if (I.err) {
+      if (failed) {
         status = FS_INST_FAILED;
-        err = fse::device_error_message(I.err);
+        err = I.err ? fse::device_error_message(I.err) : "non-finite residual";
       } else {
-        double rn = std::sqrt(I.r2);
         int tr = ts_ref(I.prm);
         if (I.ts == tr || (I.ts > tr && r->r0 < 0)) r->r0 = rn;
         if (r->r0 > 0) r->drop = rn / r->r0;
-        if (!(rn == rn)) {
-          status = FS_INST_FAILED;
-          err = "non-finite residual";
-        } else if (r->r0 > 0 && r->drop <= cfg.tol) {
-          status = FS_INST_CONVERGED;
-        } else if (I.ts >= cfg.max_steps) {
-          status = FS_INST_MAX_STEPS;
-        }
+        if (r->r0 > 0 && r->drop <= cfg.tol) status = FS_INST_CONVERGED;
+        else if (r->steps >= r->limit) status = FS_INST_MAX_STEPS;
       }
       if (status == FS_INST_RUNNING) continue;
+      fin_inst.push_back(i);
+      fin_status.push_back(status);
+      fin_err.push_back(err);
+    }
+    if (E.checkpoints_enabled() && E.checkpoint_pass(mode)) {
+      fail_group(g, fse::last_error_text());
+      return;
+    }
+    if (fin_inst.empty()) return;
+    // main.C:874-876: the body forces of the final states, one pass for all
+    std::vector<double> f;
+    bool forces_ok = false;
+    for (int st : fin_status) forces_ok = forces_ok || st == FS_INST_CONVERGED || st == FS_INST_MAX_STEPS;
+    if (forces_ok) {
+      std::vector<int> citer(E.ninst());
+      f.assign(2 * (size_t)E.ninst(), 0.0);
+      for (int k = 0; k < E.ninst(); ++k) citer[k] = E.inst[k].ts;
+      forces_ok = E.body_forces(citer.data(), f.data()) == FS_OK;
+    }
+    for (size_t k = 0; k < fin_inst.size(); ++k) {
+      const int i = fin_inst[k];
+      fse::Instance& I = E.inst[i];
+      Request* r = g->reqs[i];
+      int status = fin_status[k];
+      std::string err = fin_err[k];
       int64_t n = 4 * ((int64_t)I.nn + I.nbe);
       r->Q.assign((size_t)n, 0.0);
       if (E.get_field(i, FS_F_Q, r->Q.data(), n)) {
         status = FS_INST_FAILED;
         err = fse::last_error_text();
       }
-      if (status == FS_INST_CONVERGED || status == FS_INST_MAX_STEPS) {
-        // main.C:874-876: the body forces of the final state
-        std::vector<int> citer(E.ninst());
-        std::vector<double> f(2 * (size_t)E.ninst());
-        for (int k = 0; k < E.ninst(); ++k) citer[k] = E.inst[k].ts;
-        if (E.body_forces(citer.data(), f.data()) == FS_OK) {
-          r->forces[0] = f[2 * (size_t)i];
-          r->forces[1] = f[2 * (size_t)i + 1];
-        }
+      if (forces_ok && (status == FS_INST_CONVERGED || status == FS_INST_MAX_STEPS)) {
+        r->forces[0] = f[2 * (size_t)i];
+        r->forces[1] = f[2 * (size_t)i + 1];
       }
       I.active = false;
       --g->live;
@@ -329,12 +357,19 @@ void fs_server_config_default(fs_server_config* c) {
   c->tol = 1e-10;
   c->prep_threads = 8;
   c->use_graph = 1;
+  c->max_recoveries = 0;
+  c->checkpoint_every = 10;
+  c->max_steps_per_sqrt_node = 0.0;
+  c->limiter_freeze = 0;
+  c->limiter_fallback = 0;
 }
 
 int fs_server_create(const fs_server_config* c, fs_server** out) {
   if (!c || !out) return fail(FS_ERR_ARG, "null argument");
   if (c->precision != FS_PREC_F64 && c->precision != FS_PREC_MIXED) return fail(FS_ERR_ARG, "unknown precision");
-  if (c->max_steps < 1 || !(c->tol > 0) || c->prep_threads < 1 || c->device_budget < 0)
+  if (c->max_steps < 1 || !(c->tol > 0) || c->prep_threads < 1 || c->device_budget < 0 || c->max_recoveries < 0 ||
+      c->checkpoint_every < 1 || !(c->max_steps_per_sqrt_node >= 0) || c->limiter_freeze < 0 ||
+      c->limiter_fallback < 0)
     return fail(FS_ERR_ARG, "bad server configuration");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FS_ERR_HIP, "no HIP device available");
@@ -409,6 +444,9 @@ int fs_server_wait(fs_server* s, double timeout_s, fs_request_info* out, int* go
   out->ng = r->ng;
   out->forces[0] = r->forces[0];
   out->forces[1] = r->forces[1];
+  out->recoveries = r->rt.recoveries;
+  out->limiter_fallback = r->rt.limiter_fallback ? 1 : 0;
+  out->max_steps = r->limit;
   *got = 1;
   if (!r->error.empty()) fse::set_error(FS_ERR_STATE, r->error);
   return FS_OK;

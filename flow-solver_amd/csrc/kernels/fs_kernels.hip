@@ -172,7 +172,7 @@ __device__ __forceinline__ void candidate(bool venkat, double q, double qmin, do
 // sequential loops' values.  An instance without a limiter this iteration
 // gets phi = 1 (solve.C:343-347).  Mode bits (InstParams::lim): 1 limiter
 // on, 2 Gauss points of the k=2 phase, 4 Venkatakrishnan's function instead
-// of Barth's.
+// of Barth's, 8 frozen (phi kept).
 __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
   int n = tid >> 4, v = tid & 3, p = (tid >> 2) & 3;
@@ -180,6 +180,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
   if (!ip.active) return;
   const int lim = ip.lim;
+  if (lim & LIM_FROZEN) return;
   double* __restrict__ phi = s.phi;
   if (!(lim & LIM_ON)) {
     if ((tid & 15) == 0) phi[n] = 1.0;
@@ -598,6 +599,31 @@ __global__ void __launch_bounds__(kBlock) k_copy(size_t n, double* __restrict__ 
   if (i < n) dst[i] = src[i];
 }
 
+// one thread per real node or ghost; its instance's mode bits pick the copies
+__global__ void __launch_bounds__(kBlock) k_checkpoint(DevMesh m, double* __restrict__ Q, double* __restrict__ Qsave,
+                                                       double* __restrict__ Qphase, const int* __restrict__ mode) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.nn + m.nbe) return;
+  const int md = mode[t < m.nn ? inst_of(m, m.inst_node, t) : inst_of(m, m.inst_bedge, t - m.nn)];
+  if (md == 0) return;
+  const size_t o = 4 * (size_t)t;
+  if (md & (CKPT_RESTORE | CKPT_RESTORE_PHASE)) {
+    const double* __restrict__ src = ((md & CKPT_RESTORE) ? Qsave : Qphase) + o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Q[o + k] = src[k];
+    return;
+  }
+  double q[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = Q[o + k];
+  if (md & CKPT_SAVE)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Qsave[o + k] = q[k];
+  if (md & CKPT_SAVE_PHASE)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Qphase[o + k] = q[k];
+}
+
 // Deterministic per-instance sums of R^2 and dQ^2: one block per chunk of
 // kNormChunkNodes nodes (fixed tree order), then one wave per instance adds
 // its chunks, lane l taking chunks l, l+64, ... counted from the instance's
@@ -774,6 +800,10 @@ __global__ void __launch_bounds__(kBlock) k_roe_batch(int n, double gamma, const
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t st) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n, dst, src);
+}
+void launch_checkpoint(const DevMesh& m, double* Q, double* Qsave, double* Qphase, const int* mode, hipStream_t st) {
+  if (m.nn + m.nbe == 0) return;
+  hipLaunchKernelGGL(k_checkpoint, dim3(nblocks(m.nn + m.nbe)), dim3(kBlock), 0, st, m, Q, Qsave, Qphase, mode);
 }
 void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;

@@ -36,7 +36,9 @@ struct InstParams {
   int pad[2];
 };
 
-enum LimMode { LIM_ON = 1, LIM_GAUSS = 2, LIM_VENKAT = 4 };
+// LIM_FROZEN: the limiter pass leaves phi as the last pass wrote it (the
+// opt-in limiter freeze of batches and serving; never set on the parity path)
+enum LimMode { LIM_ON = 1, LIM_GAUSS = 2, LIM_VENKAT = 4, LIM_FROZEN = 8 };
 
 // Launch shapes chosen by the autotuner.  None of them changes what a
 // thread computes or the order it sums in, so every choice gives the same
@@ -199,6 +201,14 @@ void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st);
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st);
+// Per-instance checkpoints of the state with its ghosts (nn real nodes, then
+// the ghost of packed boundary edge b at nn + b), two slots: the latest
+// (Qsave) and the start of the current phase (Qphase).  Bits of mode[i]:
+// CKPT_SAVE / CKPT_SAVE_PHASE copy instance i's Q into the slot,
+// CKPT_RESTORE / CKPT_RESTORE_PHASE copy the slot back into Q (a restore
+// excludes saves in the same pass).
+enum CheckpointMode { CKPT_SAVE = 1, CKPT_RESTORE = 2, CKPT_SAVE_PHASE = 4, CKPT_RESTORE_PHASE = 8 };
+void launch_checkpoint(const DevMesh& m, double* Q, double* Qsave, double* Qphase, const int* mode, hipStream_t st);
 void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipStream_t st);
 // mixed precision
 // FP64 residual, ev_con/dt, FP32 analytic Jacobian blocks (off-diagonal and

@@ -307,7 +307,7 @@ typedef enum {
   FS_INST_RUNNING = 0,
   FS_INST_CONVERGED = 1,   /* residual drop reached the tolerance */
   FS_INST_MAX_STEPS = 2,   /* step limit reached first */
-  FS_INST_FAILED = 3,      /* non-physical state (fs_batch_inst_error) */
+  FS_INST_FAILED = 3,      /* non-physical state (after its recoveries, if enabled) */
   FS_INST_PAUSED = 4,      /* masked off by fs_batch_set_active */
   FS_INST_CANCELLED = 5    /* serving: cancelled before it finished */
 } fs_inst_status;
@@ -332,6 +332,39 @@ int fs_batch_get_tuning(const fs_batch* b, int* values, int* from_cache);
  * the 256-solve workload (measured), so it pays only for long tails of a
  * few running solves. */
 int fs_batch_set_compaction(fs_batch* b, int on);
+/* Recovery from non-physical states (off by default: max_recoveries 0; no
+ * counterpart in the reference, which exits, cvbc.C:330-382 and
+ * init_memory.C:257-268).  With max_recoveries > 0 every instance keeps two
+ * checkpoints of its state with its ghosts in device memory (64 bytes per
+ * node and ghost): the latest, taken after its first healthy step and then
+ * every checkpoint_every steps unless its residual is above ten times the
+ * best of its current phase, and the start of its current phase, taken
+ * after the last step of each phase.  A step that sets a device error flag
+ * or gives a non-finite residual returns the instance to its latest
+ * checkpoint, step counter included, with half its CFL target and the CFL
+ * ramp restarted from 1; after max_recoveries such restores the next
+ * failure is final (FS_INST_FAILED).  With limiter_fallback > 0 a failure of
+ * an instance that runs without a limiter (PARAMS.limit 0) in the phase
+ * where a limiter would run instead restarts that phase from its start with
+ * the Barth limiter switched on, frozen (fs_batch_set_limiter_freeze) after
+ * limiter_fallback limited steps: such an instance converges to the steady
+ * state of another discretization, one that depends on its iterate path.  Instances without a failure follow
+ * exactly the path they follow with recovery off.  Steps a restore rolled
+ * back still count in `steps` and against max_steps.  Set it before the
+ * first step. */
+int fs_batch_set_recovery(fs_batch* b, int max_recoveries, int checkpoint_every, int limiter_fallback);
+/* restores instance i has used; whether the limiter fallback switched its
+ * limiter on (either pointer may be NULL) */
+int fs_batch_inst_recovery(const fs_batch* b, int i, int* restores, int* limiter_fallback);
+/* Limiter freeze (off by default: steps 0; no counterpart in the reference,
+ * whose limiter runs every step, solve.C:440-471): once an instance's limited
+ * phase has run more than `steps` steps, the limiter pass keeps its values
+ * (phi) from then on.  The limited iteration then converges to the steady
+ * state of the discretization with those frozen limiter values instead of
+ * cycling; which values are frozen depends on the iterate path (precision,
+ * CFL schedule), and so does that steady state.  The stopping test is
+ * unchanged. */
+int fs_batch_set_limiter_freeze(fs_batch* b, int steps);
 /* convergence mask: a masked-off instance is skipped by every kernel */
 int fs_batch_set_active(fs_batch* b, int i, int active);
 /* one pseudo-time step of every active instance (each step counter advances);
@@ -362,16 +395,23 @@ int fs_mesh_footprint(const fs_mesh* m, int precision, int64_t* bytes);
  *      admitted solve resident in HBM, and overlaps the steps of all resident
  *      solves on the device (requests admitted together share one batched
  *      engine; each engine runs on its own stream).  A solve ends when
- *      ||R||/||R0|| <= tol (as fs_solve_steady), at max_steps, or on a
- *      non-physical state.  No counterpart in the reference. ---- */
+ *      ||R||/||R0|| <= tol (as fs_solve_steady), at its step limit, or on a
+ *      non-physical state that its recoveries (if enabled) did not cure.
+ *      No counterpart in the reference. ---- */
 typedef struct fs_server fs_server;
 typedef struct {
   int64_t device_budget;   /* bytes of device memory for resident solves (0: 80% of free) */
   int precision;           /* FS_PREC_F64 or FS_PREC_MIXED */
-  int max_steps;
+  int max_steps;           /* step limit of a request (see max_steps_per_sqrt_node) */
   double tol;
   int prep_threads;        /* host preprocessing workers */
   int use_graph;           /* one hipGraph launch per engine step */
+  int max_recoveries;      /* recovery policy of fs_batch_set_recovery (0: off) */
+  int checkpoint_every;
+  double max_steps_per_sqrt_node;  /* > 0: a request's limit is max(max_steps,
+                                      ceil(this * sqrt(real nodes))) */
+  int limiter_freeze;      /* fs_batch_set_limiter_freeze's steps (0: off) */
+  int limiter_fallback;    /* fs_batch_set_recovery's limiter_fallback (0: off) */
 } fs_server_config;
 typedef struct {
   int64_t id;
@@ -382,6 +422,9 @@ typedef struct {
   int64_t device_bytes;
   int nn, ng;
   double forces[2];        /* grid_io.C:1390 Compute_Body_Forces of the final state (lift, drag) */
+  int recoveries;          /* restores the recovery policy made */
+  int limiter_fallback;    /* the recovery policy switched the limiter on */
+  int max_steps;           /* the request's step limit */
 } fs_request_info;
 void fs_server_config_default(fs_server_config* c);
 int fs_server_create(const fs_server_config* c, fs_server** out);
