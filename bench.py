@@ -219,7 +219,7 @@ WORKLOAD_CONTROLS = dict(CFL=200.0, ramp=30, subits=40)
 RECOVERY = (4, 10, 50)
 # serving: a request's step limit grows with the square root of its node
 # count (the steps to converge do, measured on the batch)
-SERVE_STEPS_PER_SQRT_NODE = 6.0
+SERVE_STEPS_PER_SQRT_NODE = 8.0
 # the 600k-node transonic O-mesh: at CFL 200 and 40 sweeps the FP64 path is
 # still at a 4e-8 drop after 3000 steps and the mixed path (analytic
 # approximate Jacobian) reaches a non-physical state.  CFL 1000 brings both

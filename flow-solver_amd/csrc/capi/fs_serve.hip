@@ -447,6 +447,7 @@ int fs_server_wait(fs_server* s, double timeout_s, fs_request_info* out, int* go
   out->recoveries = r->rt.recoveries;
   out->limiter_fallback = r->rt.limiter_fallback ? 1 : 0;
   out->max_steps = r->limit;
+  out->r0 = r->r0;
   *got = 1;
   if (!r->error.empty()) fse::set_error(FS_ERR_STATE, r->error);
   return FS_OK;

@@ -135,7 +135,7 @@ class RequestInfo(C.Structure):
     _fields_ = [("id", C.c_int64), ("status", C.c_int), ("steps", C.c_int), ("drop", C.c_double),
                 ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
                 ("device_bytes", C.c_int64), ("nn", C.c_int), ("ng", C.c_int), ("forces", C.c_double * 2),
-                ("recoveries", C.c_int), ("limiter_fallback", C.c_int), ("max_steps", C.c_int)]
+                ("recoveries", C.c_int), ("limiter_fallback", C.c_int), ("max_steps", C.c_int), ("r0", C.c_double)]
 
 
 class Timing(C.Structure):

@@ -425,6 +425,7 @@ typedef struct {
   int recoveries;          /* restores the recovery policy made */
   int limiter_fallback;    /* the recovery policy switched the limiter on */
   int max_steps;           /* the request's step limit */
+  double r0;               /* ||R|| at the reference step: drop = ||R|| / r0 (-1 before it) */
 } fs_request_info;
 void fs_server_config_default(fs_server_config* c);
 int fs_server_create(const fs_server_config* c, fs_server** out);

@@ -2,10 +2,16 @@
 are on the device are admitted as memory frees up, their steps overlap, and
 every result equals the same solve run alone (bit for bit; the stated
 tolerance of the serving design is 1e-10 in the density L2 norm)."""
+import ctypes as C
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import fsutil as U
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "flow-solver_amd"))
 
 fsm = U.fsm
 pytestmark = pytest.mark.gpu
@@ -100,3 +106,76 @@ def test_serving_rejects_oversized_and_cancels():
     info = srv.wait(60.0)
     assert info is not None and info["status"] == fsm.INST_CANCELLED
     srv.close()
+
+
+def _oracle_residual(mesh, p, Q, ts):
+    """The oracle's residual at state Q (ghosts included) in the phase of
+    step ts: gradient, limiter and interior fluxes as its step forms them,
+    then the boundary fluxes (ghost update with memory)."""
+    o = U.Oracle(mesh, p)
+    o["Q"][:] = Q
+    o["nQ"][:] = Q
+    o.s.citer = ts
+    M, P, S = C.byref(o.m), C.byref(o.params), C.byref(o.s)
+    final = ts > (p.foits + p.soits if p.order == 3 else p.foits)
+    if p.order == 1 and ts > p.foits:
+        o.L.fso_gradient(M, S)
+    if final and p.limit == 1:
+        o.L.fso_limiter_barth(M, P, S)
+    o.L.fso_interior_residual(M, P, S, 0, ts)
+    assert o.L.fso_boundary_residual(M, P, S, 0) == 0
+    return o["R"].copy()
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED, fsm.FS_PREC_F64])
+def test_served_results_are_steady_states_of_the_oracle(precision):
+    """Requests drawn like the serving workload's (bump channels and O-meshes,
+    k=1 and k=2), one of them a 1M-triangle channel, served with the
+    workload's controls.  Every served state is a steady state of the
+    oracle's discretization: the oracle's residual there is within 2e-10 of
+    the request's reference residual norm (the stopping test is 1e-10 on the
+    state before the last update), and the FP64 device residual at the
+    served state matches the oracle's within 1e-12 of the face-flux scale."""
+    import workloads as W
+    cases = [
+        # 1227 x 409 nodes: 1,000,416 triangles
+        {"seed": 9001, "kind": "bump", "dims": (1227, 409), "mach": 0.5, "alpha": 0.0, "k": 1},
+        {"seed": 9002, "kind": "omesh", "dims": W.omesh_dims(60000), "mach": 0.45, "alpha": 2.0, "k": 1},
+        {"seed": 9003, "kind": "bump", "dims": W.bump_dims(30000), "mach": 0.6, "alpha": 0.0, "k": 2},
+        {"seed": 9004, "kind": "omesh", "dims": W.omesh_dims(120000), "mach": 0.7, "alpha": 1.25, "k": 2},
+        {"seed": 9005, "kind": "bump", "dims": W.bump_dims(15000), "mach": 0.4, "alpha": -1.0, "k": 1},
+    ]
+    params = [W.case_params(c, limit=(0 if c["k"] == 1 else 1), CFL=200.0, ramp=30, subits=40) for c in cases]
+    srv = fsm.Server(precision=precision, max_steps=2000, tol=TOL, prep_threads=8, max_recoveries=4,
+                     checkpoint_every=10, limiter_fallback=50, max_steps_per_sqrt_node=8.0)
+    ids = {}
+    for i, (c, p) in enumerate(zip(cases, params)):
+        ids[srv.submit(W.make_mesh(c), p)] = i
+    results = {}
+    while len(results) < len(ids):
+        info = srv.wait(300.0)
+        assert info is not None, "server timed out"
+        results[ids[info["id"]]] = (info, srv.state(info))
+        srv.release(info["id"])
+        print("served request %d: status %d, %d steps, drop %.2e, %d nodes" %
+              (ids[info["id"]], info["status"], info["steps"], info["drop"], info["nn"]), flush=True)
+    srv.close()
+    for i, (c, p) in enumerate(zip(cases, params)):
+        info, Q = results[i]
+        assert info["status"] == fsm.INST_CONVERGED and not info["limiter_fallback"]
+        m = W.make_mesh(c)
+        m.prepare(p)
+        ts = info["steps"] + 1
+        Ro = _oracle_residual(m, p, Q, ts)
+        rn = float(np.sqrt(np.sum(Ro * Ro)))
+        s = fsm.Solver(m, p, fsm.FS_PREC_F64)
+        s.set("Q", Q)
+        s.solve(ts)
+        rerr = U.residual_err(s.get("R"), Ro, U.flux_scale(m, Q, p.gamma))
+        s.close()
+        print("request %d (%s k=%d, %d triangles): oracle ||R|| / r0 = %.2e, device R err %.2e" %
+              (i, c["kind"], c["k"], m.info()["ntri"], rn / info["r0"], rerr), flush=True)
+        if i == 0:
+            assert m.info()["ntri"] >= 1_000_000
+        assert rn <= 2e-10 * info["r0"]
+        assert rerr <= 1e-12
