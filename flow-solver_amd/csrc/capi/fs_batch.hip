@@ -248,6 +248,15 @@ int fs_batch_get_tuning(const fs_batch* b, int* values, int* from_cache) {
   return FS_OK;
 }
 
+int fs_batch_set_tuning(fs_batch* b, const int* values) {
+  if (!b || !values) return fail(FS_ERR_ARG, "null argument");
+  for (auto& e : b->eng) {
+    int rc = e->set_tuning(values);
+    if (rc) return rc;
+  }
+  return FS_OK;
+}
+
 int fs_batch_set_active(fs_batch* b, int i, int active) {
   if (!b || i < 0 || i >= b->ninst()) return fail(FS_ERR_ARG, "instance out of range");
   fs_batch::Track& T = b->track[i];

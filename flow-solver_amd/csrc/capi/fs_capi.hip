@@ -432,6 +432,11 @@ int fs_solver_get_tuning(const fs_solver* s, int* values, int* from_cache) {
   return FS_OK;
 }
 
+int fs_solver_set_tuning(fs_solver* s, const int* values) {
+  if (!s || !values) return fail(FS_ERR_ARG, "null argument");
+  return s->eng.set_tuning(values);
+}
+
 int fs_build_matrix(fs_solver* s, int ts) {
   if (!s) return fail(FS_ERR_ARG, "null solver");
   if (int rc = need_f64(s)) return rc;

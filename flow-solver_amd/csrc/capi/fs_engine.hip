@@ -92,6 +92,8 @@ size_t footprint_bytes(const fs::Mesh& g, int prec, bool checkpoints) {
   // mesh image
   b += nn * (4 * 4 + 8 * 2 + 8 + 8 * 9 + 4 * 4 + 4 * 3) + ne * 4 * 6 + nsub * (8 * 3 + 4 * 2);
   b += ninc * (8 * 12 + 4) + neinc * (4 + 16) + nbe * (4 * 4 + 8 * 15) + nnz * 8 + (nn + 1) * 4 * 4 + nst * (4 + 40);
+  // LDS tiles of the k=2 derivatives: slots, neighbour lists, tile offsets
+  if (nst > 0) b += nst * (2 + 4) + (nn / fsk::kDerivTileNodes + 2) * 8;
   // state
   b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + nbe * 8 * 22;
   if (prec != FS_PREC_MIXED) b += ninc * 4 * 8;   // per-incidence face fluxes of the FP64 pass
@@ -109,7 +111,7 @@ bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
 bool valid_tuning(const fsk::Tuning& t) {
   return valid_block(t.gs_block) && (t.gs_lanes == 1 || t.gs_lanes == 4) && valid_block(t.gs32_block) &&
          (t.gs32_pre == 4 || t.gs32_pre == 6 || t.gs32_pre == 8) && valid_block(t.node_block) &&
-         valid_block(t.face_block) && valid_block(t.cell_block);
+         valid_block(t.face_block) && valid_block(t.cell_block) && (t.deriv_lds == 0 || t.deriv_lds == 1);
 }
 // FS_TUNE_CACHE, else $HOME/.cache/flowsolver/tune_cache.txt
 std::string tune_cache_path() {
@@ -334,6 +336,43 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   nbe_off[NN] = (int)binc;
   st_off[NN] = (int)sto;
 
+  // ---- LDS tiles of the k=2 derivative pass ----
+  // Consecutive runs of kDerivTileNodes nodes (fewer when their distinct
+  // stencil neighbours would exceed kDerivTileMaxU); every stencil entry gets
+  // the slot of its node in its tile's neighbour list.
+  std::vector<int> dt_off, dt_uoff, dt_unodes;
+  std::vector<unsigned short> st_loc;
+  if (NST > 0) {
+    st_loc.resize(NST);
+    std::vector<int> stamp(NN, -1), slot(NN, 0);
+    int tile = 0, tstart = 0, nu = 0;
+    dt_off.push_back(0);
+    dt_uoff.push_back(0);
+    for (int n = 0; n < NN; ++n) {
+      int add = 0;
+      for (int j = st_off[n]; j < st_off[n + 1]; ++j) add += stamp[st_nodes[j]] != tile;
+      if (n - tstart == fsk::kDerivTileNodes || nu + add > fsk::kDerivTileMaxU) {
+        ++tile;
+        tstart = n;
+        nu = 0;
+        dt_off.push_back(n);
+        dt_uoff.push_back((int)dt_unodes.size());
+      }
+      for (int j = st_off[n]; j < st_off[n + 1]; ++j) {
+        const int nb = st_nodes[j];
+        if (stamp[nb] != tile) {
+          stamp[nb] = tile;
+          slot[nb] = nu++;
+          dt_unodes.push_back(nb);
+        }
+        st_loc[j] = (unsigned short)slot[nb];
+      }
+    }
+    dt_off.push_back(NN);
+    dt_uoff.push_back((int)dt_unodes.size());
+  }
+  d.ndtile = dt_off.empty() ? 0 : (int)dt_off.size() - 1;
+
   // ---- rows in color order over all instances ----
   // Global color c holds the color-c rows of every instance, instance by
   // instance; rows of instances whose last color is c come after the others
@@ -460,6 +499,8 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = upload(&d.pdeg, pdeg)) || (rc = upload(&d.sbase, sbase)) ||
       (rc = upload(&d.color_rows, color_rows)) || (rc = upload(&d.st_off, st_off)) ||
       (rc = upload(&d.st_nodes, st_nodes)) || (rc = upload(&d.st_Wt, st_Wt)) || (rc = upload(&d.moments, mom)) ||
+      (rc = upload(&d.dt_off, dt_off)) || (rc = upload(&d.dt_uoff, dt_uoff)) || (rc = upload(&d.dt_unodes, dt_unodes)) ||
+      (rc = upload(&d.st_loc, st_loc)) ||
       (rc = upload(&nc.chunk_inst, ch_inst)) || (rc = upload(&nc.chunk_begin, ch_b)) ||
       (rc = upload(&nc.chunk_end, ch_e)) || (rc = upload(&nc.inst_chunk0, inst_ch0)))
     return rc;
@@ -515,7 +556,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       std::string k;
       fsk::Tuning t;
       if (ls >> k >> t.gs_block >> t.gs_lanes >> t.gs32_block >> t.gs32_pre >> t.node_block >> t.face_block >>
-          t.cell_block && k == key && valid_tuning(t)) {
+          t.cell_block >> t.deriv_lds && k == key && valid_tuning(t)) {
         dm.tune = t;
         tuned_from_cache_ = true;
       }
@@ -950,6 +991,19 @@ std::string Engine::tune_key() const {
   return k.str();
 }
 
+int Engine::set_tuning(const int* v) {
+  fsk::Tuning t;
+  t.gs_block = v[0]; t.gs_lanes = v[1]; t.gs32_block = v[2]; t.gs32_pre = v[3];
+  t.node_block = v[4]; t.face_block = v[5]; t.cell_block = v[6]; t.deriv_lds = v[7];
+  if (!valid_tuning(t)) return set_error(FS_ERR_ARG, "tuning value outside the candidates");
+  FSE_HIP(hipStreamSynchronize(stream));
+  dm.tune = t;
+  // captured graphs hold the old launch shapes
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+  return FS_OK;
+}
+
 int Engine::autotune(double* seconds) {
   if (steps_run_ < 1) return set_error(FS_ERR_STATE, "autotune needs one completed step (run a step first)");
   auto t_start = std::chrono::steady_clock::now();
@@ -1039,6 +1093,17 @@ int Engine::autotune(double* seconds) {
     if (t < best) { best = t; keepb = b; }
   }
   tu.cell_block = keepb;
+  if ((key & 2) && dm.ndtile > 0) {
+    // the k=2 derivatives: neighbour states gathered through L2 or staged in LDS
+    int keepl = tu.deriv_lds;
+    best = 1e30f;
+    for (int l : {0, 1}) {
+      tu.deriv_lds = l;
+      if ((rc = time_it([&] { fsk::launch_derivatives(dm, s, st); }, 3, &t))) return rc;
+      if (t < best) { best = t; keepl = l; }
+    }
+    tu.deriv_lds = keepl;
+  }
   // captured graphs hold the old launch shapes
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   graphs_.clear();
@@ -1054,7 +1119,7 @@ int Engine::autotune(double* seconds) {
     std::ofstream out(path, std::ios::app);
     if (out)
       out << tune_key() << " " << tu.gs_block << " " << tu.gs_lanes << " " << tu.gs32_block << " " << tu.gs32_pre << " "
-          << tu.node_block << " " << tu.face_block << " " << tu.cell_block << "\n";
+          << tu.node_block << " " << tu.face_block << " " << tu.cell_block << " " << tu.deriv_lds << "\n";
   }
   if (seconds)
     *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

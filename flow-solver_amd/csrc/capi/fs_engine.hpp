@@ -185,10 +185,13 @@ class Engine {
   int autotune(double* seconds);
   bool tuned_from_cache() const { return tuned_from_cache_; }
   int steps_run() const { return steps_run_; }
+  // sets the launch shapes (fs_capi.h's tuning value order); rejects
+  // values outside the candidates
+  int set_tuning(const int* v);
   void tuning_values(int* v) const {
     const fsk::Tuning& t = dm.tune;
     v[0] = t.gs_block; v[1] = t.gs_lanes; v[2] = t.gs32_block; v[3] = t.gs32_pre;
-    v[4] = t.node_block; v[5] = t.face_block; v[6] = t.cell_block;
+    v[4] = t.node_block; v[5] = t.face_block; v[6] = t.cell_block; v[7] = t.deriv_lds;
   }
 
   // state

@@ -138,6 +138,55 @@ __global__ void __launch_bounds__(kMaxBlock) k_derivatives(DevMesh m, DevState s
   s.pv[4 * (size_t)i + v] = qi - Mi[0] * a0 - Mi[1] * a1 - Mi[2] * a2 * 0.5 - Mi[3] * a3 * 0.5 - Mi[4] * a4;
 }
 
+// The same pass with a tile's neighbour states staged in LDS: the block of
+// tile t (kDerivTileNodes nodes, 16 lanes each) first copies the states of
+// the tile's distinct stencil nodes into LDS, then every lane runs the loop
+// above reading them by slot.  Each lane sums the same entries in the same
+// order, so the results are those of k_derivatives bit for bit.
+__global__ void __launch_bounds__(16 * kDerivTileNodes) k_derivatives_lds(DevMesh m, DevState s) {
+  __shared__ double sq[4 * kDerivTileMaxU];
+  const int t = xcd_block();
+  const double* __restrict__ nQ = s.nQ;
+  const int u0 = m.dt_uoff[t], nu = m.dt_uoff[t + 1] - u0;
+  for (int k = threadIdx.x; k < 4 * nu; k += blockDim.x) sq[k] = nQ[4 * (size_t)m.dt_unodes[u0 + (k >> 2)] + (k & 3)];
+  __syncthreads();
+  const int i = m.dt_off[t] + (threadIdx.x >> 4), v = threadIdx.x & 3, p = (threadIdx.x >> 2) & 3;
+  if (i >= m.dt_off[t + 1]) return;
+  const InstParams& ip = s.ip[inst_of(m, m.inst_node, i)];
+  if (!ip.active || !ip.k2) return;
+  const unsigned short* __restrict__ loc = m.st_loc;
+  const double* __restrict__ Wt = m.st_Wt;
+  const int o = m.st_off[i], e = m.st_off[i + 1];
+  const double qi = nQ[4 * (size_t)i + v];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+  for (int j = o + p; j < e; j += 4) {
+    const double dj = sq[4 * loc[j] + v] - qi;
+    const double* w = Wt + 5 * (size_t)j;
+    a0 += w[0] * dj;
+    a1 += w[1] * dj;
+    a2 += w[2] * dj;
+    a3 += w[3] * dj;
+    a4 += w[4] * dj;
+  }
+#pragma unroll
+  for (int off = 4; off <= 8; off <<= 1) {
+    a0 += __shfl_xor(a0, off, 16);
+    a1 += __shfl_xor(a1, off, 16);
+    a2 += __shfl_xor(a2, off, 16);
+    a3 += __shfl_xor(a3, off, 16);
+    a4 += __shfl_xor(a4, off, 16);
+  }
+  if (p != 0) return;
+  double* __restrict__ h = s.hess + 20 * (size_t)i + 5 * v;
+  h[0] = a0;
+  h[1] = a1;
+  h[2] = a2;
+  h[3] = a3;
+  h[4] = a4;
+  const double* Mi = m.moments + 9 * (size_t)i;
+  s.pv[4 * (size_t)i + v] = qi - Mi[0] * a0 - Mi[1] * a1 - Mi[2] * a2 * 0.5 - Mi[3] * a3 * 0.5 - Mi[4] * a4;
+}
+
 // ---------------------------------------------------------------------------
 // K4: Barth limiter, node-centric (limiter.C:39-497).  Min/max over the
 // edge-connected ring, then the node's own unlimited reconstruction at each
@@ -811,7 +860,11 @@ void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_derivatives, dim3(nblocks(16 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
+  if (m.tune.deriv_lds && m.ndtile > 0)
+    hipLaunchKernelGGL(k_derivatives_lds, dim3(m.ndtile), dim3(16 * kDerivTileNodes), 0, st, m, s);
+  else
+    hipLaunchKernelGGL(k_derivatives, dim3(nblocks(16 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m,
+                       s);
 }
 void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;

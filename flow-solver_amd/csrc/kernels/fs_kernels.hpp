@@ -52,7 +52,14 @@ struct Tuning {
   int node_block = 256;   // node assembly
   int face_block = 256;   // subedge / edge kernels
   int cell_block = 256;   // gradient, limiter, derivatives, diagonal factorisation
+  int deriv_lds = 1;      // k=2 derivatives: 0 neighbour states gathered through L2, 1 staged per tile in LDS
 };
+
+// LDS tiles of the k=2 derivative pass: runs of kDerivTileNodes consecutive
+// nodes; a tile's distinct stencil neighbours (at most kDerivTileMaxU) are
+// staged in LDS once and its stencil entries address them by slot
+constexpr int kDerivTileNodes = 32;
+constexpr int kDerivTileMaxU = 576;
 
 struct DevMesh {
   int nn, ng, ne, nsub, nbe, nnz, ncolor, ninst;
@@ -111,6 +118,11 @@ struct DevMesh {
   const int* st_off;        // nn+1 (k=2; empty stencils for k=1 instances)
   const int* st_nodes;
   const double* st_Wt;      // [entry][5] pseudo-inverse columns, weights folded
+  int ndtile;               // LDS tiles of the k=2 derivative pass (0: none)
+  const int* dt_off;        // ndtile+1: first node of each tile
+  const int* dt_uoff;       // ndtile+1: the tile's distinct neighbours in dt_unodes
+  const int* dt_unodes;
+  const unsigned short* st_loc;  // per stencil entry: slot of its node in the tile's list
   const double* moments;    // 9*nn
 };
 

@@ -27,7 +27,12 @@ _MESH_ARRAYS = [
     "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg", "bedge_gn", "quad",
 ]
 _FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS", "LHS32"]
-_TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block"]
+_TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block", "deriv_lds"]
+
+
+def _set_tuning(fn, h, cur, kw):
+    v = np.array([kw.get(k, cur[k]) for k in _TUNING], dtype=np.int32)
+    _check(fn(h, v.ctypes.data))
 
 
 def _tuning(fn, h):
@@ -231,6 +236,8 @@ def lib():
         "fs_solver_get_tuning": (i, [vp, vp, C.POINTER(i)]),
         "fs_batch_autotune": (i, [vp, C.POINTER(d)]),
         "fs_batch_get_tuning": (i, [vp, vp, C.POINTER(i)]),
+        "fs_solver_set_tuning": (i, [vp, vp]),
+        "fs_batch_set_tuning": (i, [vp, vp]),
         "fs_mesh_create": (i, [i, vp, i, vp, i, vp, i, vp, vp, vp, C.POINTER(vp)]),
         "fs_roe_flux_batch_form": (i, [i, i, d, vp, vp, vp, vp, vp]),
         "fs_state_norms": (i, [i64, vp, vp, vp, vp]),
@@ -500,6 +507,10 @@ class Solver:
     def tuning(self):
         return _tuning(lib().fs_solver_get_tuning, self._h)
 
+    def set_tuning(self, **kw):
+        """Set launch shapes by name (the others keep their values)."""
+        _set_tuning(lib().fs_solver_set_tuning, self._h, self.tuning(), kw)
+
 
 def _field_count(info, field):
     return {"Q": 4 * (info["nn"] + info["ng"]), "nQ": 4 * (info["nn"] + info["ng"]), "dQ": 4 * info["nn"],
@@ -557,6 +568,10 @@ class Batch:
 
     def tuning(self):
         return _tuning(lib().fs_batch_get_tuning, self._h)
+
+    def set_tuning(self, **kw):
+        """Set launch shapes by name (the others keep their values)."""
+        _set_tuning(lib().fs_batch_set_tuning, self._h, self.tuning(), kw)
 
     def set_active(self, i, on):
         _check(lib().fs_batch_set_active(self._h, i, 1 if on else 0))

@@ -290,10 +290,15 @@ int fs_solver_use_graph(fs_solver* s, int on);
  * $HOME/.cache/flowsolver/tune_cache.txt) under (arch, precision, k=2
  * present, log2 node count), and handles created later with the same key
  * start from it.  values: gs_block, gs_lanes, gs32_block, gs32_pre,
- * node_block, face_block, cell_block. */
-#define FS_TUNING_VALUES 7
+ * node_block, face_block, cell_block, deriv_lds (k=2 derivatives with the
+ * neighbour states staged in LDS per 32-node tile: 1, gathered through L2:
+ * 0). */
+#define FS_TUNING_VALUES 8
 int fs_solver_autotune(fs_solver* s, double* seconds);
 int fs_solver_get_tuning(const fs_solver* s, int* values, int* from_cache);
+/* sets the launch shapes (FS_TUNING_VALUES values in the order above; each
+ * must be one of the autotuner's candidates); results do not change */
+int fs_solver_set_tuning(fs_solver* s, const int* values);
 
 /* ---- batches: many independent steady solves (solve.C:44 each) packed
  *      CSR-style into one device image; every kernel of a pseudo-time step
@@ -323,6 +328,7 @@ int fs_batch_init_freestream(fs_batch* b);
 int fs_batch_use_graph(fs_batch* b, int on);
 int fs_batch_autotune(fs_batch* b, double* seconds);
 int fs_batch_get_tuning(const fs_batch* b, int* values, int* from_cache);
+int fs_batch_set_tuning(fs_batch* b, const int* values);
 /* Re-packing during fs_batch_solve_steady (off by default; no counterpart in
  * the reference): once at most half of the newest engine's instances still
  * run (and it holds at least 8), the running ones move to a new, smaller

@@ -287,3 +287,26 @@ def test_compaction_keeps_results():
         assert np.array_equal(a.get("Q"), bc.get(i, "Q"))
         np.testing.assert_array_equal(a.body_forces(st), forces[i])
         a.close()
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED])
+def test_derivative_lds_tiles_equal_gathers(precision):
+    """The k=2 derivatives with each 32-node tile's neighbour states staged in
+    LDS give the gather kernel's moments and point values bit for bit, over a
+    pack whose tiles span instances (k=1 instances have empty stencils)."""
+    meshes, params = _build(precision)
+    res = []
+    for lds in (0, 1):
+        b = fsm.Batch(meshes, params, precision)
+        b.set_tuning(deriv_lds=lds)
+        assert b.tuning()["deriv_lds"] == lds
+        b.init_freestream()
+        for _ in range(12):
+            b.step()
+        res.append([(b.get(i, "hess"), b.get(i, "pv"), b.get(i, "Q")) for i in range(len(meshes))])
+        b.close()
+    k2 = [i for i, p in enumerate(params) if p.order == 3]
+    assert any(np.any(res[1][i][0] != 0.0) for i in k2), "no instance reached the k=2 phase"
+    for i, (a, c) in enumerate(zip(*res)):
+        for x, y in zip(a, c):
+            assert np.array_equal(x, y), "instance %d" % i
