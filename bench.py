@@ -95,13 +95,24 @@ def gs_schedule(ncolor, subits):
     return out
 
 
-def algorithmic_bytes_gauss_seidel(mesh, subits, word):
-    """Compulsory HBM bytes of one linear solve: the diagonal factorisation
-    pass plus every color launch of the sweep schedule.  word = 8 (FP64) or 4.
-    Per launch of color c: its rows' off-diagonal blocks (16 words) and column
-    indices, the rows' factored diagonals (16 words), RHS and dQ (4 words each,
-    dQ read and written), and the neighbours' dQ once (4 words per node of the
-    other colors)."""
+def gs_row_bytes(word, scaled=0):
+    """Per-row bytes of one sweep launch besides the off-diagonal blocks, 12
+    of them row indices (position, degree, instance): FP64 (word 8) the
+    factored diagonal (16 words), RHS, dQ read and dQ written (4 words
+    each); FP32 (word 4) the diagonal inverse (16 words), RHS and dQ written,
+    or, with operands pre-multiplied by the inverse (scaled), only the
+    scaled right-hand side and dQ written."""
+    if word == 8:
+        return 16 * word + 12 * word + 12
+    return (8 * word + 12) if scaled else (24 * word + 12)
+
+
+def algorithmic_bytes_gauss_seidel(mesh, subits, word, scaled=0):
+    """Compulsory HBM bytes of one linear solve's sweeps: every color launch
+    of the sweep schedule.  word = 8 (FP64) or 4.  Per launch of color c: its
+    rows' off-diagonal blocks (16 words) and column indices, the per-row
+    operands (gs_row_bytes), and the neighbours' dQ once (4 words per node of
+    the other colors)."""
     import numpy as np
     info = mesh.info()
     nn = info["nn"]
@@ -112,13 +123,13 @@ def algorithmic_bytes_gauss_seidel(mesh, subits, word):
     for c in range(info["ncolor"]):
         r = rows[co[c]:co[c + 1]]
         nr, nb = len(r), int(deg[r].sum())
-        per_color.append(nb * (16 * word + 4) + nr * (16 * word + 12 * word + 12) + (nn - nr) * 4 * word)
+        per_color.append(nb * (16 * word + 4) + nr * gs_row_bytes(word, scaled) + (nn - nr) * 4 * word)
     sched = gs_schedule(info["ncolor"], subits)
     total = sum(per_color[c] for c in sched)
     return total, len(sched)
 
 
-def algorithmic_bytes_gs_batch(meshes, subits, word):
+def algorithmic_bytes_gs_batch(meshes, subits, word, scaled=0):
     """The same for the sweeps of a packed batch, launch by launch as the
     engine issues them: one launch per global color of the shared schedule
     (over the largest color count), each over that color's rows of every
@@ -136,7 +147,7 @@ def algorithmic_bytes_gs_batch(meshes, subits, word):
         for c in range(info["ncolor"]):
             r = rows[co[c]:co[c + 1]]
             nr, nb = len(r), int(deg[r].sum())
-            cols.append(nb * (16 * word + 4) + nr * (16 * word + 12 * word + 12) + (nn - nr) * 4 * word)
+            cols.append(nb * (16 * word + 4) + nr * gs_row_bytes(word, scaled) + (nn - nr) * 4 * word)
         per.append((info["ncolor"], cols))
     ncmax = max(nc for nc, _ in per)
     total, launches, last = 0, 0, -1
@@ -295,7 +306,7 @@ def batch_throughput(a, fsm, meshes, params, infos, prec):
                          "gauss_seidel": t["gs_ms"], "diag_factor": t["lu_ms"], "other": t["other_ms"]}}
     b.close()
     word = 4 if prec == fsm.FS_PREC_MIXED else 8
-    gs_bytes, launches = algorithmic_bytes_gs_batch(meshes, params[0].subits, word)
+    gs_bytes, launches = algorithmic_bytes_gs_batch(meshes, params[0].subits, word, tuning.get("gs32_scaled", 0))
     avg = t["gs_ms"] / launches
     out["gauss_seidel"] = {"launches_per_step": launches, "algorithmic_bytes_per_launch": gs_bytes / launches,
                            "avg_ms": avg, "achieved_GBs": gs_bytes / launches / (avg * 1e-3) / 1e9}
@@ -720,9 +731,10 @@ def bench_config2(a, fsm):
         for ts in range(1, warm + 1):
             s.solve(ts)
         t = s.time_steps(warm + 1, a.config2_steps)
+        scaled = s.tuning()["gs32_scaled"]
         s.close()
         word = 8 if prec == fsm.FS_PREC_F64 else 4
-        gs_bytes, gs_launches = algorithmic_bytes_gauss_seidel(mesh, p.subits, word)
+        gs_bytes, gs_launches = algorithmic_bytes_gauss_seidel(mesh, p.subits, word, scaled)
         r = {"value": evals / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s", "ms_per_step": t["step_ms"],
              "gauss_seidel": {"launches_per_step": gs_launches, "avg_ms": t["gs_ms"] / gs_launches,
                               "algorithmic_bytes_per_launch": gs_bytes / gs_launches,
@@ -789,7 +801,8 @@ def main():
                                                                         mixed["ms_per_step"]),
                      "achieved": gs["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gs["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("fsk::k_gs_color32<%d>" % mixed["tuning"]["gs32_pre"],
+                     "traffic": measured_traffic("fsk::k_gs_color32<%d, %s>" % (
+                         mixed["tuning"]["gs32_pre"], "true" if mixed["tuning"]["gs32_scaled"] else "false"),
                                                  "pmc_traffic_batch256_mixed.json") if a.batch == 256 else None,
                      "algorithmic_bytes": gs["algorithmic_bytes_per_launch"],
                      "launches_per_step": gs["launches_per_step"], "avg_ms": gs["avg_ms"]},

@@ -111,7 +111,8 @@ bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
 bool valid_tuning(const fsk::Tuning& t) {
   return valid_block(t.gs_block) && (t.gs_lanes == 1 || t.gs_lanes == 4) && valid_block(t.gs32_block) &&
          (t.gs32_pre == 4 || t.gs32_pre == 6 || t.gs32_pre == 8) && valid_block(t.node_block) &&
-         valid_block(t.face_block) && valid_block(t.cell_block) && (t.deriv_lds == 0 || t.deriv_lds == 1);
+         valid_block(t.face_block) && valid_block(t.cell_block) && (t.deriv_lds == 0 || t.deriv_lds == 1) &&
+         (t.gs32_scaled == 0 || t.gs32_scaled == 1);
 }
 // FS_TUNE_CACHE, else $HOME/.cache/flowsolver/tune_cache.txt
 std::string tune_cache_path() {
@@ -175,6 +176,10 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   if (prec != FS_PREC_F64 && prec != FS_PREC_MIXED) return set_error(FS_ERR_ARG, "unknown precision");
   precision = prec;
   subits = prm[0].subits;
+  // FP32 sweeps on operands pre-multiplied by the diagonal inverse when a
+  // linear solve runs enough sweeps to pay for the pass that forms them
+  // (measured: it pays at 40 and 160 sweeps, costs at 20)
+  dm.tune.gs32_scaled = subits >= 30 ? 1 : 0;
   iter = prm[0].iter;
   jufreq = prm[0].jufreq;
   for (int i = 0; i < B; ++i) {
@@ -556,7 +561,10 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       std::string k;
       fsk::Tuning t;
       if (ls >> k >> t.gs_block >> t.gs_lanes >> t.gs32_block >> t.gs32_pre >> t.node_block >> t.face_block >>
-          t.cell_block >> t.deriv_lds && k == key && valid_tuning(t)) {
+          t.cell_block >> t.deriv_lds && k == key) {
+        // the sweep form is the engine's own (it changes FP32 rounding)
+        t.gs32_scaled = dm.tune.gs32_scaled;
+        if (!valid_tuning(t)) continue;
         dm.tune = t;
         tuned_from_cache_ = true;
       }
@@ -994,7 +1002,7 @@ std::string Engine::tune_key() const {
 int Engine::set_tuning(const int* v) {
   fsk::Tuning t;
   t.gs_block = v[0]; t.gs_lanes = v[1]; t.gs32_block = v[2]; t.gs32_pre = v[3];
-  t.node_block = v[4]; t.face_block = v[5]; t.cell_block = v[6]; t.deriv_lds = v[7];
+  t.node_block = v[4]; t.face_block = v[5]; t.cell_block = v[6]; t.deriv_lds = v[7]; t.gs32_scaled = v[8];
   if (!valid_tuning(t)) return set_error(FS_ERR_ARG, "tuning value outside the candidates");
   FSE_HIP(hipStreamSynchronize(stream));
   dm.tune = t;
@@ -1080,6 +1088,10 @@ int Engine::autotune(double* seconds) {
   }
   keepb = tu.cell_block;
   best = 1e30f;
+  // the diagonal pass is timed in its inverse-only form: the pre-multiplied
+  // form would scale the blocks again at every trial
+  const int scaled = tu.gs32_scaled;
+  tu.gs32_scaled = 0;
   for (int b : blocks) {
     tu.cell_block = b;
     rc = time_it([&] {
@@ -1089,9 +1101,13 @@ int Engine::autotune(double* seconds) {
       if (precision == FS_PREC_MIXED) fsk::launch_diag_lu32(dm, s, st);
       else fsk::launch_diag_lu(dm, s, st);
     }, 2, &t);
-    if (rc) return rc;
+    if (rc) {
+      tu.gs32_scaled = scaled;
+      return rc;
+    }
     if (t < best) { best = t; keepb = b; }
   }
+  tu.gs32_scaled = scaled;
   tu.cell_block = keepb;
   if ((key & 2) && dm.ndtile > 0) {
     // the k=2 derivatives: neighbour states gathered through L2 or staged in LDS

@@ -53,6 +53,7 @@ struct Tuning {
   int face_block = 256;   // subedge / edge kernels
   int cell_block = 256;   // gradient, limiter, derivatives, diagonal factorisation
   int deriv_lds = 1;      // k=2 derivatives: 0 neighbour states gathered through L2, 1 staged per tile in LDS
+  int gs32_scaled = 1;    // FP32 sweeps: 0 apply the stored diagonal inverse per row, 1 operands pre-multiplied by it
 };
 
 // LDS tiles of the k=2 derivative pass: runs of kDerivTileNodes consecutive
@@ -148,7 +149,9 @@ struct DevState {
   // mixed precision (FP32 Jacobian blocks and sweeps)
   float* LHS32;      // nnz*16, color-major slots
   float* LU32;       // nn*16, color position: inverse of the diagonal block (pivoted, FP64, stored FP32)
-  float* RHS32;      // nn*4
+  float* RHS32;      // nn*4: the FP32 right-hand side; with pre-multiplied sweep operands, after the
+                     // diagonal pass, its product with the row's diagonal inverse (and the off-diagonal
+                     // blocks of LHS32 likewise)
   float* dQ32;       // nn*4
   const InstParams* ip;   // ninst, controls of the current Newton iteration
   double* inst_norms;     // 2 per instance: sum R^2, sum dQ^2

@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s
 // flux Jacobian's (0,0) entry is zero); the pivoted FP64 inverse is accurate
 // to FP32 rounding of its entries.  Row selection uses compile-time indices
 // only, so the augmented matrix stays in registers.
-__device__ __forceinline__ void inv4_pivoted(const float* A, float* out) {
+__device__ __forceinline__ void inv4_pivoted(const float* A, double* out) {
   double a[4][8];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -306,35 +306,86 @@ __device__ __forceinline__ void inv4_pivoted(const float* A, float* out) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) out[4 * i + j] = (float)a[i][4 + j];
+    for (int j = 0; j < 4; ++j) out[4 * i + j] = a[i][4 + j];
 }
 
+// One thread per row (color position t): the diagonal block's inverse Di
+// (pivoted, FP64).  Unscaled form (kScale false): Di is stored in FP32 for
+// the sweep to apply.  Scaled form: the row's sweep operands are
+// pre-multiplied by it in place instead, each rounded once to FP32 from
+// FP64 products: every off-diagonal block A -> Di A and the right-hand side
+// b -> Di b, so a sweep needs no diagonal: dQ_row = Di b - sum (Di A) dQ_nb.
+// The node pass rewrites every block and right-hand side in every Newton
+// iteration, so the scaling always starts from fresh operands; the diagonal
+// block itself is left as it is.  Both forms zero the row's dQ, the sweeps'
+// starting guess.
+template <bool kScale>
 __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstParams* __restrict__ ip,
-                                                      const float* __restrict__ LHS32, float* __restrict__ LU32,
-                                                      float* __restrict__ dQ32) {
+                                                      float* __restrict__ LHS32, float* __restrict__ LU32,
+                                                      float* __restrict__ RHS32, float* __restrict__ dQ32) {
   int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
-  float A[16], Ai[16];
+  float A[16];
+  double Di[16];
   const float4* D = reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)m.iau[n]);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     float4 r = D[k];
     A[4 * k] = r.x; A[4 * k + 1] = r.y; A[4 * k + 2] = r.z; A[4 * k + 3] = r.w;
   }
-  inv4_pivoted(A, Ai);
-  float4* o = reinterpret_cast<float4*>(LU32 + 16 * (size_t)t);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) o[k] = make_float4(Ai[4 * k], Ai[4 * k + 1], Ai[4 * k + 2], Ai[4 * k + 3]);
+  inv4_pivoted(A, Di);
   *reinterpret_cast<float4*>(dQ32 + 4 * (size_t)n) = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (!kScale) {
+    float4* o = reinterpret_cast<float4*>(LU32 + 16 * (size_t)t);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      o[k] = make_float4((float)Di[4 * k], (float)Di[4 * k + 1], (float)Di[4 * k + 2], (float)Di[4 * k + 3]);
+    return;
+  }
+  const int deg = m.pdeg[t];
+  const size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
+  for (int e = 0; e < deg; ++e) {
+    const size_t k = base + 16 * (size_t)e;
+    const float4* B = reinterpret_cast<const float4*>(LHS32 + 16 * k);
+    float b[16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float4 q = B[r];
+      b[4 * r] = q.x; b[4 * r + 1] = q.y; b[4 * r + 2] = q.z; b[4 * r + 3] = q.w;
+    }
+    float4* O = reinterpret_cast<float4*>(LHS32 + 16 * k);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double d = Di[4 * r + i];
+        c0 += d * b[4 * i];
+        c1 += d * b[4 * i + 1];
+        c2 += d * b[4 * i + 2];
+        c3 += d * b[4 * i + 3];
+      }
+      O[r] = make_float4((float)c0, (float)c1, (float)c2, (float)c3);
+    }
+  }
+  float4* R = reinterpret_cast<float4*>(RHS32 + 4 * (size_t)n);
+  const float4 rb = *R;
+  double c[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) c[r] = Di[4 * r] * rb.x + Di[4 * r + 1] * rb.y + Di[4 * r + 2] * rb.z + Di[4 * r + 3] * rb.w;
+  *R = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]);
 }
 
 // FP32 multicolor sweep over rows [row_begin,row_end) of the color order;
 // 4 lanes per row (lane v: component v), quad-wide FP32 block rows of the
 // sliced layout; the row's index, block and neighbour loads are issued
-// before the arithmetic.
-template <int kPre>
+// before the arithmetic.  Unscaled form: r = b - sum A dQ_nb, then lane v
+// applies row v of the stored diagonal inverse to the shuffled r.  Scaled
+// form (kScale): the blocks and right-hand side already carry the inverse
+// (k_diag_lu32<true>), so dQ_row[v] = c[v] - sum (Di A)[v,:] dQ_nb.
+template <int kPre, bool kScale>
 __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const InstParams* __restrict__ ip,
                                                        const float* __restrict__ LHS32, const float* __restrict__ LU32,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
@@ -368,6 +419,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const InstP
     float4 b = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)m.ja[k]);
     r -= a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
   }
+  if (kScale) {
+    dQ32[4 * (size_t)j + v] = r;
+    return;
+  }
   float b0 = __shfl(r, 0, 4), b1 = __shfl(r, 1, 4), b2 = __shfl(r, 2, 4), b3 = __shfl(r, 3, 4);
   // lane v: row v of the diagonal block's inverse
   const float4 Di = *reinterpret_cast<const float4*>(LU32 + 16 * (size_t)t + 4 * v);
@@ -392,9 +447,24 @@ void launch_resid_jac32(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_diag_lu32, dim3(nblocks(m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s.ip,
-                     s.LHS32, s.LU32, s.dQ32);
+  const dim3 grid(nblocks(m.nn, m.tune.cell_block)), block(m.tune.cell_block);
+  if (m.tune.gs32_scaled)
+    hipLaunchKernelGGL(k_diag_lu32<true>, grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32);
+  else
+    hipLaunchKernelGGL(k_diag_lu32<false>, grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32);
 }
+
+namespace {
+template <int kPre>
+void gs32_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
+  if (m.tune.gs32_scaled)
+    hipLaunchKernelGGL((k_gs_color32<kPre, true>), grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
+                       e);
+  else
+    hipLaunchKernelGGL((k_gs_color32<kPre, false>), grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
+                       e);
+}
+}  // namespace
 
 void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st) {
   std::vector<GsLaunch> sched((size_t)subits * m.ncolor + 1);
@@ -406,11 +476,11 @@ void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& 
     const int bs = m.tune.gs32_block;
     const dim3 grid(nblocks(4 * (e - b), bs));
     if (m.tune.gs32_pre <= 4)
-      hipLaunchKernelGGL(k_gs_color32<4>, grid, dim3(bs), 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b, e);
+      gs32_launch<4>(m, s, grid, dim3(bs), b, e, st);
     else if (m.tune.gs32_pre <= 6)
-      hipLaunchKernelGGL(k_gs_color32<6>, grid, dim3(bs), 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b, e);
+      gs32_launch<6>(m, s, grid, dim3(bs), b, e, st);
     else
-      hipLaunchKernelGGL(k_gs_color32<8>, grid, dim3(bs), 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b, e);
+      gs32_launch<8>(m, s, grid, dim3(bs), b, e, st);
   }
 }
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {

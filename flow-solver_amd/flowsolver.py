@@ -27,7 +27,8 @@ _MESH_ARRAYS = [
     "ja", "color", "color_off", "color_rows", "st_off", "st_nodes", "st_W", "bbc", "bseg_off", "bseg", "bedge_gn", "quad",
 ]
 _FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS", "LHS32"]
-_TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block", "deriv_lds"]
+_TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block", "deriv_lds",
+           "gs32_scaled"]
 
 
 def _set_tuning(fn, h, cur, kw):

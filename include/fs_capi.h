@@ -189,7 +189,9 @@ typedef enum {
   FS_F_EVCON,   /* f64 nn */
   FS_F_DT,      /* f64 nn */
   FS_F_LHS,     /* f64 nnz*16 row-major 4x4 blocks */
-  FS_F_LHS32,   /* mixed precision only: the FP32 blocks, widened to f64, nnz*16 */
+  FS_F_LHS32,   /* mixed precision only: the FP32 blocks, widened to f64, nnz*16 (with the
+                   pre-multiplied sweep form, PARAMS.subits >= 30, the off-diagonal blocks
+                   after a step are those products, as the sweeps read them) */
   FS_F_COUNT
 } fs_field;
 /* host <-> device copies (synchronous); n is the element count */
@@ -283,17 +285,22 @@ int fs_solver_time_steps(fs_solver* s, int ts_first, int nsteps, fs_timing* t);
 int fs_solver_use_graph(fs_solver* s, int on);
 
 /* Launch-shape autotuning (no counterpart in the reference).  Times every
- * candidate workgroup size / lanes-per-row / sweep prefetch depth of the
- * tunable kernels on the handle's own data (after at least one step) and
- * keeps the fastest; none of the choices changes any result.  The choice is
+ * candidate workgroup size / lanes-per-row / sweep prefetch depth / kernel
+ * form of the tunable kernels on the handle's own data (after at least one
+ * step) and keeps the fastest; none of the choices changes any result.  The choice is
  * appended to the cache file (FS_TUNE_CACHE, default
  * $HOME/.cache/flowsolver/tune_cache.txt) under (arch, precision, k=2
  * present, log2 node count), and handles created later with the same key
  * start from it.  values: gs_block, gs_lanes, gs32_block, gs32_pre,
  * node_block, face_block, cell_block, deriv_lds (k=2 derivatives with the
  * neighbour states staged in LDS per 32-node tile: 1, gathered through L2:
- * 0). */
-#define FS_TUNING_VALUES 8
+ * 0), gs32_scaled (FP32 sweeps on blocks and right-hand sides pre-multiplied
+ * by the diagonal inverse once per Newton iteration: 1, applying the stored
+ * inverse in every sweep: 0).  gs32_scaled is not tuned or cached: it is 1
+ * when PARAMS.subits >= 30 and can be pinned with the setter; the two forms
+ * round differently in FP32, so it moves the mixed path's iterates by FP32
+ * rounding, never its converged state. */
+#define FS_TUNING_VALUES 9
 int fs_solver_autotune(fs_solver* s, double* seconds);
 int fs_solver_get_tuning(const fs_solver* s, int* values, int* from_cache);
 /* sets the launch shapes (FS_TUNING_VALUES values in the order above; each

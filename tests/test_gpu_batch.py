@@ -173,14 +173,18 @@ def test_restart_continuation_bitexact(tmp_path):
 @pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED])
 def test_autotune_keeps_results_and_caches(precision, tmp_path, monkeypatch):
     """Tuned launch shapes give the same states bit for bit; the choice is
-    written to the cache and picked up by the next engine with the same key."""
+    written to the cache and picked up by the next engine with the same key.
+    The FP32 sweep's form (the one launch choice that changes FP32 rounding)
+    is neither tuned nor cached: pinned either way, the tuned run keeps it."""
     cache = tmp_path / "tune_cache.txt"
     monkeypatch.setenv("FS_TUNE_CACHE", str(cache))
     meshes, params = _build(precision)
 
-    def run(tune):
+    def run(tune, scaled=None):
         b = fsm.Batch(meshes, params, precision)
         b.use_graph(True)
+        if scaled is not None:
+            b.set_tuning(gs32_scaled=scaled)
         b.init_freestream()
         b.step()
         info = b.autotune() if tune else b.tuning()
@@ -190,13 +194,22 @@ def test_autotune_keeps_results_and_caches(precision, tmp_path, monkeypatch):
         b.close()
         return q, info
 
-    q0, t0 = run(False)
+    q0 = {sc: run(False, sc)[0] for sc in (0, 1)}
     assert not cache.exists() or cache.read_text() == ""
-    q1, t1 = run(True)
+    q1, t1 = run(True, 0)
     print("tuned shapes", t1, flush=True)
+    assert t1["gs32_scaled"] == 0
     assert cache.exists() and len(cache.read_text().splitlines()) == 1
-    for a, b in zip(q0, q1):
+    for a, b in zip(q0[0], q1):
         assert np.array_equal(a, b)
+    if precision == fsm.FS_PREC_F64:
+        # the FP64 path has no FP32 sweep: the knob is inert
+        for a, b in zip(q0[0], q0[1]):
+            assert np.array_equal(a, b)
+    else:
+        # the two forms round differently but solve the same system
+        for a, b in zip(q0[0], q0[1]):
+            assert np.max(np.abs(a - b)) <= 1e-4 * np.max(np.abs(a))
     b = fsm.Batch(meshes, params, precision)
     t2 = b.tuning()
     b.close()
