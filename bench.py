@@ -220,14 +220,15 @@ def parse():
 # move the solution).  Recovery, an opt-in control the reference does not
 # have: a solve that goes non-physical returns to its checkpoint with half
 # its CFL target and a restarted ramp (at most RECOVERY[0] times,
-# checkpointed every RECOVERY[1] steps); the first restore of a k=1 solve
-# also switches on the Barth limiter, frozen after RECOVERY[2] limited steps
-# (unlimited k=1 goes non-physical on transonic channels that choke at any
-# CFL target).  A solve that needed this fallback converges to the steady
-# state of the frozen-limiter discretization, which depends on its iterate
-# path: mixed and FP64 results are compared with and without it.
+# checkpointed every RECOVERY[1] steps); the first failure of a k=1 solve
+# instead restarts its linear phase with the Barth limiter, frozen after
+# RECOVERY[2] limited steps, and the next one restarts it at first order
+# (RECOVERY[3]); unlimited k=1 goes non-physical on transonic channels that
+# choke, at any CFL target.  A solve a fallback acted on converges to the
+# steady state of another discretization (the frozen-limiter one depends on
+# its iterate path): mixed and FP64 results are compared apart from it.
 WORKLOAD_CONTROLS = dict(CFL=200.0, ramp=30, subits=40)
-RECOVERY = (4, 10, 50)
+RECOVERY = (4, 10, 50, True)
 # serving: a request's step limit grows with the square root of its node
 # count (the steps to converge do, measured on the batch)
 SERVE_STEPS_PER_SQRT_NODE = 8.0
@@ -245,7 +246,7 @@ def workload_params(W, cases):
 
 def workload_controls():
     return dict(WORKLOAD_CONTROLS, tol=1e-10, limiter="k=1 none, k=2 Barth", max_recoveries=RECOVERY[0],
-                checkpoint_every=RECOVERY[1], limiter_fallback_freeze=RECOVERY[2])
+                checkpoint_every=RECOVERY[1], limiter_fallback_freeze=RECOVERY[2], first_order_fallback=RECOVERY[3])
 
 
 def apply_controls(b):
@@ -257,8 +258,10 @@ def outcome_groups(fsm, cases, status):
     """Solve outcomes by mesh kind and reconstruction order."""
     groups = {}
     for c, st in zip(cases, status):
-        g = groups.setdefault("%s_k%d" % (c["kind"], c["k"]), {"converged": 0, "max_steps": 0, "failed": 0})
-        g[{fsm.INST_CONVERGED: "converged", fsm.INST_MAX_STEPS: "max_steps"}.get(int(st), "failed")] += 1
+        g = groups.setdefault("%s_k%d" % (c["kind"], c["k"]), {"converged": 0, "max_steps": 0, "failed": 0,
+                                                                "cancelled": 0})
+        g[{fsm.INST_CONVERGED: "converged", fsm.INST_MAX_STEPS: "max_steps",
+           fsm.INST_CANCELLED: "cancelled"}.get(int(st), "failed")] += 1
     return groups
 
 
@@ -335,7 +338,8 @@ def batch_steady(a, fsm, cases, meshes, params, infos, prec):
     wall = time.perf_counter() - t0
     conv = status == fsm.INST_CONVERGED
     rec = np.array([b.recoveries(i)[0] for i in range(len(meshes))])
-    fallback = np.array([b.recoveries(i)[1] for i in range(len(meshes))])
+    fbits = np.array([b.recoveries(i)[1] for i in range(len(meshes))])
+    fallback = fbits != 0
     rho = {i: b.get(i, "Q")[0:4 * infos[i]["nn"]:4] for i in range(len(meshes)) if conv[i]}
     fell = {i for i in range(len(meshes)) if fallback[i]}
     b.close()
@@ -346,7 +350,9 @@ def batch_steady(a, fsm, cases, meshes, params, infos, prec):
            "converged_fraction": float(conv.mean()),
            "recovered_solves": int((rec > 0).sum()), "restores": int(rec.sum()),
            "converged_after_restore": int((conv & (rec > 0)).sum()),
-           "limiter_fallbacks": int(fallback.sum()), "converged_with_fallback": int((conv & fallback).sum()),
+           "limiter_fallbacks": int(((fbits & fsm.FALLBACK_LIMITER) != 0).sum()),
+           "first_order_fallbacks": int(((fbits & fsm.FALLBACK_FIRST_ORDER) != 0).sum()),
+           "converged_with_fallback": int((conv & fallback).sum()),
            "outcomes": outcome_groups(fsm, cases, status),
            "unconverged_mach_min": min([cases[i]["mach"] for i in range(len(cases)) if not conv[i]], default=None)}
     return res, rho, steps, fell
@@ -404,6 +410,7 @@ def bench_serving(a, precision):
     progress("serving: %d request meshes built" % len(meshes))
     srv = fsm.Server(precision=precision, max_steps=a.serve_max_steps, tol=1e-10, prep_threads=16,
                      max_recoveries=RECOVERY[0], checkpoint_every=RECOVERY[1], limiter_fallback=RECOVERY[2],
+                     first_order_fallback=RECOVERY[3],
                      max_steps_per_sqrt_node=SERVE_STEPS_PER_SQRT_NODE)
     res = []
 
@@ -479,9 +486,11 @@ def bench_serving(a, precision):
             "converged": int(conv.sum()), "max_steps": int((st == fsm.INST_MAX_STEPS).sum()),
             "failed": int((st == fsm.INST_FAILED).sum()),
             "converged_fraction": float(conv.mean()) if len(conv) else None,
+            "converged_fraction_of_finished": float(conv.sum()) / max(1, int((st != fsm.INST_CANCELLED).sum())),
             "recovered_requests": int(sum(1 for r in res if r["recoveries"] > 0)),
             "restores": int(sum(r["recoveries"] for r in res)),
-            "limiter_fallbacks": int(sum(r["limiter_fallback"] for r in res)),
+            "limiter_fallbacks": int(sum(1 for r in res if r["fallbacks"] & fsm.FALLBACK_LIMITER)),
+            "first_order_fallbacks": int(sum(1 for r in res if r["fallbacks"] & fsm.FALLBACK_FIRST_ORDER)),
             "step_limit_max": int(max(r["max_steps"] for r in res)) if res else None,
             "p50_time_to_converged_s": censored(lat, conv, 50), "p99_time_to_converged_s": censored(lat, conv, 99),
             "p50_admit_to_converged_s": censored(dev, conv, 50), "p99_admit_to_converged_s": censored(dev, conv, 99),
@@ -835,8 +844,8 @@ def main():
         st["compared"] = len(plain_ids)
         # solves the limiter fallback touched in either precision: frozen
         # limiter values differ between the two iterate paths
-        st["limiter_fallback_solves_compared"] = len(fb_ids)
-        st["max_density_l2_diff_limiter_fallback"] = max(diff[i] for i in fb_ids) if fb_ids else None
+        st["fallback_solves_compared"] = len(fb_ids)
+        st["max_density_l2_diff_fallback"] = max(diff[i] for i in fb_ids) if fb_ids else None
         st["time_to_solution_speedup"] = st["fp64"]["wall_s"] / st["mixed"]["wall_s"]
         line["steady"] = st
     if "plain" in sections:

@@ -130,7 +130,9 @@ int batch_step(fs_batch* b, double tol, bool stop_on_tol, bool record = false) {
       I.active = false;
       continue;
     }
-    int tr = ts_ref(I.prm);
+    // the reference step of the solve as submitted (a fallback may have
+    // changed the engine's copy of its controls)
+    int tr = ts_ref(b->params[i]);
     if (I.ts == tr || (I.ts > tr && T.r0 < 0)) T.r0 = rn;
     if (T.r0 > 0) {
       T.drop = rn / T.r0;
@@ -210,11 +212,13 @@ int fs_batch_set_compaction(fs_batch* b, int on) {
   return FS_OK;
 }
 
-int fs_batch_set_recovery(fs_batch* b, int max_recoveries, int checkpoint_every, int limiter_fallback) {
+int fs_batch_set_recovery(fs_batch* b, int max_recoveries, int checkpoint_every, int limiter_fallback,
+                          int first_order_fallback) {
   if (!b || max_recoveries < 0 || checkpoint_every < 1 || limiter_fallback < 0) return fail(FS_ERR_ARG, "bad arguments");
   b->rec.max_recoveries = max_recoveries;
   b->rec.every = checkpoint_every;
   b->rec.limiter_fallback = limiter_fallback;
+  b->rec.first_order_fallback = first_order_fallback != 0;
   if (max_recoveries > 0)
     for (auto& e : b->eng) {
       int rc = e->enable_checkpoints();
@@ -229,10 +233,10 @@ int fs_batch_set_limiter_freeze(fs_batch* b, int steps) {
   return FS_OK;
 }
 
-int fs_batch_inst_recovery(const fs_batch* b, int i, int* restores, int* limiter_fallback) {
+int fs_batch_inst_recovery(const fs_batch* b, int i, int* restores, int* fallbacks) {
   if (!b || i < 0 || i >= b->ninst()) return fail(FS_ERR_ARG, "instance out of range");
   if (restores) *restores = b->track[i].rt.recoveries;
-  if (limiter_fallback) *limiter_fallback = b->track[i].rt.limiter_fallback ? 1 : 0;
+  if (fallbacks) *fallbacks = b->track[i].rt.fallbacks;
   return FS_OK;
 }
 

@@ -905,20 +905,34 @@ bool Engine::recover_or_checkpoint(int i, bool failed, double rn, const Recovery
   Instance& I = inst[i];
   if (failed) {
     if (t.recoveries >= pol.max_recoveries) return false;
+    // the fallbacks restart the current phase from its start (the phase
+    // checkpoint holds the state the phase began from)
+    const bool at_phase = I.phase_ts >= 0 && phase_id(I.prm, I.phase_ts + 1) == phase_id(I.prm, I.ts);
+    const Phase ph = phase_for(I.prm, I.ts);
     // the limiter fallback: an unlimited solve in a phase that would limit
-    // restarts that phase from its start with the limiter on
     fs_params lim = I.prm;
     lim.limit = 1;
-    const bool fallback = pol.limiter_fallback > 0 && I.prm.limit == 0 && I.phase_ts >= 0 &&
-                          (phase_for(lim, I.ts).lim & fsk::LIM_ON) && phase_id(I.prm, I.phase_ts + 1) == phase_id(I.prm, I.ts);
+    const bool limiter = pol.limiter_fallback > 0 && I.prm.limit == 0 && at_phase &&
+                         (phase_for(lim, I.ts).lim & fsk::LIM_ON);
+    // the first-order fallback: a solve in a reconstructing phase
+    const bool first = pol.first_order_fallback && !(t.fallbacks & kFallbackFirstOrder) && at_phase &&
+                       (ph.recon_flux == 1 || ph.k2);
     int to = -1;
-    if (fallback) {
+    if (limiter) {
       mode[i] = fsk::CKPT_RESTORE_PHASE;
       to = I.phase_ts;
       I.prm.limit = 1;
       I.freeze = pol.limiter_fallback;
-      t.limiter_fallback = true;
+      t.fallbacks |= kFallbackLimiter;
       I.saved_ts = -1;   // the latest checkpoint is later than the restart
+    } else if (first) {
+      mode[i] = fsk::CKPT_RESTORE_PHASE;
+      to = I.phase_ts;
+      // the first-order start-up phase never ends: nodal face states, no
+      // gradient, derivatives or limiter from here on
+      I.prm.foits = 1 << 28;
+      t.fallbacks |= kFallbackFirstOrder;
+      I.saved_ts = -1;
     } else if (I.saved_ts >= 0) {
       mode[i] = fsk::CKPT_RESTORE;
       to = I.saved_ts;
@@ -975,6 +989,7 @@ int Engine::adopt_checkpoint(int i, Engine& from, int j) {
   T.cfl_scale = F.cfl_scale;
   T.ramp_from = F.ramp_from;
   T.prm.limit = F.prm.limit;
+  T.prm.foits = F.prm.foits;
   T.freeze = F.freeze;
   return FS_OK;
 }

@@ -82,11 +82,17 @@ struct RecoveryPolicy {
   // checkpoint) with the Barth limiter switched on, frozen after this many
   // limited steps
   int limiter_fallback = 0;
+  // the next failure of a solve in a reconstructing phase (linear or k=2)
+  // returns it to the start of that phase with first-order face states for
+  // the rest of the solve
+  bool first_order_fallback = false;
 };
+// fallback bits of RecoveryTrack::fallbacks (fs_capi.h FS_FALLBACK_*)
+constexpr int kFallbackLimiter = 1, kFallbackFirstOrder = 2;
 // per-solve bookkeeping of the policy
 struct RecoveryTrack {
   int recoveries = 0;
-  bool limiter_fallback = false;  // the fallback switched the limiter on
+  int fallbacks = 0;            // kFallback* bits of the fallbacks that acted
   int since_save = 0;
   int phase = -1;               // phase of the last step (a change resets r_best)
   double r_best = 0.0;          // smallest residual norm in the current phase

@@ -210,7 +210,8 @@ struct fs_server {
   // finished solves from one pass over the engine, finished solves masked off.
   void retire_finished(Group* g) {
     fse::Engine& E = *g->eng;
-    const fse::RecoveryPolicy pol{cfg.max_recoveries, cfg.checkpoint_every, cfg.limiter_fallback};
+    const fse::RecoveryPolicy pol{cfg.max_recoveries, cfg.checkpoint_every, cfg.limiter_fallback,
+                                  cfg.first_order_fallback != 0};
     std::vector<int> mode(E.ninst(), 0);
     std::vector<int> fin_inst, fin_status;
     std::vector<std::string> fin_err;
@@ -228,7 +229,7 @@ struct fs_server {
         status = FS_INST_FAILED;
         err = I.err ? fse::device_error_message(I.err) : "non-finite residual";
       } else {
-        int tr = ts_ref(I.prm);
+        int tr = ts_ref(r->prm);  // as submitted (a fallback may change the engine's copy)
         if (I.ts == tr || (I.ts > tr && r->r0 < 0)) r->r0 = rn;
         if (r->r0 > 0) r->drop = rn / r->r0;
         if (r->r0 > 0 && r->drop <= cfg.tol) status = FS_INST_CONVERGED;
@@ -362,6 +363,7 @@ void fs_server_config_default(fs_server_config* c) {
   c->max_steps_per_sqrt_node = 0.0;
   c->limiter_freeze = 0;
   c->limiter_fallback = 0;
+  c->first_order_fallback = 0;
 }
 
 int fs_server_create(const fs_server_config* c, fs_server** out) {
@@ -445,7 +447,7 @@ int fs_server_wait(fs_server* s, double timeout_s, fs_request_info* out, int* go
   out->forces[0] = r->forces[0];
   out->forces[1] = r->forces[1];
   out->recoveries = r->rt.recoveries;
-  out->limiter_fallback = r->rt.limiter_fallback ? 1 : 0;
+  out->fallbacks = r->rt.fallbacks;
   out->max_steps = r->limit;
   out->r0 = r->r0;
   *got = 1;

@@ -128,20 +128,21 @@ class BatchInfo(C.Structure):
 
 
 INST_RUNNING, INST_CONVERGED, INST_MAX_STEPS, INST_FAILED, INST_PAUSED, INST_CANCELLED = range(6)
+FALLBACK_LIMITER, FALLBACK_FIRST_ORDER = 1, 2
 
 
 class ServerConfig(C.Structure):
     _fields_ = [("device_budget", C.c_int64), ("precision", C.c_int), ("max_steps", C.c_int), ("tol", C.c_double),
                 ("prep_threads", C.c_int), ("use_graph", C.c_int), ("max_recoveries", C.c_int),
                 ("checkpoint_every", C.c_int), ("max_steps_per_sqrt_node", C.c_double), ("limiter_freeze", C.c_int),
-                ("limiter_fallback", C.c_int)]
+                ("limiter_fallback", C.c_int), ("first_order_fallback", C.c_int)]
 
 
 class RequestInfo(C.Structure):
     _fields_ = [("id", C.c_int64), ("status", C.c_int), ("steps", C.c_int), ("drop", C.c_double),
                 ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
                 ("device_bytes", C.c_int64), ("nn", C.c_int), ("ng", C.c_int), ("forces", C.c_double * 2),
-                ("recoveries", C.c_int), ("limiter_fallback", C.c_int), ("max_steps", C.c_int), ("r0", C.c_double)]
+                ("recoveries", C.c_int), ("fallbacks", C.c_int), ("max_steps", C.c_int), ("r0", C.c_double)]
 
 
 class Timing(C.Structure):
@@ -200,7 +201,7 @@ def lib():
         "fs_write_surface_cp": (i, [vp, C.c_char_p]),
         "fs_batch_body_forces": (i, [vp, vp]),
         "fs_batch_set_compaction": (i, [vp, i]),
-        "fs_batch_set_recovery": (i, [vp, i, i, i]),
+        "fs_batch_set_recovery": (i, [vp, i, i, i, i]),
         "fs_batch_inst_recovery": (i, [vp, i, C.POINTER(i), C.POINTER(i)]),
         "fs_batch_set_limiter_freeze": (i, [vp, i]),
         "fs_roe_flux_batch": (i, [i, d, vp, vp, vp, vp, vp]),
@@ -594,13 +595,15 @@ class Batch:
                                            status.ctypes.data))
         return steps, drop, status
 
-    def set_recovery(self, max_recoveries, checkpoint_every=10, limiter_fallback=0):
+    def set_recovery(self, max_recoveries, checkpoint_every=10, limiter_fallback=0, first_order_fallback=False):
         """Recovery from non-physical states (not in the reference; off with
         max_recoveries 0): restore the last checkpoint with half the CFL
-        target and a restarted ramp; with limiter_fallback > 0 the first
-        restore of an unlimited solve also switches on the Barth limiter,
-        frozen after that many limited steps."""
-        _check(lib().fs_batch_set_recovery(self._h, max_recoveries, checkpoint_every, limiter_fallback))
+        target and a restarted ramp; with limiter_fallback > 0 an unlimited
+        solve's first failure restarts its phase with the Barth limiter,
+        frozen after that many limited steps; with first_order_fallback the
+        next failure restarts it at first order."""
+        _check(lib().fs_batch_set_recovery(self._h, max_recoveries, checkpoint_every, limiter_fallback,
+                                           1 if first_order_fallback else 0))
 
     def set_limiter_freeze(self, steps):
         """Keep the limiter values once the limited phase has run `steps`
@@ -608,10 +611,10 @@ class Batch:
         _check(lib().fs_batch_set_limiter_freeze(self._h, steps))
 
     def recoveries(self, i):
-        """(restores, limiter fallback on) of instance i."""
+        """(restores, FALLBACK_* bits of the fallbacks that acted) of instance i."""
         n, f = C.c_int(), C.c_int()
         _check(lib().fs_batch_inst_recovery(self._h, i, C.byref(n), C.byref(f)))
-        return n.value, bool(f.value)
+        return n.value, f.value
 
     def status(self, i):
         st, steps, drop = C.c_int(), C.c_int(), C.c_double()
@@ -646,7 +649,7 @@ class Server:
 
     def __init__(self, precision=FS_PREC_MIXED, max_steps=2000, tol=1e-10, device_budget=0, prep_threads=8,
                  use_graph=True, max_recoveries=0, checkpoint_every=10, max_steps_per_sqrt_node=0.0,
-                 limiter_freeze=0, limiter_fallback=0):
+                 limiter_freeze=0, limiter_fallback=0, first_order_fallback=False):
         cfg = ServerConfig()
         lib().fs_server_config_default(C.byref(cfg))
         cfg.precision, cfg.max_steps, cfg.tol = precision, max_steps, tol
@@ -655,6 +658,7 @@ class Server:
         cfg.max_steps_per_sqrt_node = max_steps_per_sqrt_node
         cfg.limiter_freeze = limiter_freeze
         cfg.limiter_fallback = limiter_fallback
+        cfg.first_order_fallback = 1 if first_order_fallback else 0
         h = C.c_void_p()
         _check(lib().fs_server_create(C.byref(cfg), C.byref(h)))
         self._h = h

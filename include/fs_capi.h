@@ -361,14 +361,21 @@ int fs_batch_set_compaction(fs_batch* b, int on);
  * where a limiter would run instead restarts that phase from its start with
  * the Barth limiter switched on, frozen (fs_batch_set_limiter_freeze) after
  * limiter_fallback limited steps: such an instance converges to the steady
- * state of another discretization, one that depends on its iterate path.  Instances without a failure follow
+ * state of another discretization, one that depends on its iterate path.
+ * With first_order_fallback != 0 the next failure of an instance in a
+ * reconstructing phase (linear, or k=2) restarts that phase from its start
+ * with first-order face states for the rest of its solve: it converges to the
+ * first-order steady state.  Instances without a failure follow
  * exactly the path they follow with recovery off.  Steps a restore rolled
  * back still count in `steps` and against max_steps.  Set it before the
  * first step. */
-int fs_batch_set_recovery(fs_batch* b, int max_recoveries, int checkpoint_every, int limiter_fallback);
-/* restores instance i has used; whether the limiter fallback switched its
- * limiter on (either pointer may be NULL) */
-int fs_batch_inst_recovery(const fs_batch* b, int i, int* restores, int* limiter_fallback);
+int fs_batch_set_recovery(fs_batch* b, int max_recoveries, int checkpoint_every, int limiter_fallback,
+                          int first_order_fallback);
+/* the fallbacks that acted on a solve (bits) */
+enum { FS_FALLBACK_LIMITER = 1, FS_FALLBACK_FIRST_ORDER = 2 };
+/* restores instance i has used and the FS_FALLBACK_* bits of the fallbacks
+ * that acted on it (either pointer may be NULL) */
+int fs_batch_inst_recovery(const fs_batch* b, int i, int* restores, int* fallbacks);
 /* Limiter freeze (off by default: steps 0; no counterpart in the reference,
  * whose limiter runs every step, solve.C:440-471): once an instance's limited
  * phase has run more than `steps` steps, the limiter pass keeps its values
@@ -425,6 +432,7 @@ typedef struct {
                                       ceil(this * sqrt(real nodes))) */
   int limiter_freeze;      /* fs_batch_set_limiter_freeze's steps (0: off) */
   int limiter_fallback;    /* fs_batch_set_recovery's limiter_fallback (0: off) */
+  int first_order_fallback;  /* fs_batch_set_recovery's first_order_fallback (0: off) */
 } fs_server_config;
 typedef struct {
   int64_t id;
@@ -436,7 +444,7 @@ typedef struct {
   int nn, ng;
   double forces[2];        /* grid_io.C:1390 Compute_Body_Forces of the final state (lift, drag) */
   int recoveries;          /* restores the recovery policy made */
-  int limiter_fallback;    /* the recovery policy switched the limiter on */
+  int fallbacks;           /* FS_FALLBACK_* bits of the recovery policy's fallbacks that acted */
   int max_steps;           /* the request's step limit */
   double r0;               /* ||R|| at the reference step: drop = ||R|| / r0 (-1 before it) */
 } fs_request_info;

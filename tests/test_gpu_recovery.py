@@ -12,6 +12,9 @@ oracle fails it at CFL 200, 100, 50 and 20 alike).
 
 Limiter freeze: once the limited phase has run the set number of steps the
 limiter values stop changing, and before that the solve is the reference's.
+
+First-order fallback: a channel that the limiter fallback does not save
+converges at first order, to the oracle's first-order steady state.
 """
 import os
 import sys
@@ -84,7 +87,7 @@ def test_recovery_restores_checkpoint_and_leaves_healthy_solves_alone(precision)
         seen.append(q)
     st = b.status(0)
     n, fb = b.recoveries(0)
-    print("with recovery: choked solve status %d, %d steps, %d restores, limiter fallback %s" % (st[0], st[1], n, fb),
+    print("with recovery: choked solve status %d, %d steps, %d restores, fallbacks %d" % (st[0], st[1], n, fb),
           flush=True)
     assert n >= 1 and not fb
     assert st[1] == nsteps or st[0] == fsm.INST_FAILED
@@ -92,7 +95,7 @@ def test_recovery_restores_checkpoint_and_leaves_healthy_solves_alone(precision)
         assert n == 2
     for i, q in zip((1, 2), q_plain):
         assert np.array_equal(b.get(i, "Q"), q)
-        assert b.recoveries(i) == (0, False)
+        assert b.recoveries(i) == (0, 0)
     b.close()
 
 
@@ -110,7 +113,7 @@ def test_limiter_fallback_switches_the_limiter_on_at_the_first_restore(precision
             first = k + 1
             # the fallback comes with the first restore and only then, and
             # restarts the linear phase: the state after the last first-order step
-            assert fb
+            assert fb == fsm.FALLBACK_LIMITER
             assert np.array_equal(b.get(0, "Q"), seen[params[0].foits - 1])
             phi = b.get(0, "phi")
             assert np.all(phi == 1.0), "the restored solve has not run its limiter yet"
@@ -125,7 +128,7 @@ def test_limiter_fallback_switches_the_limiter_on_at_the_first_restore(precision
     if st[0] != fsm.INST_FAILED:
         assert phi.min() < 1.0, "the limiter never acted after the fallback"
     # the healthy solve never needed it
-    assert b.recoveries(1) == (0, False)
+    assert b.recoveries(1) == (0, 0)
     b.close()
 
 
@@ -157,3 +160,44 @@ def test_limiter_freeze_keeps_phi_after_the_set_steps(precision):
         assert np.array_equal(phis[t - 1], phis[last_live - 1]), "phi changed after the freeze (step %d)" % t
     ref.close()
     fz.close()
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED, fsm.FS_PREC_F64])
+def test_first_order_fallback_reaches_the_first_order_steady_state(precision):
+    """A transonic channel (151x51 bump, M 0.798) that fails unlimited and
+    again with the limiter fallback: the first-order fallback restarts its
+    linear phase with nodal face states, and the solve converges to the
+    first-order steady state, the oracle's (run first order from the
+    freestream) within 1e-8 in density."""
+    c = W.batch_cases(256)[50]
+    assert c["kind"] == "bump" and c["dims"] == (151, 51) and c["k"] == 1
+    p = _params(mach=c["mach"], alpha=c["alpha"])
+    m = W.make_mesh(c)
+    m.prepare(p)
+    b = fsm.Batch([m], [p], precision)
+    b.init_freestream()
+    b.set_recovery(4, 10, 50, True)
+    steps, drop, status = b.solve_steady(2500, 1e-10)
+    n, fb = b.recoveries(0)
+    print("status %d after %d steps, drop %.2e, %d restores, fallbacks %d" % (status[0], steps[0], drop[0], n, fb),
+          flush=True)
+    assert status[0] == fsm.INST_CONVERGED
+    assert fb & fsm.FALLBACK_FIRST_ORDER
+    rho = b.get(0, "Q")[0:4 * m.info()["nn"]:4]
+    b.close()
+    p1 = _params(mach=c["mach"], alpha=c["alpha"], foits=1 << 28)
+    o = U.Oracle(m, p1)
+    o.init_freestream()
+    r0 = None
+    for ts in range(1, 2001):
+        o.solve(ts, multicolor=1)
+        r = np.sqrt(o.s.last_rms)
+        if ts == 11:
+            r0 = r
+        if r0 and r / r0 <= 1e-11:
+            break
+    rho_o = o["Q"][0:4 * m.info()["nn"]:4]
+    diff = np.linalg.norm(rho - rho_o) / np.linalg.norm(rho_o)
+    print("oracle first order: %d steps; density L2 diff %.2e" % (ts, diff), flush=True)
+    assert r / r0 <= 1e-11
+    assert diff <= 1e-8

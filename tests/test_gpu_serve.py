@@ -162,7 +162,7 @@ def test_served_results_are_steady_states_of_the_oracle(precision):
     srv.close()
     for i, (c, p) in enumerate(zip(cases, params)):
         info, Q = results[i]
-        assert info["status"] == fsm.INST_CONVERGED and not info["limiter_fallback"]
+        assert info["status"] == fsm.INST_CONVERGED and info["fallbacks"] == 0
         m = W.make_mesh(c)
         m.prepare(p)
         ts = info["steps"] + 1
