@@ -33,6 +33,7 @@ const char* last_error_text() { return g_err.c_str(); }
 
 const char* device_error_message(int e) {
   if (e & 1) return "boundary state outside the characteristic cases (cvbc.C:330-382 exit)";
+  if (e & 4) return "a pre-multiplied sweep operand outside the FP16 range (FS_PREC_MIXED16)";
   return "non-positive or non-finite time step (init_memory.C:257-268 exit)";
 }
 }  // namespace fse

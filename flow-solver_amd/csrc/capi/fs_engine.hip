@@ -96,9 +96,11 @@ size_t footprint_bytes(const fs::Mesh& g, int prec, bool checkpoints) {
   if (nst > 0) b += nst * (2 + 4) + (nn / fsk::kDerivTileNodes + 2) * 8;
   // state
   b += (nn + ng) * 4 * 8 * 2 + nn * 8 * (4 * 3 + 8 + 1 + 20 + 4 + 2) + nbe * 8 * 22;
-  if (prec != FS_PREC_MIXED) b += ninc * 4 * 8;   // per-incidence face fluxes of the FP64 pass
+  if (prec == FS_PREC_F64) b += ninc * 4 * 8;   // per-incidence face fluxes of the FP64 pass
   if (prec == FS_PREC_MIXED)
     b += nnz * 16 * 4 + nn * (16 * 4 + 4 * 4 * 2) + neinc * (6 * 8 + 4 * 8 + 4);
+  else if (prec == FS_PREC_MIXED16)   // FP16 off-diagonal operands, FP32 diagonals, no stored inverse
+    b += (nnz - nn) * 16 * 2 + nn * (16 * 4 + 4 * 4 * 2) + neinc * (6 * 8 + 4 * 8 + 4);
   else
     b += nnz * 16 * 8 + nn * 16 * 8;
   b += (nn / fsk::kNormChunkNodes + 2) * (16 + 12);
@@ -112,7 +114,7 @@ bool valid_tuning(const fsk::Tuning& t) {
   return valid_block(t.gs_block) && (t.gs_lanes == 1 || t.gs_lanes == 4) && valid_block(t.gs32_block) &&
          (t.gs32_pre == 4 || t.gs32_pre == 6 || t.gs32_pre == 8) && valid_block(t.node_block) &&
          valid_block(t.face_block) && valid_block(t.cell_block) && (t.deriv_lds == 0 || t.deriv_lds == 1) &&
-         (t.gs32_scaled == 0 || t.gs32_scaled == 1);
+         (t.gs32_scaled == 0 || t.gs32_scaled == 1) && (t.node_fused == 0 || t.node_fused == 1);
 }
 // FS_TUNE_CACHE, else $HOME/.cache/flowsolver/tune_cache.txt
 std::string tune_cache_path() {
@@ -173,13 +175,15 @@ Engine::~Engine() {
 int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector<fs_params>& prm, int prec) {
   const int B = (int)meshes.size();
   if (B < 1 || (int)prm.size() != B) return set_error(FS_ERR_ARG, "need one parameter set per mesh");
-  if (prec != FS_PREC_F64 && prec != FS_PREC_MIXED) return set_error(FS_ERR_ARG, "unknown precision");
+  if (prec != FS_PREC_F64 && prec != FS_PREC_MIXED && prec != FS_PREC_MIXED16)
+    return set_error(FS_ERR_ARG, "unknown precision");
   precision = prec;
   subits = prm[0].subits;
   // FP32 sweeps on operands pre-multiplied by the diagonal inverse when a
   // linear solve runs enough sweeps to pay for the pass that forms them
-  // (measured: it pays at 40 and 160 sweeps, costs at 20)
-  dm.tune.gs32_scaled = subits >= 30 ? 1 : 0;
+  // (measured: it pays at 40 and 160 sweeps, costs at 20); FP16 operands
+  // exist only in that form
+  dm.tune.gs32_scaled = (subits >= 30 || prec == FS_PREC_MIXED16) ? 1 : 0;
   iter = prm[0].iter;
   jufreq = prm[0].jufreq;
   for (int i = 0; i < B; ++i) {
@@ -236,7 +240,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   std::vector<int> nsub_off(NN + 1), nedge_off(NN + 1), nbe_off(NN + 1), st_off(NN + 1);
   std::vector<int> sub_inc(2 * NS), inc_other(EINC), inc_sother(KINC), nbe_list(BINC), st_nodes(NST);
   std::vector<double> inc_n(4 * KINC), inc_fc(2 * KINC), inc_xg(6 * KINC), inc_gw(2 * EINC), st_Wt(5 * NST);
-  const bool mixed = prec == FS_PREC_MIXED;
+  const bool mixed = prec != FS_PREC_F64;
   std::vector<double> einc_geo(mixed ? 6 * EINC : 0), einc_fc(mixed ? 4 * EINC : 0);
   std::vector<int> einc_slot(mixed ? EINC : 0);
   std::vector<int> breal(NB), bbc(NB);
@@ -338,6 +342,11 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   }
   nsub_off[NN] = (int)kinc;
   nedge_off[NN] = (int)einc;
+  d.maxdeg = 0;
+  for (long long n = 0; n < NN; ++n) d.maxdeg = std::max(d.maxdeg, nedge_off[n + 1] - nedge_off[n]);
+  if (prec == FS_PREC_MIXED16 && !fsk::resid_jac32_fused_fits(d))
+    return set_error(FS_ERR_UNSUPPORTED, "FP16 sweep operands: a node has too many edges (" +
+                                             std::to_string(d.maxdeg) + ") for the node pass's LDS rows");
   nbe_off[NN] = (int)binc;
   st_off[NN] = (int)sto;
 
@@ -517,7 +526,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = alloc(&st.R, nn * 4)) || (rc = alloc(&st.RHS, nn * 4)) || (rc = alloc(&st.grad, nn * 8)) ||
       (rc = alloc(&st.phi, nn)) || (rc = alloc(&st.hess, nn * 20)) || (rc = alloc(&st.pv, nn * 4)) ||
       (rc = alloc(&st.ev_con, nn)) || (rc = alloc(&st.dt, nn)) ||
-      (prec != FS_PREC_MIXED && (rc = alloc(&st.inc_flux, (size_t)KINC * 4))) ||
+      (prec == FS_PREC_F64 && (rc = alloc(&st.inc_flux, (size_t)KINC * 4))) ||
       (rc = alloc(&st.bflux, (size_t)NB * 4)) || (rc = alloc(&st.bjac, (size_t)NB * 16)) ||
       (rc = alloc(&st.bforce, (size_t)NB * 2)) || (rc = alloc(&d_fp_, (size_t)B)) ||
       (rc = alloc(&st.err, (size_t)B)) || (rc = alloc(&st.inst_norms, 2 * (size_t)B)) ||
@@ -528,6 +537,13 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
     if ((rc = alloc(&st.LHS32, (size_t)NSLOT * 16)) || (rc = alloc(&st.LU32, nn * 16)) ||
         (rc = alloc(&st.RHS32, nn * 4)) || (rc = alloc(&st.dQ32, nn * 4)))
       return rc;
+    st.D32 = st.LHS32 + 16 * (size_t)NOFF;
+  } else if (precision == FS_PREC_MIXED16) {
+    if ((rc = alloc(&st.LHS16, (size_t)NOFF * 16)) || (rc = alloc(&st.D32, nn * 16)) ||
+        (rc = alloc(&st.RHS32, nn * 4)) || (rc = alloc(&st.dQ32, nn * 4)))
+      return rc;
+    // padding slots of the sliced layout are read by no sweep; zero them once
+    FSE_HIP(hipMemset(st.LHS16, 0, (size_t)NOFF * 16 * sizeof(unsigned short)));
   } else {
     if ((rc = alloc(&st.LHS, (size_t)NSLOT * 16)) || (rc = alloc(&st.LU, nn * 16))) return rc;
     FSE_HIP(hipMemset(st.LHS, 0, (size_t)NSLOT * 16 * sizeof(double)));
@@ -561,7 +577,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       std::string k;
       fsk::Tuning t;
       if (ls >> k >> t.gs_block >> t.gs_lanes >> t.gs32_block >> t.gs32_pre >> t.node_block >> t.face_block >>
-          t.cell_block >> t.deriv_lds && k == key) {
+          t.cell_block >> t.deriv_lds >> t.node_fused && k == key) {
         // the sweep form is the engine's own (it changes FP32 rounding)
         t.gs32_scaled = dm.tune.gs32_scaled;
         if (!valid_tuning(t)) continue;
@@ -661,9 +677,10 @@ int Engine::newton_iteration(int it, int key, bool record) {
   if (key & 2) fsk::launch_derivatives(dm, s, st);
   fsk::launch_limiter(dm, s, st);
   fsk::ColorRanges cr{color_off.data(), color_nback.data()};
-  if (precision == FS_PREC_MIXED) {
+  if (precision != FS_PREC_F64) {
     // boundary fluxes and blocks, then one node pass for the FP64 residual,
-    // ev_con/dt and every FP32 Jacobian block; FP32 sweeps
+    // ev_con/dt and every FP32 Jacobian block (and, fused, the sweep
+    // operands: then the diagonal pass launches nothing); FP32 sweeps
     if (record) FSE_HIP(hipEventRecord(ev_[0], st));
     fsk::launch_bedge(dm, s, st);
     if (record) FSE_HIP(hipEventRecord(ev_[1], st));
@@ -1010,7 +1027,8 @@ std::string Engine::tune_key() const {
   int lg = 0;
   while ((1LL << (lg + 1)) <= (long long)std::max(1, dm.nn)) ++lg;
   std::ostringstream k;
-  k << arch << ":" << (precision == FS_PREC_MIXED ? "mixed" : "f64") << ":" << (k2 ? "k2" : "k1") << ":n" << lg;
+  k << arch << ":" << (precision == FS_PREC_MIXED ? "mixed" : precision == FS_PREC_MIXED16 ? "mixed16" : "f64") << ":"
+    << (k2 ? "k2" : "k1") << ":n" << lg;
   return k.str();
 }
 
@@ -1018,7 +1036,10 @@ int Engine::set_tuning(const int* v) {
   fsk::Tuning t;
   t.gs_block = v[0]; t.gs_lanes = v[1]; t.gs32_block = v[2]; t.gs32_pre = v[3];
   t.node_block = v[4]; t.face_block = v[5]; t.cell_block = v[6]; t.deriv_lds = v[7]; t.gs32_scaled = v[8];
+  t.node_fused = v[9];
   if (!valid_tuning(t)) return set_error(FS_ERR_ARG, "tuning value outside the candidates");
+  if (precision == FS_PREC_MIXED16 && !t.gs32_scaled)
+    return set_error(FS_ERR_ARG, "FP16 sweep operands exist only pre-multiplied (gs32_scaled 1)");
   FSE_HIP(hipStreamSynchronize(stream));
   dm.tune = t;
   // captured graphs hold the old launch shapes
@@ -1052,7 +1073,7 @@ int Engine::autotune(double* seconds) {
   };
   fsk::Tuning& tu = dm.tune;
   float best, t;
-  if (precision == FS_PREC_MIXED) {
+  if (precision != FS_PREC_F64) {
     best = 1e30f;
     fsk::Tuning keep = tu;
     for (int pre : {4, 6, 8})
@@ -1080,17 +1101,35 @@ int Engine::autotune(double* seconds) {
   const int nflags = fsk::NODE_EVCON | fsk::NODE_INTERIOR | fsk::NODE_BOUNDARY | fsk::NODE_RHS | fsk::NODE_JAC;
   int keepb = tu.node_block;
   best = 1e30f;
+  // the mixed node pass with the diagonal pass (a no-op when the node pass
+  // forms the sweep operands itself), which always starts from the blocks
+  // the node pass has just written
+  auto node_pass32 = [&] {
+    fsk::launch_resid_jac32(dm, s, st);
+    fsk::launch_diag_lu32(dm, s, st);
+  };
   for (int b : blocks) {
     tu.node_block = b;
-    if (precision == FS_PREC_MIXED)
-      rc = time_it([&] { fsk::launch_resid_jac32(dm, s, st); }, 2, &t);
+    if (precision != FS_PREC_F64)
+      rc = time_it(node_pass32, 2, &t);
     else
       rc = time_it([&] { fsk::launch_node(dm, s, nflags, st); }, 2, &t);
     if (rc) return rc;
     if (t < best) { best = t; keepb = b; }
   }
   tu.node_block = keepb;
-  if (precision != FS_PREC_MIXED) {
+  if (precision == FS_PREC_MIXED && tu.gs32_scaled) {
+    // pre-multiplied operands from the node pass itself or from the diagonal pass
+    int keepf = tu.node_fused;
+    best = 1e30f;
+    for (int f : {0, 1}) {
+      tu.node_fused = f;
+      if ((rc = time_it(node_pass32, 2, &t))) return rc;
+      if (t < best) { best = t; keepf = f; }
+    }
+    tu.node_fused = keepf;
+  }
+  if (precision == FS_PREC_F64) {
     keepb = tu.face_block;
     best = 1e30f;
     for (int b : blocks) {
@@ -1114,7 +1153,7 @@ int Engine::autotune(double* seconds) {
       if (key & 2) fsk::launch_derivatives(dm, s, st);
       fsk::launch_limiter(dm, s, st);
       if (precision == FS_PREC_MIXED) fsk::launch_diag_lu32(dm, s, st);
-      else fsk::launch_diag_lu(dm, s, st);
+      else if (precision == FS_PREC_F64) fsk::launch_diag_lu(dm, s, st);
     }, 2, &t);
     if (rc) {
       tu.gs32_scaled = scaled;
@@ -1150,7 +1189,8 @@ int Engine::autotune(double* seconds) {
     std::ofstream out(path, std::ios::app);
     if (out)
       out << tune_key() << " " << tu.gs_block << " " << tu.gs_lanes << " " << tu.gs32_block << " " << tu.gs32_pre << " "
-          << tu.node_block << " " << tu.face_block << " " << tu.cell_block << " " << tu.deriv_lds << "\n";
+          << tu.node_block << " " << tu.face_block << " " << tu.cell_block << " " << tu.deriv_lds << " "
+          << tu.node_fused << "\n";
   }
   if (seconds)
     *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -1201,13 +1241,25 @@ int Engine::get_field(int i, int field, double* host, int64_t n) {
   FSE_HIP(hipStreamSynchronize(stream));
   if (field == FS_F_LHS || field == FS_F_LHS32) {
     if (field == FS_F_LHS && !ds.LHS) return set_error(FS_ERR_ARG, "FP64 blocks exist only for an FP64 solver");
-    if (field == FS_F_LHS32 && !ds.LHS32) return set_error(FS_ERR_ARG, "FP32 blocks exist only for a mixed-precision solver");
+    if (field == FS_F_LHS32 && !ds.D32) return set_error(FS_ERR_ARG, "FP32 blocks exist only for a mixed-precision solver");
     size_t all = (size_t)dm.nnz * 16;
     if (field == FS_F_LHS) {
       std::vector<double> tmp(all);
       FSE_HIP(hipMemcpy(tmp.data(), ds.LHS, all * sizeof(double), hipMemcpyDeviceToHost));
       for (size_t k = 0; k < I.slot_dev.size(); ++k)
         std::memcpy(host + 16 * k, &tmp[16 * (size_t)I.slot_dev[k]], 16 * sizeof(double));
+    } else if (ds.LHS16) {
+      // FP16 off-diagonal operands widened, FP32 diagonal blocks
+      const size_t noff = (size_t)dm.nnz - dm.nn;
+      std::vector<_Float16> h(noff * 16);
+      std::vector<float> dg((size_t)dm.nn * 16);
+      FSE_HIP(hipMemcpy(h.data(), ds.LHS16, h.size() * sizeof(_Float16), hipMemcpyDeviceToHost));
+      FSE_HIP(hipMemcpy(dg.data(), ds.D32, dg.size() * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < I.slot_dev.size(); ++k) {
+        const size_t sl = (size_t)I.slot_dev[k];
+        for (int c = 0; c < 16; ++c)
+          host[16 * k + c] = sl < noff ? (double)(float)h[16 * sl + c] : (double)dg[16 * (sl - noff) + c];
+      }
     } else {
       std::vector<float> tmp(all);
       FSE_HIP(hipMemcpy(tmp.data(), ds.LHS32, all * sizeof(float), hipMemcpyDeviceToHost));

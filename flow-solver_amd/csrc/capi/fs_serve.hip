@@ -368,7 +368,8 @@ void fs_server_config_default(fs_server_config* c) {
 
 int fs_server_create(const fs_server_config* c, fs_server** out) {
   if (!c || !out) return fail(FS_ERR_ARG, "null argument");
-  if (c->precision != FS_PREC_F64 && c->precision != FS_PREC_MIXED) return fail(FS_ERR_ARG, "unknown precision");
+  if (c->precision != FS_PREC_F64 && c->precision != FS_PREC_MIXED && c->precision != FS_PREC_MIXED16)
+    return fail(FS_ERR_ARG, "unknown precision");
   if (c->max_steps < 1 || !(c->tol > 0) || c->prep_threads < 1 || c->device_budget < 0 || c->max_recoveries < 0 ||
       c->checkpoint_every < 1 || !(c->max_steps_per_sqrt_node >= 0) || c->limiter_freeze < 0 ||
       c->limiter_fallback < 0)

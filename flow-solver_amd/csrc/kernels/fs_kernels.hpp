@@ -54,6 +54,8 @@ struct Tuning {
   int cell_block = 256;   // gradient, limiter, derivatives, diagonal factorisation
   int deriv_lds = 1;      // k=2 derivatives: 0 neighbour states gathered through L2, 1 staged per tile in LDS
   int gs32_scaled = 1;    // FP32 sweeps: 0 apply the stored diagonal inverse per row, 1 operands pre-multiplied by it
+  int node_fused = 1;     // pre-multiplied operands: 1 formed by the mixed node pass itself (the row's blocks
+                          // staged in LDS), 0 by a separate diagonal pass over the stored blocks
 };
 
 // LDS tiles of the k=2 derivative pass: runs of kDerivTileNodes consecutive
@@ -64,6 +66,7 @@ constexpr int kDerivTileMaxU = 576;
 
 struct DevMesh {
   int nn, ng, ne, nsub, nbe, nnz, ncolor, ninst;
+  int maxdeg;               // largest number of edges at one node (off-diagonal blocks of a row)
   Tuning tune;
   const int* inst_node;     // nn
   const int* inst_edge;     // ne
@@ -153,6 +156,10 @@ struct DevState {
                      // diagonal pass, its product with the row's diagonal inverse (and the off-diagonal
                      // blocks of LHS32 likewise)
   float* dQ32;       // nn*4
+  float* D32;        // nn*16 by color position: the FP32 diagonal blocks (the tail of LHS32, slots
+                     // nnz-nn.., when LHS32 exists)
+  unsigned short* LHS16;  // FS_PREC_MIXED16 only: (nnz-nn)*16 FP16 bit patterns, the off-diagonal slots
+                          // of LHS32's layout holding the pre-multiplied blocks (LHS32 is not allocated)
   const InstParams* ip;   // ninst, controls of the current Newton iteration
   double* inst_norms;     // 2 per instance: sum R^2, sum dQ^2
   double* chunk_part;     // 2 per reduction chunk
@@ -229,6 +236,14 @@ void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipS
 // FP64 residual, ev_con/dt, FP32 analytic Jacobian blocks (off-diagonal and
 // diagonal) and right-hand sides in one node pass
 void launch_resid_jac32(const DevMesh& m, const DevState& s, hipStream_t st);
+// whether launch_resid_jac32 also forms the sweep operands (diagonal
+// inverse, pre-multiplied blocks and right-hand sides, zero dQ), so no
+// diagonal pass follows it: FP16 blocks always; FP32 pre-multiplied operands
+// when the tuning asks for it and a workgroup's rows fit in LDS
+bool resid_jac32_fused(const DevMesh& m, const DevState& s);
+// whether a workgroup of the fused node pass can stage its rows' blocks in
+// LDS at the mesh's largest node degree (FS_PREC_MIXED16 requires it)
+bool resid_jac32_fused_fits(const DevMesh& m);
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st);
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st);

@@ -49,6 +49,13 @@ __device__ __forceinline__ double quad_xor(double x) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// Lane r's value in every lane of its quad (DPP quad_perm [r,r,r,r]).
+template <int kLane>
+__device__ __forceinline__ float quad_bcast(float x) {
+  constexpr int ctrl = kLane * 0x55;
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, 0xF, 0xF, false));
+}
+
 // Euler flux Jacobian dF.n/dq (jacobian.C:561-582) in FP32 from the state's
 // velocity and total energy per mass (e = E/rho)
 __device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, float u, float v, float e, float* d) {
@@ -61,6 +68,84 @@ __device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, f
   d[12] = (2.0f * ph - gE) * th; d[13] = (1.0f - g) * u * th + nx * (gE - ph);
   d[14] = (1.0f - g) * v * th + ny * (gE - ph); d[15] = g * th;
 }
+
+// Inverse of a 4x4 block in FP64 by Gauss-Jordan elimination with partial
+// pivoting, stored in FP32.  A factorization without pivoting rounded to FP32
+// loses the block where its leading pivot is small against the other entries
+// (the continuity row's diagonal is V/dt plus dissipation only, since the
+// flux Jacobian's (0,0) entry is zero); the pivoted FP64 inverse is accurate
+// to FP32 rounding of its entries.  Row selection uses compile-time indices
+// only, so the augmented matrix stays in registers.
+__device__ __forceinline__ void inv4_pivoted(const float* A, double* out) {
+  double a[4][8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a[i][j] = (double)A[4 * i + j];
+      a[i][4 + j] = (i == j) ? 1.0 : 0.0;
+    }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double best = fabs(a[k][k]);
+    int p = k;
+#pragma unroll
+    for (int r = k + 1; r < 4; ++r)
+      if (fabs(a[r][k]) > best) { best = fabs(a[r][k]); p = r; }
+#pragma unroll
+    for (int r = k + 1; r < 4; ++r)
+      if (r == p) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) { double tmp = a[k][c]; a[k][c] = a[r][c]; a[r][c] = tmp; }
+      }
+    const double ip = 1.0 / a[k][k];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) a[k][c] *= ip;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r == k) continue;
+      const double f = a[r][k];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) a[r][c] -= f * a[k][c];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[4 * i + j] = a[i][4 + j];
+}
+
+// Row r of the pre-multiplied sweep operands (r = 0..3, Di the pivoted FP64
+// inverse): row r of Di B for an off-diagonal block B, and entry r of Di b
+// for the right-hand side, FP64 products rounded once to FP32.  The diagonal
+// pass and the fused node pass both call these, so the two forms of the
+// operands agree bit for bit.
+__device__ __forceinline__ float4 scale_block_row(const double* dr, const float* b) {
+  double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double d = dr[i];
+    c0 += d * b[4 * i];
+    c1 += d * b[4 * i + 1];
+    c2 += d * b[4 * i + 2];
+    c3 += d * b[4 * i + 3];
+  }
+  return make_float4((float)c0, (float)c1, (float)c2, (float)c3);
+}
+__device__ __forceinline__ float scale_rhs_entry(const double* dr, float4 rb) {
+  return (float)(dr[0] * rb.x + dr[1] * rb.y + dr[2] * rb.z + dr[3] * rb.w);
+}
+
+// FP16 bit pattern of an FP32 value (round to nearest even); `ok` turns
+// false when it does not fit the FP16 range
+__device__ __forceinline__ unsigned int half_bits(float x, bool& ok) {
+  ok = ok && fabsf(x) < 65520.0f;
+  return (unsigned int)__builtin_bit_cast(unsigned short, (_Float16)x);
+}
+
+// per-node LDS stride (floats) of the fused node pass: the row's blocks and
+// 4 floats of padding, so consecutive nodes start in different banks
+__host__ __device__ __forceinline__ int fused_row_stride(int maxdeg) { return 16 * maxdeg + 4; }
 
 // Residual and Jacobian pass of the mixed step, one pass per node (4 lanes
 // per node), nothing re-read from an earlier pass but the boundary edges'
@@ -84,7 +169,23 @@ __device__ __forceinline__ void flux_jacobian_f32(float nx, float ny, float g, f
 // Lane v of the node's quad handles subedge (v & 1) of edge incidences
 // v >> 1, 2 + (v >> 1), ...; the two subedges of an edge meet in lanes v and
 // v ^ 1, which add their block contributions and write the edge's block.
+//
+// kOut selects what the pass leaves for the sweeps:
+//   0  the assembled FP32 blocks (off-diagonal in LHS32, diagonal in D32) and
+//      right-hand side; the diagonal pass (k_diag_lu32) follows;
+//   1  the pre-multiplied FP32 operands directly: the row's off-diagonal
+//      blocks are staged in LDS instead of written, and once the diagonal is
+//      complete every lane of the quad inverts it (inv4_pivoted, FP64), lane
+//      v writes row v of Di B for each staged block B and entry v of Di b,
+//      and zeroes dQ -- the same operations as k_diag_lu32<true>, so the
+//      operands equal the two-pass form's bit for bit, without the second
+//      pass's read and write of every block;
+//   2  as 1, with the off-diagonal operands stored in FP16 (LHS16); a value
+//      outside the FP16 range flags the instance (device error bit 4).
+// The FP32 diagonal block is stored in every form.
+template <int kOut>
 __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s) {
+  extern __shared__ float s_rows[];
   const int tid = xcd_block() * blockDim.x + threadIdx.x;
   const int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
@@ -210,12 +311,20 @@ __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s
     // lane v writes rows 2 (v & 1) and 2 (v & 1) + 1
 #pragma unroll
     for (int c = 0; c < 16; ++c) off[c] += quad_xor<1>(off[c]);
-    float row[8];
+    if (kOut == 0) {
+      float row[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) row[j] = sub ? off[8 + j] : off[j];
-    float4* blk = reinterpret_cast<float4*>(s.LHS32 + 16 * (size_t)m.einc_slot[k]) + 2 * sub;
-    blk[0] = make_float4(row[0], row[1], row[2], row[3]);
-    blk[1] = make_float4(row[4], row[5], row[6], row[7]);
+      for (int j = 0; j < 8; ++j) row[j] = sub ? off[8 + j] : off[j];
+      float4* blk = reinterpret_cast<float4*>(s.LHS32 + 16 * (size_t)m.einc_slot[k]) + 2 * sub;
+      blk[0] = make_float4(row[0], row[1], row[2], row[3]);
+      blk[1] = make_float4(row[4], row[5], row[6], row[7]);
+    } else if (sub == 0) {
+      // the whole block into the quad's LDS rows (both lanes hold it)
+      float4* blk =
+          reinterpret_cast<float4*>(s_rows + (threadIdx.x >> 2) * fused_row_stride(m.maxdeg) + 16 * (k - k0));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) blk[r] = make_float4(off[4 * r], off[4 * r + 1], off[4 * r + 2], off[4 * r + 3]);
+    }
   }
   // node sums: fixed butterfly over the quad
 #pragma unroll
@@ -256,57 +365,60 @@ __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s
       for (int c = 0; c < 4; ++c) D[c] += B[c];
     }
   }
-  *reinterpret_cast<float4*>(s.LHS32 + 16 * (size_t)m.iau[n] + 4 * v) =
-      make_float4((float)D[0], (float)D[1], (float)D[2], (float)D[3]);
+  const float4 dv = make_float4((float)D[0], (float)D[1], (float)D[2], (float)D[3]);
+  const int t = m.iau[n] - (m.nnz - m.nn);   // the row's color position
+  *reinterpret_cast<float4*>(s.D32 + 16 * (size_t)t + 4 * v) = dv;
   s.R[4 * (size_t)n + v] = Rv;
   s.RHS[4 * (size_t)n + v] = -Rv;
-  s.RHS32[4 * (size_t)n + v] = (float)(-Rv);
-}
-
-// Inverse of a 4x4 block in FP64 by Gauss-Jordan elimination with partial
-// pivoting, stored in FP32.  A factorization without pivoting rounded to FP32
-// loses the block where its leading pivot is small against the other entries
-// (the continuity row's diagonal is V/dt plus dissipation only, since the
-// flux Jacobian's (0,0) entry is zero); the pivoted FP64 inverse is accurate
-// to FP32 rounding of its entries.  Row selection uses compile-time indices
-// only, so the augmented matrix stays in registers.
-__device__ __forceinline__ void inv4_pivoted(const float* A, double* out) {
-  double a[4][8];
+  const float bv = (float)(-Rv);
+  if (kOut == 0) {
+    s.RHS32[4 * (size_t)n + v] = bv;
+    return;
+  }
+  // the stored diagonal block in every lane of the quad, and its inverse
+  float A[16];
+  {
+    const float dr[4] = {dv.x, dv.y, dv.z, dv.w};
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      a[i][j] = (double)A[4 * i + j];
-      a[i][4 + j] = (i == j) ? 1.0 : 0.0;
-    }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    double best = fabs(a[k][k]);
-    int p = k;
-#pragma unroll
-    for (int r = k + 1; r < 4; ++r)
-      if (fabs(a[r][k]) > best) { best = fabs(a[r][k]); p = r; }
-#pragma unroll
-    for (int r = k + 1; r < 4; ++r)
-      if (r == p) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c) { double tmp = a[k][c]; a[k][c] = a[r][c]; a[r][c] = tmp; }
-      }
-    const double ip = 1.0 / a[k][k];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) a[k][c] *= ip;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (r == k) continue;
-      const double f = a[r][k];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) a[r][c] -= f * a[k][c];
+    for (int c = 0; c < 4; ++c) {
+      A[c] = quad_bcast<0>(dr[c]);
+      A[4 + c] = quad_bcast<1>(dr[c]);
+      A[8 + c] = quad_bcast<2>(dr[c]);
+      A[12 + c] = quad_bcast<3>(dr[c]);
     }
   }
+  double Di[16], di[4];
+  inv4_pivoted(A, Di);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) di[i] = v == 0 ? Di[i] : v == 1 ? Di[4 + i] : v == 2 ? Di[8 + i] : Di[12 + i];
+  const float4 rb = make_float4(quad_bcast<0>(bv), quad_bcast<1>(bv), quad_bcast<2>(bv), quad_bcast<3>(bv));
+  s.RHS32[4 * (size_t)n + v] = scale_rhs_entry(di, rb);
+  s.dQ32[4 * (size_t)n + v] = 0.0f;
+  // the quad's staged blocks were written by its other lanes
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const float* rows = s_rows + (threadIdx.x >> 2) * fused_row_stride(m.maxdeg);
+  bool ok = true;
+  for (int e = 0; e < k1 - k0; ++e) {
+    float b[16];
+    const float4* B = reinterpret_cast<const float4*>(rows + 16 * e);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) out[4 * i + j] = a[i][4 + j];
+    for (int r = 0; r < 4; ++r) {
+      const float4 q = B[r];
+      b[4 * r] = q.x; b[4 * r + 1] = q.y; b[4 * r + 2] = q.z; b[4 * r + 3] = q.w;
+    }
+    const float4 o = scale_block_row(di, b);
+    const size_t slot = (size_t)m.einc_slot[k0 + e];
+    if (kOut == 1) {
+      *reinterpret_cast<float4*>(s.LHS32 + 16 * slot + 4 * v) = o;
+    } else {
+      const unsigned int h0 = half_bits(o.x, ok), h1 = half_bits(o.y, ok), h2 = half_bits(o.z, ok),
+                         h3 = half_bits(o.w, ok);
+      *reinterpret_cast<uint2*>(s.LHS16 + 16 * slot + 4 * v) = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
+    }
+  }
+  if (kOut == 2 && !ok) atomicOr(s.err + inst, 4);
 }
 
 // One thread per row (color position t): the diagonal block's inverse Di
@@ -321,15 +433,16 @@ __device__ __forceinline__ void inv4_pivoted(const float* A, double* out) {
 // starting guess.
 template <bool kScale>
 __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstParams* __restrict__ ip,
-                                                      float* __restrict__ LHS32, float* __restrict__ LU32,
-                                                      float* __restrict__ RHS32, float* __restrict__ dQ32) {
+                                                      float* __restrict__ LHS32, const float* __restrict__ D32,
+                                                      float* __restrict__ LU32, float* __restrict__ RHS32,
+                                                      float* __restrict__ dQ32) {
   int t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
   float A[16];
   double Di[16];
-  const float4* D = reinterpret_cast<const float4*>(LHS32 + 16 * (size_t)m.iau[n]);
+  const float4* D = reinterpret_cast<const float4*>(D32 + 16 * (size_t)t);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     float4 r = D[k];
@@ -357,25 +470,12 @@ __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstPa
     }
     float4* O = reinterpret_cast<float4*>(LHS32 + 16 * k);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      double c0 = 0., c1 = 0., c2 = 0., c3 = 0.;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const double d = Di[4 * r + i];
-        c0 += d * b[4 * i];
-        c1 += d * b[4 * i + 1];
-        c2 += d * b[4 * i + 2];
-        c3 += d * b[4 * i + 3];
-      }
-      O[r] = make_float4((float)c0, (float)c1, (float)c2, (float)c3);
-    }
+    for (int r = 0; r < 4; ++r) O[r] = scale_block_row(Di + 4 * r, b);
   }
   float4* R = reinterpret_cast<float4*>(RHS32 + 4 * (size_t)n);
   const float4 rb = *R;
-  double c[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) c[r] = Di[4 * r] * rb.x + Di[4 * r + 1] * rb.y + Di[4 * r + 2] * rb.z + Di[4 * r + 3] * rb.w;
-  *R = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]);
+  *R = make_float4(scale_rhs_entry(Di, rb), scale_rhs_entry(Di + 4, rb), scale_rhs_entry(Di + 8, rb),
+                   scale_rhs_entry(Di + 12, rb));
 }
 
 // FP32 multicolor sweep over rows [row_begin,row_end) of the color order;
@@ -429,6 +529,50 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const InstP
   dQ32[4 * (size_t)j + v] = Di.x * b0 + Di.y * b1 + Di.z * b2 + Di.w * b3;
 }
 
+// The pre-multiplied sweep (k_gs_color32<kPre, true>) over FP16 operands:
+// lane v reads row v of each off-diagonal block as 4 FP16 values (8 bytes;
+// a wave reads block e of its 16 rows as one contiguous 512-byte run) and
+// accumulates in FP32, dQ_row[v] = c[v] - sum (Di A)[v,:] dQ_nb.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+template <int kPre>
+__global__ void __launch_bounds__(kMaxBlock) k_gs_color16(DevMesh m, const InstParams* __restrict__ ip,
+                                                       const unsigned short* __restrict__ LHS16,
+                                                       const float* __restrict__ RHS32, float* __restrict__ dQ32,
+                                                       int row_begin, int row_end) {
+  int tid = xcd_block() * blockDim.x + threadIdx.x;
+  int t = row_begin + (tid >> 2), v = tid & 3;
+  if (t >= row_end) return;
+  int j = m.color_rows[t];
+  int inst = inst_of(m, m.pos_inst, t);
+  int deg = m.pdeg[t];
+  size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
+  float r = RHS32[4 * (size_t)j + v];
+  if (!ip[inst].active) return;
+  int nb[kPre];
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) nb[i] = (i < deg) ? m.ja[base + 16 * i] : j;
+  half4_t A[kPre];
+  float4 q[kPre];
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) {
+    if (i < deg) {
+      A[i] = *reinterpret_cast<const half4_t*>(LHS16 + 16 * (base + 16 * i) + 4 * v);
+      q[i] = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)nb[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kPre; ++i)
+    if (i < deg)
+      r -= (float)A[i].x * q[i].x + (float)A[i].y * q[i].y + (float)A[i].z * q[i].z + (float)A[i].w * q[i].w;
+  for (int e = kPre; e < deg; ++e) {
+    size_t k = base + 16 * (size_t)e;
+    const half4_t a = *reinterpret_cast<const half4_t*>(LHS16 + 16 * k + 4 * v);
+    const float4 b = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)m.ja[k]);
+    r -= (float)a.x * b.x + (float)a.y * b.y + (float)a.z * b.z + (float)a.w * b.w;
+  }
+  dQ32[4 * (size_t)j + v] = r;
+}
+
 // nQ += dQ (FP64 state, FP32 correction) and the FP64 norms' inputs
 __global__ void __launch_bounds__(kBlock) k_update_mixed(DevMesh m, const InstParams* __restrict__ ip,
                                                          double* __restrict__ nQ, const float* __restrict__ dQ32,
@@ -441,23 +585,56 @@ __global__ void __launch_bounds__(kBlock) k_update_mixed(DevMesh m, const InstPa
   nQ[i] += d;
 }
 
+namespace {
+// LDS of one workgroup of the fused node pass
+constexpr size_t kFusedLds = 65536;
+size_t fused_lds_bytes(const DevMesh& m, int block) {
+  return (size_t)(block / 4) * fused_row_stride(m.maxdeg) * sizeof(float);
+}
+// workgroup size of the fused node pass: the tuned one, halved until a
+// workgroup's staged rows fit; 0 if even 16 rows do not
+int fused_block(const DevMesh& m) {
+  int b = m.tune.node_block;
+  while (b > 64 && fused_lds_bytes(m, b) > kFusedLds) b /= 2;
+  return fused_lds_bytes(m, b) <= kFusedLds ? b : 0;
+}
+}  // namespace
+
+bool resid_jac32_fused(const DevMesh& m, const DevState& s) {
+  if (s.LHS16) return true;
+  return m.tune.gs32_scaled && m.tune.node_fused && fused_block(m) > 0;
+}
+bool resid_jac32_fused_fits(const DevMesh& m) { return fused_block(m) > 0; }
+
 void launch_resid_jac32(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_resid_jac32, dim3(nblocks(4 * m.nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st, m, s);
+  if (!resid_jac32_fused(m, s)) {
+    hipLaunchKernelGGL(k_resid_jac32<0>, dim3(nblocks(4 * m.nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st,
+                       m, s);
+    return;
+  }
+  const int b = fused_block(m);
+  const dim3 grid(nblocks(4 * m.nn, b)), block(b);
+  if (s.LHS16)
+    hipLaunchKernelGGL(k_resid_jac32<2>, grid, block, fused_lds_bytes(m, b), st, m, s);
+  else
+    hipLaunchKernelGGL(k_resid_jac32<1>, grid, block, fused_lds_bytes(m, b), st, m, s);
 }
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
-  if (m.nn == 0) return;
+  if (m.nn == 0 || resid_jac32_fused(m, s)) return;
   const dim3 grid(nblocks(m.nn, m.tune.cell_block)), block(m.tune.cell_block);
   if (m.tune.gs32_scaled)
-    hipLaunchKernelGGL(k_diag_lu32<true>, grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32);
+    hipLaunchKernelGGL(k_diag_lu32<true>, grid, block, 0, st, m, s.ip, s.LHS32, s.D32, s.LU32, s.RHS32, s.dQ32);
   else
-    hipLaunchKernelGGL(k_diag_lu32<false>, grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32);
+    hipLaunchKernelGGL(k_diag_lu32<false>, grid, block, 0, st, m, s.ip, s.LHS32, s.D32, s.LU32, s.RHS32, s.dQ32);
 }
 
 namespace {
 template <int kPre>
 void gs32_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
-  if (m.tune.gs32_scaled)
+  if (s.LHS16)
+    hipLaunchKernelGGL((k_gs_color16<kPre>), grid, block, 0, st, m, s.ip, s.LHS16, s.RHS32, s.dQ32, b, e);
+  else if (m.tune.gs32_scaled)
     hipLaunchKernelGGL((k_gs_color32<kPre, true>), grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
                        e);
   else

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libflowsolver.so")
 
 FS_PREC_F64 = 0
 FS_PREC_MIXED = 1
+FS_PREC_MIXED16 = 2
 
 _MESH_ARRAYS = [
     "x", "tri", "nn_map", "ncsn", "csn", "nnsn", "nsn", "esn", "edges", "c2e", "sub_edge", "sub_elem", "sub_n",
@@ -28,7 +29,7 @@ _MESH_ARRAYS = [
 ]
 _FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS", "LHS32"]
 _TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block", "deriv_lds",
-           "gs32_scaled"]
+           "gs32_scaled", "node_fused"]
 
 
 def _set_tuning(fn, h, cur, kw):

@@ -167,8 +167,14 @@ void fs_mesh_destroy(fs_mesh* m);
  * precision FS_PREC_MIXED: the analytic Jacobian (jacobian.C:301-658) and
  * the Gauss-Seidel sweeps in FP32 inside an FP64 defect-correction loop: the
  * residual (flux, reconstruction, limiter), the norms and the update stay
- * FP64, so the converged state is the FP64 residual's. */
-enum { FS_PREC_F64 = 0, FS_PREC_MIXED = 1 };
+ * FP64, so the converged state is the FP64 residual's.
+ * precision FS_PREC_MIXED16: FS_PREC_MIXED with the sweeps' off-diagonal
+ * operands (the Jacobian blocks pre-multiplied by their row's FP64 diagonal
+ * inverse) stored in FP16; the sweeps still compute in FP32 and everything
+ * else is as in FS_PREC_MIXED, so only the linear solve's iterates move (by
+ * the operands' rounding) and the converged state stays the FP64
+ * residual's.  An operand outside the FP16 range fails the instance. */
+enum { FS_PREC_F64 = 0, FS_PREC_MIXED = 1, FS_PREC_MIXED16 = 2 };
 int fs_solver_create(const fs_mesh* m, const fs_params* p, int precision, fs_solver** out);
 void fs_solver_destroy(fs_solver* s);
 /* init_pv with ic_0 freestream, ghosts copied (ic.C:36-99, ic.C:808-913) */
@@ -296,11 +302,14 @@ int fs_solver_use_graph(fs_solver* s, int on);
  * neighbour states staged in LDS per 32-node tile: 1, gathered through L2:
  * 0), gs32_scaled (FP32 sweeps on blocks and right-hand sides pre-multiplied
  * by the diagonal inverse once per Newton iteration: 1, applying the stored
- * inverse in every sweep: 0).  gs32_scaled is not tuned or cached: it is 1
- * when PARAMS.subits >= 30 and can be pinned with the setter; the two forms
- * round differently in FP32, so it moves the mixed path's iterates by FP32
- * rounding, never its converged state. */
-#define FS_TUNING_VALUES 9
+ * inverse in every sweep: 0), node_fused (pre-multiplied operands formed by
+ * the mixed node pass itself, with each row's blocks staged in LDS: 1, or by
+ * a separate pass over the stored blocks: 0; the same bits either way).
+ * gs32_scaled is not tuned or cached: it is 1 when PARAMS.subits >= 30 (and
+ * always with FS_PREC_MIXED16) and can be pinned with the setter; the two
+ * forms round differently in FP32, so it moves the mixed path's iterates by
+ * FP32 rounding, never its converged state. */
+#define FS_TUNING_VALUES 10
 int fs_solver_autotune(fs_solver* s, double* seconds);
 int fs_solver_get_tuning(const fs_solver* s, int* values, int* from_cache);
 /* sets the launch shapes (FS_TUNING_VALUES values in the order above; each
@@ -421,7 +430,7 @@ int fs_mesh_footprint(const fs_mesh* m, int precision, int64_t* bytes);
 typedef struct fs_server fs_server;
 typedef struct {
   int64_t device_budget;   /* bytes of device memory for resident solves (0: 80% of free) */
-  int precision;           /* FS_PREC_F64 or FS_PREC_MIXED */
+  int precision;           /* FS_PREC_F64, FS_PREC_MIXED or FS_PREC_MIXED16 */
   int max_steps;           /* step limit of a request (see max_steps_per_sqrt_node) */
   double tol;
   int prep_threads;        /* host preprocessing workers */

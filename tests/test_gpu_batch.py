@@ -53,7 +53,7 @@ def _build(precision):
     return meshes, [p for _, p in cases]
 
 
-@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED])
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED, fsm.FS_PREC_MIXED16])
 def test_batch_equals_alone(precision):
     meshes, params = _build(precision)
     print("case shapes", [(m.info()["nn"], m.info()["ncolor"]) for m in meshes], flush=True)
@@ -82,7 +82,7 @@ def test_batch_equals_alone(precision):
     assert len(set(steps.tolist())) > 1
 
 
-@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED])
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED, fsm.FS_PREC_MIXED16])
 def test_graph_replay_equals_eager(precision):
     p = _params(order=3, soits=2, limit=1, foits=2)
     m = fsm.Mesh.bump(33, 17, 0.2, 7)
@@ -323,3 +323,66 @@ def test_derivative_lds_tiles_equal_gathers(precision):
     for i, (a, c) in enumerate(zip(*res)):
         for x, y in zip(a, c):
             assert np.array_equal(x, y), "instance %d" % i
+
+
+def test_fused_operands_equal_diagonal_pass():
+    """With 40 sweeps the FP32 sweeps run on operands pre-multiplied by the
+    diagonal inverse.  Formed by the mixed node pass itself (each row's blocks
+    staged in LDS, its diagonal inverted in the same pass) they equal those of
+    the separate diagonal pass bit for bit: ten batch steps give the same
+    states, residuals and operands."""
+    meshes, params = _build(fsm.FS_PREC_MIXED)
+    params = [fsm.Params.make(**dict(p.as_dict(), subits=40)) for p in params]
+    res = []
+    for fused in (1, 0):
+        b = fsm.Batch(meshes, params, fsm.FS_PREC_MIXED)
+        b.set_tuning(node_fused=fused)
+        t = b.tuning()
+        assert t["gs32_scaled"] == 1 and t["node_fused"] == fused
+        b.init_freestream()
+        for _ in range(10):
+            b.step()
+        res.append([(b.get(i, "Q"), b.get(i, "R"), b.get(i, "LHS32")) for i in range(len(meshes))])
+        b.close()
+    for i, (a, c) in enumerate(zip(*res)):
+        print("instance %d: states, residuals and operands identical %s" %
+              (i, all(np.array_equal(x, y) for x, y in zip(a, c))), flush=True)
+        for x, y in zip(a, c):
+            assert np.array_equal(x, y), "instance %d" % i
+
+
+@pytest.mark.parametrize("order,soits,limit", [(1, 0, 0), (3, 2, 1)])
+def test_fp16_operands_are_the_fp32_operands_rounded(order, soits, limit):
+    """FS_PREC_MIXED16 keeps FS_PREC_MIXED's arithmetic and stores the
+    pre-multiplied off-diagonal sweep operands rounded to FP16: from the same
+    state, a step of each gives the same FP64 residual, the same FP32
+    diagonal blocks, and off-diagonal operands equal to the FP32 ones rounded
+    to FP16 (round to nearest even), in every phase."""
+    p = _params(order=order, soits=soits, limit=limit, foits=2, subits=40)
+    m = fsm.Mesh.bump(33, 17, 0.2, 7)
+    m.prepare(p)
+    a = fsm.Solver(m, p, fsm.FS_PREC_MIXED)
+    h = fsm.Solver(m, p, fsm.FS_PREC_MIXED16)
+    a.init_freestream()
+    h.init_freestream()
+    ia, iau = m.array("ia"), m.array("iau")
+    diag = np.zeros(ia[-1], dtype=bool)
+    diag[iau] = True
+    for ts in range(1, 7):
+        h.set("Q", a.get("Q"))
+        a.solve(ts)
+        h.solve(ts)
+        assert np.array_equal(a.get("R"), h.get("R"))
+        la = a.get("LHS32").reshape(-1, 16)
+        lh = h.get("LHS32").reshape(-1, 16)
+        assert np.array_equal(la[diag], lh[diag])
+        off32 = la[~diag].astype(np.float32)
+        want = off32.astype(np.float16).astype(np.float64)
+        nz = np.count_nonzero(off32)
+        rel = np.max(np.abs(lh[~diag] - off32)) / np.max(np.abs(off32))
+        print("ts %d: %d off-diagonal operands, %d non-zero, largest |operand| %.3e, max FP16 rounding %.2e of it" %
+              (ts, off32.size, nz, np.max(np.abs(off32)), rel), flush=True)
+        assert nz > 0
+        assert np.array_equal(lh[~diag], want)
+    a.close()
+    h.close()

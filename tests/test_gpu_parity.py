@@ -301,13 +301,15 @@ def test_mixed_fp32_jacobian_blocks():
     assert np.max(np.abs(R - Rr)) <= 1e-12 * np.max(np.abs(Rr))
 
 
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED, fsm.FS_PREC_MIXED16])
 @pytest.mark.parametrize("order,limit,soits", [(1, 0, 0), (3, 1, 10)])
-def test_mixed_converged_state_matches_fp64(order, limit, soits):
-    """Mixed precision reaches the same steady state as the FP64 path: the FP64
-    outer residual fixes the converged state; error against the oracle stated."""
+def test_mixed_converged_state_matches_fp64(order, limit, soits, precision):
+    """Mixed precision (FP32 sweep operands, or FP16 ones) reaches the same
+    steady state as the FP64 path: the FP64 outer residual fixes the
+    converged state; error against the oracle stated."""
     p, m, s, o = _pair(49, 17, 1, order=order, limit=limit, soits=soits, CFL=100.0, ramp=30, foits=10, subits=20,
                        alpha=0.0)
-    sm = fsm.Solver(m, p, fsm.FS_PREC_MIXED)
+    sm = fsm.Solver(m, p, precision)
     sm.init_freestream()
     steps_m, drop_m = sm.solve_steady(800, 1e-10)
     o.init_freestream()
