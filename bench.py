@@ -66,18 +66,21 @@ def algorithmic_bytes_edge_kernel(info, recon):
     return b
 
 
-def algorithmic_bytes_resid_jac32(info, recon):
+def algorithmic_bytes_resid_jac32(info, recon, fused=0, block_word=4):
     """Compulsory HBM bytes of one launch of the mixed path's node pass
     k_resid_jac32 (FP64 residual, ev_con/dt, FP32 Jacobian blocks):
     per node the state read (32 B), R, RHS, RHS32, ev_con and dt written
-    (32 + 32 + 16 + 8 + 8 B), cv_area read (8 B) and the FP32 diagonal block
-    written (64 B); per subedge its normal and length (24 B); per edge its two
-    node ids and two block slots (16 B) and its two FP32 off-diagonal blocks
-    written (128 B); per boundary edge its normal, flux and block read
-    (24 + 32 + 128 B).  With linear reconstruction (recon 1) also the
-    gradient, limiter value and coordinates per node (64 + 8 + 16 B) and the
-    face point per subedge (16 B)."""
-    b = info["nn"] * 200 + info["nsub"] * 24 + info["ne"] * 144 + info["nbe"] * 184
+    (32 + 32 + 16 + 8 + 8 B), cv_area and the diagonal slot read (8 + 4 B),
+    the FP32 diagonal block written (64 B) and the activity byte (1 B), and
+    when the pass forms the pre-multiplied sweep operands itself (fused) the
+    zeroed dQ (16 B); per subedge its normal and length (24 B); per edge its
+    two node ids and two block slots (16 B) and its two off-diagonal blocks
+    written (16 block words each: FP32, or FP16 operands); per boundary edge
+    its normal, flux and block read (24 + 32 + 128 B).  With linear
+    reconstruction (recon 1) also the gradient, limiter value and
+    coordinates per node (64 + 8 + 16 B) and the face point per subedge
+    (16 B)."""
+    b = info["nn"] * (205 + 16 * fused) + info["nsub"] * 24 + info["ne"] * (16 + 32 * block_word) + info["nbe"] * 184
     if recon == 1:
         b += info["nn"] * 88 + info["nsub"] * 16
     return b
@@ -96,23 +99,26 @@ def gs_schedule(ncolor, subits):
 
 
 def gs_row_bytes(word, scaled=0):
-    """Per-row bytes of one sweep launch besides the off-diagonal blocks, 12
-    of them row indices (position, degree, instance): FP64 (word 8) the
-    factored diagonal (16 words), RHS, dQ read and dQ written (4 words
-    each); FP32 (word 4) the diagonal inverse (16 words), RHS and dQ written,
-    or, with operands pre-multiplied by the inverse (scaled), only the
-    scaled right-hand side and dQ written."""
+    """Per-row bytes of one sweep launch besides the off-diagonal blocks.
+    FP64 (word 8): the factored diagonal (16 words), RHS, dQ read and dQ
+    written (4 words each) and 12 bytes of row indices (node, degree,
+    instance).  Mixed path (word 4, vectors by color position): the diagonal
+    inverse (16 words) unless the operands are pre-multiplied by it
+    (scaled), the right-hand side and dQ written (4 words each), the degree
+    and the activity byte (5 bytes)."""
     if word == 8:
         return 16 * word + 12 * word + 12
-    return (8 * word + 12) if scaled else (24 * word + 12)
+    return (8 * word + 5) if scaled else (24 * word + 5)
 
 
-def algorithmic_bytes_gauss_seidel(mesh, subits, word, scaled=0):
+def algorithmic_bytes_gauss_seidel(mesh, subits, word, scaled=0, block_word=None):
     """Compulsory HBM bytes of one linear solve's sweeps: every color launch
-    of the sweep schedule.  word = 8 (FP64) or 4.  Per launch of color c: its
-    rows' off-diagonal blocks (16 words) and column indices, the per-row
-    operands (gs_row_bytes), and the neighbours' dQ once (4 words per node of
-    the other colors)."""
+    of the sweep schedule.  word = 8 (FP64) or 4; block_word (default word)
+    = 2 for FP16 operands.  Per launch of color c: its rows' off-diagonal
+    blocks (16 block words) and column indices, the per-row operands
+    (gs_row_bytes), and the neighbours' dQ once (4 words per node of the
+    other colors)."""
+    block_word = block_word or word
     import numpy as np
     info = mesh.info()
     nn = info["nn"]
@@ -123,13 +129,13 @@ def algorithmic_bytes_gauss_seidel(mesh, subits, word, scaled=0):
     for c in range(info["ncolor"]):
         r = rows[co[c]:co[c + 1]]
         nr, nb = len(r), int(deg[r].sum())
-        per_color.append(nb * (16 * word + 4) + nr * gs_row_bytes(word, scaled) + (nn - nr) * 4 * word)
+        per_color.append(nb * (16 * block_word + 4) + nr * gs_row_bytes(word, scaled) + (nn - nr) * 4 * word)
     sched = gs_schedule(info["ncolor"], subits)
     total = sum(per_color[c] for c in sched)
     return total, len(sched)
 
 
-def algorithmic_bytes_gs_batch(meshes, subits, word, scaled=0):
+def algorithmic_bytes_gs_batch(meshes, subits, word, scaled=0, block_word=None):
     """The same for the sweeps of a packed batch, launch by launch as the
     engine issues them: one launch per global color of the shared schedule
     (over the largest color count), each over that color's rows of every
@@ -137,6 +143,7 @@ def algorithmic_bytes_gs_batch(meshes, subits, word, scaled=0):
     color is c (they repeat no turnaround), and an empty launch is not
     issued.  Returns (total bytes per linear solve, launches)."""
     import numpy as np
+    block_word = block_word or word
     per = []
     for m in meshes:
         info = m.info()
@@ -147,7 +154,7 @@ def algorithmic_bytes_gs_batch(meshes, subits, word, scaled=0):
         for c in range(info["ncolor"]):
             r = rows[co[c]:co[c + 1]]
             nr, nb = len(r), int(deg[r].sum())
-            cols.append(nb * (16 * word + 4) + nr * gs_row_bytes(word, scaled) + (nn - nr) * 4 * word)
+            cols.append(nb * (16 * block_word + 4) + nr * gs_row_bytes(word, scaled) + (nn - nr) * 4 * word)
         per.append((info["ncolor"], cols))
     ncmax = max(nc for nc, _ in per)
     total, launches, last = 0, 0, -1
@@ -207,9 +214,9 @@ def parse():
     ap.add_argument("--skip-headline", action="store_true",
                     help="run only the extras named in --sections (profiles of one section)")
     ap.add_argument("--no-config3-curved", action="store_true", help="skip the analytic-wall variant of config 3")
-    ap.add_argument("--sections", default="fp64,steady,plain,config2,config3,serving,cpu",
-                    help="extras beside the headline (a profiled run picks few): fp64, steady, plain, config2, "
-                         "config3, serving, cpu")
+    ap.add_argument("--sections", default="fp64,mixed16,steady,plain,config2,config3,serving,cpu",
+                    help="extras beside the headline (a profiled run picks few): fp64, mixed16, steady, plain, "
+                         "config2, config3, serving, cpu")
     return ap.parse_args()
 
 
@@ -294,35 +301,54 @@ def batch_throughput(a, fsm, meshes, params, infos, prec):
     b.init_freestream()
     apply_controls(b)
     b.step()
-    tuning = b.autotune()
+    b.autotune()
     for _ in range(warm - 1):
         b.step()
     tg = b.time_steps(a.steps)
     b.use_graph(False)
     t = b.time_steps(a.steps)
+    mixed_path = prec != fsm.FS_PREC_F64
+    tuning = b.tuning()
     out = {"value": t["flux_evals"] / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s", "batch": len(meshes),
            "ms_per_step": t["step_ms"], "flux_evals_per_step": t["flux_evals"], "warmup": warm,
            "graph_ms_per_step": tg["step_ms"], "graph_value": tg["flux_evals"] / (tg["step_ms"] * 1e-3),
            "device_bytes": b.info()["device_bytes"], "tuning": tuning,
-           "kernel_ms": {"boundary" if prec == fsm.FS_PREC_MIXED else "edge_flux_fd_jacobian": t["edge_ms"],
-                         "residual_jacobian_node_pass" if prec == fsm.FS_PREC_MIXED else "node_assembly": t["node_ms"],
+           "kernel_ms": {"boundary" if mixed_path else "edge_flux_fd_jacobian": t["edge_ms"],
+                         "residual_jacobian_node_pass" if mixed_path else "node_assembly": t["node_ms"],
                          "gauss_seidel": t["gs_ms"], "diag_factor": t["lu_ms"], "other": t["other_ms"]}}
     b.close()
-    word = 4 if prec == fsm.FS_PREC_MIXED else 8
-    gs_bytes, launches = algorithmic_bytes_gs_batch(meshes, params[0].subits, word, tuning.get("gs32_scaled", 0))
+    word = 4 if mixed_path else 8
+    block_word = 2 if prec == fsm.FS_PREC_MIXED16 else word
+    scaled = tuning.get("gs32_scaled", 0) if mixed_path else 0
+    gs_bytes, launches = algorithmic_bytes_gs_batch(meshes, params[0].subits, word, scaled, block_word)
     avg = t["gs_ms"] / launches
-    out["gauss_seidel"] = {"launches_per_step": launches, "algorithmic_bytes_per_launch": gs_bytes / launches,
+    out["gauss_seidel"] = {"kernel": sweep_kernel_name(prec, tuning), "launches_per_step": launches,
+                           "algorithmic_bytes_per_launch": gs_bytes / launches,
                            "avg_ms": avg, "achieved_GBs": gs_bytes / launches / (avg * 1e-3) / 1e9}
-    if prec == fsm.FS_PREC_MIXED:
-        rb = sum(algorithmic_bytes_resid_jac32(i, 0 if p.order == 3 else 1) for i, p in zip(infos, params))
+    if mixed_path:
+        fused = 1 if prec == fsm.FS_PREC_MIXED16 or (scaled and tuning.get("node_fused", 0)) else 0
+        rb = sum(algorithmic_bytes_resid_jac32(i, 0 if p.order == 3 else 1, fused, block_word)
+                 for i, p in zip(infos, params))
+        kname = "fsk::k_resid_jac32<%d>" % (2 if prec == fsm.FS_PREC_MIXED16 else fused)
         out["residual_kernel"] = {
-            "kernel": "k_resid_jac32 (FP64 residual, ev_con/dt, FP32 Jacobian blocks and right-hand sides, one "
-                      "node pass)", "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
+            "kernel": "%s (FP64 residual, ev_con/dt, FP32 Jacobian blocks and right-hand sides, one node pass%s)"
+                      % (kname, "; forms the pre-multiplied sweep operands" if fused else ""),
+            "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
             "achieved_GBs": rb / (t["node_ms"] * 1e-3) / 1e9, "frac": rb / (t["node_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "edge_flux_evals_per_s": t["flux_evals"] / (t["node_ms"] * 1e-3),
-            "traffic": measured_traffic("fsk::k_resid_jac32", "pmc_traffic_batch256_mixed.json")
-            if len(meshes) == 256 else None}
+            "traffic": measured_traffic(kname, "pmc_traffic_batch256_mixed.json") if len(meshes) == 256 else None}
     return out
+
+
+def sweep_kernel_name(prec, tuning):
+    """The sweep kernel a tuned engine of this precision launches (as rocprofv3 names it)."""
+    import fsutil as U
+    fsm = U.fsm
+    if prec == fsm.FS_PREC_F64:
+        return "fsk::k_gs_color" if tuning.get("gs_lanes", 4) == 4 else "fsk::k_gs_row"
+    if prec == fsm.FS_PREC_MIXED16:
+        return "fsk::k_gs_color16<%d>" % tuning["gs32_pre"]
+    return "fsk::k_gs_color32<%d, %s>" % (tuning["gs32_pre"], "true" if tuning["gs32_scaled"] else "false")
 
 
 def batch_steady(a, fsm, cases, meshes, params, infos, prec):
@@ -425,7 +451,7 @@ def bench_serving(a, precision):
     th = threading.Thread(target=collect)
     th.start()
     w0 = time.perf_counter()
-    name = "mixed" if precision == fsm.FS_PREC_MIXED else "fp64"
+    name = {fsm.FS_PREC_MIXED: "mixed", fsm.FS_PREC_MIXED16: "mixed16"}.get(precision, "fp64")
     last = 0.0
     rid_case = {}
     for k, (arr, m, p) in enumerate(zip(arrivals, meshes, params)):
@@ -507,7 +533,7 @@ def _config3_solves(a, fsm, mesh, p, info):
     import numpy as np
     out, rho, forces = {}, {}, {}
     evals = info["nsub"] + info["nbe"]
-    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed"), (fsm.FS_PREC_MIXED16, "mixed16")):
         s = fsm.Solver(mesh, p, prec)
         s.use_graph(True)
         s.init_freestream()
@@ -529,7 +555,7 @@ def _config3_solves(a, fsm, mesh, p, info):
                      "lift": float(forces[name][0]), "drag": float(forces[name][1])}
         s.close()
         progress("config3 %s: %d steps, drop %.2e, %.1f s" % (name, steps, drop, wall))
-    if len(rho) < 2:
+    if "fp64" not in rho or "mixed" not in rho:
         return out
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     out["time_to_solution_speedup"] = out["fp64"]["wall_s"] / out["mixed"]["wall_s"]
@@ -538,6 +564,13 @@ def _config3_solves(a, fsm, mesh, p, info):
     rel = np.abs(forces["mixed"] - forces["fp64"]) / np.abs(forces["fp64"])
     out["lift_rel_diff_mixed_vs_fp64"] = float(rel[0])
     out["drag_rel_diff_mixed_vs_fp64"] = float(rel[1])
+    if "mixed16" in rho:
+        rel = np.abs(forces["mixed16"] - forces["fp64"]) / np.abs(forces["fp64"])
+        out["mixed16_vs_fp64"] = {
+            "speedup": out["mixed16"]["value"] / out["fp64"]["value"],
+            "time_to_solution_speedup": out["fp64"]["wall_s"] / out["mixed16"]["wall_s"],
+            "density_l2_diff": float(np.linalg.norm(rho["mixed16"] - rho["fp64"]) / np.linalg.norm(rho["fp64"])),
+            "lift_rel_diff": float(rel[0]), "drag_rel_diff": float(rel[1])}
     return out
 
 
@@ -564,7 +597,7 @@ def bench_config3(a, fsm):
         # short run: the first k=2-phase steps, launched one kernel at a time
         # (no hipGraph), with kernel times from events
         warm = p.foits + p.soits
-        for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+        for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed"), (fsm.FS_PREC_MIXED16, "mixed16")):
             s = fsm.Solver(mesh, p, prec)
             s.init_freestream()
             for ts in range(1, warm + 1):
@@ -734,18 +767,21 @@ def bench_config2(a, fsm):
            "nodes": info["nn"], "subedges": info["nsub"], "bedges": info["nbe"], "lhs_blocks": info["nnz"],
            "subits": p.subits, "steps": a.config2_steps, "warmup": warm, "setup_s": t_setup,
            "jacobian_flux_evals_per_step_fp64": 8 * info["nsub"]}
-    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed")):
+    for prec, name in ((fsm.FS_PREC_F64, "fp64"), (fsm.FS_PREC_MIXED, "mixed"), (fsm.FS_PREC_MIXED16, "mixed16")):
         s = fsm.Solver(mesh, p, prec)
         s.init_freestream()
         for ts in range(1, warm + 1):
             s.solve(ts)
         t = s.time_steps(warm + 1, a.config2_steps)
-        scaled = s.tuning()["gs32_scaled"]
+        tuning = s.tuning()
+        scaled = tuning["gs32_scaled"]
         s.close()
         word = 8 if prec == fsm.FS_PREC_F64 else 4
-        gs_bytes, gs_launches = algorithmic_bytes_gauss_seidel(mesh, p.subits, word, scaled)
+        block_word = 2 if prec == fsm.FS_PREC_MIXED16 else word
+        gs_bytes, gs_launches = algorithmic_bytes_gauss_seidel(mesh, p.subits, word, scaled, block_word)
         r = {"value": evals / (t["step_ms"] * 1e-3), "unit": "edge-flux evals/s", "ms_per_step": t["step_ms"],
-             "gauss_seidel": {"launches_per_step": gs_launches, "avg_ms": t["gs_ms"] / gs_launches,
+             "gauss_seidel": {"kernel": sweep_kernel_name(prec, tuning), "launches_per_step": gs_launches,
+                              "avg_ms": t["gs_ms"] / gs_launches,
                               "algorithmic_bytes_per_launch": gs_bytes / gs_launches,
                               "achieved_GBs": gs_bytes / (t["gs_ms"] * 1e-3) / 1e9}}
         if prec == fsm.FS_PREC_F64:
@@ -755,17 +791,21 @@ def bench_config2(a, fsm):
             r["edge_kernel"] = {"kernel": "k_edge_flux (Roe flux + FD Jacobian blocks, FP64 VALU)",
                                 "avg_ms": t["edge_ms"], "algorithmic_bytes": ab,
                                 "achieved_GBs": ab / (t["edge_ms"] * 1e-3) / 1e9}
-            r["gauss_seidel"]["traffic"] = measured_traffic("fsk::k_gs_color")
+            r["gauss_seidel"]["traffic"] = measured_traffic(r["gauss_seidel"]["kernel"])
         else:
-            rb = algorithmic_bytes_resid_jac32(info, 0)
+            fused = 1 if prec == fsm.FS_PREC_MIXED16 or (scaled and tuning["node_fused"]) else 0
+            kname = "fsk::k_resid_jac32<%d>" % (2 if prec == fsm.FS_PREC_MIXED16 else fused)
+            rb = algorithmic_bytes_resid_jac32(info, 0, fused, block_word)
             r["kernel_ms"] = {"boundary": t["edge_ms"], "residual_jacobian_node_pass": t["node_ms"],
                               "gauss_seidel": t["gs_ms"], "diag_inverse": t["lu_ms"], "other": t["other_ms"]}
-            r["residual_kernel"] = {"kernel": "k_resid_jac32", "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
+            r["residual_kernel"] = {"kernel": kname, "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
                                     "achieved_GBs": rb / (t["node_ms"] * 1e-3) / 1e9,
                                     "frac": rb / (t["node_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                    "traffic": measured_traffic("fsk::k_resid_jac32")}
+                                    "traffic": measured_traffic(kname)}
+            r["gauss_seidel"]["traffic"] = measured_traffic(r["gauss_seidel"]["kernel"])
         out[name] = r
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    out["speedup_mixed16_vs_fp64"] = out["mixed16"]["value"] / out["fp64"]["value"]
     return out, mesh, p
 
 
@@ -805,14 +845,13 @@ def main():
                                     "Jacobian and Gauss-Seidel sweeps), one launch per kernel per pseudo-time step, "
                                     "final-phase steps",
                         "precision": "mixed"}, **desc),
-        "roofline": {"bound": "hbm", "kernel": "k_gs_color32 (multicolor block Gauss-Seidel sweep, FP32 blocks; "
-                                               "%.0f%% of the step)" % (100 * mixed["kernel_ms"]["gauss_seidel"] /
-                                                                        mixed["ms_per_step"]),
+        "roofline": {"bound": "hbm", "kernel": "%s (multicolor block Gauss-Seidel sweep, FP32 blocks; "
+                                               "%.0f%% of the step)" % (gs["kernel"], 100 * mixed["kernel_ms"][
+                                                   "gauss_seidel"] / mixed["ms_per_step"]),
                      "achieved": gs["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gs["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("fsk::k_gs_color32<%d, %s>" % (
-                         mixed["tuning"]["gs32_pre"], "true" if mixed["tuning"]["gs32_scaled"] else "false"),
-                                                 "pmc_traffic_batch256_mixed.json") if a.batch == 256 else None,
+                     "traffic": measured_traffic(gs["kernel"], "pmc_traffic_batch256_mixed.json")
+                     if a.batch == 256 else None,
                      "algorithmic_bytes": gs["algorithmic_bytes_per_launch"],
                      "launches_per_step": gs["launches_per_step"], "avg_ms": gs["avg_ms"]},
         "residual_kernel": mixed["residual_kernel"],
@@ -826,13 +865,43 @@ def main():
         line["fp64"] = f
         line["speedup_mixed_vs_fp64"] = mixed["value"] / f["value"]
         progress("fp64 batch: %.3f ms/step" % f["ms_per_step"])
+    if "mixed16" in sections:
+        # the same path with the sweep operands stored in FP16 (FP32 sweep
+        # arithmetic): an option beside the headline, not the headline
+        h = batch_throughput(a, fsm, meshes, params, infos, fsm.FS_PREC_MIXED16)
+        h["roofline"] = {"kernel": h["gauss_seidel"]["kernel"], "achieved": h["gauss_seidel"]["achieved_GBs"],
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": h["gauss_seidel"]["achieved_GBs"] / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(h["gauss_seidel"]["kernel"], "pmc_traffic_batch256_mixed.json")
+                         if a.batch == 256 else None}
+        line["mixed16"] = h
+        line["speedup_mixed16_vs_mixed"] = h["value"] / mixed["value"]
+        if "fp64" in line:
+            line["speedup_mixed16_vs_fp64"] = h["value"] / line["fp64"]["value"]
+        progress("mixed16 batch: %.3f ms/step" % h["ms_per_step"])
     rho, steps = {}, {}
     if "steady" in sections and a.batch_max_steps > 0:
-        st, fell = {}, set()
-        for prec, name in ((fsm.FS_PREC_MIXED, "mixed"), (fsm.FS_PREC_F64, "fp64")):
+        st, fell, fell16 = {}, set(), set()
+        runs = [(fsm.FS_PREC_MIXED, "mixed"), (fsm.FS_PREC_F64, "fp64")]
+        if "mixed16" in sections:
+            runs.insert(1, (fsm.FS_PREC_MIXED16, "mixed16"))
+        for prec, name in runs:
             st[name], rho[name], steps[name], f = batch_steady(a, fsm, cases, meshes, params, infos, prec)
-            fell |= f
+            if name == "mixed16":
+                fell16 = f
+            else:
+                fell |= f
             progress("steady %s: %d converged in %.1f s" % (name, st[name]["converged"], st[name]["wall_s"]))
+        if "mixed16" in rho:
+            # FP16 sweep operands: the same comparisons against the FP64 path
+            b16 = sorted(set(rho["fp64"]) & set(rho["mixed16"]))
+            d16 = [float(np.linalg.norm(rho["mixed16"][i] - rho["fp64"][i]) / np.linalg.norm(rho["fp64"][i]))
+                   for i in b16 if i not in fell | fell16]
+            st["mixed16_vs_fp64"] = {"converged_both": len(b16), "compared": len(d16),
+                                     "max_density_l2_diff": max(d16) if d16 else None,
+                                     "steps_total_mixed16": int(sum(steps["mixed16"][i] for i in b16)),
+                                     "steps_total_fp64": int(sum(steps["fp64"][i] for i in b16)),
+                                     "time_to_solution_speedup": st["fp64"]["wall_s"] / st["mixed16"]["wall_s"]}
         both = sorted(set(rho["fp64"]) & set(rho["mixed"]))
         diff = {i: float(np.linalg.norm(rho["mixed"][i] - rho["fp64"][i]) / np.linalg.norm(rho["fp64"][i]))
                 for i in both}
@@ -870,7 +939,7 @@ def main():
     if "config3" in sections:
         line["config3"] = bench_config3(a, fsm)
     if "serving" in sections and a.serve_seconds > 0:
-        precs = {"mixed": fsm.FS_PREC_MIXED, "fp64": fsm.FS_PREC_F64}
+        precs = {"mixed": fsm.FS_PREC_MIXED, "fp64": fsm.FS_PREC_F64, "mixed16": fsm.FS_PREC_MIXED16}
         sv = {k: bench_serving(a, precs[k]) for k in a.serve_precisions.split(",") if k}
         sv["controls"] = dict(workload_controls(), max_steps=a.serve_max_steps,
                               max_steps_per_sqrt_node=SERVE_STEPS_PER_SQRT_NODE)

@@ -423,21 +423,33 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   // ---- sliced block slots ----
   // slice s = positions 16s..16s+15; its off-diagonal blocks occupy
   // 16 * (largest row degree of the slice) slots starting at sbase[s]
+  // (when every slice padded to the largest row degree costs at most a
+  // quarter more slots, all slices get that width: a sweep then computes a
+  // row's first slot instead of loading it)
   const long long nslice = (NN + 15) / 16;
   std::vector<int> sbase(nslice + 1, 0);
   {
     long long acc = 0;
+    int wmax = 0;
     for (long long sl = 0; sl < nslice; ++sl) {
       int mx = 0;
       for (long long t = 16 * sl; t < std::min(NN, 16 * sl + 16); ++t) mx = std::max(mx, pdeg[t]);
       sbase[sl] = (int)acc;
       acc += 16LL * mx;
-      if (acc + NN >= (1LL << 31)) return set_error(FS_ERR_ARG, "batch too large for 32-bit block slots");
+      wmax = std::max(wmax, mx);
     }
+    d.slice_w = 0;
+    if (16LL * wmax * nslice <= acc + acc / 4) {
+      d.slice_w = wmax;
+      acc = 16LL * wmax * nslice;
+      for (long long sl = 0; sl <= nslice; ++sl) sbase[sl] = (int)std::min<long long>(16LL * wmax * sl, acc);
+    }
+    if (acc + NN + 16LL * fsk::kJaPad >= (1LL << 31))
+      return set_error(FS_ERR_ARG, "batch too large for 32-bit block slots");
     sbase[nslice] = (int)acc;
   }
   const long long NOFF = sbase[nslice], NSLOT = NOFF + NN;
-  std::vector<int> ja_d(NOFF, 0), iau_d(NN), roff(NN + 1, 0), tslot;
+  std::vector<int> ja_d(NOFF + 16LL * fsk::kJaPad, 0), iau_d(NN), roff(NN + 1, 0), tslot;
   tslot.reserve(NNZ);
   for (auto& I : inst) I.slot_dev.assign(I.nnz, 0);
   for (int i = 0; i < B; ++i) {
@@ -450,7 +462,8 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       for (int k = g.ia[j]; k < g.ia[j + 1] - 1; ++k) {
         long long slot = base + 16LL * (k - g.ia[j]);
         I.slot_dev[k] = (int)slot;
-        ja_d[slot] = g.ja[k] + I.noff;
+        // the mixed sweeps index their vectors by color position
+        ja_d[slot] = mixed ? pos_of[g.ja[k] + I.noff] : g.ja[k] + I.noff;
       }
       I.slot_dev[g.iau[j]] = (int)(NOFF + t);
       iau_d[J] = (int)(NOFF + t);
@@ -533,6 +546,10 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       (rc = alloc(&st.chunk_part, 2 * (size_t)std::max(1, nc.nchunk))) ||
       (rc = alloc(&d_ip_, (size_t)iter * B)))
     return rc;
+  if (mixed) {
+    if ((rc = alloc(&st.pos_act, nn))) return rc;
+    FSE_HIP(hipMemset(st.pos_act, 0, nn));
+  }
   if (precision == FS_PREC_MIXED) {
     if ((rc = alloc(&st.LHS32, (size_t)NSLOT * 16)) || (rc = alloc(&st.LU32, nn * 16)) ||
         (rc = alloc(&st.RHS32, nn * 4)) || (rc = alloc(&st.dQ32, nn * 4)))

@@ -62,11 +62,16 @@ struct Tuning {
 // nodes; a tile's distinct stencil neighbours (at most kDerivTileMaxU) are
 // staged in LDS once and its stencil entries address them by slot
 constexpr int kDerivTileNodes = 32;
+
+// block-column padding: the mixed sweeps read up to 8 column entries of a
+// row without testing its degree
+constexpr int kJaPad = 8;
 constexpr int kDerivTileMaxU = 576;
 
 struct DevMesh {
   int nn, ng, ne, nsub, nbe, nnz, ncolor, ninst;
   int maxdeg;               // largest number of edges at one node (off-diagonal blocks of a row)
+  int slice_w;              // 0, or the uniform slice width: sbase[s] = 16 slice_w s (mixed sweeps compute it)
   Tuning tune;
   const int* inst_node;     // nn
   const int* inst_edge;     // ne
@@ -115,7 +120,8 @@ struct DevMesh {
   const int* roff;          // nn+1: node n's off-diagonal entries roff[n]..roff[n+1]
   const int* tslot;         // per entry: slot of the transposed block (m,n) of entry (n,m)
   const int* iau;           // nn: slot of the diagonal block
-  const int* ja;            // per off-diagonal slot: column node (padding slots: 0)
+  const int* ja;            // per off-diagonal slot: column node (padding slots: 0); on the mixed path the
+                            // column's color position; 16 kJaPad padding entries follow the last slot
   const int* pdeg;          // nn, by position: off-diagonal blocks of the row
   const int* sbase;         // nn/16+1: first slot of each slice
   const int* color_rows;    // nn (rows grouped by color)
@@ -152,10 +158,11 @@ struct DevState {
   // mixed precision (FP32 Jacobian blocks and sweeps)
   float* LHS32;      // nnz*16, color-major slots
   float* LU32;       // nn*16, color position: inverse of the diagonal block (pivoted, FP64, stored FP32)
-  float* RHS32;      // nn*4: the FP32 right-hand side; with pre-multiplied sweep operands, after the
-                     // diagonal pass, its product with the row's diagonal inverse (and the off-diagonal
+  float* RHS32;      // nn*4 by color position: the FP32 right-hand side; with pre-multiplied sweep
+                     // operands its product with the row's diagonal inverse (and the off-diagonal
                      // blocks of LHS32 likewise)
-  float* dQ32;       // nn*4
+  float* dQ32;       // nn*4 by color position
+  unsigned char* pos_act;  // nn by color position: the row's instance is active (written before the sweeps)
   float* D32;        // nn*16 by color position: the FP32 diagonal blocks (the tail of LHS32, slots
                      // nnz-nn.., when LHS32 exists)
   unsigned short* LHS16;  // FS_PREC_MIXED16 only: (nnz-nn)*16 FP16 bit patterns, the off-diagonal slots

@@ -192,6 +192,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s
   const int inst = inst_of(m, m.inst_node, n);
   const InstParams& p = s.ip[inst];
   if (!p.active) return;
+  const int t = m.iau[n] - (m.nnz - m.nn);   // the row's color position
   const int jac = p.jac, recon = p.recon_flux;
   const double g = p.gamma, g1 = p.gamma - 1.0;
   const float gf = (float)p.gamma, gm1 = (float)(p.gamma - 1.0);
@@ -366,13 +367,12 @@ __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s
     }
   }
   const float4 dv = make_float4((float)D[0], (float)D[1], (float)D[2], (float)D[3]);
-  const int t = m.iau[n] - (m.nnz - m.nn);   // the row's color position
   *reinterpret_cast<float4*>(s.D32 + 16 * (size_t)t + 4 * v) = dv;
   s.R[4 * (size_t)n + v] = Rv;
   s.RHS[4 * (size_t)n + v] = -Rv;
   const float bv = (float)(-Rv);
   if (kOut == 0) {
-    s.RHS32[4 * (size_t)n + v] = bv;
+    s.RHS32[4 * (size_t)t + v] = bv;
     return;
   }
   // the stored diagonal block in every lane of the quad, and its inverse
@@ -392,8 +392,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s
 #pragma unroll
   for (int i = 0; i < 4; ++i) di[i] = v == 0 ? Di[i] : v == 1 ? Di[4 + i] : v == 2 ? Di[8 + i] : Di[12 + i];
   const float4 rb = make_float4(quad_bcast<0>(bv), quad_bcast<1>(bv), quad_bcast<2>(bv), quad_bcast<3>(bv));
-  s.RHS32[4 * (size_t)n + v] = scale_rhs_entry(di, rb);
-  s.dQ32[4 * (size_t)n + v] = 0.0f;
+  s.RHS32[4 * (size_t)t + v] = scale_rhs_entry(di, rb);
+  s.dQ32[4 * (size_t)t + v] = 0.0f;
   // the quad's staged blocks were written by its other lanes
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstPa
     A[4 * k] = r.x; A[4 * k + 1] = r.y; A[4 * k + 2] = r.z; A[4 * k + 3] = r.w;
   }
   inv4_pivoted(A, Di);
-  *reinterpret_cast<float4*>(dQ32 + 4 * (size_t)n) = make_float4(0.f, 0.f, 0.f, 0.f);
+  *reinterpret_cast<float4*>(dQ32 + 4 * (size_t)t) = make_float4(0.f, 0.f, 0.f, 0.f);
   if (!kScale) {
     float4* o = reinterpret_cast<float4*>(LU32 + 16 * (size_t)t);
 #pragma unroll
@@ -472,36 +472,45 @@ __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstPa
 #pragma unroll
     for (int r = 0; r < 4; ++r) O[r] = scale_block_row(Di + 4 * r, b);
   }
-  float4* R = reinterpret_cast<float4*>(RHS32 + 4 * (size_t)n);
+  float4* R = reinterpret_cast<float4*>(RHS32 + 4 * (size_t)t);
   const float4 rb = *R;
   *R = make_float4(scale_rhs_entry(Di, rb), scale_rhs_entry(Di + 4, rb), scale_rhs_entry(Di + 8, rb),
                    scale_rhs_entry(Di + 12, rb));
 }
 
-// FP32 multicolor sweep over rows [row_begin,row_end) of the color order;
-// 4 lanes per row (lane v: component v), quad-wide FP32 block rows of the
-// sliced layout; the row's index, block and neighbour loads are issued
-// before the arithmetic.  Unscaled form: r = b - sum A dQ_nb, then lane v
-// applies row v of the stored diagonal inverse to the shuffled r.  Scaled
-// form (kScale): the blocks and right-hand side already carry the inverse
-// (k_diag_lu32<true>), so dQ_row[v] = c[v] - sum (Di A)[v,:] dQ_nb.
+// Multicolor sweeps of the mixed path over positions [row_begin,row_end) of
+// the color order, 4 lanes per row (lane v: component v).  The sweep vectors
+// (right-hand side, dQ) are indexed by color position and the column indices
+// of the blocks are neighbour positions, so a wave's right-hand side reads
+// and dQ writes are contiguous.  A row needs two dependent memory round
+// trips: first its degree, activity byte, right-hand side and up to kPre
+// neighbour positions (with a uniform slice width the slot base is
+// arithmetic; the column array is padded so these loads need no bounds
+// test), then its blocks and the neighbours' dQ.
+__device__ __forceinline__ size_t gs_slot_base(const DevMesh& m, int t) {
+  return m.slice_w ? (size_t)16 * m.slice_w * (size_t)(t >> 4) + (t & 15) : (size_t)m.sbase[t >> 4] + (t & 15);
+}
+
+// FP32 blocks.  Unscaled form: r = b - sum A dQ_nb, then lane v applies row
+// v of the stored diagonal inverse to the shuffled r.  Scaled form (kScale):
+// the blocks and right-hand side already carry the inverse (k_diag_lu32<true>
+// or the fused node pass), so dQ_row[v] = c[v] - sum (Di A)[v,:] dQ_nb.
 template <int kPre, bool kScale>
-__global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const InstParams* __restrict__ ip,
+__global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const unsigned char* __restrict__ pos_act,
                                                        const float* __restrict__ LHS32, const float* __restrict__ LU32,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
                                                        int row_begin, int row_end) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
   int t = row_begin + (tid >> 2), v = tid & 3;
   if (t >= row_end) return;
-  int j = m.color_rows[t];
-  int inst = inst_of(m, m.pos_inst, t);
-  int deg = m.pdeg[t];
-  size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
-  float r = RHS32[4 * (size_t)j + v];
-  if (!ip[inst].active) return;
+  const size_t base = gs_slot_base(m, t);
+  const int deg = m.pdeg[t];
+  const int act = pos_act[t];
+  float r = RHS32[4 * (size_t)t + v];
   int nb[kPre];
 #pragma unroll
-  for (int i = 0; i < kPre; ++i) nb[i] = (i < deg) ? m.ja[base + 16 * i] : j;
+  for (int i = 0; i < kPre; ++i) nb[i] = m.ja[base + 16 * i];
+  if (!act) return;
   float4 A[kPre], q[kPre];
 #pragma unroll
   for (int i = 0; i < kPre; ++i) {
@@ -520,37 +529,36 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const InstP
     r -= a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
   }
   if (kScale) {
-    dQ32[4 * (size_t)j + v] = r;
+    dQ32[4 * (size_t)t + v] = r;
     return;
   }
   float b0 = __shfl(r, 0, 4), b1 = __shfl(r, 1, 4), b2 = __shfl(r, 2, 4), b3 = __shfl(r, 3, 4);
   // lane v: row v of the diagonal block's inverse
   const float4 Di = *reinterpret_cast<const float4*>(LU32 + 16 * (size_t)t + 4 * v);
-  dQ32[4 * (size_t)j + v] = Di.x * b0 + Di.y * b1 + Di.z * b2 + Di.w * b3;
+  dQ32[4 * (size_t)t + v] = Di.x * b0 + Di.y * b1 + Di.z * b2 + Di.w * b3;
 }
 
-// The pre-multiplied sweep (k_gs_color32<kPre, true>) over FP16 operands:
-// lane v reads row v of each off-diagonal block as 4 FP16 values (8 bytes;
-// a wave reads block e of its 16 rows as one contiguous 512-byte run) and
-// accumulates in FP32, dQ_row[v] = c[v] - sum (Di A)[v,:] dQ_nb.
+// The pre-multiplied sweep over FP16 operands: lane v reads row v of each
+// off-diagonal block as 4 FP16 values (8 bytes; a wave reads block e of its
+// 16 rows as one contiguous 512-byte run) and accumulates in FP32,
+// dQ_row[v] = c[v] - sum (Di A)[v,:] dQ_nb.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 template <int kPre>
-__global__ void __launch_bounds__(kMaxBlock) k_gs_color16(DevMesh m, const InstParams* __restrict__ ip,
+__global__ void __launch_bounds__(kMaxBlock) k_gs_color16(DevMesh m, const unsigned char* __restrict__ pos_act,
                                                        const unsigned short* __restrict__ LHS16,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
                                                        int row_begin, int row_end) {
   int tid = xcd_block() * blockDim.x + threadIdx.x;
   int t = row_begin + (tid >> 2), v = tid & 3;
   if (t >= row_end) return;
-  int j = m.color_rows[t];
-  int inst = inst_of(m, m.pos_inst, t);
-  int deg = m.pdeg[t];
-  size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
-  float r = RHS32[4 * (size_t)j + v];
-  if (!ip[inst].active) return;
+  const size_t base = gs_slot_base(m, t);
+  const int deg = m.pdeg[t];
+  const int act = pos_act[t];
+  float r = RHS32[4 * (size_t)t + v];
   int nb[kPre];
 #pragma unroll
-  for (int i = 0; i < kPre; ++i) nb[i] = (i < deg) ? m.ja[base + 16 * i] : j;
+  for (int i = 0; i < kPre; ++i) nb[i] = m.ja[base + 16 * i];
+  if (!act) return;
   half4_t A[kPre];
   float4 q[kPre];
 #pragma unroll
@@ -570,7 +578,16 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color16(DevMesh m, const InstP
     const float4 b = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)m.ja[k]);
     r -= (float)a.x * b.x + (float)a.y * b.y + (float)a.z * b.z + (float)a.w * b.w;
   }
-  dQ32[4 * (size_t)j + v] = r;
+  dQ32[4 * (size_t)t + v] = r;
+}
+
+// The activity byte of every color position (its instance takes part in
+// this Newton iteration), read by the sweeps with the row's other data
+__global__ void __launch_bounds__(kBlock) k_pos_active(DevMesh m, const InstParams* __restrict__ ip,
+                                                       unsigned char* __restrict__ pos_act) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m.nn) return;
+  pos_act[t] = ip[inst_of(m, m.pos_inst, t)].active ? 1 : 0;
 }
 
 // nQ += dQ (FP64 state, FP32 correction) and the FP64 norms' inputs
@@ -579,8 +596,10 @@ __global__ void __launch_bounds__(kBlock) k_update_mixed(DevMesh m, const InstPa
                                                          double* __restrict__ dQ) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 4 * m.nn) return;
-  if (!ip[inst_of(m, m.inst_node, i >> 2)].active) return;
-  double d = (double)dQ32[i];
+  const int n = i >> 2;
+  if (!ip[inst_of(m, m.inst_node, n)].active) return;
+  // dQ32 is indexed by color position
+  double d = (double)dQ32[4 * (size_t)(m.iau[n] - (m.nnz - m.nn)) + (i & 3)];
   dQ[i] = d;
   nQ[i] += d;
 }
@@ -633,17 +652,18 @@ namespace {
 template <int kPre>
 void gs32_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
   if (s.LHS16)
-    hipLaunchKernelGGL((k_gs_color16<kPre>), grid, block, 0, st, m, s.ip, s.LHS16, s.RHS32, s.dQ32, b, e);
+    hipLaunchKernelGGL((k_gs_color16<kPre>), grid, block, 0, st, m, s.pos_act, s.LHS16, s.RHS32, s.dQ32, b, e);
   else if (m.tune.gs32_scaled)
-    hipLaunchKernelGGL((k_gs_color32<kPre, true>), grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
+    hipLaunchKernelGGL((k_gs_color32<kPre, true>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
                        e);
   else
-    hipLaunchKernelGGL((k_gs_color32<kPre, false>), grid, block, 0, st, m, s.ip, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
+    hipLaunchKernelGGL((k_gs_color32<kPre, false>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
                        e);
 }
 }  // namespace
 
 void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st) {
+  if (m.nn > 0) hipLaunchKernelGGL(k_pos_active, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.ip, s.pos_act);
   std::vector<GsLaunch> sched((size_t)subits * m.ncolor + 1);
   int ns = gs_schedule(m.ncolor, subits, sched.data());
   for (int i = 0; i < ns; ++i) {
