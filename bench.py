@@ -346,9 +346,10 @@ def sweep_kernel_name(prec, tuning):
     fsm = U.fsm
     if prec == fsm.FS_PREC_F64:
         return "fsk::k_gs_color" if tuning.get("gs_lanes", 4) == 4 else "fsk::k_gs_row"
+    tf = lambda v: "true" if v else "false"
     if prec == fsm.FS_PREC_MIXED16:
-        return "fsk::k_gs_color16<%d>" % tuning["gs32_pre"]
-    return "fsk::k_gs_color32<%d, %s>" % (tuning["gs32_pre"], "true" if tuning["gs32_scaled"] else "false")
+        return "fsk::k_gs_color16<%d, %s>" % (tuning["gs32_pre"], tf(tuning["gs32_dpp"]))
+    return "fsk::k_gs_color32<%d, %s, %s>" % (tuning["gs32_pre"], tf(tuning["gs32_scaled"]), tf(tuning["gs32_dpp"]))
 
 
 def batch_steady(a, fsm, cases, meshes, params, infos, prec):

@@ -114,7 +114,8 @@ bool valid_tuning(const fsk::Tuning& t) {
   return valid_block(t.gs_block) && (t.gs_lanes == 1 || t.gs_lanes == 4) && valid_block(t.gs32_block) &&
          (t.gs32_pre == 4 || t.gs32_pre == 6 || t.gs32_pre == 8) && valid_block(t.node_block) &&
          valid_block(t.face_block) && valid_block(t.cell_block) && (t.deriv_lds == 0 || t.deriv_lds == 1) &&
-         (t.gs32_scaled == 0 || t.gs32_scaled == 1) && (t.node_fused == 0 || t.node_fused == 1);
+         (t.gs32_scaled == 0 || t.gs32_scaled == 1) && (t.node_fused == 0 || t.node_fused == 1) &&
+         (t.gs32_dpp == 0 || t.gs32_dpp == 1);
 }
 // FS_TUNE_CACHE, else $HOME/.cache/flowsolver/tune_cache.txt
 std::string tune_cache_path() {
@@ -594,7 +595,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
       std::string k;
       fsk::Tuning t;
       if (ls >> k >> t.gs_block >> t.gs_lanes >> t.gs32_block >> t.gs32_pre >> t.node_block >> t.face_block >>
-          t.cell_block >> t.deriv_lds >> t.node_fused && k == key) {
+          t.cell_block >> t.deriv_lds >> t.node_fused >> t.gs32_dpp && k == key) {
         // the sweep form is the engine's own (it changes FP32 rounding)
         t.gs32_scaled = dm.tune.gs32_scaled;
         if (!valid_tuning(t)) continue;
@@ -1054,6 +1055,7 @@ int Engine::set_tuning(const int* v) {
   t.gs_block = v[0]; t.gs_lanes = v[1]; t.gs32_block = v[2]; t.gs32_pre = v[3];
   t.node_block = v[4]; t.face_block = v[5]; t.cell_block = v[6]; t.deriv_lds = v[7]; t.gs32_scaled = v[8];
   t.node_fused = v[9];
+  t.gs32_dpp = v[10];
   if (!valid_tuning(t)) return set_error(FS_ERR_ARG, "tuning value outside the candidates");
   if (precision == FS_PREC_MIXED16 && !t.gs32_scaled)
     return set_error(FS_ERR_ARG, "FP16 sweep operands exist only pre-multiplied (gs32_scaled 1)");
@@ -1093,15 +1095,18 @@ int Engine::autotune(double* seconds) {
   if (precision != FS_PREC_F64) {
     best = 1e30f;
     fsk::Tuning keep = tu;
-    for (int pre : {4, 6, 8})
-      for (int b : blocks) {
-        tu.gs32_pre = pre;
-        tu.gs32_block = b;
-        if ((rc = time_it([&] { fsk::launch_gs32_sweeps(dm, s, cr, 2, st); }, 2, &t))) return rc;
-        if (t < best) { best = t; keep.gs32_pre = pre; keep.gs32_block = b; }
-      }
+    for (int dpp : {0, 1})
+      for (int pre : {4, 6, 8})
+        for (int b : blocks) {
+          tu.gs32_dpp = dpp;
+          tu.gs32_pre = pre;
+          tu.gs32_block = b;
+          if ((rc = time_it([&] { fsk::launch_gs32_sweeps(dm, s, cr, 2, st); }, 2, &t))) return rc;
+          if (t < best) { best = t; keep.gs32_pre = pre; keep.gs32_block = b; keep.gs32_dpp = dpp; }
+        }
     tu.gs32_pre = keep.gs32_pre;
     tu.gs32_block = keep.gs32_block;
+    tu.gs32_dpp = keep.gs32_dpp;
   } else {
     best = 1e30f;
     fsk::Tuning keep = tu;
@@ -1207,7 +1212,7 @@ int Engine::autotune(double* seconds) {
     if (out)
       out << tune_key() << " " << tu.gs_block << " " << tu.gs_lanes << " " << tu.gs32_block << " " << tu.gs32_pre << " "
           << tu.node_block << " " << tu.face_block << " " << tu.cell_block << " " << tu.deriv_lds << " "
-          << tu.node_fused << "\n";
+          << tu.node_fused << " " << tu.gs32_dpp << "\n";
   }
   if (seconds)
     *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

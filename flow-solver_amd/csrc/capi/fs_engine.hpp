@@ -198,7 +198,7 @@ class Engine {
     const fsk::Tuning& t = dm.tune;
     v[0] = t.gs_block; v[1] = t.gs_lanes; v[2] = t.gs32_block; v[3] = t.gs32_pre;
     v[4] = t.node_block; v[5] = t.face_block; v[6] = t.cell_block; v[7] = t.deriv_lds; v[8] = t.gs32_scaled;
-    v[9] = t.node_fused;
+    v[9] = t.node_fused; v[10] = t.gs32_dpp;
   }
 
   // state

@@ -56,6 +56,8 @@ struct Tuning {
   int gs32_scaled = 1;    // FP32 sweeps: 0 apply the stored diagonal inverse per row, 1 operands pre-multiplied by it
   int node_fused = 1;     // pre-multiplied operands: 1 formed by the mixed node pass itself (the row's blocks
                           // staged in LDS), 0 by a separate diagonal pass over the stored blocks
+  int gs32_dpp = 0;       // mixed sweeps: 0 every lane loads a neighbour's whole dQ, 1 one component per lane
+                          // shared within the quad by DPP broadcasts
 };
 
 // LDS tiles of the k=2 derivative pass: runs of kDerivTileNodes consecutive

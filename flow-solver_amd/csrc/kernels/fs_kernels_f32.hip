@@ -495,7 +495,19 @@ __device__ __forceinline__ size_t gs_slot_base(const DevMesh& m, int t) {
 // v of the stored diagonal inverse to the shuffled r.  Scaled form (kScale):
 // the blocks and right-hand side already carry the inverse (k_diag_lu32<true>
 // or the fused node pass), so dQ_row[v] = c[v] - sum (Di A)[v,:] dQ_nb.
-template <int kPre, bool kScale>
+// Neighbour dQ of a sweep row: kDpp false, every lane loads the neighbour's
+// 4 components (16 bytes, the same address in all 4 lanes of the quad);
+// kDpp true, lane v loads component v and the quad exchanges them with DPP
+// broadcasts (a quarter of the load work; the same values, so the same
+// bits).  Every lane of the quad must be active.
+template <bool kDpp>
+__device__ __forceinline__ float4 gs_neighbour(const float* __restrict__ dQ32, int nb, int v, float own) {
+  if (!kDpp) return *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)nb);
+  (void)v;
+  return make_float4(quad_bcast<0>(own), quad_bcast<1>(own), quad_bcast<2>(own), quad_bcast<3>(own));
+}
+
+template <int kPre, bool kScale, bool kDpp>
 __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const unsigned char* __restrict__ pos_act,
                                                        const float* __restrict__ LHS32, const float* __restrict__ LU32,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
@@ -512,20 +524,28 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const unsig
   for (int i = 0; i < kPre; ++i) nb[i] = m.ja[base + 16 * i];
   if (!act) return;
   float4 A[kPre], q[kPre];
+  float qv[kPre];
 #pragma unroll
   for (int i = 0; i < kPre; ++i) {
     if (i < deg) {
       A[i] = *reinterpret_cast<const float4*>(LHS32 + 16 * (base + 16 * i) + 4 * v);
-      q[i] = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)nb[i]);
+      if (kDpp)
+        qv[i] = dQ32[4 * (size_t)nb[i] + v];
+      else
+        q[i] = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)nb[i]);
     }
   }
 #pragma unroll
   for (int i = 0; i < kPre; ++i)
-    if (i < deg) r -= A[i].x * q[i].x + A[i].y * q[i].y + A[i].z * q[i].z + A[i].w * q[i].w;
+    if (i < deg) {
+      const float4 b = kDpp ? gs_neighbour<true>(dQ32, 0, v, qv[i]) : q[i];
+      r -= A[i].x * b.x + A[i].y * b.y + A[i].z * b.z + A[i].w * b.w;
+    }
   for (int e = kPre; e < deg; ++e) {
     size_t k = base + 16 * (size_t)e;
     float4 a = *reinterpret_cast<const float4*>(LHS32 + 16 * k + 4 * v);
-    float4 b = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)m.ja[k]);
+    const int nbk = m.ja[k];
+    const float4 b = gs_neighbour<kDpp>(dQ32, nbk, v, kDpp ? dQ32[4 * (size_t)nbk + v] : 0.0f);
     r -= a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
   }
   if (kScale) {
@@ -543,7 +563,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const unsig
 // 16 rows as one contiguous 512-byte run) and accumulates in FP32,
 // dQ_row[v] = c[v] - sum (Di A)[v,:] dQ_nb.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-template <int kPre>
+template <int kPre, bool kDpp>
 __global__ void __launch_bounds__(kMaxBlock) k_gs_color16(DevMesh m, const unsigned char* __restrict__ pos_act,
                                                        const unsigned short* __restrict__ LHS16,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
@@ -561,21 +581,28 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color16(DevMesh m, const unsig
   if (!act) return;
   half4_t A[kPre];
   float4 q[kPre];
+  float qv[kPre];
 #pragma unroll
   for (int i = 0; i < kPre; ++i) {
     if (i < deg) {
       A[i] = *reinterpret_cast<const half4_t*>(LHS16 + 16 * (base + 16 * i) + 4 * v);
-      q[i] = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)nb[i]);
+      if (kDpp)
+        qv[i] = dQ32[4 * (size_t)nb[i] + v];
+      else
+        q[i] = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)nb[i]);
     }
   }
 #pragma unroll
   for (int i = 0; i < kPre; ++i)
-    if (i < deg)
-      r -= (float)A[i].x * q[i].x + (float)A[i].y * q[i].y + (float)A[i].z * q[i].z + (float)A[i].w * q[i].w;
+    if (i < deg) {
+      const float4 b = kDpp ? gs_neighbour<true>(dQ32, 0, v, qv[i]) : q[i];
+      r -= (float)A[i].x * b.x + (float)A[i].y * b.y + (float)A[i].z * b.z + (float)A[i].w * b.w;
+    }
   for (int e = kPre; e < deg; ++e) {
     size_t k = base + 16 * (size_t)e;
     const half4_t a = *reinterpret_cast<const half4_t*>(LHS16 + 16 * k + 4 * v);
-    const float4 b = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)m.ja[k]);
+    const int nbk = m.ja[k];
+    const float4 b = gs_neighbour<kDpp>(dQ32, nbk, v, kDpp ? dQ32[4 * (size_t)nbk + v] : 0.0f);
     r -= (float)a.x * b.x + (float)a.y * b.y + (float)a.z * b.z + (float)a.w * b.w;
   }
   dQ32[4 * (size_t)t + v] = r;
@@ -590,18 +617,24 @@ __global__ void __launch_bounds__(kBlock) k_pos_active(DevMesh m, const InstPara
   pos_act[t] = ip[inst_of(m, m.pos_inst, t)].active ? 1 : 0;
 }
 
-// nQ += dQ (FP64 state, FP32 correction) and the FP64 norms' inputs
+// nQ += dQ (FP64 state, FP32 correction) and the FP64 norms' inputs; one
+// thread per node (dQ32 is indexed by color position)
 __global__ void __launch_bounds__(kBlock) k_update_mixed(DevMesh m, const InstParams* __restrict__ ip,
                                                          double* __restrict__ nQ, const float* __restrict__ dQ32,
                                                          double* __restrict__ dQ) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 4 * m.nn) return;
-  const int n = i >> 2;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= m.nn) return;
   if (!ip[inst_of(m, m.inst_node, n)].active) return;
-  // dQ32 is indexed by color position
-  double d = (double)dQ32[4 * (size_t)(m.iau[n] - (m.nnz - m.nn)) + (i & 3)];
-  dQ[i] = d;
-  nQ[i] += d;
+  const float4 d = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)(m.iau[n] - (m.nnz - m.nn)));
+  double2* q = reinterpret_cast<double2*>(nQ + 4 * (size_t)n);
+  double2* o = reinterpret_cast<double2*>(dQ + 4 * (size_t)n);
+  const double2 d01 = make_double2((double)d.x, (double)d.y), d23 = make_double2((double)d.z, (double)d.w);
+  double2 q01 = q[0], q23 = q[1];
+  q01.x += d01.x; q01.y += d01.y; q23.x += d23.x; q23.y += d23.y;
+  o[0] = d01;
+  o[1] = d23;
+  q[0] = q01;
+  q[1] = q23;
 }
 
 namespace {
@@ -649,16 +682,23 @@ void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 
 namespace {
+template <int kPre, bool kDpp>
+void gs32_launch_form(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
+  if (s.LHS16)
+    hipLaunchKernelGGL((k_gs_color16<kPre, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS16, s.RHS32, s.dQ32, b, e);
+  else if (m.tune.gs32_scaled)
+    hipLaunchKernelGGL((k_gs_color32<kPre, true, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32,
+                       s.dQ32, b, e);
+  else
+    hipLaunchKernelGGL((k_gs_color32<kPre, false, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32,
+                       s.dQ32, b, e);
+}
 template <int kPre>
 void gs32_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
-  if (s.LHS16)
-    hipLaunchKernelGGL((k_gs_color16<kPre>), grid, block, 0, st, m, s.pos_act, s.LHS16, s.RHS32, s.dQ32, b, e);
-  else if (m.tune.gs32_scaled)
-    hipLaunchKernelGGL((k_gs_color32<kPre, true>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
-                       e);
+  if (m.tune.gs32_dpp)
+    gs32_launch_form<kPre, true>(m, s, grid, block, b, e, st);
   else
-    hipLaunchKernelGGL((k_gs_color32<kPre, false>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32, s.dQ32, b,
-                       e);
+    gs32_launch_form<kPre, false>(m, s, grid, block, b, e, st);
 }
 }  // namespace
 
@@ -682,7 +722,7 @@ void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& 
 }
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_update_mixed, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s.ip, s.nQ, s.dQ32, s.dQ);
+  hipLaunchKernelGGL(k_update_mixed, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.ip, s.nQ, s.dQ32, s.dQ);
 }
 
 }  // namespace fsk

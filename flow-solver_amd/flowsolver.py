@@ -29,7 +29,7 @@ _MESH_ARRAYS = [
 ]
 _FIELDS = ["Q", "nQ", "dQ", "R", "RHS", "grad", "phi", "hess", "pv", "ev_con", "dt", "LHS", "LHS32"]
 _TUNING = ["gs_block", "gs_lanes", "gs32_block", "gs32_pre", "node_block", "face_block", "cell_block", "deriv_lds",
-           "gs32_scaled", "node_fused"]
+           "gs32_scaled", "node_fused", "gs32_dpp"]
 
 
 def _set_tuning(fn, h, cur, kw):
