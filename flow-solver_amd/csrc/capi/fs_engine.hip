@@ -1110,15 +1110,24 @@ int Engine::autotune(double* seconds) {
   } else {
     best = 1e30f;
     fsk::Tuning keep = tu;
-    for (int lanes : {4, 1})
+    // 1 thread per row, or 4 lanes per row with each prefetch depth and gather form
+    const int forms[][3] = {{1, 6, 0}, {4, 4, 0}, {4, 6, 0}, {4, 8, 0}, {4, 4, 1}, {4, 6, 1}, {4, 8, 1}};
+    for (const auto& f : forms)
       for (int b : blocks) {
-        tu.gs_lanes = lanes;
+        tu.gs_lanes = f[0];
+        tu.gs32_pre = f[1];
+        tu.gs32_dpp = f[2];
         tu.gs_block = b;
         if ((rc = time_it([&] { fsk::launch_gs_sweeps(dm, s, cr, 2, st); }, 2, &t))) return rc;
-        if (t < best) { best = t; keep.gs_lanes = lanes; keep.gs_block = b; }
+        if (t < best) {
+          best = t;
+          keep.gs_lanes = f[0]; keep.gs32_pre = f[1]; keep.gs32_dpp = f[2]; keep.gs_block = b;
+        }
       }
     tu.gs_lanes = keep.gs_lanes;
     tu.gs_block = keep.gs_block;
+    tu.gs32_pre = keep.gs32_pre;
+    tu.gs32_dpp = keep.gs32_dpp;
   }
   const int nflags = fsk::NODE_EVCON | fsk::NODE_INTERIOR | fsk::NODE_BOUNDARY | fsk::NODE_RHS | fsk::NODE_JAC;
   int keepb = tu.node_block;

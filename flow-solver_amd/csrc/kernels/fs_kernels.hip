@@ -559,11 +559,27 @@ __global__ void __launch_bounds__(kMaxBlock) k_diag_lu(DevMesh m, const InstPara
   for (int j = 0; j < 4; ++j) dQ[4 * (size_t)n + j] = 0.0;
 }
 
+// Lane r's double in every lane of its quad (DPP quad_perm [r,r,r,r] on
+// both halves).  Every lane of the quad must be active.
+template <int kLane>
+__device__ __forceinline__ double quad_bcast_d(double x) {
+  constexpr int ctrl = kLane * 0x55;
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // rows [row_begin,row_end) of the color-sorted order; 4 lanes per row, lane v
 // forms component v of RHS - sum_offdiag A dQ in the row's block order (a
 // wave's 16 rows read block e of their rows as one contiguous 2 KB run of the
 // sliced layout), then every lane of the quad solves with the factored
-// diagonal.
+// diagonal.  The first kPre column ids are loaded together (the column array
+// is padded, so without a degree test), then those blocks and neighbour
+// states, before any arithmetic: the products are the same and are summed
+// in the same order as one block at a time.  kDpp: lane v loads component v
+// of each neighbour's dQ and the quad exchanges them (the same values).
+template <int kPre, bool kDpp>
 __global__ void __launch_bounds__(kMaxBlock) k_gs_color(DevMesh m, const InstParams* __restrict__ ip,
                                                      const double* __restrict__ LHS, const double* __restrict__ LU,
                                                      const double* __restrict__ RHS, double* __restrict__ dQ,
@@ -574,10 +590,46 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color(DevMesh m, const InstPar
   int j = m.color_rows[t];
   int inst = inst_of(m, m.pos_inst, t);
   int deg = m.pdeg[t];
-  size_t base = (size_t)m.sbase[t >> 4] + (t & 15);
+  const size_t base =
+      m.slice_w ? (size_t)16 * m.slice_w * (size_t)(t >> 4) + (t & 15) : (size_t)m.sbase[t >> 4] + (t & 15);
   double r = RHS[4 * (size_t)j + v];
+  int nb[kPre];
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) nb[i] = m.ja[base + 16 * i];
   if (!ip[inst].active) return;
-  for (int e = 0; e < deg; ++e) {
+  double2 A01[kPre], A23[kPre], q01[kPre], q23[kPre];
+  double qv[kPre];
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) {
+    if (i < deg) {
+      const double2* Ab = reinterpret_cast<const double2*>(LHS + 16 * (base + 16 * i) + 4 * v);
+      A01[i] = Ab[0];
+      A23[i] = Ab[1];
+      if (kDpp) {
+        qv[i] = dQ[4 * (size_t)nb[i] + v];
+      } else {
+        const double2* qb = reinterpret_cast<const double2*>(dQ + 4 * (size_t)nb[i]);
+        q01[i] = qb[0];
+        q23[i] = qb[1];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) {
+    if (i < deg) {
+      double q0, q1, q2, q3;
+      if (kDpp) {
+        q0 = quad_bcast_d<0>(qv[i]);
+        q1 = quad_bcast_d<1>(qv[i]);
+        q2 = quad_bcast_d<2>(qv[i]);
+        q3 = quad_bcast_d<3>(qv[i]);
+      } else {
+        q0 = q01[i].x; q1 = q01[i].y; q2 = q23[i].x; q3 = q23[i].y;
+      }
+      r -= (A01[i].x * q0 + A01[i].y * q1 + A23[i].x * q2 + A23[i].y * q3);
+    }
+  }
+  for (int e = kPre; e < deg; ++e) {
     size_t k = base + 16 * (size_t)e;
     const double* A = LHS + 16 * k + 4 * v;
     const double* q = dQ + 4 * (size_t)m.ja[k];
@@ -911,6 +963,16 @@ void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st) {
                      s.LU, s.dQ);
 }
 
+namespace {
+template <int kPre>
+void gs_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
+  if (m.tune.gs32_dpp)
+    hipLaunchKernelGGL((k_gs_color<kPre, true>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+  else
+    hipLaunchKernelGGL((k_gs_color<kPre, false>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+}
+}  // namespace
+
 void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr, int subits, hipStream_t st) {
   std::vector<GsLaunch> sched((size_t)subits * m.ncolor + 1);
   int ns = gs_schedule(m.ncolor, subits, sched.data());
@@ -921,9 +983,12 @@ void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr
     const int bs = m.tune.gs_block;
     if (m.tune.gs_lanes == 1)
       hipLaunchKernelGGL(k_gs_row, dim3(nblocks(e - b, bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+    else if (m.tune.gs32_pre <= 4)
+      gs_launch<4>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
+    else if (m.tune.gs32_pre <= 6)
+      gs_launch<6>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
     else
-      hipLaunchKernelGGL(k_gs_color, dim3(nblocks(4 * (e - b), bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS,
-                         s.dQ, b, e);
+      gs_launch<8>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
   }
 }
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st) {

@@ -48,7 +48,8 @@ struct Tuning {
   int gs_block = 256;     // FP64 sweep workgroup size
   int gs_lanes = 4;       // FP64 sweep: 4 lanes per row (one per component) or 1 thread per row
   int gs32_block = 256;   // FP32 sweep workgroup size
-  int gs32_pre = 6;       // FP32 sweep: row blocks whose loads are issued before the arithmetic (4, 6, 8)
+  int gs32_pre = 6;       // the engine's sweep (FP32 or FP16 operands; FP64 with 4 lanes per row): row blocks
+                          // whose loads are issued before the arithmetic (4, 6, 8)
   int node_block = 256;   // node assembly
   int face_block = 256;   // subedge / edge kernels
   int cell_block = 256;   // gradient, limiter, derivatives, diagonal factorisation
@@ -56,8 +57,8 @@ struct Tuning {
   int gs32_scaled = 1;    // FP32 sweeps: 0 apply the stored diagonal inverse per row, 1 operands pre-multiplied by it
   int node_fused = 1;     // pre-multiplied operands: 1 formed by the mixed node pass itself (the row's blocks
                           // staged in LDS), 0 by a separate diagonal pass over the stored blocks
-  int gs32_dpp = 0;       // mixed sweeps: 0 every lane loads a neighbour's whole dQ, 1 one component per lane
-                          // shared within the quad by DPP broadcasts
+  int gs32_dpp = 0;       // the engine's 4-lane sweep: 0 every lane loads a neighbour's whole dQ, 1 one
+                          // component per lane shared within the quad by DPP broadcasts
 };
 
 // LDS tiles of the k=2 derivative pass: runs of kDerivTileNodes consecutive

@@ -388,24 +388,31 @@ def test_fp16_operands_are_the_fp32_operands_rounded(order, soits, limit):
     h.close()
 
 
-@pytest.mark.parametrize("precision,subits", [(fsm.FS_PREC_MIXED, 12), (fsm.FS_PREC_MIXED, 40),
-                                              (fsm.FS_PREC_MIXED16, 40)])
+@pytest.mark.parametrize("precision,subits", [(fsm.FS_PREC_F64, 12), (fsm.FS_PREC_MIXED, 12),
+                                              (fsm.FS_PREC_MIXED, 40), (fsm.FS_PREC_MIXED16, 40)])
 def test_sweep_gather_forms_equal(precision, subits):
-    """The mixed sweeps read a neighbour's dQ either whole in every lane of
-    the row's quad or one component per lane, exchanged within the quad:
-    the same values in the same arithmetic, so the same states bit for bit
-    (stored-inverse and pre-multiplied forms, FP32 and FP16 operands)."""
+    """The 4-lane sweeps read a neighbour's dQ either whole in every lane of
+    the row's quad or one component per lane, exchanged within the quad, and
+    issue a row's first 4 (or 6, 8) block loads before the arithmetic: the
+    same values in the same arithmetic, so the same states bit for bit (FP64;
+    stored-inverse and pre-multiplied forms, FP32 and FP16 operands).  The
+    FP64 path's one-thread-per-row sweep gives the same bits as well."""
     meshes, params = _build(precision)
     params = [fsm.Params.make(**dict(p.as_dict(), subits=subits)) for p in params]
     res = []
-    for dpp in (0, 1):
+    forms = [(0, 4), (1, 4), (1, 8)] + ([(-1, 6)] if precision == fsm.FS_PREC_F64 else [])
+    for dpp, pre in forms:
         b = fsm.Batch(meshes, params, precision)
-        b.set_tuning(gs32_dpp=dpp, gs32_pre=4)
+        if dpp < 0:
+            b.set_tuning(gs_lanes=1)
+        else:
+            b.set_tuning(gs32_dpp=dpp, gs32_pre=pre)
         b.init_freestream()
         for _ in range(8):
             b.step()
         res.append([(b.get(i, "Q"), b.get(i, "dQ")) for i in range(len(meshes))])
         b.close()
-    for i, (a, c) in enumerate(zip(*res)):
-        for x, y in zip(a, c):
-            assert np.array_equal(x, y), "instance %d" % i
+    for other in res[1:]:
+        for i, (a, c) in enumerate(zip(res[0], other)):
+            for x, y in zip(a, c):
+                assert np.array_equal(x, y), "instance %d" % i
