@@ -138,6 +138,17 @@ __global__ void __launch_bounds__(kMaxBlock) k_derivatives(DevMesh m, DevState s
   s.pv[4 * (size_t)i + v] = qi - Mi[0] * a0 - Mi[1] * a1 - Mi[2] * a2 * 0.5 - Mi[3] * a3 * 0.5 - Mi[4] * a4;
 }
 
+// Lane r's double in every lane of its quad (DPP quad_perm [r,r,r,r] on
+// both halves).  Every lane of the quad must be active.
+template <int kLane>
+__device__ __forceinline__ double quad_bcast_d(double x) {
+  constexpr int ctrl = kLane * 0x55;
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // The same pass with a tile's neighbour states staged in LDS: the block of
 // tile t (kDerivTileNodes nodes, 16 lanes each) first copies the states of
 // the tile's distinct stencil nodes into LDS, then every lane runs the loop
@@ -159,14 +170,16 @@ __global__ void __launch_bounds__(16 * kDerivTileNodes) k_derivatives_lds(DevMes
   const int o = m.st_off[i], e = m.st_off[i + 1];
   const double qi = nQ[4 * (size_t)i + v];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+  // the 4 lanes of a quad (variables 0-3) share stencil entry j: lane v loads
+  // its weight v and the quad exchanges them (weight 4 is loaded by all)
   for (int j = o + p; j < e; j += 4) {
+    const double wv = Wt[5 * (size_t)j + v], w4 = Wt[5 * (size_t)j + 4];
     const double dj = sq[4 * loc[j] + v] - qi;
-    const double* w = Wt + 5 * (size_t)j;
-    a0 += w[0] * dj;
-    a1 += w[1] * dj;
-    a2 += w[2] * dj;
-    a3 += w[3] * dj;
-    a4 += w[4] * dj;
+    a0 += quad_bcast_d<0>(wv) * dj;
+    a1 += quad_bcast_d<1>(wv) * dj;
+    a2 += quad_bcast_d<2>(wv) * dj;
+    a3 += quad_bcast_d<3>(wv) * dj;
+    a4 += w4 * dj;
   }
 #pragma unroll
   for (int off = 4; off <= 8; off <<= 1) {
@@ -199,12 +212,27 @@ __global__ void __launch_bounds__(16 * kDerivTileNodes) k_derivatives_lds(DevMes
 // first so a wave divides once per candidate whichever side its lanes take;
 // the quotient is the same operation on the same operands as in the
 // two-branch form.
+//
+// Barth's phi is min(1, min of the quotients), and IEEE division rounds
+// monotonically, so a quotient can lower the running minimum ph (<= 1, a
+// representable value) only if num/d < ph exactly.  fma(ph, d, -num) rounds
+// ph d - num once, which keeps its sign; the division runs only when that
+// sign allows a smaller quotient (or is zero, or the operands are not
+// finite), so the minimum and every bit of phi are those of dividing every
+// candidate, and a smooth region divides nowhere.
 template <bool kVenkat>
 __device__ __forceinline__ void limiter_candidate(double q, double qmin, double qmax, double qf, double* ph) {
   const double d = qf - q;
   const double num = (d > 0.) ? (qmax - q) : (qmin - q);
+  if (!kVenkat) {
+    if (!(d > 0. || d < 0.)) return;   // the candidate is 1: no effect on ph <= 1
+    const double r = fma(*ph, d, -num);
+    if ((d > 0. && r < 0.) || (d < 0. && r > 0.)) return;
+    *ph = dmin(*ph, dmin(1.0, num / d));
+    return;
+  }
   double t = num / d;
-  t = kVenkat ? (t * t + 2. * t) / (t * t + t + 2.) : dmin(1.0, t);
+  t = (t * t + 2. * t) / (t * t + t + 2.);
   *ph = dmin(*ph, (d > 0. || d < 0.) ? t : 1.0);
 }
 
@@ -239,6 +267,29 @@ __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
   const double q = nQ[4 * (size_t)n + v];
   double qmin = q, qmax = q, ph = 1.0;
   const int eb = m.nedge_off[n], ee = m.nedge_off[n + 1];
+  const int sb = m.nsub_off[n], se = m.nsub_off[n + 1];
+  const bool gauss = lim & LIM_GAUSS;
+  // k=2 phase: the node's derivatives, point value and the Gauss points of
+  // its first face incidences are requested before the neighbour min/max,
+  // so the two load chains overlap (up to three incidences per lane and pass)
+  constexpr int kU = 3;
+  double xg[kU][6];
+  double h0 = 0., h1 = 0., h2 = 0., h3 = 0., h4 = 0., p0 = 0.;
+  auto load_xg = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int k = k0 + 4 * u;
+      const double* src = m.inc_xg + 6 * (size_t)(k < se ? k : k0);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) xg[u][c] = src[c];
+    }
+  };
+  if (gauss) {
+    const double* h = s.hess + 20 * (size_t)n + 5 * v;
+    h0 = h[0]; h1 = h[1]; h2 = h[2]; h3 = h[3]; h4 = h[4];
+    p0 = s.pv[4 * (size_t)n + v];
+    if (sb + p < se) load_xg(sb + p);
+  }
   // two edges per lane and pass, both loads in flight together
   for (int k = eb + p; k < ee; k += 8) {
     const bool two = k + 4 < ee;
@@ -255,23 +306,9 @@ __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
   }
   const double xn0 = m.x[2 * (size_t)n], xn1 = m.x[2 * (size_t)n + 1];
   const bool venkat = lim & LIM_VENKAT;
-  const int sb = m.nsub_off[n], se = m.nsub_off[n + 1];
-  if (lim & LIM_GAUSS) {
-    const double* h = s.hess + 20 * (size_t)n + 5 * v;
-    const double h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3], h4 = h[4];
-    const double p0 = s.pv[4 * (size_t)n + v];
-    // up to three face incidences per lane and pass, their Gauss points
-    // loaded before the first candidate
-    constexpr int kU = 3;
+  if (gauss) {
     for (int k0 = sb + p; k0 < se; k0 += 4 * kU) {
-      double xg[kU][6];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int k = k0 + 4 * u;
-        const double* src = m.inc_xg + 6 * (size_t)(k < se ? k : k0);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) xg[u][c] = src[c];
-      }
+      if (k0 != sb + p) load_xg(k0);
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         if (k0 + 4 * u >= se) break;
@@ -557,17 +594,6 @@ __global__ void __launch_bounds__(kMaxBlock) k_diag_lu(DevMesh m, const InstPara
   for (int k = 0; k < 16; ++k) LU[16 * (size_t)t + k] = A[k];
 #pragma unroll
   for (int j = 0; j < 4; ++j) dQ[4 * (size_t)n + j] = 0.0;
-}
-
-// Lane r's double in every lane of its quad (DPP quad_perm [r,r,r,r] on
-// both halves).  Every lane of the quad must be active.
-template <int kLane>
-__device__ __forceinline__ double quad_bcast_d(double x) {
-  constexpr int ctrl = kLane * 0x55;
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), ctrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // rows [row_begin,row_end) of the color-sorted order; 4 lanes per row, lane v
