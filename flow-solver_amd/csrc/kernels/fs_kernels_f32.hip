@@ -217,32 +217,53 @@ __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s
   int k = k0 + (v >> 1);
   int ent_n = 0;
   double geo_n[3] = {0.0, 0.0, 0.0}, qb_n[4] = {1.0, 0.0, 0.0, 1.0};
-  if (k < k1) {
-    ent_n = m.inc_other[k];
-    const double* geo = m.einc_geo + 6 * (size_t)k + 3 * sub;
+  // linear reconstruction (recon 1): the node's own gradient, limiter value
+  // and coordinates once, and the next face's face point and the other
+  // node's gradient, limiter value and coordinates one iteration ahead
+  double ga[8], xa[2], phia = 0.0;
+  double gb_n[8], xb_n[2] = {0.0, 0.0}, fc_n[2] = {0.0, 0.0}, phib_n = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) gb_n[j] = 0.0;
+  auto load_face = [&](int kk) {
+    ent_n = m.inc_other[kk];
+    const double* geo = m.einc_geo + 6 * (size_t)kk + 3 * sub;
 #pragma unroll
     for (int c = 0; c < 3; ++c) geo_n[c] = geo[c];
     const size_t on = (size_t)(ent_n & 0x7fffffff);
 #pragma unroll
     for (int j = 0; j < 4; ++j) qb_n[j] = nQ[4 * on + j];
+    if (recon == 1) {
+      const double* fc = m.einc_fc + 4 * (size_t)kk + 2 * sub;
+      fc_n[0] = fc[0];
+      fc_n[1] = fc[1];
+      phib_n = s.phi[on];
+      xb_n[0] = m.x[2 * on];
+      xb_n[1] = m.x[2 * on + 1];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) gb_n[j] = grad[8 * on + j];
+    }
+  };
+  if (recon == 1) {
+    phia = s.phi[n];
+    xa[0] = m.x[2 * (size_t)n];
+    xa[1] = m.x[2 * (size_t)n + 1];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ga[j] = grad[8 * (size_t)n + j];
   }
+  if (k < k1) load_face(k);
   for (; k < k1; k += 2) {
     const int ent = ent_n;
-    const int o = ent & 0x7fffffff;
     const bool isR = ent < 0;
     const double nx = geo_n[0], ny = geo_n[1], len = geo_n[2];
-    double qb[4];
+    double qb[4], gb[8], xb[2], fc[2];
+    const double phib = phib_n;
 #pragma unroll
     for (int j = 0; j < 4; ++j) qb[j] = qb_n[j];
-    if (k + 2 < k1) {
-      ent_n = m.inc_other[k + 2];
-      const double* geo = m.einc_geo + 6 * (size_t)(k + 2) + 3 * sub;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) geo_n[c] = geo[c];
-      const size_t on = (size_t)(ent_n & 0x7fffffff);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) qb_n[j] = nQ[4 * on + j];
-    }
+    for (int j = 0; j < 8; ++j) gb[j] = gb_n[j];
+    xb[0] = xb_n[0]; xb[1] = xb_n[1];
+    fc[0] = fc_n[0]; fc[1] = fc_n[1];
+    if (k + 2 < k1) load_face(k + 2);
     PrimW Pb = prim_w(qb, g1);
     // this lane's contribution to block (n, other)
     float off[16];
@@ -255,14 +276,12 @@ __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s
       PrimW Pa = Pn;
       if (recon == 1) {
         // reconstruct.C:181-205 at the subedge's face point, for both ends
-        const double* fc = m.einc_fc + 4 * (size_t)k + 2 * sub;
-        const double phia = s.phi[n], phib = s.phi[o];
-        const double dxa0 = fc[0] - m.x[2 * (size_t)n], dxa1 = fc[1] - m.x[2 * (size_t)n + 1];
-        const double dxb0 = fc[0] - m.x[2 * (size_t)o], dxb1 = fc[1] - m.x[2 * (size_t)o + 1];
+        const double dxa0 = fc[0] - xa[0], dxa1 = fc[1] - xa[1];
+        const double dxb0 = fc[0] - xb[0], dxb1 = fc[1] - xb[1];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          qa[j] += phia * grad[8 * (size_t)n + 2 * j] * dxa0 + phia * grad[8 * (size_t)n + 2 * j + 1] * dxa1;
-          qb[j] += phib * grad[8 * (size_t)o + 2 * j] * dxb0 + phib * grad[8 * (size_t)o + 2 * j + 1] * dxb1;
+          qa[j] += phia * ga[2 * j] * dxa0 + phia * ga[2 * j + 1] * dxa1;
+          qb[j] += phib * gb[2 * j] * dxb0 + phib * gb[2 * j + 1] * dxb1;
         }
         Pa = prim_w(qa, g1);
         Pb = prim_w(qb, g1);
