@@ -336,7 +336,7 @@ def batch_throughput(a, fsm, meshes, params, infos, prec):
             "avg_ms": t["node_ms"], "algorithmic_bytes": rb,
             "achieved_GBs": rb / (t["node_ms"] * 1e-3) / 1e9, "frac": rb / (t["node_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "edge_flux_evals_per_s": t["flux_evals"] / (t["node_ms"] * 1e-3),
-            "traffic": measured_traffic(kname, "pmc_traffic_batch256_mixed.json") if len(meshes) == 256 else None}
+            "traffic": measured_traffic(kname, "pmc_traffic_batch256.json") if len(meshes) == 256 else None}
     return out
 
 
@@ -851,7 +851,7 @@ def main():
                                                    "gauss_seidel"] / mixed["ms_per_step"]),
                      "achieved": gs["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gs["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(gs["kernel"], "pmc_traffic_batch256_mixed.json")
+                     "traffic": measured_traffic(gs["kernel"], "pmc_traffic_batch256.json")
                      if a.batch == 256 else None,
                      "algorithmic_bytes": gs["algorithmic_bytes_per_launch"],
                      "launches_per_step": gs["launches_per_step"], "avg_ms": gs["avg_ms"]},
@@ -873,7 +873,7 @@ def main():
         h["roofline"] = {"kernel": h["gauss_seidel"]["kernel"], "achieved": h["gauss_seidel"]["achieved_GBs"],
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": h["gauss_seidel"]["achieved_GBs"] / HBM_PEAK_GBS,
-                         "traffic": measured_traffic(h["gauss_seidel"]["kernel"], "pmc_traffic_batch256_mixed.json")
+                         "traffic": measured_traffic(h["gauss_seidel"]["kernel"], "pmc_traffic_batch256.json")
                          if a.batch == 256 else None}
         line["mixed16"] = h
         line["speedup_mixed16_vs_mixed"] = h["value"] / mixed["value"]
