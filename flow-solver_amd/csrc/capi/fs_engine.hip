@@ -112,7 +112,7 @@ namespace {
 bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
 bool valid_tuning(const fsk::Tuning& t) {
   return valid_block(t.gs_block) && (t.gs_lanes == 1 || t.gs_lanes == 4) && valid_block(t.gs32_block) &&
-         (t.gs32_pre == 4 || t.gs32_pre == 6 || t.gs32_pre == 8) && valid_block(t.node_block) &&
+         (t.gs32_pre == 0 || t.gs32_pre == 4 || t.gs32_pre == 6 || t.gs32_pre == 8) && valid_block(t.node_block) &&
          valid_block(t.face_block) && valid_block(t.cell_block) && (t.deriv_lds == 0 || t.deriv_lds == 1) &&
          (t.gs32_scaled == 0 || t.gs32_scaled == 1) && (t.node_fused == 0 || t.node_fused == 1) &&
          (t.gs32_dpp == 0 || t.gs32_dpp == 1);
@@ -1111,7 +1111,7 @@ int Engine::autotune(double* seconds) {
     best = 1e30f;
     fsk::Tuning keep = tu;
     // 1 thread per row, or 4 lanes per row with each prefetch depth and gather form
-    const int forms[][3] = {{1, 6, 0}, {4, 4, 0}, {4, 6, 0}, {4, 8, 0}, {4, 4, 1}, {4, 6, 1}, {4, 8, 1}};
+    const int forms[][3] = {{1, 6, 0}, {4, 0, 0}, {4, 4, 0}, {4, 6, 0}, {4, 8, 0}, {4, 4, 1}, {4, 6, 1}, {4, 8, 1}};
     for (const auto& f : forms)
       for (int b : blocks) {
         tu.gs_lanes = f[0];

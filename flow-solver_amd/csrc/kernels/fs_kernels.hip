@@ -619,12 +619,14 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color(DevMesh m, const InstPar
   const size_t base =
       m.slice_w ? (size_t)16 * m.slice_w * (size_t)(t >> 4) + (t & 15) : (size_t)m.sbase[t >> 4] + (t & 15);
   double r = RHS[4 * (size_t)j + v];
-  int nb[kPre];
+  // kPre 0: every block in the loop below, one at a time
+  constexpr int kN = kPre > 0 ? kPre : 1;
+  int nb[kN];
 #pragma unroll
   for (int i = 0; i < kPre; ++i) nb[i] = m.ja[base + 16 * i];
   if (!ip[inst].active) return;
-  double2 A01[kPre], A23[kPre], q01[kPre], q23[kPre];
-  double qv[kPre];
+  double2 A01[kN], A23[kN], q01[kN], q23[kN];
+  double qv[kN];
 #pragma unroll
   for (int i = 0; i < kPre; ++i) {
     if (i < deg) {
@@ -992,6 +994,10 @@ void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st) {
 namespace {
 template <int kPre>
 void gs_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
+  if (kPre == 0) {   // no prefetch: the gather form does not apply
+    hipLaunchKernelGGL((k_gs_color<0, false>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+    return;
+  }
   if (m.tune.gs32_dpp)
     hipLaunchKernelGGL((k_gs_color<kPre, true>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
   else
@@ -1009,6 +1015,8 @@ void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr
     const int bs = m.tune.gs_block;
     if (m.tune.gs_lanes == 1)
       hipLaunchKernelGGL(k_gs_row, dim3(nblocks(e - b, bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+    else if (m.tune.gs32_pre == 0)
+      gs_launch<0>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
     else if (m.tune.gs32_pre <= 4)
       gs_launch<4>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
     else if (m.tune.gs32_pre <= 6)

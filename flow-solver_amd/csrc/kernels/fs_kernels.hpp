@@ -49,7 +49,8 @@ struct Tuning {
   int gs_lanes = 4;       // FP64 sweep: 4 lanes per row (one per component) or 1 thread per row
   int gs32_block = 256;   // FP32 sweep workgroup size
   int gs32_pre = 6;       // the engine's sweep (FP32 or FP16 operands; FP64 with 4 lanes per row): row blocks
-                          // whose loads are issued before the arithmetic (4, 6, 8)
+                          // whose loads are issued before the arithmetic (4, 6, 8; 0: the FP64 sweep loads
+                          // block by block, the mixed sweeps take it as 4)
   int node_block = 256;   // node assembly
   int face_block = 256;   // subedge / edge kernels
   int cell_block = 256;   // gradient, limiter, derivatives, diagonal factorisation

@@ -307,7 +307,9 @@ int fs_solver_use_graph(fs_solver* s, int on);
  * a separate pass over the stored blocks: 0; the same bits either way),
  * gs32_dpp (the 4-lane sweeps: each lane loads one component of a
  * neighbour's dQ and the row's 4 lanes exchange them: 1, or every lane loads
- * all four: 0).  gs32_pre and gs32_dpp apply to the FP64 4-lane sweep too.
+ * all four: 0).  gs32_pre and gs32_dpp apply to the FP64 4-lane sweep too;
+ * gs32_pre 0 (FP64: no prefetch, blocks loaded one at a time) is a candidate
+ * there. 
  * gs32_scaled is not tuned or cached: it is 1 when PARAMS.subits >= 30 (and
  * always with FS_PREC_MIXED16) and can be pinned with the setter; the two
  * forms round differently in FP32, so it moves the mixed path's iterates by

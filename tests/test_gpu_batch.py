@@ -400,7 +400,7 @@ def test_sweep_gather_forms_equal(precision, subits):
     meshes, params = _build(precision)
     params = [fsm.Params.make(**dict(p.as_dict(), subits=subits)) for p in params]
     res = []
-    forms = [(0, 4), (1, 4), (1, 8)] + ([(-1, 6)] if precision == fsm.FS_PREC_F64 else [])
+    forms = [(0, 4), (1, 4), (1, 8)] + ([(-1, 6), (0, 0)] if precision == fsm.FS_PREC_F64 else [])
     for dpp, pre in forms:
         b = fsm.Batch(meshes, params, precision)
         if dpp < 0:
