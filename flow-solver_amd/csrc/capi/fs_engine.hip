@@ -508,6 +508,16 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   inst_ch0[B] = (int)ch_inst.size();
   nc.nchunk = (int)ch_inst.size();
 
+  pos_inst_h_ = pos_inst;
+  {
+    // work-list capacity: every chunk of every sweep range
+    long long cap = 64;
+    for (int c = 0; c < ncmax; ++c)
+      cap += (color_off[c + 1] - color_off[c] + fsk::kGsChunk - 1) / fsk::kGsChunk +
+             (color_nback[c] + fsk::kGsChunk - 1) / fsk::kGsChunk;
+    int rc0 = alloc(&d_chunks_, (size_t)cap);
+    if (rc0) return rc0;
+  }
   int rc = 0;
   if ((rc = upload(&d.inst_node, inst_node)) || (rc = upload(&d.inst_edge, inst_edge)) ||
       (rc = upload(&d.inst_bedge, inst_bedge)) || (rc = upload(&d.pos_inst, pos_inst)) || (rc = upload(&d.x, x)) ||
@@ -686,6 +696,69 @@ int Engine::step_key() const {
   return any ? key : -1;
 }
 
+void Engine::drop_graphs() {
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+}
+
+fsk::ColorRanges Engine::color_ranges() const {
+  fsk::ColorRanges cr{color_off.data(), color_nback.data()};
+  if (chunks_on_) {
+    cr.chunk_off = chunk_off_h_.data();
+    cr.chunks = d_chunks_;
+  }
+  return cr;
+}
+
+int Engine::refresh_work_lists() {
+  const int B = ninst();
+  long long act = 0;
+  bool covered = chunks_on_;
+  for (int i = 0; i < B; ++i) {
+    if (!inst[i].active) continue;
+    act += inst[i].nn;
+    if (chunks_on_ && !chunk_inst_[i]) covered = false;
+  }
+  const bool want = act > 0 && 4 * act <= 3LL * dm.nn;
+  if (!want) {
+    if (chunks_on_) {
+      chunks_on_ = false;
+      dm.xcd_run = 0;
+      drop_graphs();
+    }
+    return FS_OK;
+  }
+  if (covered && 4 * act > 3 * chunk_nodes_) return FS_OK;
+  // an earlier list may still feed enqueued sweeps
+  FSE_HIP(hipStreamSynchronize(stream));
+  const int nc = (int)color_nback.size();
+  chunk_off_h_.assign(2 * (size_t)nc + 1, 0);
+  chunk_h_.clear();
+  for (int c = 0; c < nc; ++c)
+    for (int bwd = 0; bwd < 2; ++bwd) {
+      const int b = color_off[c], e = bwd ? b + color_nback[c] : color_off[c + 1];
+      for (int t0 = b; t0 < e; t0 += fsk::kGsChunk) {
+        const int t1 = std::min(e, t0 + fsk::kGsChunk);
+        for (int t = t0; t < t1; ++t)
+          if (inst[pos_inst_h_[t]].active) {
+            chunk_h_.push_back(t0);
+            break;
+          }
+      }
+      chunk_off_h_[2 * (size_t)c + bwd + 1] = (int)chunk_h_.size();
+    }
+  if (!chunk_h_.empty())
+    FSE_HIP(hipMemcpy(d_chunks_, chunk_h_.data(), chunk_h_.size() * sizeof(int), hipMemcpyHostToDevice));
+  chunk_inst_.assign(B, 0);
+  for (int i = 0; i < B; ++i) chunk_inst_[i] = inst[i].active ? 1 : 0;
+  chunk_nodes_ = act;
+  chunks_on_ = true;
+  // the running instances' nodes and rows are spread over all XCDs
+  dm.xcd_run = fsk::kXcdRun;
+  drop_graphs();
+  return FS_OK;
+}
+
 // one Newton iteration of solve.C:328-909 over every active instance
 int Engine::newton_iteration(int it, int key, bool record) {
   fsk::DevState s = ds;
@@ -694,7 +767,7 @@ int Engine::newton_iteration(int it, int key, bool record) {
   if (key & 1) fsk::launch_gradient(dm, s, st);
   if (key & 2) fsk::launch_derivatives(dm, s, st);
   fsk::launch_limiter(dm, s, st);
-  fsk::ColorRanges cr{color_off.data(), color_nback.data()};
+  const fsk::ColorRanges cr = color_ranges();
   if (precision != FS_PREC_F64) {
     // boundary fluxes and blocks, then one node pass for the FP64 residual,
     // ev_con/dt and every FP32 Jacobian block (and, fused, the sweep
@@ -752,7 +825,8 @@ int Engine::step(bool advance, bool graph, bool record) {
         if (I.active) ++I.ts;
     int key = step_key();
     if (key < 0) return FS_OK;
-    int rc = upload_params(1);
+    int rc = refresh_work_lists();
+    if (!rc) rc = upload_params(1);
     if (!rc) rc = enqueue_step(key, true);
     if (!rc) rc = enqueue_readback();
     if (rc) return rc;
@@ -788,7 +862,8 @@ int Engine::step_begin(bool advance, bool graph) {
       if (I.active) ++I.ts;
   int key = step_key();
   if (key < 0) return FS_OK;
-  int rc = upload_params(1);
+  int rc = refresh_work_lists();
+  if (!rc) rc = upload_params(1);
   if (rc) return rc;
   if (graph) {
     auto itg = graphs_.find(key);

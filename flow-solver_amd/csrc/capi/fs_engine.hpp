@@ -234,6 +234,22 @@ class Engine {
   int* h_rmode_ = nullptr;            // pinned staging, ninst
   hipEvent_t ev_[8] = {};
   std::map<int, hipGraphExec_t> graphs_;
+  // Sweep work lists (fsk::ColorRanges::chunks): once at most three quarters
+  // of the nodes belong to active instances, each sweep launch covers only
+  // the 16-row chunks of its range that hold an active row.  Rebuilt when an
+  // instance outside the lists becomes active or the active nodes fall below
+  // three quarters of those at the last build; every rebuild drops the
+  // captured graphs (their launch grids change).
+  std::vector<int> pos_inst_h_;       // instance of each color position
+  std::vector<int> chunk_off_h_;      // 2 ncolor + 1
+  std::vector<int> chunk_h_;          // first position of each chunk
+  std::vector<char> chunk_inst_;      // instances the lists cover
+  int* d_chunks_ = nullptr;
+  bool chunks_on_ = false;
+  long long chunk_nodes_ = 0;         // active nodes when the lists were built
+  int refresh_work_lists();
+  fsk::ColorRanges color_ranges() const;
+  void drop_graphs();
 
   template <typename T>
   int alloc(T** out, size_t count);

@@ -32,7 +32,7 @@ __device__ __forceinline__ double dmax(double x, double y) { return x >= y ? x :
 // K2: least-squares gradient, node-centric gather (gradient.C:363-403)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMaxBlock) k_gradient(DevMesh m, DevState s) {
-  int n = xcd_block() * blockDim.x + threadIdx.x;
+  int n = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
   if (!ip.active || !ip.grad) return;
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_gradient(DevMesh m, DevState s) {
 // entries per step and the 4 variable lanes of an entry share its cache line.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMaxBlock) k_derivatives(DevMesh m, DevState s) {
-  int tid = xcd_block() * blockDim.x + threadIdx.x;
+  int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
   int i = tid >> 4, v = tid & 3, p = (tid >> 2) & 3;
   if (i >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, i)];
@@ -156,7 +156,7 @@ __device__ __forceinline__ double quad_bcast_d(double x) {
 // order, so the results are those of k_derivatives bit for bit.
 __global__ void __launch_bounds__(16 * kDerivTileNodes) k_derivatives_lds(DevMesh m, DevState s) {
   __shared__ double sq[4 * kDerivTileMaxU];
-  const int t = xcd_block();
+  const int t = xcd_block(m.xcd_run);
   const double* __restrict__ nQ = s.nQ;
   const int u0 = m.dt_uoff[t], nu = m.dt_uoff[t + 1] - u0;
   for (int k = threadIdx.x; k < 4 * nu; k += blockDim.x) sq[k] = nQ[4 * (size_t)m.dt_unodes[u0 + (k >> 2)] + (k & 3)];
@@ -251,7 +251,7 @@ __device__ __forceinline__ void candidate(bool venkat, double q, double qmin, do
 // on, 2 Gauss points of the k=2 phase, 4 Venkatakrishnan's function instead
 // of Barth's, 8 frozen (phi kept).
 __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
-  int tid = xcd_block() * blockDim.x + threadIdx.x;
+  int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
   int n = tid >> 4, v = tid & 3, p = (tid >> 2) & 3;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
 // is accumulated over the subedges in the reference's order.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st) {
-  int tid = xcd_block() * blockDim.x + threadIdx.x;
+  int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
   int e = tid >> 1, side = tid & 1;
   if (e >= m.ne) return;
   const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_bedge(DevMesh m, DevState s) {
 // 4 lanes per node: lane v owns residual component v and row v of the
 // diagonal block, so the quad reads each 128-byte block contiguously.
 __global__ void __launch_bounds__(kMaxBlock) k_node(DevMesh m, DevState s, int flags) {
-  int tid = xcd_block() * blockDim.x + threadIdx.x;
+  int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
   int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
   const int inst = inst_of(m, m.inst_node, n);
@@ -581,7 +581,7 @@ __global__ void __launch_bounds__(kBlock) k_zero_offdiag(DevMesh m, double* __re
 __global__ void __launch_bounds__(kMaxBlock) k_diag_lu(DevMesh m, const InstParams* __restrict__ ip,
                                                     const double* __restrict__ LHS, double* __restrict__ LU,
                                                     double* __restrict__ dQ) {
-  int t = xcd_block() * blockDim.x + threadIdx.x;
+  int t = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
@@ -609,9 +609,16 @@ template <int kPre, bool kDpp>
 __global__ void __launch_bounds__(kMaxBlock) k_gs_color(DevMesh m, const InstParams* __restrict__ ip,
                                                      const double* __restrict__ LHS, const double* __restrict__ LU,
                                                      const double* __restrict__ RHS, double* __restrict__ dQ,
-                                                     int row_begin, int row_end) {
-  int tid = xcd_block() * blockDim.x + threadIdx.x;
-  int t = row_begin + (tid >> 2), v = tid & 3;
+                                                     const int* __restrict__ chunks, int nchunks, int row_begin,
+                                                     int row_end) {
+  int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
+  // with a work list of nchunks chunks: row (tid / 4) % 16 of the wave's chunk
+  int t = row_begin + (tid >> 2);
+  if (chunks) {
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    t = w < nchunks ? chunks[w] + ((tid >> 2) & (kGsChunk - 1)) : row_end;
+  }
+  const int v = tid & 3;
   if (t >= row_end) return;
   int j = m.color_rows[t];
   int inst = inst_of(m, m.pos_inst, t);
@@ -683,8 +690,12 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_color(DevMesh m, const InstPar
 __global__ void __launch_bounds__(kMaxBlock) k_gs_row(DevMesh m, const InstParams* __restrict__ ip,
                                                       const double* __restrict__ LHS, const double* __restrict__ LU,
                                                       const double* __restrict__ RHS, double* __restrict__ dQ,
-                                                      int row_begin, int row_end) {
-  int t = row_begin + xcd_block() * blockDim.x + threadIdx.x;
+                                                      const int* __restrict__ chunks, int nchunks, int row_begin,
+                                                      int row_end) {
+  const int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
+  // with a work list of nchunks chunks: row tid % 16 of chunk tid / 16
+  int t = row_begin + tid;
+  if (chunks) t = tid / kGsChunk < nchunks ? chunks[tid / kGsChunk] + (tid & (kGsChunk - 1)) : row_end;
   if (t >= row_end) return;
   int j = m.color_rows[t];
   int inst = inst_of(m, m.pos_inst, t);
@@ -993,15 +1004,16 @@ void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st) {
 
 namespace {
 template <int kPre>
-void gs_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
+void gs_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, const int* ch, int nch, int b, int e,
+               hipStream_t st) {
   if (kPre == 0) {   // no prefetch: the gather form does not apply
-    hipLaunchKernelGGL((k_gs_color<0, false>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+    hipLaunchKernelGGL((k_gs_color<0, false>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, ch, nch, b, e);
     return;
   }
   if (m.tune.gs32_dpp)
-    hipLaunchKernelGGL((k_gs_color<kPre, true>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+    hipLaunchKernelGGL((k_gs_color<kPre, true>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, ch, nch, b, e);
   else
-    hipLaunchKernelGGL((k_gs_color<kPre, false>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+    hipLaunchKernelGGL((k_gs_color<kPre, false>), grid, block, 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, ch, nch, b, e);
 }
 }  // namespace
 
@@ -1013,16 +1025,26 @@ void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr
     int b = cr.off[c], e = sched[i].backward ? b + cr.nback[c] : cr.off[c + 1];
     if (e <= b) continue;
     const int bs = m.tune.gs_block;
+    const int* ch = nullptr;
+    int nch = 0, nrows = e - b;
+    if (cr.chunks) {
+      const int r = 2 * c + sched[i].backward;
+      ch = cr.chunks + cr.chunk_off[r];
+      nch = cr.chunk_off[r + 1] - cr.chunk_off[r];
+      nrows = kGsChunk * nch;
+      if (nrows == 0) continue;
+    }
     if (m.tune.gs_lanes == 1)
-      hipLaunchKernelGGL(k_gs_row, dim3(nblocks(e - b, bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, b, e);
+      hipLaunchKernelGGL(k_gs_row, dim3(nblocks(nrows, bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, ch, nch,
+                         b, e);
     else if (m.tune.gs32_pre == 0)
-      gs_launch<0>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
+      gs_launch<0>(m, s, dim3(nblocks(4 * nrows, bs)), dim3(bs), ch, nch, b, e, st);
     else if (m.tune.gs32_pre <= 4)
-      gs_launch<4>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
+      gs_launch<4>(m, s, dim3(nblocks(4 * nrows, bs)), dim3(bs), ch, nch, b, e, st);
     else if (m.tune.gs32_pre <= 6)
-      gs_launch<6>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
+      gs_launch<6>(m, s, dim3(nblocks(4 * nrows, bs)), dim3(bs), ch, nch, b, e, st);
     else
-      gs_launch<8>(m, s, dim3(nblocks(4 * (e - b), bs)), dim3(bs), b, e, st);
+      gs_launch<8>(m, s, dim3(nblocks(4 * nrows, bs)), dim3(bs), ch, nch, b, e, st);
   }
 }
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st) {

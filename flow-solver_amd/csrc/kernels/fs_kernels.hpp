@@ -76,6 +76,7 @@ struct DevMesh {
   int nn, ng, ne, nsub, nbe, nnz, ncolor, ninst;
   int maxdeg;               // largest number of edges at one node (off-diagonal blocks of a row)
   int slice_w;              // 0, or the uniform slice width: sbase[s] = 16 slice_w s (mixed sweeps compute it)
+  int xcd_run;              // xcd_block's run length (0: one contiguous range per XCD)
   Tuning tune;
   const int* inst_node;     // nn
   const int* inst_edge;     // ne
@@ -176,15 +177,30 @@ struct DevState {
   double* chunk_part;     // 2 per reduction chunk
 };
 
-// Workgroups are dispatched to the 8 XCDs round-robin.  Renumbered this way,
-// XCD x runs one contiguous range of the grid, so the neighbour gathers of
-// adjacent rows and faces are served by the same L2 instead of being fetched
-// into eight.  A bijection of [0, gridDim.x) for any grid size.
-__device__ __forceinline__ int xcd_block() {
-  const int G = gridDim.x, b = blockIdx.x;
-  const int q = G >> 3, r = G & 7, x = b & 7;
-  return x * q + (x < r ? x : r) + (b >> 3);
+// Workgroups are dispatched to the 8 XCDs round-robin.  Renumbered this way
+// (run 0), XCD x runs one contiguous range of the grid, so the neighbour
+// gathers of adjacent rows and faces are served by the same L2 instead of
+// being fetched into eight.  With run R > 0 the grid is dealt to the XCDs in
+// runs of R consecutive workgroups instead (the blocks past the last full
+// round of 8 R split contiguously): work that sits in one part of the grid,
+// the instances still running in a batch, then spreads over all eight.  A
+// bijection of [0, gridDim.x) for any grid size; which XCD runs a workgroup
+// changes no result.
+__device__ __forceinline__ int xcd_block(int run) {
+  const int G = gridDim.x, b = blockIdx.x, x = b & 7;
+  int base = 0, n = G, bb = b;
+  if (run > 0) {
+    const int S = 8 * run, full = G / S, k = b >> 3;
+    if (k < full * run) return (k / run) * S + x * run + (k % run);
+    base = full * S;
+    n = G - base;
+    bb = b - base;
+  }
+  const int q = n >> 3, r = n & 7;
+  return base + x * q + (x < r ? x : r) + (bb >> 3);
 }
+// the run length used once a batch's work lists are on
+constexpr int kXcdRun = 16;
 
 // Instance of element `idx` (a single-instance launch skips the lookup).
 __device__ __forceinline__ int inst_of(const DevMesh& m, const int* map, int idx) {
@@ -220,7 +236,15 @@ int gs_schedule(int ncolor, int subits, GsLaunch* out);
 struct ColorRanges {
   const int* off;     // ncolor+1, host
   const int* nback;   // ncolor, host: rows of a color visited by backward launches
+  // Optional work lists (both null: every row of a launch's range): range
+  // 2c + b (color c forward, b = 0, or backward, b = 1) launches only the
+  // kGsChunk-row chunks chunks[chunk_off[2c + b] ..], each a first position,
+  // so rows of stopped instances cost nothing once most have stopped.
+  const int* chunk_off = nullptr;  // 2 ncolor + 1, host
+  const int* chunks = nullptr;     // device
 };
+// rows per work-list chunk: one wave of the 4-lane sweeps
+constexpr int kGsChunk = 16;
 
 // Per-instance modes come from DevState::ip.
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t st);

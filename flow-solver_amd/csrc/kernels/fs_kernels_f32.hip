@@ -186,7 +186,7 @@ __host__ __device__ __forceinline__ int fused_row_stride(int maxdeg) { return 16
 template <int kOut>
 __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s) {
   extern __shared__ float s_rows[];
-  const int tid = xcd_block() * blockDim.x + threadIdx.x;
+  const int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
   const int n = tid >> 2, v = tid & 3;
   if (n >= m.nn) return;
   const int inst = inst_of(m, m.inst_node, n);
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstPa
                                                       float* __restrict__ LHS32, const float* __restrict__ D32,
                                                       float* __restrict__ LU32, float* __restrict__ RHS32,
                                                       float* __restrict__ dQ32) {
-  int t = xcd_block() * blockDim.x + threadIdx.x;
+  int t = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
@@ -510,6 +510,18 @@ __device__ __forceinline__ size_t gs_slot_base(const DevMesh& m, int t) {
   return m.slice_w ? (size_t)16 * m.slice_w * (size_t)(t >> 4) + (t & 15) : (size_t)m.sbase[t >> 4] + (t & 15);
 }
 
+// The row of a 4-lane sweep thread: the launch range from row_begin, or,
+// with a work list of nchunks chunks, row (tid / 4) % 16 of the wave's chunk
+// (one wave per chunk; its first position is wave-uniform, a scalar load);
+// row_end (no row) past the last chunk.
+__device__ __forceinline__ int gs_row(const int* __restrict__ chunks, int nchunks, int row_begin, int row_end,
+                                      int tid) {
+  if (!chunks) return row_begin + (tid >> 2);
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (w >= nchunks) return row_end;
+  return chunks[w] + ((tid >> 2) & (kGsChunk - 1));
+}
+
 // FP32 blocks.  Unscaled form: r = b - sum A dQ_nb, then lane v applies row
 // v of the stored diagonal inverse to the shuffled r.  Scaled form (kScale):
 // the blocks and right-hand side already carry the inverse (k_diag_lu32<true>
@@ -530,9 +542,10 @@ template <int kPre, bool kScale, bool kDpp>
 __global__ void __launch_bounds__(kMaxBlock) k_gs_color32(DevMesh m, const unsigned char* __restrict__ pos_act,
                                                        const float* __restrict__ LHS32, const float* __restrict__ LU32,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
-                                                       int row_begin, int row_end) {
-  int tid = xcd_block() * blockDim.x + threadIdx.x;
-  int t = row_begin + (tid >> 2), v = tid & 3;
+                                                       const int* __restrict__ chunks, int nchunks, int row_begin,
+                                                       int row_end) {
+  int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
+  int t = gs_row(chunks, nchunks, row_begin, row_end, tid), v = tid & 3;
   if (t >= row_end) return;
   const size_t base = gs_slot_base(m, t);
   const int deg = m.pdeg[t];
@@ -586,9 +599,10 @@ template <int kPre, bool kDpp>
 __global__ void __launch_bounds__(kMaxBlock) k_gs_color16(DevMesh m, const unsigned char* __restrict__ pos_act,
                                                        const unsigned short* __restrict__ LHS16,
                                                        const float* __restrict__ RHS32, float* __restrict__ dQ32,
-                                                       int row_begin, int row_end) {
-  int tid = xcd_block() * blockDim.x + threadIdx.x;
-  int t = row_begin + (tid >> 2), v = tid & 3;
+                                                       const int* __restrict__ chunks, int nchunks, int row_begin,
+                                                       int row_end) {
+  int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
+  int t = gs_row(chunks, nchunks, row_begin, row_end, tid), v = tid & 3;
   if (t >= row_end) return;
   const size_t base = gs_slot_base(m, t);
   const int deg = m.pdeg[t];
@@ -702,22 +716,25 @@ void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
 
 namespace {
 template <int kPre, bool kDpp>
-void gs32_launch_form(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
+void gs32_launch_form(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, const int* ch, int nch, int b,
+                      int e, hipStream_t st) {
   if (s.LHS16)
-    hipLaunchKernelGGL((k_gs_color16<kPre, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS16, s.RHS32, s.dQ32, b, e);
+    hipLaunchKernelGGL((k_gs_color16<kPre, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS16, s.RHS32, s.dQ32, ch, nch, b,
+                       e);
   else if (m.tune.gs32_scaled)
     hipLaunchKernelGGL((k_gs_color32<kPre, true, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32,
-                       s.dQ32, b, e);
+                       s.dQ32, ch, nch, b, e);
   else
     hipLaunchKernelGGL((k_gs_color32<kPre, false, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32,
-                       s.dQ32, b, e);
+                       s.dQ32, ch, nch, b, e);
 }
 template <int kPre>
-void gs32_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, int b, int e, hipStream_t st) {
+void gs32_launch(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, const int* ch, int nch, int b, int e,
+                 hipStream_t st) {
   if (m.tune.gs32_dpp)
-    gs32_launch_form<kPre, true>(m, s, grid, block, b, e, st);
+    gs32_launch_form<kPre, true>(m, s, grid, block, ch, nch, b, e, st);
   else
-    gs32_launch_form<kPre, false>(m, s, grid, block, b, e, st);
+    gs32_launch_form<kPre, false>(m, s, grid, block, ch, nch, b, e, st);
 }
 }  // namespace
 
@@ -730,13 +747,22 @@ void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& 
     int b = cr.off[c], e = sched[i].backward ? b + cr.nback[c] : cr.off[c + 1];
     if (e <= b) continue;
     const int bs = m.tune.gs32_block;
-    const dim3 grid(nblocks(4 * (e - b), bs));
+    const int* ch = nullptr;
+    int nch = 0, nthreads = 4 * (e - b);
+    if (cr.chunks) {
+      const int r = 2 * c + sched[i].backward;
+      ch = cr.chunks + cr.chunk_off[r];
+      nch = cr.chunk_off[r + 1] - cr.chunk_off[r];
+      nthreads = 4 * kGsChunk * nch;
+      if (nthreads == 0) continue;
+    }
+    const dim3 grid(nblocks(nthreads, bs));
     if (m.tune.gs32_pre <= 4)
-      gs32_launch<4>(m, s, grid, dim3(bs), b, e, st);
+      gs32_launch<4>(m, s, grid, dim3(bs), ch, nch, b, e, st);
     else if (m.tune.gs32_pre <= 6)
-      gs32_launch<6>(m, s, grid, dim3(bs), b, e, st);
+      gs32_launch<6>(m, s, grid, dim3(bs), ch, nch, b, e, st);
     else
-      gs32_launch<8>(m, s, grid, dim3(bs), b, e, st);
+      gs32_launch<8>(m, s, grid, dim3(bs), ch, nch, b, e, st);
   }
 }
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {

@@ -37,7 +37,7 @@ CASES = [
 MAX_STEPS, TOL = 400, 1e-10
 
 
-@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED, fsm.FS_PREC_F64])
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED, fsm.FS_PREC_F64, fsm.FS_PREC_MIXED16])
 def test_serving_results_equal_alone(precision):
     params = [_params(**kw) for _, kw in CASES]
     meshes = []
@@ -127,7 +127,7 @@ def _oracle_residual(mesh, p, Q, ts):
     return o["R"].copy()
 
 
-@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED, fsm.FS_PREC_F64])
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED, fsm.FS_PREC_F64, fsm.FS_PREC_MIXED16])
 def test_served_results_are_steady_states_of_the_oracle(precision):
     """Requests drawn like the serving workload's (bump channels and O-meshes,
     k=1 and k=2), one of them a 1M-triangle channel, served with the
