@@ -508,13 +508,26 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   inst_ch0[B] = (int)ch_inst.size();
   nc.nchunk = (int)ch_inst.size();
 
-  pos_inst_h_ = pos_inst;
   {
-    // work-list capacity: every chunk of every sweep range
-    long long cap = 64;
+    // each instance's positions within each color
+    color_segs_.assign(ncmax, {});
+    for (int c = 0; c < ncmax; ++c)
+      for (int t = color_off[c]; t < color_off[c + 1]; ++t) {
+        auto& v = color_segs_[c];
+        if (!v.empty() && v.back().inst == pos_inst[t] && v.back().t1 == t)
+          ++v.back().t1;
+        else
+          v.push_back({t, t + 1, pos_inst[t]});
+      }
+    dt_off_h_ = dt_off;
+    // work-list capacity: every chunk of every sweep range, of the nodes,
+    // edges and positions (one more per instance and color for the
+    // boundaries), and every tile
+    long long cap = 64 + (long long)dt_off.size();
     for (int c = 0; c < ncmax; ++c)
       cap += (color_off[c + 1] - color_off[c] + fsk::kGsChunk - 1) / fsk::kGsChunk +
              (color_nback[c] + fsk::kGsChunk - 1) / fsk::kGsChunk;
+    cap += 2 * (NN / fsk::kWorkChunk) + NE / fsk::kWorkChunk + 3LL * B + (long long)B * ncmax + 8;
     int rc0 = alloc(&d_chunks_, (size_t)cap);
     if (rc0) return rc0;
   }
@@ -724,31 +737,70 @@ int Engine::refresh_work_lists() {
     if (chunks_on_) {
       chunks_on_ = false;
       dm.xcd_run = 0;
+      dm.node_chunks = dm.edge_chunks = dm.pos_chunks = dm.tile_list = nullptr;
+      dm.n_node_chunks = dm.n_edge_chunks = dm.n_pos_chunks = dm.n_tile_list = 0;
       drop_graphs();
     }
     return FS_OK;
   }
   if (covered && 4 * act > 3 * chunk_nodes_) return FS_OK;
-  // an earlier list may still feed enqueued sweeps
+  // an earlier list may still feed enqueued launches
   FSE_HIP(hipStreamSynchronize(stream));
   const int nc = (int)color_nback.size();
+  constexpr int W = fsk::kWorkChunk;
   chunk_off_h_.assign(2 * (size_t)nc + 1, 0);
   chunk_h_.clear();
+  // appends the first elements of the chunks (of `width` elements from
+  // `origin`) covering [a, e); ranges arrive in increasing order
+  auto add = [&](int origin, int width, int a, int e, size_t from) {
+    if (e <= a) return;
+    for (int k = (a - origin) / width; k <= (e - 1 - origin) / width; ++k) {
+      const int first = origin + k * width;
+      if (chunk_h_.size() == from || chunk_h_.back() < first) chunk_h_.push_back(first);
+    }
+  };
+  // sweeps: per color, forward over the whole color and backward over its
+  // first nback rows, in 16-row chunks from the range's start
   for (int c = 0; c < nc; ++c)
     for (int bwd = 0; bwd < 2; ++bwd) {
       const int b = color_off[c], e = bwd ? b + color_nback[c] : color_off[c + 1];
-      for (int t0 = b; t0 < e; t0 += fsk::kGsChunk) {
-        const int t1 = std::min(e, t0 + fsk::kGsChunk);
-        for (int t = t0; t < t1; ++t)
-          if (inst[pos_inst_h_[t]].active) {
-            chunk_h_.push_back(t0);
-            break;
-          }
-      }
+      const size_t from = chunk_h_.size();
+      for (const Seg& g : color_segs_[c])
+        if (inst[g.inst].active) add(b, fsk::kGsChunk, std::max(b, g.t0), std::min(e, g.t1), from);
       chunk_off_h_[2 * (size_t)c + bwd + 1] = (int)chunk_h_.size();
     }
-  if (!chunk_h_.empty())
-    FSE_HIP(hipMemcpy(d_chunks_, chunk_h_.data(), chunk_h_.size() * sizeof(int), hipMemcpyHostToDevice));
+  // nodes, edges, color positions: 64-element chunks from 0
+  const size_t node0 = chunk_h_.size();
+  for (int i = 0; i < B; ++i)
+    if (inst[i].active) add(0, W, inst[i].noff, inst[i].noff + inst[i].nn, node0);
+  const size_t edge0 = chunk_h_.size();
+  for (int i = 0; i < B; ++i)
+    if (inst[i].active) add(0, W, inst[i].eoff, inst[i].eoff + inst[i].ne, edge0);
+  const size_t pos0 = chunk_h_.size();
+  for (int c = 0; c < nc; ++c)
+    for (const Seg& g : color_segs_[c])
+      if (inst[g.inst].active) add(0, W, g.t0, g.t1, pos0);
+  // LDS derivative tiles holding nodes of running k=2 instances
+  const size_t tile0 = chunk_h_.size();
+  if (dm.ndtile > 0)
+    for (int i = 0; i < B; ++i) {
+      const Instance& I = inst[i];
+      if (!I.active || I.prm.order != 3 || I.nn == 0) continue;
+      const int a = (int)(std::upper_bound(dt_off_h_.begin(), dt_off_h_.end(), I.noff) - dt_off_h_.begin()) - 1;
+      const int z = (int)(std::upper_bound(dt_off_h_.begin(), dt_off_h_.end(), I.noff + I.nn - 1) - dt_off_h_.begin()) - 1;
+      for (int t = a; t <= z; ++t)
+        if (chunk_h_.size() == tile0 || chunk_h_.back() < t) chunk_h_.push_back(t);
+    }
+  const size_t end = chunk_h_.size();
+  if (end > 0) FSE_HIP(hipMemcpy(d_chunks_, chunk_h_.data(), end * sizeof(int), hipMemcpyHostToDevice));
+  dm.node_chunks = d_chunks_ + node0;
+  dm.n_node_chunks = (int)(edge0 - node0);
+  dm.edge_chunks = d_chunks_ + edge0;
+  dm.n_edge_chunks = (int)(pos0 - edge0);
+  dm.pos_chunks = d_chunks_ + pos0;
+  dm.n_pos_chunks = (int)(tile0 - pos0);
+  dm.tile_list = d_chunks_ + tile0;
+  dm.n_tile_list = (int)(end - tile0);
   chunk_inst_.assign(B, 0);
   for (int i = 0; i < B; ++i) chunk_inst_[i] = inst[i].active ? 1 : 0;
   chunk_nodes_ = act;
@@ -805,15 +857,14 @@ int Engine::newton_iteration(int it, int key, bool record) {
 }
 
 int Engine::enqueue_step(int key, bool record) {
-  size_t ntot = ((size_t)dm.nn + dm.ng) * 4;
   // nQ = Q for real and ghost nodes (solve.C:319-325)
-  fsk::launch_copy(ds.nQ, ds.Q, ntot, stream);
+  fsk::launch_copy_state(dm, ds.nQ, ds.Q, stream);
   for (int it = 1; it <= iter; ++it) {
     int rc = newton_iteration(it, key, record && it == iter);
     if (rc) return rc;
   }
   // Q = nQ for real and ghost nodes (solve.C:913-931)
-  fsk::launch_copy(ds.Q, ds.nQ, ntot, stream);
+  fsk::launch_copy_state(dm, ds.Q, ds.nQ, stream);
   FSE_HIP(hipGetLastError());
   return FS_OK;
 }

@@ -240,9 +240,17 @@ class Engine {
   // instance outside the lists becomes active or the active nodes fall below
   // three quarters of those at the last build; every rebuild drops the
   // captured graphs (their launch grids change).
-  std::vector<int> pos_inst_h_;       // instance of each color position
+  // The node, edge and position kernels get lists of the 64-element chunks
+  // holding elements of running instances, the LDS derivative pass a list of
+  // its tiles (fsk::DevMesh::*_chunks, tile_list); the workgroups are dealt
+  // to the XCDs in runs meanwhile (fsk::DevMesh::xcd_run).
+  struct Seg {
+    int t0, t1, inst;   // color positions [t0, t1) of one instance
+  };
+  std::vector<std::vector<Seg>> color_segs_;   // per color, in position order
+  std::vector<int> dt_off_h_;         // first node of each LDS derivative tile
   std::vector<int> chunk_off_h_;      // 2 ncolor + 1
-  std::vector<int> chunk_h_;          // first position of each chunk
+  std::vector<int> chunk_h_;          // sweep chunks' first positions, then the other lists
   std::vector<char> chunk_inst_;      // instances the lists cover
   int* d_chunks_ = nullptr;
   bool chunks_on_ = false;

@@ -32,7 +32,7 @@ __device__ __forceinline__ double dmax(double x, double y) { return x >= y ? x :
 // K2: least-squares gradient, node-centric gather (gradient.C:363-403)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMaxBlock) k_gradient(DevMesh m, DevState s) {
-  int n = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
+  int n = work_elem(m.node_chunks, m.n_node_chunks, xcd_block(m.xcd_run) * blockDim.x + threadIdx.x, m.nn);
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
   if (!ip.active || !ip.grad) return;
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_gradient(DevMesh m, DevState s) {
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMaxBlock) k_derivatives(DevMesh m, DevState s) {
   int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
-  int i = tid >> 4, v = tid & 3, p = (tid >> 2) & 3;
+  int i = work_elem(m.node_chunks, m.n_node_chunks, tid >> 4, m.nn), v = tid & 3, p = (tid >> 2) & 3;
   if (i >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, i)];
   if (!ip.active || !ip.k2) return;
@@ -156,15 +156,21 @@ __device__ __forceinline__ double quad_bcast_d(double x) {
 // order, so the results are those of k_derivatives bit for bit.
 __global__ void __launch_bounds__(16 * kDerivTileNodes) k_derivatives_lds(DevMesh m, DevState s) {
   __shared__ double sq[4 * kDerivTileMaxU];
-  const int t = xcd_block(m.xcd_run);
+  int t = xcd_block(m.xcd_run);
+  if (m.tile_list) t = m.tile_list[t];   // one workgroup per listed tile
   const double* __restrict__ nQ = s.nQ;
+  const int i = m.dt_off[t] + (threadIdx.x >> 4), v = threadIdx.x & 3, p = (threadIdx.x >> 2) & 3;
+  bool mine = false;
+  if (i < m.dt_off[t + 1]) {
+    const InstParams& ip = s.ip[inst_of(m, m.inst_node, i)];
+    mine = ip.active && ip.k2;
+  }
+  // a tile none of whose nodes needs derivatives this iteration stages nothing
+  if (!__syncthreads_or(mine)) return;
   const int u0 = m.dt_uoff[t], nu = m.dt_uoff[t + 1] - u0;
   for (int k = threadIdx.x; k < 4 * nu; k += blockDim.x) sq[k] = nQ[4 * (size_t)m.dt_unodes[u0 + (k >> 2)] + (k & 3)];
   __syncthreads();
-  const int i = m.dt_off[t] + (threadIdx.x >> 4), v = threadIdx.x & 3, p = (threadIdx.x >> 2) & 3;
-  if (i >= m.dt_off[t + 1]) return;
-  const InstParams& ip = s.ip[inst_of(m, m.inst_node, i)];
-  if (!ip.active || !ip.k2) return;
+  if (!mine) return;
   const unsigned short* __restrict__ loc = m.st_loc;
   const double* __restrict__ Wt = m.st_Wt;
   const int o = m.st_off[i], e = m.st_off[i + 1];
@@ -252,7 +258,7 @@ __device__ __forceinline__ void candidate(bool venkat, double q, double qmin, do
 // of Barth's, 8 frozen (phi kept).
 __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
   int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
-  int n = tid >> 4, v = tid & 3, p = (tid >> 2) & 3;
+  int n = work_elem(m.node_chunks, m.n_node_chunks, tid >> 4, m.nn), v = tid & 3, p = (tid >> 2) & 3;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
   if (!ip.active) return;
@@ -354,7 +360,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMaxBlock) k_edge_flux(DevMesh m, DevState st) {
   int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
-  int e = tid >> 1, side = tid & 1;
+  int e = work_elem(m.edge_chunks, m.n_edge_chunks, tid >> 1, m.ne), side = tid & 1;
   if (e >= m.ne) return;
   const InstParams& ip = st.ip[inst_of(m, m.inst_edge, e)];
   if (!ip.active) return;
@@ -484,7 +490,7 @@ __global__ void __launch_bounds__(kBlock) k_bedge(DevMesh m, DevState s) {
 // diagonal block, so the quad reads each 128-byte block contiguously.
 __global__ void __launch_bounds__(kMaxBlock) k_node(DevMesh m, DevState s, int flags) {
   int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
-  int n = tid >> 2, v = tid & 3;
+  int n = work_elem(m.node_chunks, m.n_node_chunks, tid >> 2, m.nn), v = tid & 3;
   if (n >= m.nn) return;
   const int inst = inst_of(m, m.inst_node, n);
   const InstParams& p = s.ip[inst];
@@ -581,7 +587,7 @@ __global__ void __launch_bounds__(kBlock) k_zero_offdiag(DevMesh m, double* __re
 __global__ void __launch_bounds__(kMaxBlock) k_diag_lu(DevMesh m, const InstParams* __restrict__ ip,
                                                     const double* __restrict__ LHS, double* __restrict__ LU,
                                                     double* __restrict__ dQ) {
-  int t = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
+  int t = work_elem(m.pos_chunks, m.n_pos_chunks, xcd_block(m.xcd_run) * blockDim.x + threadIdx.x, m.nn);
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
@@ -728,15 +734,27 @@ __global__ void __launch_bounds__(kMaxBlock) k_gs_row(DevMesh m, const InstParam
 // nQ += dQ over the real nodes (solve.C:901-907)
 __global__ void __launch_bounds__(kBlock) k_update(DevMesh m, const InstParams* __restrict__ ip,
                                                    double* __restrict__ nQ, const double* __restrict__ dQ) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 4 * m.nn) return;
-  if (!ip[inst_of(m, m.inst_node, i >> 2)].active) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = work_elem(m.node_chunks, m.n_node_chunks, k >> 2, m.nn);
+  if (n >= m.nn) return;
+  if (!ip[inst_of(m, m.inst_node, n)].active) return;
+  const size_t i = 4 * (size_t)n + (k & 3);
   nQ[i] += dQ[i];
 }
 
 __global__ void __launch_bounds__(kBlock) k_copy(size_t n, double* __restrict__ dst, const double* __restrict__ src) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
+}
+
+// the listed nodes' 4 words (one thread per word)
+__global__ void __launch_bounds__(kBlock) k_copy_nodes(DevMesh m, double* __restrict__ dst,
+                                                       const double* __restrict__ src) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = work_elem(m.node_chunks, m.n_node_chunks, k >> 2, m.nn);
+  if (n >= m.nn) return;
+  const size_t i = 4 * (size_t)n + (k & 3);
+  dst[i] = src[i];
 }
 
 // one thread per real node or ghost; its instance's mode bits pick the copies
@@ -941,29 +959,47 @@ void launch_copy(double* dst, const double* src, size_t n, hipStream_t st) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n, dst, src);
 }
+void launch_copy_state(const DevMesh& m, double* dst, const double* src, hipStream_t st) {
+  if (!m.node_chunks) {
+    launch_copy(dst, src, ((size_t)m.nn + m.ng) * 4, st);
+    return;
+  }
+  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  if (nn > 0) hipLaunchKernelGGL(k_copy_nodes, dim3(nblocks(4 * nn)), dim3(kBlock), 0, st, m, dst, src);
+  launch_copy(dst + 4 * (size_t)m.nn, src + 4 * (size_t)m.nn, 4 * (size_t)m.ng, st);
+}
 void launch_checkpoint(const DevMesh& m, double* Q, double* Qsave, double* Qphase, const int* mode, hipStream_t st) {
   if (m.nn + m.nbe == 0) return;
   hipLaunchKernelGGL(k_checkpoint, dim3(nblocks(m.nn + m.nbe)), dim3(kBlock), 0, st, m, Q, Qsave, Qphase, mode);
 }
 void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_gradient, dim3(nblocks(m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
+  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  if (nn == 0) return;
+  hipLaunchKernelGGL(k_gradient, dim3(nblocks(nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
 }
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
   if (m.tune.deriv_lds && m.ndtile > 0)
-    hipLaunchKernelGGL(k_derivatives_lds, dim3(m.ndtile), dim3(16 * kDerivTileNodes), 0, st, m, s);
+    hipLaunchKernelGGL(k_derivatives_lds, dim3(m.tile_list ? m.n_tile_list : m.ndtile), dim3(16 * kDerivTileNodes),
+                       0, st, m, s);
   else
-    hipLaunchKernelGGL(k_derivatives, dim3(nblocks(16 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m,
+    hipLaunchKernelGGL(k_derivatives,
+                       dim3(nblocks(16 * work_count(m.node_chunks, m.n_node_chunks, m.nn), m.tune.cell_block)),
+                       dim3(m.tune.cell_block), 0, st, m,
                        s);
 }
 void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_limiter, dim3(nblocks(16 * m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
+  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  if (nn == 0) return;
+  hipLaunchKernelGGL(k_limiter, dim3(nblocks(16 * nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
 }
 void launch_edge_flux(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.ne == 0) return;
-  hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(2 * m.ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
+  const int ne = work_count(m.edge_chunks, m.n_edge_chunks, m.ne);
+  if (ne == 0) return;
+  hipLaunchKernelGGL(k_edge_flux, dim3(nblocks(2 * ne, m.tune.face_block)), dim3(m.tune.face_block), 0, st, m, s);
 }
 void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nbe == 0) return;
@@ -971,7 +1007,9 @@ void launch_bedge(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 void launch_node(const DevMesh& m, const DevState& s, int flags, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_node, dim3(nblocks(4 * m.nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st, m, s, flags);
+  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  if (nn == 0) return;
+  hipLaunchKernelGGL(k_node, dim3(nblocks(4 * nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st, m, s, flags);
 }
 void launch_zero_offdiag(const DevMesh& m, const DevState& s, hipStream_t st) {
   size_t n = (size_t)m.nnz * 16;
@@ -998,7 +1036,9 @@ int gs_schedule(int ncolor, int subits, GsLaunch* out) {
 
 void launch_diag_lu(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_diag_lu, dim3(nblocks(m.nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s.ip, s.LHS,
+  const int np = work_count(m.pos_chunks, m.n_pos_chunks, m.nn);
+  if (np == 0) return;
+  hipLaunchKernelGGL(k_diag_lu, dim3(nblocks(np, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s.ip, s.LHS,
                      s.LU, s.dQ);
 }
 
@@ -1049,7 +1089,9 @@ void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr
 }
 void launch_update(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_update, dim3(nblocks(4 * m.nn)), dim3(kBlock), 0, st, m, s.ip, s.nQ, s.dQ);
+  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  if (nn == 0) return;
+  hipLaunchKernelGGL(k_update, dim3(nblocks(4 * nn)), dim3(kBlock), 0, st, m, s.ip, s.nQ, s.dQ);
 }
 void launch_norms(const DevMesh& m, const DevState& s, const NormChunks& c, hipStream_t st) {
   if (c.nchunk > 0)

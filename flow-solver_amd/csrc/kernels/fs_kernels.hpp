@@ -77,6 +77,14 @@ struct DevMesh {
   int maxdeg;               // largest number of edges at one node (off-diagonal blocks of a row)
   int slice_w;              // 0, or the uniform slice width: sbase[s] = 16 slice_w s (mixed sweeps compute it)
   int xcd_run;              // xcd_block's run length (0: one contiguous range per XCD)
+  // Work lists of a batch most of whose instances have stopped (null: every
+  // element): kWorkChunk-element chunks holding elements of running
+  // instances, as first elements, and the LDS derivative tiles holding nodes
+  // of running instances.
+  const int* node_chunks;  int n_node_chunks;
+  const int* edge_chunks;  int n_edge_chunks;
+  const int* pos_chunks;   int n_pos_chunks;    // color positions
+  const int* tile_list;    int n_tile_list;
   Tuning tune;
   const int* inst_node;     // nn
   const int* inst_edge;     // ne
@@ -202,6 +210,20 @@ __device__ __forceinline__ int xcd_block(int run) {
 // the run length used once a batch's work lists are on
 constexpr int kXcdRun = 16;
 
+// The element of virtual index vn (a thread index divided by its lanes per
+// element) under a work list of 64-element chunks: element vn % 64 of the
+// (vn / 64)-th listed chunk, or `none` past the list; without a list, vn.  A
+// wave's virtual indices lie in one chunk for any lanes per element, so the
+// chunk's first element is a scalar load.
+constexpr int kWorkChunk = 64;
+__device__ __forceinline__ int work_elem(const int* __restrict__ chunks, int nchunks, int vn, int none) {
+  if (!chunks) return vn;
+  const int c = __builtin_amdgcn_readfirstlane(vn / kWorkChunk);
+  return c < nchunks ? chunks[c] + (vn % kWorkChunk) : none;
+}
+// elements a launch covers: every one, or the listed chunks
+inline int work_count(const int* chunks, int nchunks, int n) { return chunks ? kWorkChunk * nchunks : n; }
+
 // Instance of element `idx` (a single-instance launch skips the lookup).
 __device__ __forceinline__ int inst_of(const DevMesh& m, const int* map, int idx) {
   return m.ninst == 1 ? 0 : map[idx];
@@ -248,6 +270,9 @@ constexpr int kGsChunk = 16;
 
 // Per-instance modes come from DevState::ip.
 void launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
+// dst = src for the state with ghosts (4 words per node and ghost): the
+// nodes of the work list's chunks (every node without one) and every ghost
+void launch_copy_state(const DevMesh& m, double* dst, const double* src, hipStream_t st);
 void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st);

@@ -187,7 +187,7 @@ template <int kOut>
 __global__ void __launch_bounds__(kMaxBlock) k_resid_jac32(DevMesh m, DevState s) {
   extern __shared__ float s_rows[];
   const int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
-  const int n = tid >> 2, v = tid & 3;
+  const int n = work_elem(m.node_chunks, m.n_node_chunks, tid >> 2, m.nn), v = tid & 3;
   if (n >= m.nn) return;
   const int inst = inst_of(m, m.inst_node, n);
   const InstParams& p = s.ip[inst];
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_diag_lu32(DevMesh m, const InstPa
                                                       float* __restrict__ LHS32, const float* __restrict__ D32,
                                                       float* __restrict__ LU32, float* __restrict__ RHS32,
                                                       float* __restrict__ dQ32) {
-  int t = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
+  int t = work_elem(m.pos_chunks, m.n_pos_chunks, xcd_block(m.xcd_run) * blockDim.x + threadIdx.x, m.nn);
   if (t >= m.nn) return;
   int n = m.color_rows[t];
   if (!ip[inst_of(m, m.pos_inst, t)].active) return;
@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(kBlock) k_pos_active(DevMesh m, const InstPara
 __global__ void __launch_bounds__(kBlock) k_update_mixed(DevMesh m, const InstParams* __restrict__ ip,
                                                          double* __restrict__ nQ, const float* __restrict__ dQ32,
                                                          double* __restrict__ dQ) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = work_elem(m.node_chunks, m.n_node_chunks, blockIdx.x * blockDim.x + threadIdx.x, m.nn);
   if (n >= m.nn) return;
   if (!ip[inst_of(m, m.inst_node, n)].active) return;
   const float4 d = *reinterpret_cast<const float4*>(dQ32 + 4 * (size_t)(m.iau[n] - (m.nnz - m.nn)));
@@ -693,13 +693,15 @@ bool resid_jac32_fused_fits(const DevMesh& m) { return fused_block(m) > 0; }
 
 void launch_resid_jac32(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
+  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  if (nn == 0) return;
   if (!resid_jac32_fused(m, s)) {
-    hipLaunchKernelGGL(k_resid_jac32<0>, dim3(nblocks(4 * m.nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st,
+    hipLaunchKernelGGL(k_resid_jac32<0>, dim3(nblocks(4 * nn, m.tune.node_block)), dim3(m.tune.node_block), 0, st,
                        m, s);
     return;
   }
   const int b = fused_block(m);
-  const dim3 grid(nblocks(4 * m.nn, b)), block(b);
+  const dim3 grid(nblocks(4 * nn, b)), block(b);
   if (s.LHS16)
     hipLaunchKernelGGL(k_resid_jac32<2>, grid, block, fused_lds_bytes(m, b), st, m, s);
   else
@@ -707,7 +709,9 @@ void launch_resid_jac32(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 void launch_diag_lu32(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0 || resid_jac32_fused(m, s)) return;
-  const dim3 grid(nblocks(m.nn, m.tune.cell_block)), block(m.tune.cell_block);
+  const int np = work_count(m.pos_chunks, m.n_pos_chunks, m.nn);
+  if (np == 0) return;
+  const dim3 grid(nblocks(np, m.tune.cell_block)), block(m.tune.cell_block);
   if (m.tune.gs32_scaled)
     hipLaunchKernelGGL(k_diag_lu32<true>, grid, block, 0, st, m, s.ip, s.LHS32, s.D32, s.LU32, s.RHS32, s.dQ32);
   else
@@ -767,7 +771,9 @@ void launch_gs32_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& 
 }
 void launch_update_mixed(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  hipLaunchKernelGGL(k_update_mixed, dim3(nblocks(m.nn)), dim3(kBlock), 0, st, m, s.ip, s.nQ, s.dQ32, s.dQ);
+  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  if (nn == 0) return;
+  hipLaunchKernelGGL(k_update_mixed, dim3(nblocks(nn)), dim3(kBlock), 0, st, m, s.ip, s.nQ, s.dQ32, s.dQ);
 }
 
 }  // namespace fsk
