@@ -786,8 +786,11 @@ int Engine::refresh_work_lists() {
     for (int i = 0; i < B; ++i) {
       const Instance& I = inst[i];
       if (!I.active || I.prm.order != 3 || I.nn == 0) continue;
-      const int a = (int)(std::upper_bound(dt_off_h_.begin(), dt_off_h_.end(), I.noff) - dt_off_h_.begin()) - 1;
-      const int z = (int)(std::upper_bound(dt_off_h_.begin(), dt_off_h_.end(), I.noff + I.nn - 1) - dt_off_h_.begin()) - 1;
+      // the tiles containing the instance's first and last node
+      auto tile_of = [&](int node) {
+        return (int)(std::upper_bound(dt_off_h_.begin(), dt_off_h_.end(), node) - dt_off_h_.begin()) - 1;
+      };
+      const int a = tile_of(I.noff), z = tile_of(I.noff + I.nn - 1);
       for (int t = a; t <= z; ++t)
         if (chunk_h_.size() == tile0 || chunk_h_.back() < t) chunk_h_.push_back(t);
     }

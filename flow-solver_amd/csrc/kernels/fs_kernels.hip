@@ -1075,8 +1075,8 @@ void launch_gs_sweeps(const DevMesh& m, const DevState& s, const ColorRanges& cr
       if (nrows == 0) continue;
     }
     if (m.tune.gs_lanes == 1)
-      hipLaunchKernelGGL(k_gs_row, dim3(nblocks(nrows, bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, ch, nch,
-                         b, e);
+      hipLaunchKernelGGL(k_gs_row, dim3(nblocks(nrows, bs)), dim3(bs), 0, st, m, s.ip, s.LHS, s.LU, s.RHS, s.dQ, ch,
+                         nch, b, e);
     else if (m.tune.gs32_pre == 0)
       gs_launch<0>(m, s, dim3(nblocks(4 * nrows, bs)), dim3(bs), ch, nch, b, e, st);
     else if (m.tune.gs32_pre <= 4)

@@ -723,8 +723,8 @@ template <int kPre, bool kDpp>
 void gs32_launch_form(const DevMesh& m, const DevState& s, dim3 grid, dim3 block, const int* ch, int nch, int b,
                       int e, hipStream_t st) {
   if (s.LHS16)
-    hipLaunchKernelGGL((k_gs_color16<kPre, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS16, s.RHS32, s.dQ32, ch, nch, b,
-                       e);
+    hipLaunchKernelGGL((k_gs_color16<kPre, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS16, s.RHS32, s.dQ32, ch,
+                       nch, b, e);
   else if (m.tune.gs32_scaled)
     hipLaunchKernelGGL((k_gs_color32<kPre, true, kDpp>), grid, block, 0, st, m, s.pos_act, s.LHS32, s.LU32, s.RHS32,
                        s.dQ32, ch, nch, b, e);
