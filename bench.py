@@ -230,12 +230,15 @@ def parse():
 # checkpointed every RECOVERY[1] steps); the first failure of a k=1 solve
 # instead restarts its linear phase with the Barth limiter, frozen after
 # RECOVERY[2] limited steps, and the next one restarts it at first order
-# (RECOVERY[3]); unlimited k=1 goes non-physical on transonic channels that
-# choke, at any CFL target.  A solve a fallback acted on converges to the
+# (RECOVERY[3]) at the solve's own CFL target; unlimited k=1 goes
+# non-physical on transonic channels that choke, at any CFL target.  A solve
+# whose residual has not halved in STALL_STEPS_PER_SQRT_NODE * sqrt(nodes)
+# steps moves on to the next fallback.  A solve a fallback acted on converges to the
 # steady state of another discretization (the frozen-limiter one depends on
 # its iterate path): mixed and FP64 results are compared apart from it.
 WORKLOAD_CONTROLS = dict(CFL=200.0, ramp=30, subits=40)
 RECOVERY = (4, 10, 50, True)
+STALL_STEPS_PER_SQRT_NODE = 2.0
 # serving: a request's step limit grows with the square root of its node
 # count (the steps to converge do, measured on the batch)
 SERVE_STEPS_PER_SQRT_NODE = 8.0
@@ -253,12 +256,14 @@ def workload_params(W, cases):
 
 def workload_controls():
     return dict(WORKLOAD_CONTROLS, tol=1e-10, limiter="k=1 none, k=2 Barth", max_recoveries=RECOVERY[0],
-                checkpoint_every=RECOVERY[1], limiter_fallback_freeze=RECOVERY[2], first_order_fallback=RECOVERY[3])
+                checkpoint_every=RECOVERY[1], limiter_fallback_freeze=RECOVERY[2], first_order_fallback=RECOVERY[3],
+                stall_steps_per_sqrt_node=STALL_STEPS_PER_SQRT_NODE)
 
 
 def apply_controls(b):
     """The workloads' opt-in recovery policy on a batch."""
     b.set_recovery(*RECOVERY)
+    b.set_stall(STALL_STEPS_PER_SQRT_NODE)
 
 
 def outcome_groups(fsm, cases, status):
@@ -383,6 +388,7 @@ def batch_steady(a, fsm, cases, meshes, params, infos, prec):
            "converged_after_restore": int((conv & (rec > 0)).sum()),
            "limiter_fallbacks": int(((fbits & fsm.FALLBACK_LIMITER) != 0).sum()),
            "first_order_fallbacks": int(((fbits & fsm.FALLBACK_FIRST_ORDER) != 0).sum()),
+           "stall_fallbacks": int(((fbits & fsm.FALLBACK_STALL) != 0).sum()),
            "converged_with_fallback": int((conv & fallback).sum()),
            "outcomes": outcome_groups(fsm, cases, status),
            "unconverged_mach_min": min([cases[i]["mach"] for i in range(len(cases)) if not conv[i]], default=None)}
@@ -441,7 +447,7 @@ def bench_serving(a, precision):
     progress("serving: %d request meshes built" % len(meshes))
     srv = fsm.Server(precision=precision, max_steps=a.serve_max_steps, tol=1e-10, prep_threads=16,
                      max_recoveries=RECOVERY[0], checkpoint_every=RECOVERY[1], limiter_fallback=RECOVERY[2],
-                     first_order_fallback=RECOVERY[3],
+                     first_order_fallback=RECOVERY[3], stall_steps_per_sqrt_node=STALL_STEPS_PER_SQRT_NODE,
                      max_steps_per_sqrt_node=SERVE_STEPS_PER_SQRT_NODE)
     res = []
 
@@ -522,6 +528,7 @@ def bench_serving(a, precision):
             "restores": int(sum(r["recoveries"] for r in res)),
             "limiter_fallbacks": int(sum(1 for r in res if r["fallbacks"] & fsm.FALLBACK_LIMITER)),
             "first_order_fallbacks": int(sum(1 for r in res if r["fallbacks"] & fsm.FALLBACK_FIRST_ORDER)),
+            "stall_fallbacks": int(sum(1 for r in res if r["fallbacks"] & fsm.FALLBACK_STALL)),
             "step_limit_max": int(max(r["max_steps"] for r in res)) if res else None,
             "p50_time_to_converged_s": censored(lat, conv, 50), "p99_time_to_converged_s": censored(lat, conv, 99),
             "p50_admit_to_converged_s": censored(dev, conv, 50), "p99_admit_to_converged_s": censored(dev, conv, 99),

@@ -227,6 +227,12 @@ int fs_batch_set_recovery(fs_batch* b, int max_recoveries, int checkpoint_every,
   return FS_OK;
 }
 
+int fs_batch_set_stall(fs_batch* b, double steps_per_sqrt_node) {
+  if (!b || !(steps_per_sqrt_node >= 0)) return fail(FS_ERR_ARG, "bad arguments");
+  b->rec.stall_steps_per_sqrt_node = steps_per_sqrt_node;
+  return FS_OK;
+}
+
 int fs_batch_set_limiter_freeze(fs_batch* b, int steps) {
   if (!b || steps < 0) return fail(FS_ERR_ARG, "bad arguments");
   for (int i = 0; i < b->ninst(); ++i) b->inst(i).freeze = steps;

@@ -1067,58 +1067,84 @@ bool Engine::recover_or_checkpoint(int i, bool failed, double rn, const Recovery
                                    std::vector<int>& mode) {
   if (pol.max_recoveries <= 0 || !qsave_) return false;
   Instance& I = inst[i];
-  if (failed) {
-    if (t.recoveries >= pol.max_recoveries) return false;
-    // the fallbacks restart the current phase from its start (the phase
-    // checkpoint holds the state the phase began from)
+  // the fallbacks restart the current phase from its start (the phase
+  // checkpoint holds the state the phase began from); returns the step
+  // restored to, or -1 when none applies
+  bool cfl_reset = false;
+  auto fallback = [&]() -> int {
     const bool at_phase = I.phase_ts >= 0 && phase_id(I.prm, I.phase_ts + 1) == phase_id(I.prm, I.ts);
+    if (!at_phase) return -1;
     const Phase ph = phase_for(I.prm, I.ts);
     // the limiter fallback: an unlimited solve in a phase that would limit
     fs_params lim = I.prm;
     lim.limit = 1;
-    const bool limiter = pol.limiter_fallback > 0 && I.prm.limit == 0 && at_phase &&
-                         (phase_for(lim, I.ts).lim & fsk::LIM_ON);
-    // the first-order fallback: a solve in a reconstructing phase
-    const bool first = pol.first_order_fallback && !(t.fallbacks & kFallbackFirstOrder) && at_phase &&
-                       (ph.recon_flux == 1 || ph.k2);
-    int to = -1;
-    if (limiter) {
-      mode[i] = fsk::CKPT_RESTORE_PHASE;
-      to = I.phase_ts;
+    if (pol.limiter_fallback > 0 && I.prm.limit == 0 && (phase_for(lim, I.ts).lim & fsk::LIM_ON)) {
       I.prm.limit = 1;
       I.freeze = pol.limiter_fallback;
       t.fallbacks |= kFallbackLimiter;
-      I.saved_ts = -1;   // the latest checkpoint is later than the restart
-    } else if (first) {
-      mode[i] = fsk::CKPT_RESTORE_PHASE;
-      to = I.phase_ts;
+    } else if (pol.first_order_fallback && !(t.fallbacks & kFallbackFirstOrder) && (ph.recon_flux == 1 || ph.k2)) {
       // the first-order start-up phase never ends: nodal face states, no
-      // gradient, derivatives or limiter from here on
+      // gradient, derivatives or limiter from here on; the CFL target the
+      // restores lowered returns to the solve's own (the first-order
+      // iteration is the robust one, and at a quarter of the target its
+      // convergence is linear and slow)
       I.prm.foits = 1 << 28;
+      I.cfl_scale = 1.0;
+      cfl_reset = true;
       t.fallbacks |= kFallbackFirstOrder;
-      I.saved_ts = -1;
-    } else if (I.saved_ts >= 0) {
-      mode[i] = fsk::CKPT_RESTORE;
-      to = I.saved_ts;
-    } else if (I.phase_ts >= 0) {
-      mode[i] = fsk::CKPT_RESTORE_PHASE;
-      to = I.phase_ts;
     } else {
-      return false;
+      return -1;
     }
+    mode[i] = fsk::CKPT_RESTORE_PHASE;
+    I.saved_ts = -1;   // the latest checkpoint is later than the restart
+    return I.phase_ts;
+  };
+  auto restart = [&](int to) {
     ++t.recoveries;
     I.err = 0;
     I.ts = to;
     I.ramp_from = to;
-    I.cfl_scale *= 0.5;
     t.since_save = 0;
     t.phase = -1;
+  };
+  if (failed) {
+    if (t.recoveries >= pol.max_recoveries) return false;
+    int to = fallback();
+    if (to < 0) {
+      if (I.saved_ts >= 0) {
+        mode[i] = fsk::CKPT_RESTORE;
+        to = I.saved_ts;
+      } else if (I.phase_ts >= 0) {
+        mode[i] = fsk::CKPT_RESTORE_PHASE;
+        to = I.phase_ts;
+      } else {
+        return false;
+      }
+    }
+    restart(to);
+    if (!cfl_reset) I.cfl_scale *= 0.5;
     return true;
   }
   const int pid = phase_id(I.prm, I.ts);
   if (pid != t.phase) {
     t.phase = pid;
     t.r_best = rn;
+    t.r_mark = rn;
+    t.since_mark = 0;
+  } else if (rn < 0.5 * t.r_mark) {
+    t.r_mark = rn;
+    t.since_mark = 0;
+  } else {
+    ++t.since_mark;
+  }
+  if (pol.stall_steps_per_sqrt_node > 0 && t.recoveries < pol.max_recoveries &&
+      t.since_mark >= std::max(kStallMinSteps, (int)std::ceil(pol.stall_steps_per_sqrt_node * std::sqrt((double)I.nn)))) {
+    const int to = fallback();
+    if (to >= 0) {
+      t.fallbacks |= kFallbackStall;
+      restart(to);
+      return true;
+    }
   }
   ++t.since_save;
   if ((I.saved_ts < 0 || t.since_save >= std::max(1, pol.every)) && rn <= 10.0 * t.r_best) {

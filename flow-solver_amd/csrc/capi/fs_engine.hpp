@@ -86,9 +86,15 @@ struct RecoveryPolicy {
   // returns it to the start of that phase with first-order face states for
   // the rest of the solve
   bool first_order_fallback = false;
+  // > 0: a solve whose residual norm has not fallen below half its lowest
+  // mark of the phase for ceil(this * sqrt(nodes)) steps (at least
+  // kStallMinSteps) has stalled; a stall hands it to the next fallback that
+  // applies, at the CFL it runs with (no fallback left: it runs on)
+  double stall_steps_per_sqrt_node = 0.0;
 };
+constexpr int kStallMinSteps = 100;
 // fallback bits of RecoveryTrack::fallbacks (fs_capi.h FS_FALLBACK_*)
-constexpr int kFallbackLimiter = 1, kFallbackFirstOrder = 2;
+constexpr int kFallbackLimiter = 1, kFallbackFirstOrder = 2, kFallbackStall = 4;
 // per-solve bookkeeping of the policy
 struct RecoveryTrack {
   int recoveries = 0;
@@ -96,6 +102,8 @@ struct RecoveryTrack {
   int since_save = 0;
   int phase = -1;               // phase of the last step (a change resets r_best)
   double r_best = 0.0;          // smallest residual norm in the current phase
+  double r_mark = 0.0;          // stall test: the last mark of the phase
+  int since_mark = 0;           // steps since r_mark was set
 };
 
 // Device bytes that packing `m` into an engine of precision `prec` costs.
@@ -170,7 +178,9 @@ class Engine {
   // the limiter fallback, the start of its phase.  A healthy step is
   // checkpointed after the first healthy step and then every `every` steps,
   // unless its residual is above ten times the best of its phase, and the
-  // last step of a phase also saves the phase checkpoint.  Marks the copies
+  // last step of a phase also saves the phase checkpoint.  A healthy step
+  // that completes a stall (RecoveryPolicy::stall_steps_per_sqrt_node)
+  // starts the next fallback that applies, at the same CFL.  Marks the copies
   // in mode[i] (fsk::CheckpointMode bits); returns whether the instance was
   // restored.  A failed step that is not restored (policy off, or out of
   // recoveries) is left to the caller.

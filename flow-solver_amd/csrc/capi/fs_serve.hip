@@ -211,7 +211,7 @@ struct fs_server {
   void retire_finished(Group* g) {
     fse::Engine& E = *g->eng;
     const fse::RecoveryPolicy pol{cfg.max_recoveries, cfg.checkpoint_every, cfg.limiter_fallback,
-                                  cfg.first_order_fallback != 0};
+                                  cfg.first_order_fallback != 0, cfg.stall_steps_per_sqrt_node};
     std::vector<int> mode(E.ninst(), 0);
     std::vector<int> fin_inst, fin_status;
     std::vector<std::string> fin_err;
@@ -364,6 +364,7 @@ void fs_server_config_default(fs_server_config* c) {
   c->limiter_freeze = 0;
   c->limiter_fallback = 0;
   c->first_order_fallback = 0;
+  c->stall_steps_per_sqrt_node = 0.0;
 }
 
 int fs_server_create(const fs_server_config* c, fs_server** out) {
@@ -372,7 +373,7 @@ int fs_server_create(const fs_server_config* c, fs_server** out) {
     return fail(FS_ERR_ARG, "unknown precision");
   if (c->max_steps < 1 || !(c->tol > 0) || c->prep_threads < 1 || c->device_budget < 0 || c->max_recoveries < 0 ||
       c->checkpoint_every < 1 || !(c->max_steps_per_sqrt_node >= 0) || c->limiter_freeze < 0 ||
-      c->limiter_fallback < 0)
+      c->limiter_fallback < 0 || !(c->stall_steps_per_sqrt_node >= 0))
     return fail(FS_ERR_ARG, "bad server configuration");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FS_ERR_HIP, "no HIP device available");

@@ -129,14 +129,15 @@ class BatchInfo(C.Structure):
 
 
 INST_RUNNING, INST_CONVERGED, INST_MAX_STEPS, INST_FAILED, INST_PAUSED, INST_CANCELLED = range(6)
-FALLBACK_LIMITER, FALLBACK_FIRST_ORDER = 1, 2
+FALLBACK_LIMITER, FALLBACK_FIRST_ORDER, FALLBACK_STALL = 1, 2, 4
 
 
 class ServerConfig(C.Structure):
     _fields_ = [("device_budget", C.c_int64), ("precision", C.c_int), ("max_steps", C.c_int), ("tol", C.c_double),
                 ("prep_threads", C.c_int), ("use_graph", C.c_int), ("max_recoveries", C.c_int),
                 ("checkpoint_every", C.c_int), ("max_steps_per_sqrt_node", C.c_double), ("limiter_freeze", C.c_int),
-                ("limiter_fallback", C.c_int), ("first_order_fallback", C.c_int)]
+                ("limiter_fallback", C.c_int), ("first_order_fallback", C.c_int),
+                ("stall_steps_per_sqrt_node", C.c_double)]
 
 
 class RequestInfo(C.Structure):
@@ -205,6 +206,7 @@ def lib():
         "fs_batch_set_recovery": (i, [vp, i, i, i, i]),
         "fs_batch_inst_recovery": (i, [vp, i, C.POINTER(i), C.POINTER(i)]),
         "fs_batch_set_limiter_freeze": (i, [vp, i]),
+        "fs_batch_set_stall": (i, [vp, d]),
         "fs_roe_flux_batch": (i, [i, d, vp, vp, vp, vp, vp]),
         "fs_solve_steady": (i, [vp, i, d, C.POINTER(i), C.POINTER(d)]),
         "fs_solver_time_steps": (i, [vp, i, i, vp]),
@@ -606,6 +608,13 @@ class Batch:
         _check(lib().fs_batch_set_recovery(self._h, max_recoveries, checkpoint_every, limiter_fallback,
                                            1 if first_order_fallback else 0))
 
+    def set_stall(self, steps_per_sqrt_node):
+        """Stall test of the recovery policy (not in the reference; 0: off):
+        a solve whose residual has not halved its lowest mark of the phase
+        for max(100, ceil(steps_per_sqrt_node * sqrt(nodes))) steps starts
+        the next fallback that applies, at its CFL."""
+        _check(lib().fs_batch_set_stall(self._h, steps_per_sqrt_node))
+
     def set_limiter_freeze(self, steps):
         """Keep the limiter values once the limited phase has run `steps`
         steps (not in the reference; 0: off)."""
@@ -650,7 +659,7 @@ class Server:
 
     def __init__(self, precision=FS_PREC_MIXED, max_steps=2000, tol=1e-10, device_budget=0, prep_threads=8,
                  use_graph=True, max_recoveries=0, checkpoint_every=10, max_steps_per_sqrt_node=0.0,
-                 limiter_freeze=0, limiter_fallback=0, first_order_fallback=False):
+                 limiter_freeze=0, limiter_fallback=0, first_order_fallback=False, stall_steps_per_sqrt_node=0.0):
         cfg = ServerConfig()
         lib().fs_server_config_default(C.byref(cfg))
         cfg.precision, cfg.max_steps, cfg.tol = precision, max_steps, tol
@@ -660,6 +669,7 @@ class Server:
         cfg.limiter_freeze = limiter_freeze
         cfg.limiter_fallback = limiter_fallback
         cfg.first_order_fallback = 1 if first_order_fallback else 0
+        cfg.stall_steps_per_sqrt_node = stall_steps_per_sqrt_node
         h = C.c_void_p()
         _check(lib().fs_server_create(C.byref(cfg), C.byref(h)))
         self._h = h

@@ -385,8 +385,18 @@ int fs_batch_set_compaction(fs_batch* b, int on);
  * first step. */
 int fs_batch_set_recovery(fs_batch* b, int max_recoveries, int checkpoint_every, int limiter_fallback,
                           int first_order_fallback);
-/* the fallbacks that acted on a solve (bits) */
-enum { FS_FALLBACK_LIMITER = 1, FS_FALLBACK_FIRST_ORDER = 2 };
+/* Stall test of the recovery policy (off by default: 0; no counterpart in
+ * the reference).  A healthy instance whose residual norm has not fallen
+ * below half its lowest mark of the current phase for
+ * max(100, ceil(steps_per_sqrt_node * sqrt(nodes))) steps has stalled: the
+ * stall starts the next fallback of fs_batch_set_recovery that applies to it
+ * (limiter, then first order; it counts as a restore) at the CFL it runs
+ * with.  With no fallback left the instance runs on.  Instances that do not
+ * stall follow exactly the path they follow with the test off. */
+int fs_batch_set_stall(fs_batch* b, double steps_per_sqrt_node);
+/* the fallbacks that acted on a solve (bits; FS_FALLBACK_STALL: a stall
+ * started at least one of them) */
+enum { FS_FALLBACK_LIMITER = 1, FS_FALLBACK_FIRST_ORDER = 2, FS_FALLBACK_STALL = 4 };
 /* restores instance i has used and the FS_FALLBACK_* bits of the fallbacks
  * that acted on it (either pointer may be NULL) */
 int fs_batch_inst_recovery(const fs_batch* b, int i, int* restores, int* fallbacks);
@@ -447,6 +457,7 @@ typedef struct {
   int limiter_freeze;      /* fs_batch_set_limiter_freeze's steps (0: off) */
   int limiter_fallback;    /* fs_batch_set_recovery's limiter_fallback (0: off) */
   int first_order_fallback;  /* fs_batch_set_recovery's first_order_fallback (0: off) */
+  double stall_steps_per_sqrt_node;  /* fs_batch_set_stall's test (0: off) */
 } fs_server_config;
 typedef struct {
   int64_t id;

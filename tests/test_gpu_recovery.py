@@ -10,6 +10,9 @@ with recovery off.  The failing solve is a k=1 unlimited transonic channel
 non-physical about ten steps into its linear phase at any CFL target (the
 oracle fails it at CFL 200, 100, 50 and 20 alike).
 
+Stall test: a solve whose residual stops falling moves on to the next fallback;
+the solves that do not stall are untouched.
+
 Limiter freeze: once the limited phase has run the set number of steps the
 limiter values stop changing, and before that the solve is the reference's.
 
@@ -201,3 +204,48 @@ def test_first_order_fallback_reaches_the_first_order_steady_state(precision):
     print("oracle first order: %d steps; density L2 diff %.2e" % (ts, diff), flush=True)
     assert r / r0 <= 1e-11
     assert diff <= 1e-8
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_MIXED16, fsm.FS_PREC_MIXED])
+def test_stall_starts_the_next_fallback(precision):
+    """A transonic channel (180x60 bump, M 0.745, a request of the serving
+    stream) fails unlimited and restarts with the limiter fallback.  With
+    FP16 sweep operands its frozen-limiter iteration then cycles near a 1e-1
+    drop and reaches the step limit; with the stall test (2 sqrt(nodes) steps
+    without halving the residual) the stall starts the first-order fallback
+    and the solve converges.  With FP32 operands the frozen limiter takes
+    other values (the frozen state depends on the iterate path), the solve
+    converges without stalling, and the stall test changes nothing.  The
+    solve beside it never stalls and follows exactly the path it follows with
+    the test off."""
+    c = W.serving_cases(287)[139]
+    assert c["kind"] == "bump" and c["dims"] == (180, 60) and c["k"] == 1
+    cases = [(W.make_mesh(c), _params(mach=c["mach"], alpha=c["alpha"])),
+             (fsm.Mesh.bump(33, 17, 0.2, 1), _params(mach=0.5, alpha=0.5))]
+    for m, p in cases:
+        m.prepare(p)
+    out = {}
+    for stall in (0.0, 2.0):
+        b = fsm.Batch([m for m, _ in cases], [p for _, p in cases], precision)
+        b.init_freestream()
+        b.set_recovery(4, 10, 50, True)
+        b.set_stall(stall)
+        steps, drop, status = b.solve_steady(2000, 1e-10)
+        out[stall] = (steps, drop, status, [b.recoveries(i) for i in range(2)], [b.get(i, "Q") for i in range(2)])
+        print("stall %.0f: status %s steps %s drop %s recoveries %s" % (stall, status, steps, drop, out[stall][3]),
+              flush=True)
+        b.close()
+    steps, drop, status, rec, q = out[0.0]
+    steps_s, drop_s, status_s, rec_s, q_s = out[2.0]
+    assert status[1] == fsm.INST_CONVERGED and rec[1] == (0, 0)
+    assert status_s[1] == fsm.INST_CONVERGED and rec_s[1] == (0, 0) and steps_s[1] == steps[1]
+    assert np.array_equal(q_s[1], q[1])
+    assert rec[0][1] == fsm.FALLBACK_LIMITER
+    if precision == fsm.FS_PREC_MIXED16:
+        assert status[0] == fsm.INST_MAX_STEPS
+        assert status_s[0] == fsm.INST_CONVERGED
+        assert rec_s[0][1] == fsm.FALLBACK_LIMITER | fsm.FALLBACK_FIRST_ORDER | fsm.FALLBACK_STALL
+    else:
+        assert status[0] == fsm.INST_CONVERGED
+        assert status_s[0] == fsm.INST_CONVERGED and rec_s[0] == rec[0] and steps_s[0] == steps[0]
+        assert np.array_equal(q_s[0], q[0])
