@@ -23,9 +23,10 @@
  *   fso_linear_solve        reference build (dQ per step); dense solve on tiny systems
  *   fso_solve               reference build step histories
  *   analytic Jacobian       (fso_set_analytic_jacobian) restates jacobian.C:301-658, which the
- *                           reference never calls: pinned by closed forms (flux Jacobian
- *                           against finite differences of the physical flux; the Roe matrix
- *                           against |A| of the averaged state) in the tests
+ *                           reference's solve never calls: bit for bit against the reference
+ *                           build's own compute_approximate_jacobian_roe over every face at
+ *                           unequal reconstructed face states (oracle/ref_pointwise jac mode);
+ *                           equal to the forward difference where the face states are equal
  *   fso_build_pinv          numpy pinv and quadratic exactness (the reference build
  *                           cannot run the k=2 factorization: LAPACK is absent)
  *   fso_derivatives         quadratic exactness of hess/point_value (tests);

@@ -16,8 +16,14 @@
 //           face into SYS_MEM's LHS; Build_CRS; Compute_Edge_Weights
 //   device: Set_Boundary_Values and cv_calc_eigenvalue_contribution (device
 //           passes over the GRID's twin)
+// and, in jac mode, writes the reference's compute_approximate_jacobian_roe
+// accumulated over every face at the linear phase's reconstructed (unequal)
+// face states, with the state and limiter values it started from, for the
+// oracle's restatement of that Jacobian to be compared with:
+//   jac_nQ.bin (nn+ng+1)*4, jac_phi.bin (nn+1)*8, jac_lhs.bin nnz*16 doubles
+//   (the reference's 1-based arrays as they are), jac_meta.txt "nn ng nnz ts"
 //
-// Usage: ref_pointwise <case_dir> host|device
+// Usage: ref_pointwise <case_dir> host|device|jac
 //   case_dir holds euler.params (order 3), bc.dat and grid.mesh.
 #include <cmath>
 #include <cstdio>
@@ -70,10 +76,11 @@ static double* zeros(long n) { return (double*)calloc((size_t)n, sizeof(double))
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: ref_pointwise <case_dir> host|device\n");
+    std::fprintf(stderr, "usage: ref_pointwise <case_dir> host|device|jac\n");
     return 2;
   }
   const bool device = std::strcmp(argv[2], "device") == 0;
+  const bool jac = std::strcmp(argv[2], "jac") == 0;
   if (chdir(argv[1]) != 0) {
     std::fprintf(stderr, "cannot chdir to %s\n", argv[1]);
     return 2;
@@ -138,6 +145,44 @@ int main(int argc, char** argv) {
     }
   }
   const int phase_ts[3] = {p.foits, p.foits + 1, p.foits + p.soits + 1};
+
+  if (jac) {
+    // the reference's approximate Jacobian over every face, linear phase
+    const int nnz = smem.ia[nn + 1];
+    if (!smem.LHS) smem.LHS = zeros((long)nnz * 16);
+    std::memset(smem.LHS, 0, (size_t)nnz * 16 * sizeof(double));
+    const int ts = phase_ts[1];
+    grid.citer = ts;
+    for (int s = 1; s <= grid.nsubedges; ++s) {
+      const int e = grid.subedges[s * 2];
+      const int L = grid.edges[e * 2], R = grid.edges[e * 2 + 1];
+      const double nx = grid.xn_subedges[s * 3], ny = grid.xn_subedges[s * 3 + 1], len = grid.xn_subedges[s * 3 + 2];
+      double QL[4], QR[4], f[4];
+      Reconstruct(s, L, R, QL, QR, &grid, p, ts);
+      Roe_flux_centered(nx, ny, len, p.gamma, QL, QR, f);
+      compute_approximate_jacobian_roe(&grid, &smem, p, L, R, QL, QR, f, nx, ny, len);
+    }
+    struct {
+      const char* name;
+      const double* a;
+      long n;
+    } outs[3] = {{"jac_nQ.bin", grid.nQ, ntot}, {"jac_phi.bin", grid.phi, (long)(nn + 1) * 8},
+                 {"jac_lhs.bin", smem.LHS, (long)nnz * 16}};
+    for (const auto& o : outs) {
+      FILE* f = std::fopen(o.name, "wb");
+      if (!f || std::fwrite(o.a, sizeof(double), (size_t)o.n, f) != (size_t)o.n) {
+        std::fprintf(stderr, "cannot write %s\n", o.name);
+        return 2;
+      }
+      std::fclose(f);
+    }
+    FILE* f = std::fopen("jac_meta.txt", "w");
+    if (!f) return 2;
+    std::fprintf(f, "%d %d %d %d\n", nn, ng, nnz, ts);
+    std::fclose(f);
+    std::printf("wrote the approximate Jacobian of %d faces\n", grid.nsubedges);
+    return 0;
+  }
 
   if (!device) {
     // ---- reconstruction at face points and Gauss points ----
