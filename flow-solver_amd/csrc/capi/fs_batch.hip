@@ -71,6 +71,7 @@ int maybe_compact(fs_batch* b) {
   if (!rc && E.checkpoints_enabled()) rc = N->enable_checkpoints();
   if (rc) return rc;
   N->dm.tune = E.dm.tune;
+  N->defer_k2 = E.defer_k2;
   FSE_HIP(hipStreamSynchronize(E.stream));
   for (size_t k = 0; k < live.size(); ++k) {
     fse::Instance& from = b->inst(live[k]);
@@ -233,8 +234,17 @@ int fs_batch_set_stall(fs_batch* b, double steps_per_sqrt_node) {
   return FS_OK;
 }
 
+int fs_batch_defer_k2(fs_batch* b, int on) {
+  if (!b) return fail(FS_ERR_ARG, "null batch");
+  for (auto& e : b->eng) e->defer_k2 = on != 0;
+  return FS_OK;
+}
+
 int fs_batch_set_limiter_freeze(fs_batch* b, int steps) {
   if (!b || steps < 0) return fail(FS_ERR_ARG, "bad arguments");
+  // a freeze changes which limiter values a deferred evaluation would form
+  for (auto& e : b->eng)
+    if (int rc = e->ensure_k2_fields()) return rc;
   for (int i = 0; i < b->ninst(); ++i) b->inst(i).freeze = steps;
   return FS_OK;
 }

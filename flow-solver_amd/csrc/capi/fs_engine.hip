@@ -105,6 +105,8 @@ size_t footprint_bytes(const fs::Mesh& g, int prec, bool checkpoints) {
     b += nnz * 16 * 8 + nn * 16 * 8;
   b += (nn / fsk::kNormChunkNodes + 2) * (16 + 12);
   if (checkpoints) b += (nn + ng) * 4 * 8 * 2 + 4;
+  // the start state of the steps deferring the k=2 fields
+  if (nst > 0) b += (nn + ng) * 4 * 8;
   return b;
 }
 
@@ -574,6 +576,10 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
     if ((rc = alloc(&st.pos_act, nn))) return rc;
     FSE_HIP(hipMemset(st.pos_act, 0, nn));
   }
+  k2_stale_.assign(B, 0);
+  bool any_k2 = false;
+  for (int i = 0; i < B; ++i) any_k2 = any_k2 || prm[i].order == 3;
+  if (any_k2 && iter == 1 && (rc = alloc(&qpre_, ntot * 4))) return rc;
   if (precision == FS_PREC_MIXED) {
     if ((rc = alloc(&st.LHS32, (size_t)NSLOT * 16)) || (rc = alloc(&st.LU32, nn * 16)) ||
         (rc = alloc(&st.RHS32, nn * 4)) || (rc = alloc(&st.dQ32, nn * 4)))
@@ -672,7 +678,61 @@ fsk::InstParams Engine::params_for(int i, int it) const {
   // solve.C:335-338: update on the first iteration and every jufreq-th
   p.jac = (it == 1 || (it - 1) % jufreq == 0) ? 1 : 0;
   p.active = I.active ? 1 : 0;
+  if (k2_deferred(I)) {
+    p.k2_deferred = 1;
+    p.k2 = 0;
+    p.lim |= fsk::LIM_FROZEN;
+  }
   return p;
+}
+
+bool Engine::k2_deferred(const Instance& I) const {
+  return defer_k2 && !k2_trial_ && qpre_ && iter == 1 && I.freeze == 0 && phase_for(I.prm, I.ts).k2;
+}
+
+// after upload_params of a step: which instances' k=2 fields that step leaves
+// to ensure_k2_fields (a step that evaluates them makes them current)
+void Engine::mark_k2_stale() {
+  for (int i = 0; i < ninst(); ++i)
+    if (inst[i].active) k2_stale_[i] = h_ip_[i].k2_deferred ? 1 : 0;
+}
+
+int Engine::ensure_k2_fields(const std::vector<char>* want) {
+  const int B = ninst();
+  bool any = false;
+  for (int i = 0; i < B; ++i) any = any || (k2_stale_[i] && (!want || (*want)[i]));
+  if (!any) return FS_OK;
+  // the staging buffer and the device table may still feed a step
+  FSE_HIP(hipStreamSynchronize(stream));
+  for (int i = 0; i < B; ++i) {
+    const bool on = k2_stale_[i] && (!want || (*want)[i]);
+    // the controls of the instance's last step (its counter has not moved
+    // since), with the k=2 passes on
+    const bool trial = k2_trial_;
+    k2_trial_ = true;
+    fsk::InstParams p = params_for(i, 1);
+    k2_trial_ = trial;
+    p.active = on ? 1 : 0;
+    h_ip_[i] = p;
+  }
+  FSE_HIP(hipMemcpyAsync(d_ip_, h_ip_, (size_t)B * sizeof(fsk::InstParams), hipMemcpyHostToDevice, stream));
+  // every node and tile (the work lists may no longer hold a finished instance)
+  fsk::DevMesh m = dm;
+  m.node_chunks = nullptr;
+  m.n_node_chunks = 0;
+  m.tile_list = nullptr;
+  m.n_tile_list = 0;
+  m.xcd_run = 0;
+  fsk::DevState s = ds;
+  s.nQ = qpre_;
+  s.ip = d_ip_;
+  fsk::launch_derivatives(m, s, stream);
+  fsk::launch_limiter(m, s, stream);
+  FSE_HIP(hipGetLastError());
+  FSE_HIP(hipStreamSynchronize(stream));
+  for (int i = 0; i < B; ++i)
+    if (!want || (*want)[i]) k2_stale_[i] = 0;
+  return FS_OK;
 }
 
 int Engine::upload_params(int it) {
@@ -688,6 +748,8 @@ int Engine::upload_params(int it) {
 }
 
 int Engine::upload_params_custom(const fsk::InstParams& p) {
+  // a single pass may overwrite the deferred fields: evaluate them first
+  if (int rc = ensure_k2_fields()) return rc;
   FSE_HIP(hipStreamSynchronize(stream));
   for (int i = 0; i < ninst(); ++i) h_ip_[i] = p;
   FSE_HIP(hipMemcpyAsync(d_ip_, h_ip_, (size_t)ninst() * sizeof(fsk::InstParams), hipMemcpyHostToDevice, stream));
@@ -704,7 +766,7 @@ int Engine::step_key() const {
     any = true;
     Phase ph = phase_for(I.prm, I.ts);
     if (ph.gradient) key |= 1;
-    if (ph.k2) key |= 2;
+    if (ph.k2 && !k2_deferred(I)) key |= 2;
   }
   return any ? key : -1;
 }
@@ -860,8 +922,12 @@ int Engine::newton_iteration(int it, int key, bool record) {
 }
 
 int Engine::enqueue_step(int key, bool record) {
-  // nQ = Q for real and ghost nodes (solve.C:319-325)
-  fsk::launch_copy_state(dm, ds.nQ, ds.Q, stream);
+  // nQ = Q for real and ghost nodes (solve.C:319-325); instances deferring
+  // their k=2 fields keep this state
+  if (qpre_)
+    fsk::launch_copy_state_save(dm, d_ip_, ds.nQ, ds.Q, qpre_, stream);
+  else
+    fsk::launch_copy_state(dm, ds.nQ, ds.Q, stream);
   for (int it = 1; it <= iter; ++it) {
     int rc = newton_iteration(it, key, record && it == iter);
     if (rc) return rc;
@@ -881,6 +947,7 @@ int Engine::step(bool advance, bool graph, bool record) {
     if (key < 0) return FS_OK;
     int rc = refresh_work_lists();
     if (!rc) rc = upload_params(1);
+    if (!rc) mark_k2_stale();
     if (!rc) rc = enqueue_step(key, true);
     if (!rc) rc = enqueue_readback();
     if (rc) return rc;
@@ -919,6 +986,7 @@ int Engine::step_begin(bool advance, bool graph) {
   int rc = refresh_work_lists();
   if (!rc) rc = upload_params(1);
   if (rc) return rc;
+  mark_k2_stale();
   if (graph) {
     auto itg = graphs_.find(key);
     if (itg == graphs_.end()) {
@@ -962,7 +1030,8 @@ int Engine::step_end() {
 // evaluates the per-bedge terms; their sum runs here in the reference's
 // bedge order over the instance's walls (odd bc), so the result is the
 // reference's sequential sum.
-int Engine::body_forces(const int* citer, double* forces) {
+int Engine::body_forces(const int* citer, double* forces, const std::vector<char>* want) {
+  if (int rc = ensure_k2_fields(want)) return rc;
   const int B = ninst();
   for (int i = 0; i < B; ++i) {
     const fs_params& p = inst[i].prm;
@@ -1227,8 +1296,15 @@ int Engine::autotune(double* seconds) {
   auto t_start = std::chrono::steady_clock::now();
   int rc = sync();
   if (rc) return rc;
-  fsk::DevState s = ds;  // controls of the last step's first Newton iteration
+  // the trials run the k=2 passes as a step that does not defer them would
+  // (they overwrite the fields; deferred ones are still evaluated later from
+  // the kept state)
+  k2_trial_ = true;
+  rc = upload_params(1);
   const int key = step_key() < 0 ? 0 : step_key();
+  k2_trial_ = false;
+  if (rc) return rc;
+  fsk::DevState s = ds;  // controls of the last step's first Newton iteration
   fsk::ColorRanges cr{color_off.data(), color_nback.data()};
   const int blocks[] = {64, 128, 256, 512};
   hipStream_t st = stream;
@@ -1423,6 +1499,11 @@ int Engine::get_field(int i, int field, double* host, int64_t n) {
   int rc = field_count(i, field, &cnt);
   if (rc) return rc;
   if (n != cnt) return set_error(FS_ERR_ARG, "size mismatch: expected " + std::to_string(cnt));
+  if (field == FS_F_PHI || field == FS_F_HESS || field == FS_F_PV) {
+    std::vector<char> one(ninst(), 0);
+    one[i] = 1;
+    if ((rc = ensure_k2_fields(&one))) return rc;
+  }
   const Instance& I = inst[i];
   FSE_HIP(hipStreamSynchronize(stream));
   if (field == FS_F_LHS || field == FS_F_LHS32) {
@@ -1469,6 +1550,12 @@ int Engine::set_field(int i, int field, const double* host, int64_t n) {
   int rc = field_count(i, field, &cnt);
   if (rc) return rc;
   if (n != cnt) return set_error(FS_ERR_ARG, "size mismatch: expected " + std::to_string(cnt));
+  {
+    // the instance's deferred fields come from the state before this write
+    std::vector<char> one(ninst(), 0);
+    one[i] = 1;
+    if ((rc = ensure_k2_fields(&one))) return rc;
+  }
   const Instance& I = inst[i];
   FSE_HIP(hipStreamSynchronize(stream));
   if (field == FS_F_LHS32) return set_error(FS_ERR_ARG, "FP32 blocks are read-only");

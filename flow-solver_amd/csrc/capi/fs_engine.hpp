@@ -150,8 +150,25 @@ class Engine {
   bool step_ready() const { return !pending_ || hipStreamQuery(stream) != hipErrorNotReady; }
 
   // Compute_Body_Forces of every instance from its resident state (lift,
-  // drag: 2 per instance); citer[i] is the step the state comes from.
-  int body_forces(const int* citer, double* forces);
+  // drag: 2 per instance); citer[i] is the step the state comes from.  With
+  // `want`, only the instances it marks are evaluated (the others' values
+  // are not meaningful).
+  int body_forces(const int* citer, double* forces, const std::vector<char>* want = nullptr);
+
+  // Deferred k=2 fields.  In the k=2 phase neither the residual nor the
+  // Jacobian reads the k=2 derivatives, point values or limiter values (the
+  // Gauss-point interior flux is overwritten by the nodal one,
+  // residual.C:208-290, and the boundary Gauss branch is disabled,
+  // residual.C:446); the reference still evaluates them every step
+  // (solve.C:396-472), and its body forces, Cp and field reads see the values
+  // of the last step.  A step of an instance in that phase (one Newton
+  // iteration, no limiter freeze) therefore skips them and keeps the state it
+  // started from; ensure_k2_fields evaluates them from that state for the
+  // marked (or all) instances whose last step skipped them, giving the
+  // values the step would have written.  Every output that reads them calls
+  // it first.  defer_k2 = false evaluates them every step instead.
+  int ensure_k2_fields(const std::vector<char>* want = nullptr);
+  bool defer_k2 = true;
 
   // copies the per-instance device error flags into Instance::err (and clears
   // them on the device)
@@ -242,6 +259,11 @@ class Engine {
   double* qphase_ = nullptr;          // Q with ghosts at the start of the phase, 4 * (nn + ng)
   int* d_rmode_ = nullptr;            // ninst
   int* h_rmode_ = nullptr;            // pinned staging, ninst
+  double* qpre_ = nullptr;            // state (with ghosts) the last step of a deferring instance started from
+  std::vector<char> k2_stale_;        // per instance: its last step deferred the k=2 fields
+  bool k2_trial_ = false;             // autotune: the k=2 passes run as they would without deferral
+  bool k2_deferred(const Instance& I) const;
+  void mark_k2_stale();
   hipEvent_t ev_[8] = {};
   std::map<int, hipGraphExec_t> graphs_;
   // Sweep work lists (fsk::ColorRanges::chunks): once at most three quarters

@@ -251,9 +251,11 @@ struct fs_server {
     for (int st : fin_status) forces_ok = forces_ok || st == FS_INST_CONVERGED || st == FS_INST_MAX_STEPS;
     if (forces_ok) {
       std::vector<int> citer(E.ninst());
+      std::vector<char> want(E.ninst(), 0);
       f.assign(2 * (size_t)E.ninst(), 0.0);
       for (int k = 0; k < E.ninst(); ++k) citer[k] = E.inst[k].ts;
-      forces_ok = E.body_forces(citer.data(), f.data()) == FS_OK;
+      for (int k : fin_inst) want[k] = 1;
+      forces_ok = E.body_forces(citer.data(), f.data(), &want) == FS_OK;
     }
     for (size_t k = 0; k < fin_inst.size(); ++k) {
       const int i = fin_inst[k];

@@ -757,6 +757,35 @@ __global__ void __launch_bounds__(kBlock) k_copy_nodes(DevMesh m, double* __rest
   dst[i] = src[i];
 }
 
+// k_copy_nodes which also keeps the elements of instances whose k=2 fields
+// are deferred this step in `save`
+__global__ void __launch_bounds__(kBlock) k_copy_nodes_save(DevMesh m, const InstParams* __restrict__ ip,
+                                                            double* __restrict__ dst, const double* __restrict__ src,
+                                                            double* __restrict__ save) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = work_elem(m.node_chunks, m.n_node_chunks, k >> 2, m.nn);
+  if (n >= m.nn) return;
+  const size_t i = 4 * (size_t)n + (k & 3);
+  const double v = src[i];
+  dst[i] = v;
+  const InstParams& p = ip[inst_of(m, m.inst_node, n)];
+  if (p.active && p.k2_deferred) save[i] = v;
+}
+// the ghosts: ghost g is the outer state of boundary edge g
+__global__ void __launch_bounds__(kBlock) k_copy_ghosts_save(DevMesh m, const InstParams* __restrict__ ip,
+                                                             double* __restrict__ dst, const double* __restrict__ src,
+                                                             double* __restrict__ save) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 4 * m.ng) return;
+  const int g = k >> 2;
+  const size_t i = 4 * ((size_t)m.nn + g) + (k & 3);
+  const double v = src[i];
+  dst[i] = v;
+  if (g >= m.nbe) return;
+  const InstParams& p = ip[inst_of(m, m.inst_bedge, g)];
+  if (p.active && p.k2_deferred) save[i] = v;
+}
+
 // one thread per real node or ghost; its instance's mode bits pick the copies
 __global__ void __launch_bounds__(kBlock) k_checkpoint(DevMesh m, double* __restrict__ Q, double* __restrict__ Qsave,
                                                        double* __restrict__ Qphase, const int* __restrict__ mode) {
@@ -967,6 +996,13 @@ void launch_copy_state(const DevMesh& m, double* dst, const double* src, hipStre
   const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
   if (nn > 0) hipLaunchKernelGGL(k_copy_nodes, dim3(nblocks(4 * nn)), dim3(kBlock), 0, st, m, dst, src);
   launch_copy(dst + 4 * (size_t)m.nn, src + 4 * (size_t)m.nn, 4 * (size_t)m.ng, st);
+}
+void launch_copy_state_save(const DevMesh& m, const InstParams* ip, double* dst, const double* src, double* save,
+                            hipStream_t st) {
+  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  if (nn > 0) hipLaunchKernelGGL(k_copy_nodes_save, dim3(nblocks(4 * nn)), dim3(kBlock), 0, st, m, ip, dst, src, save);
+  if (m.ng > 0)
+    hipLaunchKernelGGL(k_copy_ghosts_save, dim3(nblocks(4 * m.ng)), dim3(kBlock), 0, st, m, ip, dst, src, save);
 }
 void launch_checkpoint(const DevMesh& m, double* Q, double* Qsave, double* Qphase, const int* mode, hipStream_t st) {
   if (m.nn + m.nbe == 0) return;

@@ -33,7 +33,12 @@ struct InstParams {
   int lim;         // LimMode bits; 0: phi = 1
   int jac;         // Jacobian update this iteration
   int active;      // instance takes part in this step
-  int pad[2];
+  // k=2 phase with the derivatives and limiter deferred (k2 0 and lim
+  // LIM_FROZEN then): no residual or Jacobian of that phase reads them, so
+  // they are evaluated when an output does, from the state the step started
+  // from, which the step's first copy keeps for the instance
+  int k2_deferred;
+  int pad;
 };
 
 // LIM_FROZEN: the limiter pass leaves phi as the last pass wrote it (the
@@ -273,6 +278,10 @@ void launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
 // dst = src for the state with ghosts (4 words per node and ghost): the
 // nodes of the work list's chunks (every node without one) and every ghost
 void launch_copy_state(const DevMesh& m, double* dst, const double* src, hipStream_t st);
+// the same copy, which also writes the real nodes and ghosts of instances
+// with ip[i].k2_deferred (and active) to `save`
+void launch_copy_state_save(const DevMesh& m, const InstParams* ip, double* dst, const double* src, double* save,
+                            hipStream_t st);
 void launch_gradient(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st);
 void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st);

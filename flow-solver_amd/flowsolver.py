@@ -216,6 +216,7 @@ def lib():
         "fs_batch_get_info": (i, [vp, vp]),
         "fs_batch_init_freestream": (i, [vp]),
         "fs_batch_use_graph": (i, [vp, i]),
+        "fs_batch_defer_k2": (i, [vp, i]),
         "fs_batch_set_active": (i, [vp, i, i]),
         "fs_batch_step": (i, [vp, vp, vp]),
         "fs_batch_solve_steady": (i, [vp, i, d, vp, vp, vp]),
@@ -565,6 +566,13 @@ class Batch:
 
     def use_graph(self, on=True):
         _check(lib().fs_batch_use_graph(self._h, 1 if on else 0))
+
+    def defer_k2(self, on=True):
+        """Deferred k=2 fields (default on; not in the reference): the k=2
+        phase's derivatives and limiter, which no residual or Jacobian of
+        that phase reads, are evaluated when an output reads them, from the
+        state the last step started from (the same bits as every step)."""
+        _check(lib().fs_batch_defer_k2(self._h, 1 if on else 0))
 
     def autotune(self):
         t = C.c_double()

@@ -347,6 +347,18 @@ void fs_batch_destroy(fs_batch* b);
 int fs_batch_get_info(const fs_batch* b, fs_batch_info* info);
 int fs_batch_init_freestream(fs_batch* b);
 int fs_batch_use_graph(fs_batch* b, int on);
+/* Deferred k=2 fields (on by default; no counterpart in the reference).  In
+ * the k=2 phase no residual or Jacobian reads the k=2 derivatives, point
+ * values or limiter values (residual.C:208-290 overwrites the Gauss-point
+ * flux with the nodal one; the boundary Gauss branch, residual.C:446, is
+ * disabled), which solve.C:396-472 still evaluates every step.  With
+ * deferral a step skips them and keeps the state it started from, and every
+ * output that reads them (body forces, field reads, single passes) first
+ * evaluates them from that state: all results are those of evaluation
+ * every step, bit for bit (instances with one Newton iteration per step and
+ * no limiter freeze; others evaluate them every step).  on = 0 evaluates
+ * them every step. */
+int fs_batch_defer_k2(fs_batch* b, int on);
 int fs_batch_autotune(fs_batch* b, double* seconds);
 int fs_batch_get_tuning(const fs_batch* b, int* values, int* from_cache);
 int fs_batch_set_tuning(fs_batch* b, const int* values);

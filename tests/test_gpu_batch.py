@@ -83,6 +83,49 @@ def test_batch_equals_alone(precision):
 
 
 @pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED, fsm.FS_PREC_MIXED16])
+def test_deferred_k2_fields_equal_every_step(precision):
+    """The k=2 phase's derivatives, point values and limiter values deferred
+    to the outputs that read them (the default) against evaluated every step
+    as the reference does: the residual norms and states of every step, the
+    fields read two steps before the end (the deferred run then goes on
+    deferring), and the body forces and fields at the end, bit for bit.  The
+    k=2 instances reach their k=2 phase by step 10 (foits 5, soits at most
+    4)."""
+    meshes, params = _build(precision)
+    n = len(meshes)
+    nsteps = 16
+    runs = {}
+    for defer in (False, True):
+        b = fsm.Batch(meshes, params, precision)
+        b.init_freestream()
+        b.use_graph(True)
+        b.defer_k2(defer)
+        hist, mid = [], None
+        for ts in range(1, nsteps + 1):
+            r2, dq = b.step()
+            hist.append((r2, dq, [b.get(i, "Q") for i in range(n)]))
+            if ts == nsteps - 2:
+                mid = [[b.get(i, f) for f in ("phi", "hess", "pv")] for i in range(n)]
+        forces = b.body_forces()
+        end = [[b.get(i, f) for f in ("phi", "hess", "pv")] for i in range(n)]
+        runs[defer] = (hist, mid, forces, end)
+        b.close()
+    (h0, m0, f0, e0), (h1, m1, f1, e1) = runs[False], runs[True]
+    for ts in range(nsteps):
+        assert np.array_equal(h0[ts][0], h1[ts][0]) and np.array_equal(h0[ts][1], h1[ts][1]), "norms, step %d" % (ts + 1)
+        for i in range(n):
+            assert np.array_equal(h0[ts][2][i], h1[ts][2][i]), "state of instance %d, step %d" % (i, ts + 1)
+    for i in range(n):
+        for k, f in enumerate(("phi", "hess", "pv")):
+            assert np.array_equal(m0[i][k], m1[i][k]), "%s of instance %d, step %d" % (f, i, nsteps - 2)
+            assert np.array_equal(e0[i][k], e1[i][k]), "%s of instance %d, end" % (f, i)
+    assert np.array_equal(f0, f1)
+    # the k=2 instances did run their k=2 phase, with limiter values below 1
+    k2 = [i for i, p in enumerate(params) if p.order == 3 and p.limit == 1]
+    assert k2 and all(np.min(e1[i][0]) < 1.0 for i in k2)
+
+
+@pytest.mark.parametrize("precision", [fsm.FS_PREC_F64, fsm.FS_PREC_MIXED, fsm.FS_PREC_MIXED16])
 def test_graph_replay_equals_eager(precision):
     p = _params(order=3, soits=2, limit=1, foits=2)
     m = fsm.Mesh.bump(33, 17, 0.2, 7)
