@@ -85,6 +85,7 @@ int maybe_compact(fs_batch* b) {
     // limiter values, which a frozen limiter keeps from step to step
     FSE_HIP(hipMemcpyAsync(N->ds.phi + to.noff, E.ds.phi + from.noff, (size_t)from.nn * sizeof(double),
                            hipMemcpyDeviceToDevice, N->stream));
+    N->phi_written((int)k);
     to.ts = from.ts;
     to.freeze = from.freeze;
     to.active = true;

@@ -577,6 +577,7 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
     FSE_HIP(hipMemset(st.pos_act, 0, nn));
   }
   k2_stale_.assign(B, 0);
+  phi_one_.assign(B, 1);   // phi starts at 1 everywhere (filled below)
   bool any_k2 = false;
   for (int i = 0; i < B; ++i) any_k2 = any_k2 || prm[i].order == 3;
   if (any_k2 && iter == 1 && (rc = alloc(&qpre_, ntot * 4))) return rc;
@@ -683,7 +684,16 @@ fsk::InstParams Engine::params_for(int i, int it) const {
     p.k2 = 0;
     p.lim |= fsk::LIM_FROZEN;
   }
+  // a limiter pass without a limiter writes phi = 1: nothing to do where
+  // every phi of the instance is 1 already
+  if (!(p.lim & (fsk::LIM_ON | fsk::LIM_FROZEN)) && phi_one_[i]) p.lim |= fsk::LIM_FROZEN;
   return p;
+}
+
+// what a limiter pass with mode `lim` leaves in instance i's phi
+void Engine::note_phi(int i, int lim) {
+  if (lim & fsk::LIM_FROZEN) return;
+  phi_one_[i] = (lim & fsk::LIM_ON) ? 0 : 1;
 }
 
 bool Engine::k2_deferred(const Instance& I) const {
@@ -694,7 +704,10 @@ bool Engine::k2_deferred(const Instance& I) const {
 // to ensure_k2_fields (a step that evaluates them makes them current)
 void Engine::mark_k2_stale() {
   for (int i = 0; i < ninst(); ++i)
-    if (inst[i].active) k2_stale_[i] = h_ip_[i].k2_deferred ? 1 : 0;
+    if (inst[i].active) {
+      k2_stale_[i] = h_ip_[i].k2_deferred ? 1 : 0;
+      note_phi(i, h_ip_[i].lim);
+    }
 }
 
 int Engine::ensure_k2_fields(const std::vector<char>* want) {
@@ -714,6 +727,7 @@ int Engine::ensure_k2_fields(const std::vector<char>* want) {
     k2_trial_ = trial;
     p.active = on ? 1 : 0;
     h_ip_[i] = p;
+    if (on) note_phi(i, p.lim);
   }
   FSE_HIP(hipMemcpyAsync(d_ip_, h_ip_, (size_t)B * sizeof(fsk::InstParams), hipMemcpyHostToDevice, stream));
   // every node and tile (the work lists may no longer hold a finished instance)
@@ -750,6 +764,8 @@ int Engine::upload_params(int it) {
 int Engine::upload_params_custom(const fsk::InstParams& p) {
   // a single pass may overwrite the deferred fields: evaluate them first
   if (int rc = ensure_k2_fields()) return rc;
+  // and may write any phi
+  std::fill(phi_one_.begin(), phi_one_.end(), 0);
   FSE_HIP(hipStreamSynchronize(stream));
   for (int i = 0; i < ninst(); ++i) h_ip_[i] = p;
   FSE_HIP(hipMemcpyAsync(d_ip_, h_ip_, (size_t)ninst() * sizeof(fsk::InstParams), hipMemcpyHostToDevice, stream));
@@ -767,6 +783,7 @@ int Engine::step_key() const {
     Phase ph = phase_for(I.prm, I.ts);
     if (ph.gradient) key |= 1;
     if (ph.k2 && !k2_deferred(I)) key |= 2;
+    if (!(params_for((int)(&I - inst.data()), 1).lim & fsk::LIM_FROZEN)) key |= 4;
   }
   return any ? key : -1;
 }
@@ -883,7 +900,7 @@ int Engine::newton_iteration(int it, int key, bool record) {
   hipStream_t st = stream;
   if (key & 1) fsk::launch_gradient(dm, s, st);
   if (key & 2) fsk::launch_derivatives(dm, s, st);
-  fsk::launch_limiter(dm, s, st);
+  if (key & 4) fsk::launch_limiter(dm, s, st);
   const fsk::ColorRanges cr = color_ranges();
   if (precision != FS_PREC_F64) {
     // boundary fluxes and blocks, then one node pass for the FP64 residual,
@@ -1304,6 +1321,8 @@ int Engine::autotune(double* seconds) {
   const int key = step_key() < 0 ? 0 : step_key();
   k2_trial_ = false;
   if (rc) return rc;
+  for (int i = 0; i < ninst(); ++i)
+    if (inst[i].active) note_phi(i, h_ip_[i].lim);   // the trials run the limiter pass
   fsk::DevState s = ds;  // controls of the last step's first Newton iteration
   fsk::ColorRanges cr{color_off.data(), color_nback.data()};
   const int blocks[] = {64, 128, 256, 512};
@@ -1556,6 +1575,7 @@ int Engine::set_field(int i, int field, const double* host, int64_t n) {
     one[i] = 1;
     if ((rc = ensure_k2_fields(&one))) return rc;
   }
+  if (field == FS_F_PHI) phi_one_[i] = 0;
   const Instance& I = inst[i];
   FSE_HIP(hipStreamSynchronize(stream));
   if (field == FS_F_LHS32) return set_error(FS_ERR_ARG, "FP32 blocks are read-only");

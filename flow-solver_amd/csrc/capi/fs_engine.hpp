@@ -169,6 +169,8 @@ class Engine {
   // it first.  defer_k2 = false evaluates them every step instead.
   int ensure_k2_fields(const std::vector<char>* want = nullptr);
   bool defer_k2 = true;
+  // instance i's phi was written from outside the engine's passes (re-packing)
+  void phi_written(int i) { phi_one_[i] = 0; }
 
   // copies the per-instance device error flags into Instance::err (and clears
   // them on the device)
@@ -264,6 +266,10 @@ class Engine {
   bool k2_trial_ = false;             // autotune: the k=2 passes run as they would without deferral
   bool k2_deferred(const Instance& I) const;
   void mark_k2_stale();
+  // per instance: every phi is 1 (a limiter pass without a limiter would
+  // write nothing new, so it is skipped and, with no other work, not launched)
+  std::vector<char> phi_one_;
+  void note_phi(int i, int lim);
   hipEvent_t ev_[8] = {};
   std::map<int, hipGraphExec_t> graphs_;
   // Sweep work lists (fsk::ColorRanges::chunks): once at most three quarters
