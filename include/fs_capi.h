@@ -390,8 +390,9 @@ int fs_batch_set_compaction(fs_batch* b, int on);
  * state of another discretization, one that depends on its iterate path.
  * With first_order_fallback != 0 the next failure of an instance in a
  * reconstructing phase (linear, or k=2) restarts that phase from its start
- * with first-order face states for the rest of its solve: it converges to the
- * first-order steady state.  Instances without a failure follow
+ * with first-order face states for the rest of its solve, at its own CFL
+ * target again (the halvings of the earlier restores are undone): it
+ * converges to the first-order steady state.  Instances without a failure follow
  * exactly the path they follow with recovery off.  Steps a restore rolled
  * back still count in `steps` and against max_steps.  Set it before the
  * first step. */
