@@ -578,6 +578,8 @@ int Engine::create(const std::vector<const fs::Mesh*>& meshes, const std::vector
   }
   k2_stale_.assign(B, 0);
   phi_one_.assign(B, 1);   // phi starts at 1 everywhere (filled below)
+  lim_set_.assign(B, 2);   // no list built yet
+  if (B > 1 && (rc = alloc(&d_lim_chunks_, nn / fsk::kWorkChunk + (size_t)B + 2))) return rc;
   bool any_k2 = false;
   for (int i = 0; i < B; ++i) any_k2 = any_k2 || prm[i].order == 3;
   if (any_k2 && iter == 1 && (rc = alloc(&qpre_, ntot * 4))) return rc;
@@ -710,6 +712,41 @@ void Engine::mark_k2_stale() {
     }
 }
 
+// after upload_params of a step
+int Engine::refresh_limiter_list() {
+  if (!d_lim_chunks_) return FS_OK;
+  const int B = ninst();
+  std::vector<char> need(B, 0);
+  long long nodes = 0;
+  for (int i = 0; i < B; ++i)
+    if (inst[i].active && !(h_ip_[i].lim & fsk::LIM_FROZEN)) {
+      need[i] = 1;
+      nodes += inst[i].nn;
+    }
+  if (need == lim_set_) return FS_OK;
+  lim_set_ = need;
+  if (4 * nodes > 3LL * dm.nn) {
+    dm.lim_chunks = nullptr;
+    dm.n_lim_chunks = 0;
+  } else {
+    lim_h_.clear();
+    for (int i = 0; i < B; ++i) {
+      const Instance& I = inst[i];
+      if (!need[i] || I.nn == 0) continue;
+      for (int c = I.noff / fsk::kWorkChunk; c <= (I.noff + I.nn - 1) / fsk::kWorkChunk; ++c)
+        lim_h_.push_back(c * fsk::kWorkChunk);
+    }
+    std::sort(lim_h_.begin(), lim_h_.end());
+    lim_h_.erase(std::unique(lim_h_.begin(), lim_h_.end()), lim_h_.end());
+    if (!lim_h_.empty())
+      FSE_HIP(hipMemcpy(d_lim_chunks_, lim_h_.data(), lim_h_.size() * sizeof(int), hipMemcpyHostToDevice));
+    dm.lim_chunks = d_lim_chunks_;
+    dm.n_lim_chunks = (int)lim_h_.size();
+  }
+  drop_graphs();
+  return FS_OK;
+}
+
 int Engine::ensure_k2_fields(const std::vector<char>* want) {
   const int B = ninst();
   bool any = false;
@@ -736,6 +773,8 @@ int Engine::ensure_k2_fields(const std::vector<char>* want) {
   m.n_node_chunks = 0;
   m.tile_list = nullptr;
   m.n_tile_list = 0;
+  m.lim_chunks = nullptr;
+  m.n_lim_chunks = 0;
   m.xcd_run = 0;
   fsk::DevState s = ds;
   s.nQ = qpre_;
@@ -965,6 +1004,7 @@ int Engine::step(bool advance, bool graph, bool record) {
     int rc = refresh_work_lists();
     if (!rc) rc = upload_params(1);
     if (!rc) mark_k2_stale();
+    if (!rc) rc = refresh_limiter_list();
     if (!rc) rc = enqueue_step(key, true);
     if (!rc) rc = enqueue_readback();
     if (rc) return rc;
@@ -1004,6 +1044,7 @@ int Engine::step_begin(bool advance, bool graph) {
   if (!rc) rc = upload_params(1);
   if (rc) return rc;
   mark_k2_stale();
+  if ((rc = refresh_limiter_list())) return rc;
   if (graph) {
     auto itg = graphs_.find(key);
     if (itg == graphs_.end()) {
@@ -1323,6 +1364,11 @@ int Engine::autotune(double* seconds) {
   if (rc) return rc;
   for (int i = 0; i < ninst(); ++i)
     if (inst[i].active) note_phi(i, h_ip_[i].lim);   // the trials run the limiter pass
+  // the limiter trials cover the nodes the step's work lists cover; the next
+  // step rebuilds the limiter's own list
+  dm.lim_chunks = nullptr;
+  dm.n_lim_chunks = 0;
+  lim_set_.assign(ninst(), 2);
   fsk::DevState s = ds;  // controls of the last step's first Newton iteration
   fsk::ColorRanges cr{color_off.data(), color_nback.data()};
   const int blocks[] = {64, 128, 256, 512};

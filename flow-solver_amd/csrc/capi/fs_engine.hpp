@@ -270,6 +270,14 @@ class Engine {
   // write nothing new, so it is skipped and, with no other work, not launched)
   std::vector<char> phi_one_;
   void note_phi(int i, int lim);
+  // the limiter pass's node list (fsk::DevMesh::lim_chunks, batches only):
+  // the chunks holding nodes of the instances whose pass has work, when they
+  // hold at most three quarters of the nodes; rebuilt when that set changes
+  // (the graphs are dropped then)
+  int* d_lim_chunks_ = nullptr;
+  std::vector<char> lim_set_;
+  std::vector<int> lim_h_;
+  int refresh_limiter_list();
   hipEvent_t ev_[8] = {};
   std::map<int, hipGraphExec_t> graphs_;
   // Sweep work lists (fsk::ColorRanges::chunks): once at most three quarters

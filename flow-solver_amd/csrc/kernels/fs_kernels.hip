@@ -258,7 +258,9 @@ __device__ __forceinline__ void candidate(bool venkat, double q, double qmin, do
 // of Barth's, 8 frozen (phi kept).
 __global__ void __launch_bounds__(kMaxBlock) k_limiter(DevMesh m, DevState s) {
   int tid = xcd_block(m.xcd_run) * blockDim.x + threadIdx.x;
-  int n = work_elem(m.node_chunks, m.n_node_chunks, tid >> 4, m.nn), v = tid & 3, p = (tid >> 2) & 3;
+  const bool own = m.lim_chunks != nullptr;
+  int n = work_elem(own ? m.lim_chunks : m.node_chunks, own ? m.n_lim_chunks : m.n_node_chunks, tid >> 4, m.nn);
+  const int v = tid & 3, p = (tid >> 2) & 3;
   if (n >= m.nn) return;
   const InstParams& ip = s.ip[inst_of(m, m.inst_node, n)];
   if (!ip.active) return;
@@ -1027,7 +1029,7 @@ void launch_derivatives(const DevMesh& m, const DevState& s, hipStream_t st) {
 }
 void launch_limiter(const DevMesh& m, const DevState& s, hipStream_t st) {
   if (m.nn == 0) return;
-  const int nn = work_count(m.node_chunks, m.n_node_chunks, m.nn);
+  const int nn = m.lim_chunks ? kWorkChunk * m.n_lim_chunks : work_count(m.node_chunks, m.n_node_chunks, m.nn);
   if (nn == 0) return;
   hipLaunchKernelGGL(k_limiter, dim3(nblocks(16 * nn, m.tune.cell_block)), dim3(m.tune.cell_block), 0, st, m, s);
 }

@@ -90,6 +90,9 @@ struct DevMesh {
   const int* edge_chunks;  int n_edge_chunks;
   const int* pos_chunks;   int n_pos_chunks;    // color positions
   const int* tile_list;    int n_tile_list;
+  // the limiter pass's own list (null: node_chunks): chunks holding nodes of
+  // the instances whose limiter pass has work this step
+  const int* lim_chunks;   int n_lim_chunks;
   Tuning tune;
   const int* inst_node;     // nn
   const int* inst_edge;     // ne
