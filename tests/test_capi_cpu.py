@@ -141,3 +141,15 @@ def test_mesh_from_arrays_validates_input():
     for bad_tri, bad_quad, bad_seg in (([(0, 1, 9)], quad, seg), (tri, [(1, 2, 5, 6)], seg), (tri, quad, [(0, 7)])):
         with pytest.raises(fsm.FlowSolverError):
             fsm.Mesh.from_arrays(x, bad_tri, bad_quad, [0, len(bad_seg)], bad_seg, [1])
+
+
+def test_solve_controls_reject_bad_arguments():
+    """The stall test, the k=2 deferral switch and the server's stall field
+    check their arguments before touching a device."""
+    L = fsm.lib()
+    assert L.fs_batch_set_stall(None, 2.0) == -1
+    assert L.fs_batch_defer_k2(None, 1) == -1
+    for bad in (-1.0, float("nan")):
+        with pytest.raises(fsm.FlowSolverError) as e:
+            fsm.Server(stall_steps_per_sqrt_node=bad)
+        assert e.value.code == -1
